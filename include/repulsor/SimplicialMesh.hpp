@@ -1,0 +1,253 @@
+#pragma once
+// Simplicial mesh of dimension DOM_DIM in R^3 with the reference's caching behaviour
+// (src/SimplicialMesh.hpp:13-869): the cluster tree and the block cluster tree are persistent
+// (built once from the coordinates frozen at construction), while energy/metric values are cleared
+// by SemiStaticUpdate. The GPU context (librepulsor_hip.so) is owned by the mesh, like the
+// reference's persistent cache entries.
+
+#include "Common.hpp"
+#include "Settings.hpp"
+#include "SimplexData.hpp"
+#include "ClusterTree.hpp"
+#include "BlockClusterTree.hpp"
+#include "../repulsor_hip.h"
+
+#include <map>
+#include <unordered_map>
+
+namespace Repulsor
+{
+    // RAII wrapper of an rp_ctx.
+    class DeviceContext
+    {
+        rp_ctx * ctx = nullptr;
+    public:
+        explicit DeviceContext( int device )
+        {
+            if( rp_ctx_create( device, &ctx ) != RP_OK )
+            {
+                ctx = nullptr;
+                throw std::runtime_error( std::string("rp_ctx_create failed: ") + rp_last_error() );
+            }
+        }
+        ~DeviceContext() { if( ctx ) { rp_ctx_destroy( ctx ); } }
+        DeviceContext( const DeviceContext & ) = delete;
+        DeviceContext & operator=( const DeviceContext & ) = delete;
+        rp_ctx * get() const { return ctx; }
+    };
+
+    inline bool Check( int status, const std::string & where )
+    {
+        if( status != RP_OK ) { eprint( where + ": " + rp_last_error() ); return false; }
+        return true;
+    }
+
+    template<int DOM_DIM_, int AMB_DIM_ = 3>
+    class SimplicialMesh
+    {
+        static_assert( AMB_DIM_ == 3, "only meshes in R^3 are supported" );
+        static_assert( DOM_DIM_ == 1 || DOM_DIM_ == 2, "only curves and surfaces are supported" );
+
+    public:
+        static constexpr int DOM_DIM  = DOM_DIM_;
+        static constexpr int AMB_DIM  = AMB_DIM_;
+        static constexpr int SIZE     = DOM_DIM + 1;
+        static constexpr int NEAR_DIM = SimplexData<DOM_DIM>::NEAR_DIM;
+        static constexpr int FAR_DIM  = SimplexData<DOM_DIM>::FAR_DIM;
+
+        using Tree_T = ClusterTree<DOM_DIM>;
+        using BCT_T  = BlockClusterTree<DOM_DIM>;
+
+        ClusterTreeSettings      cluster_tree_settings;
+        BlockClusterTreeSettings block_cluster_tree_settings;
+        AdaptivitySettings       adaptivity_settings;
+
+    protected:
+        int threads = 1;
+        int device  = 0;
+        std::vector<double> V_coords, V_coords_frozen, V_charges;
+        std::vector<Int>    simplices;
+        bool have_charges = false;
+
+        mutable std::unique_ptr<Tree_T> tree;
+        mutable std::unique_ptr<BCT_T>  bct;
+
+        // device state
+        mutable std::unique_ptr<DeviceContext> dev;
+        mutable bool dev_topology = false;
+        mutable bool dev_data     = false;
+
+    public:
+        // Row-major V x 3 coordinates and T x (n+1) simplices, as the reference's pointer constructor
+        // (src/SimplicialMesh.hpp:143-155).
+        template<typename ExtReal, typename ExtInt>
+        SimplicialMesh( const ExtReal * V_coords_, const LInt vertex_count_,
+                        const ExtInt * simplices_, const LInt simplex_count_,
+                        const LInt thread_count_ = 1, const double * vertex_charges_ = nullptr )
+        :   threads( (int)std::max<LInt>( 1, thread_count_ ) )
+        {
+            V_coords.assign( (size_t)(3 * vertex_count_), 0.0 );
+            for( LInt k = 0; k < 3 * vertex_count_; ++k ) { V_coords[(size_t)k] = (double)V_coords_[k]; }
+            V_coords_frozen = V_coords;
+            simplices.assign( (size_t)(SIZE * simplex_count_), 0 );
+            for( LInt k = 0; k < SIZE * simplex_count_; ++k ) { simplices[(size_t)k] = (Int)simplices_[k]; }
+            if( vertex_charges_ )
+            {
+                have_charges = true;
+                V_charges.assign( vertex_charges_, vertex_charges_ + vertex_count_ );
+            }
+            cluster_tree_settings.thread_count = threads;
+        }
+
+        LInt VertexCount()  const { return (LInt)V_coords.size() / 3; }
+        LInt SimplexCount() const { return (LInt)simplices.size() / SIZE; }
+        int  AmbDim()       const { return AMB_DIM; }
+        int  DomDim()       const { return DOM_DIM; }
+        int  ThreadCount()  const { return threads; }
+        void SetDevice( int d ) { device = d; }
+
+        const std::vector<double> & VertexCoordinates() const { return V_coords; }
+        const std::vector<Int>    & Simplices()         const { return simplices; }
+        const double * VertexCharges() const { return have_charges ? V_charges.data() : nullptr; }
+
+        // ---------------------------------------------------------------- persistent cache
+        const Tree_T & GetClusterTree() const
+        {
+            if( !tree )
+            {
+                ScopedTimer timer("SimplicialMesh::GetClusterTree");
+                const LInt T = SimplexCount();
+                std::vector<double> P_near( (size_t)(NEAR_DIM * T) ), P_far( (size_t)(FAR_DIM * T) );
+                std::vector<double> diffop( (size_t)(3 * SIZE * T) );
+                std::vector<Int> sorted( (size_t)(SIZE * T) );
+                ParallelFor( T, threads, [&]( LInt i )
+                {
+                    SimplexData<DOM_DIM> ker;
+                    ker.Read( V_coords.data(), simplices.data(), VertexCharges(), i );
+                    ker.WriteNear( &P_near[(size_t)(NEAR_DIM * i)] );
+                    ker.WriteFar ( &P_far [(size_t)(FAR_DIM  * i)] );
+                    ker.WriteDiffOp( &diffop[(size_t)(3 * SIZE * i)] );
+                    for( int l = 0; l < SIZE; ++l ) { sorted[(size_t)(SIZE * i + l)] = ker.s_simplex[l]; }
+                });
+                ClusterTreeSettings s = cluster_tree_settings;
+                if( s.thread_count <= 0 ) { s.thread_count = threads; }
+                tree = std::make_unique<Tree_T>( V_coords_frozen.data(), simplices.data(), T,
+                                                 P_near.data(), P_far.data(), diffop.data(), sorted.data(),
+                                                 s, threads );
+            }
+            return *tree;
+        }
+
+        const BCT_T & GetBlockClusterTree() const
+        {
+            if( !bct )
+            {
+                ScopedTimer timer("SimplicialMesh::GetBlockClusterTree");
+                // The adaptivity settings overwrite the near-field parameters (src/SimplicialMesh.hpp:592-595).
+                BlockClusterTreeSettings s = block_cluster_tree_settings;
+                s.near_field_separation_parameter   = adaptivity_settings.theta;
+                s.near_field_intersection_parameter = adaptivity_settings.intersection_theta;
+                s.max_refinement                    = adaptivity_settings.max_refinement;
+                bct = std::make_unique<BCT_T>( GetClusterTree(), s, threads );
+            }
+            return *bct;
+        }
+
+        // Reads new coordinates without rebuilding the tree (src/SimplicialMesh.hpp:354-376).
+        void SemiStaticUpdate( const double * V_new, const bool transp = false )
+        {
+            ScopedTimer timer("SimplicialMesh::SemiStaticUpdate");
+            const LInt V = VertexCount();
+            if( transp ) { for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { V_coords[(size_t)(3 * v + k)] = V_new[k * V + v]; } } }
+            else { std::copy( V_new, V_new + 3 * V, V_coords.begin() ); }
+            ClearCache();
+            if( tree )
+            {
+                const LInt T = SimplexCount();
+                std::vector<double> P_near( (size_t)(NEAR_DIM * T) ), P_far( (size_t)(FAR_DIM * T) );
+                ComputeNearFarData<DOM_DIM>( V_coords.data(), simplices.data(), VertexCharges(), T, P_near.data(), P_far.data(), threads );
+                tree->SemiStaticUpdate( P_near.data(), P_far.data() );
+            }
+            dev_data = false;
+        }
+
+        // ---------------------------------------------------------------- value cache
+        mutable std::map<std::string,double> scalar_cache;
+        mutable std::map<std::string,std::vector<double>> vector_cache;
+        mutable std::map<std::string,int> device_state;   // e.g. which metric the device holds
+
+        void ClearCache() const
+        {
+            scalar_cache.clear();
+            vector_cache.clear();
+            device_state.clear();
+        }
+
+        // ---------------------------------------------------------------- device
+        // Uploads topology once and the per-state data after every SemiStaticUpdate.
+        rp_ctx * Device() const
+        {
+            if( !dev ) { dev = std::make_unique<DeviceContext>( device ); dev_topology = false; dev_data = false; }
+            rp_ctx * c = dev->get();
+            if( !dev_topology )
+            {
+                const Tree_T & S = GetClusterTree();
+                const BCT_T & B = GetBlockClusterTree();
+                if( B.VeryNearFieldInteractionCount() > 0 )
+                {
+                    throw std::runtime_error( "very-near field is non-empty (" + std::to_string(B.VeryNearFieldInteractionCount())
+                        + " pairs); the adaptive very-near-field kernel is not implemented on the GPU path" );
+                }
+                auto chk = [&]( int st, const char * w ) { if( st != RP_OK ) { throw std::runtime_error( std::string(w) + ": " + rp_last_error() ); } };
+                chk( rp_upload_mesh( c, DOM_DIM, VertexCount(), SimplexCount(), simplices.data(), VertexCharges() ), "rp_upload_mesh" );
+                chk( rp_upload_tree( c, S.ClusterCount(), S.C_left.data(), S.C_right.data(), S.C_begin.data(), S.C_end.data(),
+                                     S.P_ordering.data(), S.pre_cols.data(), S.pre_vals.data() ), "rp_upload_tree" );
+                const NearTiles & Tl = B.tiles;
+                chk( rp_upload_near_tiles( c, Tl.leaf_count, Tl.leaf_begin.data(), Tl.leaf_end.data(), Tl.col_ptr.data(),
+                                           Tl.col_prim.data(), Tl.col_mask.data() ), "rp_upload_near_tiles" );
+                chk( rp_upload_far_pattern( c, B.far.rows, B.far.rp.data(), B.far.ci.data() ), "rp_upload_far_pattern" );
+                dev_topology = true;
+                dev_data = false;
+            }
+            if( !dev_data )
+            {
+                const Tree_T & S = GetClusterTree();
+                auto st = rp_upload_data( c, V_coords.data(), S.P_near.data(), S.C_far.data() );
+                if( st != RP_OK ) { throw std::runtime_error( std::string("rp_upload_data: ") + rp_last_error() ); }
+                dev_data = true;
+                device_state.clear();
+            }
+            return c;
+        }
+
+        bool DeviceReady() const { return dev != nullptr; }
+
+        // Converts near metric values from the device tile layout to the reference's CSR order
+        // (full symmetric pattern of NearCSR()).
+        void NearTileValuesToCSR( const double * tile_vals, const SymmetricCSR & A, double * csr_vals ) const
+        {
+            const NearTiles & Tl = GetBlockClusterTree().tiles;
+            for( Int l = 0; l < Tl.leaf_count; ++l )
+            {
+                const Int ib = Tl.leaf_begin[(size_t)l];
+                const Int nI = Tl.leaf_end[(size_t)l] - ib;
+                for( Int i = ib; i < ib + nI; ++i )
+                {
+                    LInt pos = A.rp[(size_t)i];
+                    LInt m = 0;
+                    for( LInt k = Tl.col_ptr[(size_t)l]; k < Tl.col_ptr[(size_t)l + 1]; ++k, ++m )
+                    {
+                        if( (Tl.col_mask[(size_t)k] >> (i - ib)) & 1u )
+                        {
+                            const LInt slot = Tl.val_ptr[(size_t)l] + m * nI + (i - ib);
+                            csr_vals[2 * pos]     = tile_vals[2 * slot];
+                            csr_vals[2 * pos + 1] = tile_vals[2 * slot + 1];
+                            ++pos;
+                        }
+                    }
+                }
+            }
+        }
+    };
+
+} // namespace Repulsor

@@ -1,0 +1,109 @@
+#pragma once
+// TangentPointMetric0 with the reference's public surface
+// (src/Metrics/TangentPointMetric0.hpp:29-126, src/Metrics/MetricBase.hpp:34-134,
+//  src/Metrics/MetricDimRestricted.hpp:28-297): MetricValues, MultiplyMetric (pointer and
+// vector overloads), MetricNorm. The metric blocks are assembled and applied on the GPU.
+
+#include "SimplicialMesh.hpp"
+
+namespace Repulsor
+{
+    template<typename Mesh_T>
+    class TangentPointMetric0
+    {
+    public:
+        static constexpr int DOM_DIM = Mesh_T::DOM_DIM;
+        static constexpr int AMB_DIM = 3;
+
+    protected:
+        double q, p, s;
+        Precision precision = Precision::FP64;
+
+    public:
+        explicit TangentPointMetric0( const double p_ ) : q( p_ ), p( 2 * p_ ), s( (2 * p_ - DOM_DIM) / p_ ) {}
+        TangentPointMetric0( const double q_, const double p_ ) : q( q_ ), p( p_ ), s( (p_ - DOM_DIM) / q_ ) {}
+
+        void SetPrecision( Precision prec ) { precision = prec; }
+        Precision GetPrecision() const { return precision; }
+
+        std::string ClassName() const
+        {
+            char buf[128];
+            std::snprintf( buf, sizeof(buf), "TangentPointMetric0<%d,3>(%g,%g,%g)%s", DOM_DIM, q, p, s, precision == Precision::Mixed ? "[mixed]" : "" );
+            return std::string(buf);
+        }
+
+        // Assembles the metric blocks on the device (cached until SemiStaticUpdate).
+        bool MetricValues( const Mesh_T & M ) const
+        {
+            try
+            {
+                rp_ctx * c = M.Device();
+                const std::string tag = ClassName();
+                auto it = M.device_state.find( "metric" );
+                if( it != M.device_state.end() && M.device_state.count( "metric:" + tag ) ) { return true; }
+                ScopedTimer timer( "TangentPointMetric0::MetricValues" );
+                if( !Check( rp_tp0_compute( c, RP_METRIC, q, p, (int)precision, nullptr ), ClassName() + "::MetricValues" ) ) { return false; }
+                for( auto k = M.device_state.begin(); k != M.device_state.end(); )
+                {
+                    if( k->first.rfind( "metric", 0 ) == 0 ) { k = M.device_state.erase(k); } else { ++k; }
+                }
+                M.device_state["metric"] = 1;
+                M.device_state["metric:" + tag] = 1;
+                return true;
+            }
+            catch( const std::exception & e ) { eprint( ClassName() + "::MetricValues: " + e.what() ); return false; }
+        }
+
+        // Metric blocks in the reference's layout: NF values over the full near CSR (2 per nnz),
+        // NF diagonal (2 per primitive), FF values over the far CSR, FF diagonal.
+        bool DownloadMetricValues( const Mesh_T & M, double * nf_csr, double * nf_diag, double * ff_vals, double * ff_diag ) const
+        {
+            if( !MetricValues( M ) ) { return false; }
+            const auto & B = M.GetBlockClusterTree();
+            std::vector<double> tile( (size_t)std::max<LInt>( 1, 2 * B.tiles.val_ptr.back() ) );
+            if( !Check( rp_download_metric( M.Device(), nf_csr ? tile.data() : nullptr, nf_diag, ff_vals, ff_diag ), ClassName() ) ) { return false; }
+            if( nf_csr ) { M.NearTileValuesToCSR( tile.data(), B.NearCSR(), nf_csr ); }
+            return true;
+        }
+
+        // Y = alpha * A X + beta * Y (MetricDimRestricted::MultiplyMetric, MetricDimRestricted.hpp:109-139).
+        // nrhs must equal 3; otherwise an error is printed and Y is left untouched.
+        bool MultiplyMetric( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,
+                             const double beta, double * Y, const LInt ldY, const int nrhs,
+                             const bool VF_flag = true, const bool NF_flag = true, const bool FF_flag = true,
+                             const bool device_ptrs = false ) const
+        {
+            if( nrhs != AMB_DIM )
+            {
+                eprint( ClassName() + "::MultiplyMetric: nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
+                return false;
+            }
+            if( !MetricValues( M ) ) { return false; }
+            ScopedTimer timer( "TangentPointMetric0::MultiplyMetric" );
+            return Check( rp_multiply_metric( M.Device(), alpha, X, ldX, beta, Y, ldY, nrhs,
+                                              VF_flag, NF_flag, FF_flag, device_ptrs ? RP_DEVICE_PTR : 0 ),
+                          ClassName() + "::MultiplyMetric" );
+        }
+
+        // Vector overload (MetricBase.hpp:46-59): X, Y are V x 3.
+        bool MultiplyMetric( const Mesh_T & M, const double alpha, const std::vector<double> & X,
+                             const double beta, std::vector<double> & Y ) const
+        {
+            return MultiplyMetric( M, alpha, X.data(), 3, beta, Y.data(), 3, 3 );
+        }
+
+        // sqrt(<X, A X>) (MetricBase.hpp:81-100).
+        double MetricNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs ) const
+        {
+            const LInt V = M.VertexCount();
+            std::vector<double> Xc( (size_t)(3 * V) ), Yc( (size_t)(3 * V), 0.0 );
+            for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { Xc[(size_t)(3 * v + k)] = X[ldX * v + k]; } }
+            if( !MultiplyMetric( M, 1.0, Xc.data(), 3, 0.0, Yc.data(), 3, nrhs ) ) { return 0; }
+            double s2 = 0;
+            for( size_t k = 0; k < Xc.size(); ++k ) { s2 += Xc[k] * Yc[k]; }
+            return std::sqrt( s2 );
+        }
+    };
+
+} // namespace Repulsor

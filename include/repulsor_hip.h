@@ -1,0 +1,122 @@
+/* C-ABI of the MI355X tangent-point library (librepulsor_hip.so).
+ *
+ * This is the drop-in boundary at the traversor seam of HenrikSchumacher/Repulsor: the header-only
+ * C++ host code (include/repulsor/) keeps building the cluster tree, the block cluster tree and the
+ * per-primitive data on the host, and hands them to these entry points, which run the
+ * TangentPointEnergy0 near/far kernels, derivative collection, TangentPointMetric0's
+ * MultiplyMetric and Solve on one AMD Instinct GPU (gfx950).
+ *
+ * Conventions: plain pointers and sizes only; all arrays row-major; int status return
+ * (0 = RP_OK); rp_last_error() gives a thread-local message. Host pointers unless the
+ * RP_DEVICE_PTR flag is passed. Calls are synchronous on return. A context is not reentrant.
+ */
+#ifndef REPULSOR_HIP_H
+#define REPULSOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP_OK            0
+#define RP_EINVAL        1
+#define RP_EHIP          2
+#define RP_ENOMEM        3
+#define RP_ESTAGNATION   4
+#define RP_EUNSUPPORTED  5
+#define RP_ENOTREADY     6
+
+/* rp_tp0_compute flags */
+#define RP_ENERGY        1u
+#define RP_DIFF          2u
+#define RP_METRIC        4u
+
+/* pointer-location flag */
+#define RP_DEVICE_PTR    1
+
+/* precision */
+#define RP_FP64          0
+#define RP_MIXED         1
+
+typedef struct rp_ctx rp_ctx;
+
+/* Thread-local message of the last failing call. */
+const char * rp_last_error(void);
+
+/* Library version string. */
+const char * rp_version(void);
+
+/* Context on HIP device `device` with its own stream (default priority).
+ * Replaces: the mesh's persistent cache entries that own ClusterTree/BlockClusterTree
+ * (reference src/SimplicialMesh.hpp:470-607). */
+int rp_ctx_create( int device, rp_ctx ** out );
+int rp_ctx_destroy( rp_ctx * ctx );
+
+/* Mesh topology: simplices T x (dom_dim+1) in the caller's order, optional vertex charges (V).
+ * Replaces: the SimplicialMesh pointer constructor (reference src/SimplicialMesh.hpp:143-185). */
+int rp_upload_mesh( rp_ctx * ctx, int dom_dim, int64_t vertex_count, int64_t simplex_count,
+                    const int32_t * simplices, const double * vertex_charges );
+
+/* Cluster tree arrays (depth-first numbering, C clusters), P_ordering (tree position -> simplex),
+ * and the mixed pre/post operator: pre_cols T x (n+1) vertex ids, pre_vals T x 4 x (n+1).
+ * Replaces: ClusterTree construction outputs C_left/C_right/C_begin/C_end/P_ordering and
+ * mi_pre/mi_post (reference src/ClusterTree.hpp:115-180, 410-639). */
+int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
+                    const int32_t * C_left, const int32_t * C_right,
+                    const int32_t * C_begin, const int32_t * C_end,
+                    const int32_t * P_ordering,
+                    const int32_t * pre_cols, const double * pre_vals );
+
+/* Near field as leaf tiles: per leaf I the primitive range [leaf_begin, leaf_end), the
+ * concatenated column primitives of all leaves sharing a non-admissible block with I
+ * (col_ptr has leaf_count+1 entries), and a row-bit mask per column (bit ii: pair
+ * (leaf_begin+ii, column) is a Near pair). Leaves hold at most 32 primitives.
+ * Replaces: BlockClusterTree::Near() pattern (reference src/BlockClusterTree.hpp:234-330). */
+int rp_upload_near_tiles( rp_ctx * ctx, int32_t leaf_count,
+                          const int32_t * leaf_begin, const int32_t * leaf_end,
+                          const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask );
+
+/* Far field: full symmetric CSR over clusters (rows = cluster_count).
+ * Replaces: BlockClusterTree::Far() (reference src/BlockClusterTree.hpp:321-324). */
+int rp_upload_far_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp, const int32_t * ci );
+
+/* Per-state data, after construction and after every SemiStaticUpdate:
+ * V_coords V x 3, P_near T x (1+3(n+1)+6) and C_far C x 10 in tree order.
+ * Replaces: ClusterTree::SemiStaticUpdate data path (reference src/ClusterTree.hpp:983-993). */
+int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near, const double * C_far );
+
+/* TangentPointEnergy0 kernels over the near and far fields (flags: RP_ENERGY | RP_DIFF | RP_METRIC).
+ * energy_out may be NULL. RP_METRIC stores the metric blocks on the device for rp_multiply_metric.
+ * Replaces: TP0_Traversor<...>::Compute (reference src/Energies/TangentPointEnergy0/TP0_Traversor.hpp:129-253). */
+int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precision, double * energy_out );
+
+/* Moment derivatives of the last RP_DIFF compute: P_D_near T x ND, P_D_far T x 10, original order.
+ * Replaces: ClusterTree::CollectDerivatives (reference src/ClusterTree.hpp:1115-1208). */
+int rp_download_derivatives( rp_ctx * ctx, double * P_D_near, double * P_D_far );
+
+/* Y = alpha * dE/dV + beta * Y for the last RP_DIFF compute (V x 3, leading dimension ldY).
+ * Replaces: SimplicialMesh::Assemble_ClusterTree_Derivatives (reference src/SimplicialMesh.hpp:651-675). */
+int rp_assemble_vertex_gradient( rp_ctx * ctx, double alpha, double beta, double * Y, int64_t ldY, int ptr_flags );
+
+/* Metric blocks of the last RP_METRIC compute in tile layout (nf: 2 x slots, ordered [leaf][column][row]),
+ * NF diagonal T x 2, FF values nnz x 2 (CSR order of the far pattern), FF diagonal C x 2.
+ * Any pointer may be NULL. Replaces: MetricValueContainer read-out (reference src/FMM/FMM_Configurator.hpp:5-20). */
+int rp_download_metric( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * ff_vals, double * ff_diag );
+
+/* Y = alpha * A X + beta * Y, X and Y V x nrhs (nrhs must be 3), ld >= 3.
+ * Replaces: MetricDimRestricted::MultiplyMetric (reference src/Metrics/MetricDimRestricted.hpp:109-139). */
+int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ldX,
+                        double beta, double * Y, int64_t ldY, int nrhs,
+                        int vf_flag, int nf_flag, int ff_flag, int ptr_flags );
+
+/* Device timing of the last call of each entry point, in milliseconds (HIP events on the
+ * context's stream). which: 0 = tp0_compute, 1 = assemble_vertex_gradient, 2 = multiply_metric,
+ * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM. */
+int rp_last_timing( rp_ctx * ctx, int which, double * ms );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REPULSOR_HIP_H */

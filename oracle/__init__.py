@@ -1,0 +1,162 @@
+"""ORACLE -- test infrastructure only.
+
+Python binding of the CPU restatement of the reference's tangent-point path (oracle/tp0_oracle.cpp,
+built into oracle/_build/liboracle.so by oracle/Makefile). Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this module, as the checker or the timed CPU baseline.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _load(name):
+    path = os.path.join(_HERE, "_build", name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} missing: run make -C oracle")
+    return C.CDLL(path)
+
+
+_lib = _load("liboracle.so")
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+
+
+def _p(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_simplex = _f("orc_simplex_data", None, [C.c_int, C.c_int64, _d, _i32, _d, _d, _d, _d, _i32])
+_tp0 = _f("orc_tp0_compute", C.c_double, [C.c_int, C.c_int64, C.c_int64, _d, _d, _i64, _i32, _i64, _i64, _i32, _i64,
+                                          C.c_double, C.c_double, C.c_int, _d, _d, _d, _d, _d, _d])
+_collect = _f("orc_collect_derivatives", None, [C.c_int, C.c_int64, C.c_int64, _i32, _i32, _i32, _i32, _i32, _d, _d, _d, _d])
+_hulls = _f("orc_dnearfar_to_hulls", None, [C.c_int, C.c_int64, _d, _i32, _d, _d, _d, _d])
+_assemble = _f("orc_assemble", None, [C.c_int, C.c_int64, C.c_int64, _i32, _d, C.c_double, C.c_double, _d, C.c_int64])
+_mm = _f("orc_multiply_metric", None, [C.c_int, C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32, _i32, _i32, _d,
+                                       _i64, _i32, _d, _d, _i64, _i32, _d, _d,
+                                       C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int])
+_allpairs = _f("orc_allpairs_energy", C.c_double, [C.c_int, C.c_int64, _d, C.c_double, C.c_double])
+_allpairs_metric = _f("orc_allpairs_metric", None, [C.c_int, C.c_int64, _d, C.c_double, C.c_double, _d])
+
+
+def c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def c32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def c64i(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def simplex_data(V, S, charges=None):
+    """SimplexDataKernel restatement: near (T x ND), far (T x 10), diffop (T x 3 x (n+1)), sorted simplices."""
+    V, S = c64(V), c32(S)
+    n = S.shape[1] - 1
+    T = S.shape[0]
+    nd = 1 + 3 * (n + 1) + 6
+    near = np.zeros((T, nd)); far = np.zeros((T, 10)); dop = np.zeros((T, 3, n + 1)); srt = np.zeros_like(S)
+    _simplex(n, T, _p(V), _p(S, _i32), _p(None if charges is None else c64(charges)), _p(near), _p(far), _p(dop), _p(srt, _i32))
+    return near, far, dop, srt
+
+
+def tp0_compute(n, P_near, C_far, near_pat, far_pat, q, p, flags):
+    """TP0_Traversor::Compute restatement. *_pat = (rp, ci, diag) full symmetric CSR. flags 1|2|4."""
+    P_near, C_far = c64(P_near), c64(C_far)
+    T, Cn = P_near.shape[0], C_far.shape[0]
+    nrp, nci, ndg = c64i(near_pat[0]), c32(near_pat[1]), c64i(near_pat[2])
+    frp, fci, fdg = c64i(far_pat[0]), c32(far_pat[1]), c64i(far_pat[2])
+    nd = P_near.shape[1]
+    out = {}
+    PDn = np.zeros((T, nd)) if flags & 2 else None
+    CDf = np.zeros((Cn, 10)) if flags & 2 else None
+    nfv = np.zeros((max(1, len(nci)), 2)) if flags & 4 else None
+    ffv = np.zeros((max(1, len(fci)), 2)) if flags & 4 else None
+    nfd = np.zeros((T, 2)) if flags & 4 else None
+    ffd = np.zeros((Cn, 2)) if flags & 4 else None
+    en = _tp0(n, T, Cn, _p(P_near), _p(C_far), _p(nrp, _i64), _p(nci if len(nci) else np.zeros(1, np.int32), _i32), _p(ndg, _i64),
+              _p(frp, _i64), _p(fci if len(fci) else np.zeros(1, np.int32), _i32), _p(fdg, _i64), q, p, flags,
+              _p(PDn), _p(CDf), _p(nfv), _p(ffv), _p(nfd), _p(ffd))
+    out["energy"] = en
+    if flags & 2:
+        out["P_D_near_tree"], out["C_D_far"] = PDn, CDf
+    if flags & 4:
+        out["NF"], out["FF"], out["NF_diag"], out["FF_diag"] = nfv[: len(nci)], ffv[: len(fci)], nfd, ffd
+    return out
+
+
+def collect_derivatives(n, tree, P_D_near_tree, C_D_far):
+    T = len(tree["P_ordering"])
+    Cn = len(tree["C_left"])
+    nd = 1 + 3 * (n + 1) + 6
+    Cd = c64(C_D_far).copy()
+    PDn = np.zeros((T, nd)); PDf = np.zeros((T, 10))
+    _collect(n, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
+             _p(c32(tree["C_end"]), _i32), _p(c32(tree["P_inverse_ordering"]), _i32), _p(c64(P_D_near_tree)), _p(Cd), _p(PDn), _p(PDf))
+    return PDn, PDf
+
+
+def dnearfar_to_hulls(V, S, N, F, charges=None):
+    V, S = c64(V), c32(S)
+    n = S.shape[1] - 1
+    B = np.zeros((S.shape[0], n + 1, 3))
+    _hulls(n, S.shape[0], _p(V), _p(S, _i32), _p(None if charges is None else c64(charges)), _p(c64(N)), _p(c64(F)), _p(B))
+    return B
+
+
+def assemble(S, nV, B, alpha=1.0, beta=0.0, Y=None):
+    S = c32(S)
+    n = S.shape[1] - 1
+    Y = np.zeros((nV, 3)) if Y is None else Y
+    _assemble(n, nV, S.shape[0], _p(S, _i32), _p(c64(B)), alpha, beta, _p(Y), Y.shape[1])
+    return Y
+
+
+def differential(n, V, S, tree, P_D_near_tree, C_D_far, charges=None):
+    """Full Assemble_ClusterTree_Derivatives restatement: collect, chain rule, assemble."""
+    N, F = collect_derivatives(n, tree, P_D_near_tree, C_D_far)
+    B = dnearfar_to_hulls(V, S, N, F, charges)
+    return assemble(S, len(V), B), N, F
+
+
+def multiply_metric(n, nV, tree, pre_cols, pre_vals, near_pat, nf_vals, nf_diag, far_pat, ff_vals, ff_diag,
+                    alpha, X, beta, Y, nf=True, ff=True):
+    T = len(tree["P_ordering"])
+    Cn = len(tree["C_left"])
+    X = c64(X)
+    z = np.zeros(1, np.int32)
+    nci = c32(near_pat[1]); fci = c32(far_pat[1])
+    _mm(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
+        _p(c32(tree["C_end"]), _i32), _p(c32(pre_cols), _i32), _p(c64(pre_vals)),
+        _p(c64i(near_pat[0]), _i64), _p(nci if len(nci) else z, _i32), _p(c64(nf_vals) if len(nci) else np.zeros(2)), _p(c64(nf_diag)),
+        _p(c64i(far_pat[0]), _i64), _p(fci if len(fci) else z, _i32), _p(c64(ff_vals) if len(fci) else np.zeros(2)), _p(c64(ff_diag)),
+        alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], int(nf), int(ff))
+    return Y
+
+
+def allpairs_energy(n, P_far, q, p):
+    P_far = c64(P_far)
+    return _allpairs(n, P_far.shape[0], _p(P_far), q, p)
+
+
+def allpairs_metric(n, P_far, q, p):
+    P_far = c64(P_far)
+    T = P_far.shape[0]
+    out = np.zeros((T, T, 2))
+    _allpairs_metric(n, T, _p(P_far), q, p, _p(out))
+    return out

@@ -1,0 +1,663 @@
+// ORACLE -- test infrastructure only.
+//
+// CPU restatement (FP64, single thread, reference summation order) of the reference's
+// tangent-point hot path. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// load this library, and only as the checker or the timed CPU baseline; the product path never
+// calls it.
+//
+// Restated reference functions (file:line in HenrikSchumacher/Repulsor):
+//   orc_simplex_data          SimplexDataKernel::ReadSimplex/WriteNear/WriteFar/WriteDiffOp
+//                             src/SimplexDataKernel.hpp:92-238
+//   orc_tp0_compute           TP0_Traversor::Compute / NF_Compute / FF_Compute
+//                             src/Energies/TangentPointEnergy0/TP0_Traversor.hpp:129-376,
+//                             FMM_Traversor::Compute src/FMM/FMM_Traversor.hpp:56-147,
+//                             FMM_Kernel_NF::loadS/loadT/writeS/writeT src/FMM/FMM_Kernel_NF.hpp:169-245,
+//                             FMM_Kernel_FF::loadS/loadT/writeS/writeT src/FMM/FMM_Kernel_FF.hpp:162-217,
+//                             TP0_Kernel_NF::Compute src/Energies/TangentPointEnergy0/TP0_Kernel_NF.hpp:140-338,
+//                             TP0_Kernel_FF::Compute src/Energies/TangentPointEnergy0/TP0_Kernel_FF.hpp:130-319
+//   orc_collect_derivatives   ClusterTree::CollectNearFieldDerivatives / CollectFarFieldDerivatives
+//                             src/ClusterTree.hpp:1115-1208, PercolateDown src/ClusterTree/Percolate_Parallel.hpp:187-287
+//   orc_dnearfar_to_hulls     SimplicialMeshDetails<2,3>::DNearFarToHulls src/SimplicialMesh/SimplicialMeshDetails.hpp:1097-1410
+//                             and <1,3> :529-736 (spec CodeGeneration/SimplicialMesh/Code_DNearFar_to_Hulls.m),
+//                             re-derived by hand from the moment map (a, a*hull, a*P) / (a, a*x, a*P).
+//   orc_assemble              DerivativeAssembler().Dot<3> src/SimplicialMesh.hpp:625-675
+//   orc_multiply_metric       MetricDimRestricted::MultiplyMetric src/Metrics/MetricDimRestricted.hpp:109-139,
+//                             ClusterTree::Pre/Post src/ClusterTree.hpp:770-943,
+//                             TP0_Traversor::MultiplyMetric src/Energies/TangentPointEnergy0/TP0_Traversor.hpp:384-505,
+//                             TP0_Kernel_MultiplyMetric_Tiny::ApplyBlock .../TP0_Kernel_MultiplyMetric_Tiny.hpp:89-138
+//   orc_allpairs_energy       TangentPointEnergy_AllPairs idea (src/Energies/TangentPointEnergy/TP_AllPairs_Traversor.hpp):
+//                             sum over all unordered pairs of a_i a_j E_ij, no tree.
+//
+// The `Power` of the Tensors library (absent from the reference checkout) is restated as binary
+// powering for integer exponents and std::pow for real exponents.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+typedef int32_t Int;
+typedef int64_t LInt;
+
+namespace
+{
+    struct Neumaier
+    {
+        double s = 0, c = 0;
+        void add( double x )
+        {
+            const double t = s + x;
+            if( std::abs(s) >= std::abs(x) ) { c += (s - t) + x; } else { c += (x - t) + s; }
+            s = t;
+        }
+        double value() const { return s + c; }
+    };
+
+    double IntPow( double x, LInt n )
+    {
+        if( n < 0 ) { return 1.0 / IntPow( x, -n ); }
+        double r = 1.0, b = x;
+        while( n > 0 ) { if( n & 1 ) { r *= b; } b *= b; n >>= 1; }
+        return r;
+    }
+
+    int lin_k3( int i, int j ) { if( i > j ) { int t = i; i = j; j = t; } return i * 3 - (i * (i - 1)) / 2 + (j - i); }
+
+    // Exponent specialisation of TP0_Traversor::Compute (TP0_Traversor.hpp:169-245).
+    struct Exponents
+    {
+        double q, p, q_half, p_half;
+        bool q_int, p_int;
+        LInt q_half_i, p_half_i;
+        int q_flag;    // 2, 0, or -1 (invalid)
+        double s, s_exp;
+
+        Exponents( double q_, double p_, int n )
+        {
+            q = q_; p = p_;
+            q_half = q / 2; p_half = p / 2;
+            q_half_i = (LInt)q_half; p_half_i = (LInt)p_half;
+            q_int = (q_half == (double)q_half_i);
+            p_int = (p_half == (double)p_half_i);
+            q_flag = (q_half >= 1) ? 2 : ((q_half == 0) ? 0 : -1);
+            s = (p - n) / q;
+            s_exp = -0.5 * (2.0 * (s - 1.0) + n);
+        }
+        double PowQ( double x, double e_real, LInt e_int ) const { return q_int ? IntPow(x, e_int) : std::pow(x, e_real); }
+        double PowP( double x, double e_real, LInt e_int ) const { return p_int ? IntPow(x, e_int) : std::pow(x, e_real); }
+    };
+
+    // One pair evaluation of TP0_Kernel_{NF,FF}::Compute. x,P: S side; y,Q: T side.
+    // DX/DY layout: [0] area part, then `ncoord` coordinate slots (near: hull vertices, far: center),
+    // then 6 projector slots. Returns the energy term (without symmetry factor, which is 1).
+    struct PairOut { double result, a0, a1; };
+
+    PairOut PairCompute( const Exponents & ex, int n, bool energy, bool diff, bool metric,
+        double a, const double * x, const double * P, double b, const double * y, const double * Q,
+        double * DX, double * DY, int n_hull_S, int n_hull_T, double S_scale, double T_scale )
+    {
+        double v[3], Pv[3] = {0,0,0}, Qv[3] = {0,0,0}, V[6] = {0,0,0,0,0,0};
+        double r2 = 0, rCosPhi_2 = 0, rCosPsi_2 = 0;
+        for( int i = 0; i < 3; ++i ) { v[i] = y[i] - x[i]; r2 += v[i] * v[i]; }
+        for( int i = 0; i < 3; ++i )
+        {
+            for( int j = 0; j < 3; ++j )
+            {
+                const int k = lin_k3(i,j);
+                Pv[i] += P[k] * v[j];
+                Qv[i] += Q[k] * v[j];
+                if( j >= i ) { V[k] = (1.0 + (double)(i != j)) * v[i] * v[j]; }
+            }
+            rCosPhi_2 += v[i] * Pv[i];
+            rCosPsi_2 += v[i] * Qv[i];
+        }
+        PairOut out{0,0,0};
+        if( energy || diff )
+        {
+            const double r_minus_p_minus_2 = ex.PowP( r2, -ex.p_half - 1, -ex.p_half_i - 1 );
+            const double r_minus_p = r_minus_p_minus_2 * r2;
+            double rCosPhi_q_minus_2, rCosPsi_q_minus_2, rCosPhi_q, rCosPsi_q;
+            if( ex.q_flag >= 2 )
+            {
+                rCosPhi_q_minus_2 = ex.PowQ( std::abs(rCosPhi_2), ex.q_half - 1, ex.q_half_i - 1 );
+                rCosPsi_q_minus_2 = ex.PowQ( std::abs(rCosPsi_2), ex.q_half - 1, ex.q_half_i - 1 );
+                rCosPhi_q = rCosPhi_q_minus_2 * rCosPhi_2;
+                rCosPsi_q = rCosPsi_q_minus_2 * rCosPsi_2;
+            }
+            else
+            {
+                rCosPhi_q_minus_2 = 1;
+                rCosPsi_q_minus_2 = 1;
+                rCosPhi_q = ex.PowQ( std::abs(rCosPhi_2), ex.q_half, ex.q_half_i );
+                rCosPsi_q = ex.PowQ( std::abs(rCosPsi_2), ex.q_half, ex.q_half_i );
+            }
+            const double Num = rCosPhi_q + rCosPsi_q;
+            const double E = Num * r_minus_p;
+            if( energy ) { out.result = a * E * b; }
+            if( diff )
+            {
+                const double factor = (ex.q_flag == 0) ? 0.0 : ex.q * r_minus_p;
+                const double K_xy   = (ex.q_flag == 0) ? 0.0 : factor * rCosPhi_q_minus_2;
+                const double K_yx   = (ex.q_flag == 0) ? 0.0 : factor * rCosPsi_q_minus_2;
+                const double H      = ex.p * r_minus_p_minus_2 * Num;
+                double dEdvx = 0, dEdvy = 0;
+                for( int i = 0; i < 3; ++i )
+                {
+                    const double dEdv = ((ex.q_flag == 0) ? 0.0 : K_xy * Pv[i] + K_yx * Qv[i]) - H * v[i];
+                    dEdvx += dEdv * x[i];
+                    dEdvy += dEdv * y[i];
+                    for( int ii = 0; ii < n_hull_S; ++ii ) { DX[1 + 3 * ii + i] -= S_scale * b * dEdv; }
+                    for( int ii = 0; ii < n_hull_T; ++ii ) { DY[1 + 3 * ii + i] += T_scale * a * dEdv; }
+                }
+                DX[0] += b * ( E - 0.5 * factor * rCosPhi_q + dEdvx );
+                DY[0] += a * ( E - 0.5 * factor * rCosPsi_q - dEdvy );
+                const double half_b_K_xy = 0.5 * b * K_xy;
+                const double half_a_K_yx = 0.5 * a * K_yx;
+                for( int k = 0; k < 6; ++k )
+                {
+                    DX[1 + 3 * n_hull_S + k] += half_b_K_xy * V[k];
+                    DY[1 + 3 * n_hull_T + k] += half_a_K_yx * V[k];
+                }
+            }
+        }
+        if( metric )
+        {
+            const double r4 = r2 * r2;
+            const double a_1 = -2.0 * std::pow( r2, ex.s_exp );
+            const double a_0 = 0.5 * (rCosPhi_2 + rCosPsi_2) / r4 * a_1;
+            out.a0 = a_0; out.a1 = a_1;
+        }
+        (void)n;
+        return out;
+    }
+}
+
+extern "C"
+{
+
+// SimplexDataKernel restatement for all simplices (original order).
+// near: T x (1 + 3(n+1) + 6), far: T x 10, diffop: T x 3 x (n+1) (columns = sorted simplex),
+// sorted: T x (n+1).
+void orc_simplex_data( int n, LInt T, const double * V, const Int * S, const double * charges,
+                       double * near, double * far, double * diffop, Int * sorted )
+{
+    const int SIZE = n + 1;
+    const int ND = 1 + 3 * SIZE + 6;
+    for( LInt t = 0; t < T; ++t )
+    {
+        Int s[3], ss[3];
+        for( int l = 0; l < SIZE; ++l ) { s[l] = S[SIZE * t + l]; ss[l] = s[l]; }
+        std::sort( ss, ss + SIZE );
+        double hull[3][3], center[3];
+        for( int l = 0; l < SIZE; ++l ) { for( int k = 0; k < 3; ++k ) { hull[l][k] = V[3 * (LInt)s[l] + k]; } }
+        double charge = charges ? charges[s[0]] : 1.0;
+        for( int k = 0; k < 3; ++k ) { center[k] = hull[0][k]; }
+        for( int l = 1; l < SIZE; ++l ) { for( int k = 0; k < 3; ++k ) { center[k] += hull[l][k]; } charge += charges ? charges[s[l]] : 1.0; }
+        for( int k = 0; k < 3; ++k ) { center[k] *= 1.0 / SIZE; }
+        charge *= 1.0 / SIZE;
+        double df[3][2];
+        for( int l = 0; l < n; ++l ) { for( int k = 0; k < 3; ++k ) { df[k][l] = V[3 * (LInt)ss[l+1] + k] - V[3 * (LInt)ss[0] + k]; } }
+        double g[2][2];
+        for( int r = 0; r < n; ++r ) { for( int c = 0; c < n; ++c ) { double acc = 0; for( int k = 0; k < 3; ++k ) { acc += df[k][r] * df[k][c]; } g[r][c] = acc; } }
+        double a = charge * (n == 2 ? 0.5 : 1.0);
+        double ginv[2][2];
+        if( n == 2 )
+        {
+            const double det = g[0][0] * g[1][1] - g[0][1] * g[1][0];
+            a *= std::sqrt(det);
+            ginv[0][0] =  g[1][1] / det; ginv[1][1] = g[0][0] / det;
+            ginv[0][1] = -g[0][1] / det; ginv[1][0] = -g[1][0] / det;
+        }
+        else
+        {
+            a *= std::sqrt(g[0][0]);
+            ginv[0][0] = 1.0 / g[0][0];
+        }
+        double dfd[2][3];
+        for( int l = 0; l < n; ++l ) { for( int k = 0; k < 3; ++k ) { double acc = 0; for( int m = 0; m < n; ++m ) { acc += ginv[l][m] * df[k][m]; } dfd[l][k] = acc; } }
+        double P[3][3];
+        for( int r = 0; r < 3; ++r ) { for( int c = 0; c < 3; ++c ) { double acc = 0; for( int l = 0; l < n; ++l ) { acc += df[r][l] * dfd[l][c]; } P[r][c] = (r == c ? 1.0 : 0.0) - acc; } }
+        double * nr = near + ND * t;
+        nr[0] = a;
+        for( int l = 0; l < SIZE; ++l ) { for( int k = 0; k < 3; ++k ) { nr[1 + 3 * l + k] = hull[l][k]; } }
+        const int pi[6] = {0,0,0,1,1,2}, pj[6] = {0,1,2,1,2,2};
+        for( int k = 0; k < 6; ++k ) { nr[1 + 3 * SIZE + k] = P[pi[k]][pj[k]]; }
+        double * fr = far + 10 * t;
+        fr[0] = a;
+        for( int k = 0; k < 3; ++k ) { fr[1 + k] = center[k]; }
+        for( int k = 0; k < 6; ++k ) { fr[4 + k] = P[pi[k]][pj[k]]; }
+        if( diffop )
+        {
+            double * D = diffop + 3 * SIZE * t;
+            for( int k = 0; k < 3; ++k )
+            {
+                D[SIZE * k] = 0;
+                for( int l = 0; l < n; ++l ) { D[SIZE * k] -= dfd[l][k]; D[SIZE * k + l + 1] = dfd[l][k]; }
+            }
+        }
+        if( sorted ) { for( int l = 0; l < SIZE; ++l ) { sorted[SIZE * t + l] = ss[l]; } }
+    }
+}
+
+// TP0_Traversor::Compute for a symmetric BCT (S == T). Patterns are full symmetric CSR with
+// `diag` = first strictly-upper position per row. flags: 1 energy, 2 differential, 4 metric.
+// P_near: T x ND (tree order), C_far: C x 10.
+// Outputs (any may be NULL when not requested): P_D_near T x ND, C_D_far C x 10 (both zeroed here),
+// nf_vals N_nf x 2, ff_vals N_ff x 2 (full pattern, lower part filled from the upper part),
+// nf_diag T x 2, ff_diag C x 2. Returns the energy (compensated sum).
+double orc_tp0_compute( int n, LInt T, LInt C, const double * P_near, const double * C_far,
+    const LInt * nrp, const Int * nci, const LInt * ndiag,
+    const LInt * frp, const Int * fci, const LInt * fdiag,
+    double q, double p, int flags,
+    double * P_D_near, double * C_D_far, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag )
+{
+    const bool energy = flags & 1, diff = flags & 2, metric = flags & 4;
+    const Exponents ex( q, p, n );
+    if( ex.q_flag < 0 ) { return std::nan(""); }
+    const int SIZE = n + 1;
+    const int ND = 1 + 3 * SIZE + 6;
+    Neumaier en;
+    if( diff )
+    {
+        std::memset( P_D_near, 0, sizeof(double) * (size_t)(ND * T) );
+        std::memset( C_D_far,  0, sizeof(double) * (size_t)(10 * C) );
+    }
+    if( metric )
+    {
+        std::memset( nf_diag, 0, sizeof(double) * (size_t)(2 * T) );
+        std::memset( ff_diag, 0, sizeof(double) * (size_t)(2 * C) );
+    }
+    const double scale = 1.0 / SIZE;
+    // ---- near field ----
+    if( nrp[T] > 0 )
+    {
+        for( LInt i = 0; i < T; ++i )
+        {
+            const LInt kb = ndiag[i], ke = nrp[i + 1];
+            if( ke <= kb ) { continue; }
+            const double * X = P_near + ND * i;
+            const double a = X[0];
+            double x[3];
+            for( int k = 0; k < 3; ++k ) { x[k] = scale * X[1 + k]; for( int kk = 1; kk < SIZE; ++kk ) { x[k] += scale * X[1 + 3 * kk + k]; } }
+            double DX[16] = {0};
+            double ii_block[2] = {0,0};
+            for( LInt k = kb; k < ke; ++k )
+            {
+                const Int j = nci[k];
+                const double * Y = P_near + ND * j;
+                const double b = Y[0];
+                double y[3];
+                for( int kk = 0; kk < 3; ++kk ) { y[kk] = scale * Y[1 + kk]; for( int h = 1; h < SIZE; ++h ) { y[kk] += scale * Y[1 + 3 * h + kk]; } }
+                double DY[16] = {0};
+                const PairOut o = PairCompute( ex, n, energy, diff, metric, a, x, X + 1 + 3 * SIZE, b, y, Y + 1 + 3 * SIZE,
+                                               DX, DY, SIZE, SIZE, scale, scale );
+                if( energy ) { en.add( o.result ); }
+                if( diff ) { for( int h = 0; h < ND; ++h ) { P_D_near[ND * j + h] += DY[h]; } }
+                if( metric )
+                {
+                    nf_vals[2 * k] = o.a0; nf_vals[2 * k + 1] = o.a1;
+                    ii_block[0] -= (b / a) * o.a0;
+                    ii_block[1] -= (b / a) * o.a1;
+                    nf_diag[2 * j]     += -(a / b) * o.a0;
+                    nf_diag[2 * j + 1] += -(a / b) * o.a1;
+                }
+            }
+            if( diff ) { for( int h = 0; h < ND; ++h ) { P_D_near[ND * i + h] += DX[h]; } }
+            if( metric ) { nf_diag[2 * i] += ii_block[0]; nf_diag[2 * i + 1] += ii_block[1]; }
+        }
+        if( metric )
+        {
+            // FillLowerTriangleFromUpperTriangle.
+            for( LInt i = 0; i < T; ++i )
+            {
+                for( LInt k = nrp[i]; k < ndiag[i]; ++k )
+                {
+                    const Int j = nci[k];
+                    const Int * b = nci + ndiag[j];
+                    const Int * e = nci + nrp[j + 1];
+                    const Int * f = std::lower_bound( b, e, (Int)i );
+                    const LInt pos = f - nci;
+                    nf_vals[2 * k] = nf_vals[2 * pos]; nf_vals[2 * k + 1] = nf_vals[2 * pos + 1];
+                }
+            }
+        }
+    }
+    // ---- far field ----
+    if( frp[C] > 0 )
+    {
+        for( LInt i = 0; i < C; ++i )
+        {
+            const LInt kb = fdiag[i], ke = frp[i + 1];
+            if( ke <= kb ) { continue; }
+            const double * X = C_far + 10 * i;
+            const double a = X[0];
+            double DX[10] = {0};
+            double ii_block[2] = {0,0};
+            for( LInt k = kb; k < ke; ++k )
+            {
+                const Int j = fci[k];
+                const double * Y = C_far + 10 * j;
+                const double b = Y[0];
+                double DY[10] = {0};
+                const PairOut o = PairCompute( ex, n, energy, diff, metric, a, X + 1, X + 4, b, Y + 1, Y + 4, DX, DY, 1, 1, 1.0, 1.0 );
+                if( energy ) { en.add( o.result ); }
+                if( diff ) { for( int h = 0; h < 10; ++h ) { C_D_far[10 * j + h] += DY[h]; } }
+                if( metric )
+                {
+                    ff_vals[2 * k] = o.a0; ff_vals[2 * k + 1] = o.a1;
+                    ii_block[0] -= (b / a) * o.a0;
+                    ii_block[1] -= (b / a) * o.a1;
+                    ff_diag[2 * j]     += -(a / b) * o.a0;
+                    ff_diag[2 * j + 1] += -(a / b) * o.a1;
+                }
+            }
+            if( diff ) { for( int h = 0; h < 10; ++h ) { C_D_far[10 * i + h] += DX[h]; } }
+            if( metric ) { ff_diag[2 * i] += ii_block[0]; ff_diag[2 * i + 1] += ii_block[1]; }
+        }
+        if( metric )
+        {
+            for( LInt i = 0; i < C; ++i )
+            {
+                for( LInt k = frp[i]; k < fdiag[i]; ++k )
+                {
+                    const Int j = fci[k];
+                    const Int * f = std::lower_bound( fci + fdiag[j], fci + frp[j + 1], (Int)i );
+                    const LInt pos = f - fci;
+                    ff_vals[2 * k] = ff_vals[2 * pos]; ff_vals[2 * k + 1] = ff_vals[2 * pos + 1];
+                }
+            }
+        }
+    }
+    return en.value();
+}
+
+// CollectFarFieldDerivatives + CollectNearFieldDerivatives. Inputs in tree order; outputs in the
+// original primitive order. C_D_far is used as scratch (percolated down in place).
+void orc_collect_derivatives( int n, LInt T, LInt C, const Int * C_left, const Int * C_right,
+    const Int * C_begin, const Int * C_end, const Int * P_inverse_ordering,
+    const double * P_D_near_tree, double * C_D_far, double * P_D_near, double * P_D_far )
+{
+    const int ND = 1 + 3 * (n + 1) + 6;
+    // PercolateDown: parents before children in depth-first numbering.
+    for( LInt c = 0; c < C; ++c )
+    {
+        const Int L = C_left[c];
+        if( L >= 0 )
+        {
+            const Int R = C_right[c];
+            for( int h = 0; h < 10; ++h ) { C_D_far[10 * L + h] = C_D_far[10 * c + h] + C_D_far[10 * L + h]; }
+            for( int h = 0; h < 10; ++h ) { C_D_far[10 * R + h] = C_D_far[10 * c + h] + C_D_far[10 * R + h]; }
+        }
+    }
+    std::vector<double> P_out( (size_t)(10 * T), 0.0 );
+    for( LInt c = 0; c < C; ++c )
+    {
+        if( C_left[c] < 0 )
+        {
+            for( Int i = C_begin[c]; i < C_end[c]; ++i ) { for( int h = 0; h < 10; ++h ) { P_out[10 * (size_t)i + h] = C_D_far[10 * c + h]; } }
+        }
+    }
+    for( LInt i = 0; i < T; ++i )
+    {
+        const LInt from = P_inverse_ordering[i];
+        for( int h = 0; h < 10; ++h ) { P_D_far[10 * i + h] = P_out[10 * from + h]; }
+        for( int h = 0; h < ND; ++h ) { P_D_near[ND * i + h] = P_D_near_tree[ND * from + h]; }
+    }
+}
+
+// Chain rule from moment derivatives to vertex coordinates (DNearFarToHulls).
+// N: T x ND near derivatives, F: T x 10 far derivatives (original order); B: T x (n+1) x 3.
+// Moment map per simplex with vertices X_0..X_n (input order):
+//   triangles: nu = (X_1-X_0) x (X_2-X_0), vol = |nu|/2, pr = nu nu^T / |nu|^2
+//   edges:     tau = X_1-X_0,              vol = |tau|,  pr = I - tau tau^T / |tau|^2
+//   near moments (vol, vol*hull, vol*pr_packed), far moments (vol, vol*mean(hull), vol*pr_packed);
+//   B = charge * ( N . d(near moments)/dX + F . d(far moments)/dX ).
+void orc_dnearfar_to_hulls( int n, LInt T, const double * V, const Int * S, const double * charges,
+    const double * N, const double * F, double * B )
+{
+    const int SIZE = n + 1;
+    const int ND = 1 + 3 * SIZE + 6;
+    const int pi[6] = {0,0,0,1,1,2}, pj[6] = {0,1,2,1,2,2};
+    for( LInt t = 0; t < T; ++t )
+    {
+        double X[3][3];
+        double charge = 0;
+        for( int l = 0; l < SIZE; ++l ) { for( int k = 0; k < 3; ++k ) { X[l][k] = V[3 * (LInt)S[SIZE * t + l] + k]; } charge += charges ? charges[S[SIZE * t + l]] : 1.0; }
+        charge *= 1.0 / SIZE;
+        const double * Nt = N + ND * t;
+        const double * Ft = F + 10 * t;
+        double vol, pr[3][3], w[3];   // w: nu (triangles) or tau (edges)
+        double ww;
+        if( n == 2 )
+        {
+            const double e1[3] = { X[1][0]-X[0][0], X[1][1]-X[0][1], X[1][2]-X[0][2] };
+            const double e2[3] = { X[2][0]-X[0][0], X[2][1]-X[0][1], X[2][2]-X[0][2] };
+            w[0] = e1[1]*e2[2] - e1[2]*e2[1]; w[1] = e1[2]*e2[0] - e1[0]*e2[2]; w[2] = e1[0]*e2[1] - e1[1]*e2[0];
+            ww = w[0]*w[0] + w[1]*w[1] + w[2]*w[2];
+            vol = 0.5 * std::sqrt(ww);
+            for( int r = 0; r < 3; ++r ) { for( int c = 0; c < 3; ++c ) { pr[r][c] = w[r] * w[c] / ww; } }
+        }
+        else
+        {
+            for( int k = 0; k < 3; ++k ) { w[k] = X[1][k] - X[0][k]; }
+            ww = w[0]*w[0] + w[1]*w[1] + w[2]*w[2];
+            vol = std::sqrt(ww);
+            for( int r = 0; r < 3; ++r ) { for( int c = 0; c < 3; ++c ) { pr[r][c] = (r == c ? 1.0 : 0.0) - w[r] * w[c] / ww; } }
+        }
+        double mean[3] = {0,0,0};
+        for( int l = 0; l < SIZE; ++l ) { for( int k = 0; k < 3; ++k ) { mean[k] += X[l][k] / SIZE; } }
+        for( int j = 0; j < SIZE; ++j )
+        {
+            for( int c = 0; c < 3; ++c )
+            {
+                // dw = d w / d X_{j,c}
+                double dw[3] = {0,0,0};
+                if( n == 2 )
+                {
+                    // w = (X1-X0) x (X2-X0); d/dX1 = e_c x (X2-X0); d/dX2 = (X1-X0) x e_c; d/dX0 = -(sum).
+                    const double e1[3] = { X[1][0]-X[0][0], X[1][1]-X[0][1], X[1][2]-X[0][2] };
+                    const double e2[3] = { X[2][0]-X[0][0], X[2][1]-X[0][1], X[2][2]-X[0][2] };
+                    double ec[3] = {0,0,0}; ec[c] = 1.0;
+                    double a1[3] = { ec[1]*e2[2]-ec[2]*e2[1], ec[2]*e2[0]-ec[0]*e2[2], ec[0]*e2[1]-ec[1]*e2[0] };
+                    double a2[3] = { e1[1]*ec[2]-e1[2]*ec[1], e1[2]*ec[0]-e1[0]*ec[2], e1[0]*ec[1]-e1[1]*ec[0] };
+                    for( int k = 0; k < 3; ++k )
+                    {
+                        if( j == 1 ) { dw[k] = a1[k]; }
+                        else if( j == 2 ) { dw[k] = a2[k]; }
+                        else { dw[k] = -a1[k] - a2[k]; }
+                    }
+                }
+                else
+                {
+                    dw[c] = (j == 1) ? 1.0 : -1.0;
+                }
+                const double wdw = w[0]*dw[0] + w[1]*dw[1] + w[2]*dw[2];
+                const double dvol = (n == 2) ? 0.5 * wdw / std::sqrt(ww) : wdw / std::sqrt(ww);
+                // d(w w^T / ww) = (dw w^T + w dw^T)/ww - 2 w w^T (w.dw)/ww^2
+                double dpr[3][3];
+                for( int r = 0; r < 3; ++r )
+                {
+                    for( int s = 0; s < 3; ++s )
+                    {
+                        const double d = (dw[r] * w[s] + w[r] * dw[s]) / ww - 2.0 * w[r] * w[s] * wdw / (ww * ww);
+                        dpr[r][s] = (n == 2) ? d : -d;
+                    }
+                }
+                double acc = (Nt[0] + Ft[0]) * dvol;
+                for( int l = 0; l < SIZE; ++l )
+                {
+                    for( int k = 0; k < 3; ++k )
+                    {
+                        acc += Nt[1 + 3 * l + k] * ( dvol * X[l][k] + ((l == j && k == c) ? vol : 0.0) );
+                    }
+                }
+                for( int k = 0; k < 3; ++k )
+                {
+                    acc += Ft[1 + k] * ( dvol * mean[k] + ((k == c) ? vol / SIZE : 0.0) );
+                }
+                for( int k = 0; k < 6; ++k )
+                {
+                    const double m = dvol * pr[pi[k]][pj[k]] + vol * dpr[pi[k]][pj[k]];
+                    acc += (Nt[1 + 3 * SIZE + k] + Ft[4 + k]) * m;
+                }
+                B[(LInt)3 * SIZE * t + 3 * j + c] = charge * acc;
+            }
+        }
+    }
+}
+
+// Y = alpha * A^T B + beta * Y, A = binary T(n+1) x V corner-to-vertex matrix (DerivativeAssembler).
+// Summation per vertex in increasing corner index.
+void orc_assemble( int n, LInt V, LInt T, const Int * S, const double * B, double alpha, double beta, double * Y, LInt ldY )
+{
+    const int SIZE = n + 1;
+    std::vector<double> acc( (size_t)(3 * V), 0.0 );
+    std::vector<char> touched( (size_t)V, 0 );
+    for( LInt r = 0; r < T * SIZE; ++r )
+    {
+        const Int v = S[r];
+        for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += B[3 * r + k]; }
+    }
+    for( LInt v = 0; v < V; ++v )
+    {
+        for( int k = 0; k < 3; ++k )
+        {
+            Y[ldY * v + k] = alpha * acc[3 * v + k] + (beta == 0.0 ? 0.0 : beta * Y[ldY * v + k]);
+        }
+    }
+}
+
+// Pre (MixedOrder) -> NF/FF block products -> Post, for 3 right-hand sides.
+// pre_cols T x (n+1), pre_vals T x 4 x (n+1) (tree order; row 0 averaging, rows 1..3 derivative),
+// nf_vals/ff_vals full-pattern (a0,a1), nf_diag T x 2, ff_diag C x 2.
+void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
+    const Int * C_left, const Int * C_right, const Int * C_begin, const Int * C_end,
+    const Int * pre_cols, const double * pre_vals,
+    const LInt * nrp, const Int * nci, const double * nf_vals, const double * nf_diag,
+    const LInt * frp, const Int * fci, const double * ff_vals, const double * ff_diag,
+    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag )
+{
+    const int SIZE = n + 1;
+    std::vector<double> P_in( (size_t)(12 * T), 0.0 ), P_out( (size_t)(12 * T), 0.0 );
+    std::vector<double> C_in( (size_t)(12 * C), 0.0 ), C_out( (size_t)(12 * C), 0.0 );
+    for( LInt i = 0; i < T; ++i )
+    {
+        for( int r = 0; r < 4; ++r )
+        {
+            for( int k = 0; k < 3; ++k )
+            {
+                double s = 0;
+                for( int l = 0; l < SIZE; ++l ) { s += pre_vals[4 * SIZE * i + SIZE * r + l] * X[ldX * pre_cols[SIZE * i + l] + k]; }
+                P_in[12 * i + 3 * r + k] = s;
+            }
+        }
+    }
+    // PrimitivesToClusters + PercolateUp (children before parents: reverse depth-first order).
+    for( LInt c = C; c-- > 0; )
+    {
+        if( C_left[c] < 0 )
+        {
+            for( int h = 0; h < 12; ++h ) { double s = 0; for( Int i = C_begin[c]; i < C_end[c]; ++i ) { s += P_in[12 * (size_t)i + h]; } C_in[12 * c + h] = s; }
+        }
+        else
+        {
+            for( int h = 0; h < 12; ++h ) { C_in[12 * c + h] = C_in[12 * C_left[c] + h] + C_in[12 * C_right[c] + h]; }
+        }
+    }
+    auto apply = [&]( const LInt rows, const LInt * rp, const Int * ci, const double * vals, const double * diag,
+                      const std::vector<double> & xin, std::vector<double> & yout )
+    {
+        for( LInt i = 0; i < rows; ++i )
+        {
+            double y[12] = {0};
+            for( LInt k = rp[i]; k < rp[i + 1]; ++k )
+            {
+                const double * x = &xin[12 * (size_t)ci[k]];
+                const double a0 = vals[2 * k], a1 = vals[2 * k + 1];
+                for( int h = 0; h < 3; ++h ) { y[h] += a0 * x[h]; }
+                for( int h = 3; h < 12; ++h ) { y[h] += a1 * x[h]; }
+            }
+            const double * x = &xin[12 * (size_t)i];
+            for( int h = 0; h < 3; ++h ) { y[h] += diag[2 * i] * x[h]; }
+            for( int h = 3; h < 12; ++h ) { y[h] += diag[2 * i + 1] * x[h]; }
+            for( int h = 0; h < 12; ++h ) { yout[12 * (size_t)i + h] = y[h]; }
+        }
+    };
+    if( nf_flag && nrp[T] > 0 ) { apply( T, nrp, nci, nf_vals, nf_diag, P_in, P_out ); }
+    if( ff_flag && frp[C] > 0 ) { apply( C, frp, fci, ff_vals, ff_diag, C_in, C_out ); }
+    // Post: PercolateDown, ClustersToPrimitives(add), mi_post.
+    for( LInt c = 0; c < C; ++c )
+    {
+        const Int L = C_left[c];
+        if( L >= 0 )
+        {
+            const Int R = C_right[c];
+            for( int h = 0; h < 12; ++h ) { C_out[12 * L + h] = C_out[12 * c + h] + C_out[12 * L + h]; }
+            for( int h = 0; h < 12; ++h ) { C_out[12 * R + h] = C_out[12 * c + h] + C_out[12 * R + h]; }
+        }
+    }
+    for( LInt c = 0; c < C; ++c )
+    {
+        if( C_left[c] < 0 )
+        {
+            for( Int i = C_begin[c]; i < C_end[c]; ++i ) { for( int h = 0; h < 12; ++h ) { P_out[12 * (size_t)i + h] += C_out[12 * c + h]; } }
+        }
+    }
+    std::vector<double> acc( (size_t)(3 * V), 0.0 );
+    for( LInt i = 0; i < T; ++i )
+    {
+        for( int l = 0; l < SIZE; ++l )
+        {
+            const Int v = pre_cols[SIZE * i + l];
+            for( int r = 0; r < 4; ++r )
+            {
+                const double w = pre_vals[4 * SIZE * i + SIZE * r + l];
+                for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += w * P_out[12 * i + 3 * r + k]; }
+            }
+        }
+    }
+    for( LInt v = 0; v < V; ++v )
+    {
+        for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = alpha * acc[3 * v + k] + (beta == 0.0 ? 0.0 : beta * Y[ldY * v + k]); }
+    }
+}
+
+// Brute-force energy: sum over all unordered pairs i < j of a_i a_j E_ij from far rows
+// (area, barycenter, projector), no hierarchy.
+double orc_allpairs_energy( int n, LInt T, const double * P_far, double q, double p )
+{
+    const Exponents ex( q, p, n );
+    Neumaier en;
+    double DX[16], DY[16];
+    for( LInt i = 0; i < T; ++i )
+    {
+        const double * X = P_far + 10 * i;
+        for( LInt j = i + 1; j < T; ++j )
+        {
+            const double * Y = P_far + 10 * j;
+            en.add( PairCompute( ex, n, true, false, false, X[0], X + 1, X + 4, Y[0], Y + 1, Y + 4, DX, DY, 1, 1, 1.0, 1.0 ).result );
+        }
+    }
+    return en.value();
+}
+
+// Brute-force dense metric values for all ordered pairs i != j (T x T x 2), far rows as input.
+void orc_allpairs_metric( int n, LInt T, const double * P_far, double q, double p, double * vals )
+{
+    const Exponents ex( q, p, n );
+    double DX[16], DY[16];
+    for( LInt i = 0; i < T; ++i )
+    {
+        const double * X = P_far + 10 * i;
+        for( LInt j = 0; j < T; ++j )
+        {
+            if( i == j ) { vals[2 * (T * i + j)] = 0; vals[2 * (T * i + j) + 1] = 0; continue; }
+            const double * Y = P_far + 10 * j;
+            const PairOut o = PairCompute( ex, n, false, false, true, X[0], X + 1, X + 4, Y[0], Y + 1, Y + 4, DX, DY, 1, 1, 1.0, 1.0 );
+            vals[2 * (T * i + j)] = o.a0; vals[2 * (T * i + j) + 1] = o.a1;
+        }
+    }
+}
+
+} // extern "C"

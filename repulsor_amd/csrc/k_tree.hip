@@ -1,0 +1,468 @@
+// Data preparation, derivative collection, chain rule to vertices, and the cluster-tree passes and
+// block SpMMs of MultiplyMetric.
+//
+// Reference counterparts:
+//   k_prep_prims / k_prep_clusters   FMM_Kernel_NF::loadS (barycenter = sum_k (1/(n+1)) hull_k, FMM_Kernel_NF.hpp:169-196),
+//                                    FMM_Kernel_FF::loadS (FMM_Kernel_FF.hpp:162-179)
+//   k_leaf_far                       CollectFarFieldDerivatives: PercolateDown + ClustersToPrimitives (ClusterTree.hpp:1164-1208)
+//   k_hulls                          DNearFarToHulls<2,3>/<1,3> (SimplicialMeshDetails.hpp:529-736, 1097-1410)
+//   k_assemble                       DerivativeAssembler().Dot<3> (SimplicialMesh.hpp:625-675)
+//   k_pre, k_up_leaves, k_up_level   ClusterTree::Pre: mi_pre.Dot, PrimitivesToClusters, PercolateUp (ClusterTree.hpp:770-857)
+//   k_nf_spmm, k_ff_spmm             TP0_Traversor::NF/FF_MultiplyMetric + DiagonalKernelMatrix (TP0_Traversor.hpp:426-505)
+//   k_down_leaves, k_post            ClusterTree::Post: PercolateDown, ClustersToPrimitives(add), mi_post.Dot (ClusterTree.hpp:860-943)
+
+#include "rp_launch.hpp"
+
+namespace rp
+{
+    // ---------------------------------------------------------------- data preparation
+    __global__ void k_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        const double * X = near + (int64_t)nd * i;
+        const double s = 1.0 / size;
+        a[i] = X[0];
+        for( int k = 0; k < 3; ++k )
+        {
+            double xk = s * X[1 + k];
+            for( int h = 1; h < size; ++h ) { xk += s * X[1 + 3 * h + k]; }
+            x[k * T + i] = xk;
+        }
+        for( int k = 0; k < 6; ++k ) { P[k * T + i] = X[1 + 3 * size + k]; }
+    }
+
+    __global__ void k_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= C ) { return; }
+        const double * X = far + 10 * i;
+        a[i] = X[0];
+        for( int k = 0; k < 3; ++k ) { x[k * C + i] = X[1 + k]; }
+        for( int k = 0; k < 6; ++k ) { P[k * C + i] = X[4 + k]; }
+    }
+
+    __global__ void k_leaf_origin( int64_t L, int64_t C, const int32_t * leaf_cluster, const double * cx, double * origin )
+    {
+        const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( l >= L ) { return; }
+        const int32_t c = leaf_cluster[l];
+        for( int k = 0; k < 3; ++k ) { origin[k * L + l] = cx[k * C + c]; }
+    }
+
+    // ---------------------------------------------------------------- derivative collection
+    // Root-to-leaf sum of the far-field moment derivatives, in the order of the reference's
+    // PercolateDown (child = parent + child, parents first). Output 10 x L.
+    template<int DIM>
+    __global__ void k_leaf_path_sum( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent,
+                                     const double * src, int64_t src_stride_soa, double * dst, int dst_soa )
+    {
+        const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( l >= L ) { return; }
+        int32_t path[96];
+        int depth = 0;
+        int32_t c = leaf_cluster[l];
+        while( c >= 0 && depth < 96 ) { path[depth++] = c; c = parent[c]; }
+        double acc[DIM];
+        // SoA source: src[k * C + c]; AoS source: src[DIM * c + k]
+        for( int k = 0; k < DIM; ++k ) { acc[k] = src_stride_soa ? src[k * C + path[depth - 1]] : src[(int64_t)DIM * path[depth - 1] + k]; }
+        for( int d = depth - 2; d >= 0; --d )
+        {
+            const int32_t cc = path[d];
+            for( int k = 0; k < DIM; ++k ) { acc[k] = acc[k] + ( src_stride_soa ? src[k * C + cc] : src[(int64_t)DIM * cc + k] ); }
+        }
+        for( int k = 0; k < DIM; ++k )
+        {
+            if( dst_soa ) { dst[k * L + l] = acc[k]; } else { dst[(int64_t)DIM * l + k] = acc[k]; }
+        }
+    }
+
+    // Chain rule from the moment derivatives (near: a, a*hull, a*P; far: a, a*x, a*P) to the vertex
+    // coordinates of every simplex. With w = normal (X1-X0)x(X2-X0) for triangles or tangent X1-X0
+    // for edges, vol = |w|/2 or |w|, and M the symmetric matrix of the projector weights
+    // W_k = N_P,k + F_P,k (M_aa = W_aa, M_ab = W_ab/2):
+    //   triangles: pr = w w^T/|w|^2, G = d(vol*K + vol*W.pr)/dw = ((K - S) w + 2 M w) / (2|w|), S = w.Mw/|w|^2
+    //   edges:     pr = I - w w^T/|w|^2, G = ((K + tr W - w.Mw/|w|^2 + 2 w.Mw/|w|^2) w - 2 M w) / |w|
+    //   K = N_0 + F_0 + sum_l N_hull,l . X_l + F_x . mean(X)
+    //   dB/dX_{j,c} = G . dw/dX_{j,c} + vol (N_hull,j,c + F_x,c / (n+1))
+    // with dw/dX_{j,c} = e_c x u_j (u_0 = e1 - e2, u_1 = e2, u_2 = -e1) for triangles and -/+ e_c for edges.
+    template<int N>
+    __global__ void k_hulls( int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim,
+                             const int32_t * S, const double * V, const double * charges,
+                             const double * nf_dx, const double * leaf_far, double * B )
+    {
+        constexpr int SIZE = N + 1;
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        const int64_t t = P_ordering[i];
+        const int64_t l = leaf_of_prim[i];
+        double X[SIZE][3];
+        double charge = 0;
+        for( int j = 0; j < SIZE; ++j )
+        {
+            const int32_t v = S[SIZE * t + j];
+            for( int c = 0; c < 3; ++c ) { X[j][c] = V[3 * (int64_t)v + c]; }
+            charge += charges ? charges[v] : 1.0;
+        }
+        charge *= 1.0 / SIZE;
+        double Nd[10], Fd[10];
+        for( int k = 0; k < 10; ++k ) { Nd[k] = nf_dx[k * T + i]; Fd[k] = leaf_far[k * L + l]; }
+        // Nd: [0] d/da, [1..3] d/d(a hull_l) (same for every l), [4..9] d/d(a P)
+        double mean[3] = { 0, 0, 0 };
+        for( int j = 0; j < SIZE; ++j ) { for( int c = 0; c < 3; ++c ) { mean[c] += X[j][c] * (1.0 / SIZE); } }
+        double K = Nd[0] + Fd[0];
+        for( int j = 0; j < SIZE; ++j ) { for( int c = 0; c < 3; ++c ) { K += Nd[1 + c] * X[j][c]; } }
+        for( int c = 0; c < 3; ++c ) { K += Fd[1 + c] * mean[c]; }
+        double W[6];
+        for( int k = 0; k < 6; ++k ) { W[k] = Nd[4 + k] + Fd[4 + k]; }
+        double w[3];
+        const double e1[3] = { X[1][0] - X[0][0], X[1][1] - X[0][1], X[1][2] - X[0][2] };
+        double e2[3] = { 0, 0, 0 };
+        if constexpr ( N == 2 )
+        {
+            for( int c = 0; c < 3; ++c ) { e2[c] = X[2][c] - X[0][c]; }
+            w[0] = e1[1] * e2[2] - e1[2] * e2[1];
+            w[1] = e1[2] * e2[0] - e1[0] * e2[2];
+            w[2] = e1[0] * e2[1] - e1[1] * e2[0];
+        }
+        else
+        {
+            for( int c = 0; c < 3; ++c ) { w[c] = e1[c]; }
+        }
+        const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+        const double nn = sqrt( ww );
+        const double Mw[3] = {
+            W[0] * w[0] + 0.5 * W[1] * w[1] + 0.5 * W[2] * w[2],
+            0.5 * W[1] * w[0] + W[3] * w[1] + 0.5 * W[4] * w[2],
+            0.5 * W[2] * w[0] + 0.5 * W[4] * w[1] + W[5] * w[2] };
+        const double wMw = w[0] * Mw[0] + w[1] * Mw[1] + w[2] * Mw[2];
+        double G[3], vol;
+        if constexpr ( N == 2 )
+        {
+            vol = 0.5 * nn;
+            const double Sv = wMw / ww;
+            for( int c = 0; c < 3; ++c ) { G[c] = ( (K - Sv) * w[c] + 2.0 * Mw[c] ) / (2.0 * nn); }
+        }
+        else
+        {
+            vol = nn;
+            const double trW = W[0] + W[3] + W[5];
+            const double Qv = wMw / ww;
+            const double Sedge = trW - Qv;
+            for( int c = 0; c < 3; ++c ) { G[c] = ( (K + Sedge + 2.0 * Qv) * w[c] - 2.0 * Mw[c] ) / nn; }
+        }
+        for( int j = 0; j < SIZE; ++j )
+        {
+            double d[3];
+            if constexpr ( N == 2 )
+            {
+                double u[3];
+                for( int c = 0; c < 3; ++c ) { u[c] = (j == 0) ? (e1[c] - e2[c]) : ((j == 1) ? e2[c] : -e1[c]); }
+                // (u x G)_c = G . (e_c x u)
+                d[0] = u[1] * G[2] - u[2] * G[1];
+                d[1] = u[2] * G[0] - u[0] * G[2];
+                d[2] = u[0] * G[1] - u[1] * G[0];
+            }
+            else
+            {
+                const double sgn = (j == 0) ? -1.0 : 1.0;
+                for( int c = 0; c < 3; ++c ) { d[c] = sgn * G[c]; }
+            }
+            for( int c = 0; c < 3; ++c )
+            {
+                B[ (SIZE * t + j) * 3 + c ] = charge * ( d[c] + vol * ( Nd[1 + c] + Fd[1 + c] * (1.0 / SIZE) ) );
+            }
+        }
+    }
+
+    // Y[v] = alpha * sum_{corners r of v, increasing} B[r] + beta * Y[v]
+    __global__ void k_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B,
+                                double alpha, double beta, double * Y, int64_t ldY )
+    {
+        const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( v >= V ) { return; }
+        double s0 = 0, s1 = 0, s2 = 0;
+        for( int64_t k = vc_ptr[v]; k < vc_ptr[v + 1]; ++k )
+        {
+            const int64_t r = vc_corner[k];
+            s0 += B[3 * r]; s1 += B[3 * r + 1]; s2 += B[3 * r + 2];
+        }
+        double * y = Y + ldY * v;
+        if( beta == 0.0 ) { y[0] = alpha * s0; y[1] = alpha * s1; y[2] = alpha * s2; }
+        else { y[0] = alpha * s0 + beta * y[0]; y[1] = alpha * s1 + beta * y[1]; y[2] = alpha * s2 + beta * y[2]; }
+    }
+
+    // ---------------------------------------------------------------- MultiplyMetric: Pre
+    // P_in[i][r][k] = sum_l pre_vals[i][r][l] X[pre_cols[i][l]][k]   (4 x 3 per primitive)
+    template<int SIZE>
+    __global__ void k_pre( int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        double x[SIZE][3];
+        for( int l = 0; l < SIZE; ++l )
+        {
+            const int64_t v = cols[SIZE * i + l];
+            for( int k = 0; k < 3; ++k ) { x[l][k] = X[ldX * v + k]; }
+        }
+        const double * pv = vals + 4 * SIZE * i;
+        double out[12];
+        for( int r = 0; r < 4; ++r )
+        {
+            for( int k = 0; k < 3; ++k )
+            {
+                double s = 0;
+                for( int l = 0; l < SIZE; ++l ) { s += pv[SIZE * r + l] * x[l][k]; }
+                out[3 * r + k] = s;
+            }
+        }
+        double2 * o = reinterpret_cast<double2 *>( P_in + 12 * i );
+        for( int h = 0; h < 6; ++h ) { o[h] = make_double2( out[2 * h], out[2 * h + 1] ); }
+    }
+
+    // Leaf sums of P_in (PrimitivesToClusters).
+    __global__ void k_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end,
+                                 const double * P_in, double * C_in )
+    {
+        const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t l = g / 12;
+        const int h = (int)(g % 12);
+        if( l >= L ) { return; }
+        const int32_t c = leaf_cluster[l];
+        double s = 0;
+        for( int32_t i = C_begin[c]; i < C_end[c]; ++i ) { s += P_in[12 * (int64_t)i + h]; }
+        C_in[12 * (int64_t)c + h] = s;
+    }
+
+    // One depth level of PercolateUp: C_in[c] = C_in[left] + C_in[right].
+    __global__ void k_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in )
+    {
+        const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t q = g / 12;
+        const int h = (int)(g % 12);
+        if( q >= n ) { return; }
+        const int32_t c = nodes[q];
+        C_in[12 * (int64_t)c + h] = C_in[12 * (int64_t)C_left[c] + h] + C_in[12 * (int64_t)C_right[c] + h];
+    }
+
+    // ---------------------------------------------------------------- MultiplyMetric: near SpMM
+    // y_i = sum_j diag(a0, a1, a1, a1) x_j + diag_i x_i over the leaf tiles; x, y are 4 x 3 per primitive.
+    template<typename VT>
+    __global__ __launch_bounds__(256) void k_nf_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
+        const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
+        const VT * vals, const double * diag, const double * xin, double * yout )
+    {
+        __shared__ double red[4][64][12];
+        const int w = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        const int64_t I = (int64_t)blockIdx.x * 4 + w;
+        const bool wave_ok = I < L;
+        double acc[12];
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
+        int nI = 1, nS = 64, ii = 0;
+        int64_t i = 0;
+        if( wave_ok )
+        {
+            const int ib = leaf_begin[I];
+            nI = leaf_end[I] - ib;
+            nS = 64 / nI;
+            ii = lane % nI;
+            const int c = lane / nI;
+            i = ib + ii;
+            if( lane < nS * nI )
+            {
+                const int64_t cb = col_ptr[I];
+                const int64_t M_ = col_ptr[I + 1] - cb;
+                const int64_t vb = val_ptr[I];
+                for( int64_t m = c; m < M_; m += nS )
+                {
+                    if( (col_mask[cb + m] >> ii) & 1u )
+                    {
+                        const int32_t j = col_prim[cb + m];
+                        const VT av = vals[vb + m * nI + ii];
+                        const double a0 = (double)av.x, a1 = (double)av.y;
+                        const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)j );
+                        const double2 x01 = xj[0], x23 = xj[1], x45 = xj[2], x67 = xj[3], x89 = xj[4], xab = xj[5];
+                        acc[0] += a0 * x01.x; acc[1] += a0 * x01.y; acc[2] += a0 * x23.x;
+                        acc[3] += a1 * x23.y; acc[4] += a1 * x45.x; acc[5] += a1 * x45.y;
+                        acc[6] += a1 * x67.x; acc[7] += a1 * x67.y; acc[8] += a1 * x89.x;
+                        acc[9] += a1 * x89.y; acc[10] += a1 * xab.x; acc[11] += a1 * xab.y;
+                    }
+                }
+            }
+        }
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { red[w][lane][k] = acc[k]; }
+        __syncthreads();
+        if( wave_ok && lane < nI )
+        {
+            double s[12];
+            #pragma unroll
+            for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
+            for( int c = 0; c < nS; ++c )
+            {
+                #pragma unroll
+                for( int k = 0; k < 12; ++k ) { s[k] += red[w][lane + c * nI][k]; }
+            }
+            const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
+            const double * xi = xin + 12 * i;
+            for( int k = 0; k < 3; ++k )  { s[k] += d0 * xi[k]; }
+            for( int k = 3; k < 12; ++k ) { s[k] += d1 * xi[k]; }
+            double2 * o = reinterpret_cast<double2 *>( yout + 12 * i );
+            for( int h = 0; h < 6; ++h ) { o[h] = make_double2( s[2 * h], s[2 * h + 1] ); }
+        }
+    }
+
+    template<typename T>
+    __device__ __forceinline__ T wave_sum64( T v )
+    {
+        #pragma unroll
+        for( int off = 32; off > 0; off >>= 1 ) { v += __shfl_xor( v, off, 64 ); }
+        return v;
+    }
+
+    template<typename VT>
+    __global__ __launch_bounds__(256) void k_ff_spmm( int64_t C, const int64_t * rp, const int32_t * ci,
+        const VT * vals, const double * diag, const double * xin, double * yout )
+    {
+        const int w = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        const int64_t i = (int64_t)blockIdx.x * 4 + w;
+        if( i >= C ) { return; }
+        double acc[12];
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
+        for( int64_t k = rp[i] + lane; k < rp[i + 1]; k += 64 )
+        {
+            const int32_t j = ci[k];
+            const VT av = vals[k];
+            const double a0 = (double)av.x, a1 = (double)av.y;
+            const double * xj = xin + 12 * (int64_t)j;
+            #pragma unroll
+            for( int h = 0; h < 3; ++h ) { acc[h] += a0 * xj[h]; }
+            #pragma unroll
+            for( int h = 3; h < 12; ++h ) { acc[h] += a1 * xj[h]; }
+        }
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { acc[k] = wave_sum64( acc[k] ); }
+        if( lane == 0 )
+        {
+            const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
+            const double * xi = xin + 12 * i;
+            for( int k = 0; k < 3; ++k )  { acc[k] += d0 * xi[k]; }
+            for( int k = 3; k < 12; ++k ) { acc[k] += d1 * xi[k]; }
+            for( int k = 0; k < 12; ++k ) { yout[12 * i + k] = acc[k]; }
+        }
+    }
+
+    // ---------------------------------------------------------------- MultiplyMetric: Post
+    // Y[v] = alpha * sum_{(i,slot) of v, increasing i} sum_r pre_vals[i][r][slot] (P_out[i][r] + leaf_out[leaf(i)][r]) + beta Y[v]
+    template<int SIZE>
+    __global__ void k_post( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_idx_of_prim,
+                            const double * pre_vals, const double * P_out, const double * leaf_out, int use_far,
+                            double alpha, double beta, double * Y, int64_t ldY )
+    {
+        const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( v >= V ) { return; }
+        double s[3] = { 0, 0, 0 };
+        for( int64_t k = vp_ptr[v]; k < vp_ptr[v + 1]; ++k )
+        {
+            const int64_t e = vp_entry[k];
+            const int64_t i = e / SIZE;
+            const int slot = (int)(e % SIZE);
+            const double * po = P_out + 12 * i;
+            const double * lo = use_far ? leaf_out + 12 * (int64_t)leaf_idx_of_prim[i] : nullptr;
+            for( int r = 0; r < 4; ++r )
+            {
+                const double wgt = pre_vals[4 * SIZE * i + SIZE * r + slot];
+                for( int c = 0; c < 3; ++c )
+                {
+                    const double pval = use_far ? po[3 * r + c] + lo[3 * r + c] : po[3 * r + c];
+                    s[c] += wgt * pval;
+                }
+            }
+        }
+        double * y = Y + ldY * v;
+        if( beta == 0.0 ) { for( int c = 0; c < 3; ++c ) { y[c] = alpha * s[c]; } }
+        else { for( int c = 0; c < 3; ++c ) { y[c] = alpha * s[c] + beta * y[c]; } }
+    }
+
+    __global__ void k_fill( double * p, int64_t n, double v )
+    {
+        const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( g < n ) { p[g] = v; }
+    }
+
+    // ---------------------------------------------------------------- host launchers
+    static inline dim3 grid1( int64_t n, int b = 256 ) { return dim3( (unsigned)std::max<int64_t>( 1, (n + b - 1) / b ) ); }
+
+    void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, hipStream_t s )
+    { if( T ) { hipLaunchKernelGGL( k_prep_prims, grid1(T), dim3(256), 0, s, T, nd, size, near, a, x, P ); } }
+
+    void launch_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P, hipStream_t s )
+    { if( C ) { hipLaunchKernelGGL( k_prep_clusters, grid1(C), dim3(256), 0, s, C, far, a, x, P ); } }
+
+    void launch_leaf_origin( int64_t L, int64_t C, const int32_t * leaf_cluster, const double * cx, double * origin, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( k_leaf_origin, grid1(L), dim3(256), 0, s, L, C, leaf_cluster, cx, origin ); } }
+
+    void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * ff_dx, double * leaf_far, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<10>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, ff_dx, (int64_t)1, leaf_far, 1 ); } }
+
+    void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<12>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
+
+    void launch_hulls( int n, int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim, const int32_t * S,
+                       const double * V, const double * charges, const double * nf_dx, const double * leaf_far, double * B, hipStream_t s )
+    {
+        if( !T ) { return; }
+        if( n == 2 ) { hipLaunchKernelGGL( (k_hulls<2>), grid1(T), dim3(256), 0, s, T, L, P_ordering, leaf_of_prim, S, V, charges, nf_dx, leaf_far, B ); }
+        else         { hipLaunchKernelGGL( (k_hulls<1>), grid1(T), dim3(256), 0, s, T, L, P_ordering, leaf_of_prim, S, V, charges, nf_dx, leaf_far, B ); }
+    }
+
+    void launch_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )
+    { if( V ) { hipLaunchKernelGGL( k_assemble, grid1(V), dim3(256), 0, s, V, vc_ptr, vc_corner, B, alpha, beta, Y, ldY ); } }
+
+    void launch_pre( int size, int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in, hipStream_t s )
+    {
+        if( !T ) { return; }
+        if( size == 3 ) { hipLaunchKernelGGL( (k_pre<3>), grid1(T), dim3(256), 0, s, T, cols, vals, X, ldX, P_in ); }
+        else            { hipLaunchKernelGGL( (k_pre<2>), grid1(T), dim3(256), 0, s, T, cols, vals, X, ldX, P_in ); }
+    }
+
+    void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( k_up_leaves, grid1(12 * L), dim3(256), 0, s, L, leaf_cluster, C_begin, C_end, P_in, C_in ); } }
+
+    void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s )
+    { if( n ) { hipLaunchKernelGGL( k_up_level, grid1(12 * n), dim3(256), 0, s, n, nodes, C_left, C_right, C_in ); } }
+
+    void launch_nf_spmm( bool mixed, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp, const int32_t * cpr, const uint32_t * cm,
+                         const int64_t * vp, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
+    {
+        if( !L ) { return; }
+        const dim3 grid( (unsigned)((L + 3) / 4) );
+        if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2>), grid, dim3(256), 0, s, L, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, yout ); }
+        else        { hipLaunchKernelGGL( (k_nf_spmm<double2>), grid, dim3(256), 0, s, L, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, yout ); }
+    }
+
+    void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
+    {
+        if( !C ) { return; }
+        const dim3 grid( (unsigned)((C + 3) / 4) );
+        if( mixed ) { hipLaunchKernelGGL( (k_ff_spmm<float2>), grid, dim3(256), 0, s, C, rp, ci, (const float2 *)vals, diag, xin, yout ); }
+        else        { hipLaunchKernelGGL( (k_ff_spmm<double2>), grid, dim3(256), 0, s, C, rp, ci, (const double2 *)vals, diag, xin, yout ); }
+    }
+
+    void launch_post( int size, int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_idx_of_prim,
+                      const double * pre_vals, const double * P_out, const double * leaf_out, int use_far,
+                      double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )
+    {
+        if( !V ) { return; }
+        if( size == 3 ) { hipLaunchKernelGGL( (k_post<3>), grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, leaf_idx_of_prim, pre_vals, P_out, leaf_out, use_far, alpha, beta, Y, ldY ); }
+        else            { hipLaunchKernelGGL( (k_post<2>), grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, leaf_idx_of_prim, pre_vals, P_out, leaf_out, use_far, alpha, beta, Y, ldY ); }
+    }
+
+    void launch_fill( double * p, int64_t n, double v, hipStream_t s )
+    { if( n ) { hipLaunchKernelGGL( k_fill, grid1(n), dim3(256), 0, s, p, n, v ); } }
+
+} // namespace rp

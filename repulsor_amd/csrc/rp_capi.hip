@@ -1,0 +1,591 @@
+// C-ABI entry points of librepulsor_hip.so (declared in include/repulsor_hip.h).
+
+#include "rp_internal.hpp"
+#include "k_pair.hpp"
+#include "rp_launch.hpp"
+
+#include <cstring>
+#include <exception>
+#include <new>
+
+namespace rp
+{
+    static thread_local std::string g_last_error;
+    void set_error( const std::string & msg ) { g_last_error = msg; }
+
+    struct InvalidArg { std::string msg; int status; };
+
+    static void require( bool cond, const std::string & msg, int status = RP_EINVAL )
+    {
+        if( !cond ) { throw InvalidArg{ msg, status }; }
+    }
+
+    template<typename F>
+    static int guarded( const char * name, F && f )
+    {
+        try
+        {
+            f();
+            return RP_OK;
+        }
+        catch( const InvalidArg & e ) { set_error( std::string(name) + ": " + e.msg ); return e.status; }
+        catch( const HipError & e )   { set_error( std::string(name) + ": " + e.msg ); return e.status; }
+        catch( const std::bad_alloc & ) { set_error( std::string(name) + ": host allocation failed" ); return RP_ENOMEM; }
+        catch( const std::exception & e ) { set_error( std::string(name) + ": " + e.what() ); return RP_EINVAL; }
+    }
+
+    struct EventScope
+    {
+        rp_ctx * ctx; int which;
+        EventScope( rp_ctx * c, int w ) : ctx(c), which(w) { RP_HIP( hipEventRecord( ctx->ev[which][0], ctx->stream ) ); }
+        void stop() { RP_HIP( hipEventRecord( ctx->ev[which][1], ctx->stream ) ); }
+    };
+
+    static void finish_timing( rp_ctx * ctx, int which )
+    {
+        float ms = 0;
+        RP_HIP( hipEventElapsedTime( &ms, ctx->ev[which][0], ctx->ev[which][1] ) );
+        ctx->timings.ms[which] = ms;
+    }
+
+    static int exponent_kind( double q, double p )
+    {
+        if( q == 6.0 && p == 12.0 ) { return KIND_Q6P12; }
+        if( q == 2.0 && p == 4.0 )  { return KIND_Q2P4; }
+        return KIND_GENERIC;
+    }
+
+    static void pack_rows( const double * src, int64_t ld, int64_t n, std::vector<double> & dst )
+    {
+        dst.resize( (size_t)(3 * n) );
+        for( int64_t v = 0; v < n; ++v ) { for( int k = 0; k < 3; ++k ) { dst[3 * v + k] = src[ld * v + k]; } }
+    }
+
+    // Leaf-path sums of the far-field derivatives (needed by both read-out paths).
+    static void collect_far( rp_ctx * ctx )
+    {
+        ctx->leaf_far.require( (size_t)(10 * ctx->L) );
+        launch_leaf_far( ctx->L, ctx->C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->ff_dx.get(), ctx->leaf_far.get(), ctx->stream );
+    }
+}
+
+using namespace rp;
+
+extern "C"
+{
+
+const char * rp_last_error(void) { return g_last_error.c_str(); }
+
+const char * rp_version(void) { return "repulsor_hip 0.1 (gfx950)"; }
+
+int rp_ctx_create( int device, rp_ctx ** out )
+{
+    return guarded( "rp_ctx_create", [&]()
+    {
+        require( out != nullptr, "out is NULL" );
+        *out = nullptr;
+        int count = 0;
+        RP_HIP( hipGetDeviceCount( &count ) );
+        require( device >= 0 && device < count, "no such HIP device", RP_EHIP );
+        RP_HIP( hipSetDevice( device ) );
+        rp_ctx * c = new rp_ctx();
+        c->device = device;
+        try
+        {
+            RP_HIP( hipStreamCreateWithFlags( &c->stream, hipStreamNonBlocking ) );
+            for( int k = 0; k < 8; ++k ) { RP_HIP( hipEventCreate( &c->ev[k][0] ) ); RP_HIP( hipEventCreate( &c->ev[k][1] ) ); }
+        }
+        catch( ... ) { delete c; throw; }
+        *out = c;
+    });
+}
+
+int rp_ctx_destroy( rp_ctx * ctx )
+{
+    if( !ctx ) { return RP_OK; }
+    (void)hipSetDevice( ctx->device );
+    if( ctx->stream ) { (void)hipStreamSynchronize( ctx->stream ); }
+    for( int k = 0; k < 8; ++k ) { (void)hipEventDestroy( ctx->ev[k][0] ); (void)hipEventDestroy( ctx->ev[k][1] ); }
+    if( ctx->stream ) { (void)hipStreamDestroy( ctx->stream ); }
+    delete ctx;
+    return RP_OK;
+}
+
+int rp_upload_mesh( rp_ctx * ctx, int dom_dim, int64_t vertex_count, int64_t simplex_count,
+                    const int32_t * simplices, const double * vertex_charges )
+{
+    return guarded( "rp_upload_mesh", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require( dom_dim == 1 || dom_dim == 2, "only curves (dom_dim 1) and surfaces (dom_dim 2) in R^3 are supported", RP_EUNSUPPORTED );
+        require( vertex_count > 0 && simplex_count > 0 && simplices != nullptr, "empty mesh" );
+        require( vertex_count < (int64_t)INT32_MAX && simplex_count * (dom_dim + 1) < (int64_t)INT32_MAX, "mesh too large for 32-bit indices" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int size = dom_dim + 1;
+        for( int64_t r = 0; r < simplex_count * size; ++r )
+        {
+            require( simplices[r] >= 0 && simplices[r] < vertex_count, "simplex vertex index out of range" );
+        }
+        ctx->dom_dim = dom_dim;
+        ctx->size = size;
+        ctx->nd = 1 + 3 * size + 6;
+        ctx->V = vertex_count;
+        ctx->T = simplex_count;
+        ctx->simplices.upload( simplices, (size_t)(simplex_count * size), ctx->stream );
+        ctx->have_charges = ( vertex_charges != nullptr );
+        if( vertex_charges ) { ctx->charges.upload( vertex_charges, (size_t)vertex_count, ctx->stream ); }
+        // vertex -> corner CSR, corners increasing
+        std::vector<int64_t> ptr( (size_t)vertex_count + 1, 0 );
+        for( int64_t r = 0; r < simplex_count * size; ++r ) { ++ptr[(size_t)simplices[r] + 1]; }
+        for( int64_t v = 0; v < vertex_count; ++v ) { ptr[(size_t)v + 1] += ptr[(size_t)v]; }
+        std::vector<int32_t> corner( (size_t)(simplex_count * size) );
+        std::vector<int64_t> pos( ptr.begin(), ptr.end() - 1 );
+        for( int64_t r = 0; r < simplex_count * size; ++r ) { corner[(size_t)pos[(size_t)simplices[r]]++] = (int32_t)r; }
+        ctx->vc_ptr.upload( ptr.data(), ptr.size(), ctx->stream );
+        ctx->vc_corner.upload( corner.data(), corner.size(), ctx->stream );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->have_tree = ctx->have_near = ctx->have_far = ctx->have_data = false;
+        ctx->have_diff = ctx->have_metric = false;
+    });
+}
+
+int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
+                    const int32_t * C_left, const int32_t * C_right,
+                    const int32_t * C_begin, const int32_t * C_end,
+                    const int32_t * P_ordering,
+                    const int32_t * pre_cols, const double * pre_vals )
+{
+    return guarded( "rp_upload_tree", [&]()
+    {
+        require( ctx != nullptr && ctx->T > 0, "upload the mesh first", RP_ENOTREADY );
+        require( cluster_count > 0 && C_left && C_right && C_begin && C_end && P_ordering && pre_cols && pre_vals, "missing tree arrays" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t C = cluster_count, T = ctx->T;
+        const int size = ctx->size;
+        std::vector<int32_t> parent( (size_t)C, -1 ), depth( (size_t)C, 0 );
+        std::vector<int32_t> leaves;
+        for( int64_t c = 0; c < C; ++c )
+        {
+            require( C_begin[c] >= 0 && C_end[c] <= T && C_begin[c] < C_end[c], "bad cluster range" );
+            const int32_t L = C_left[c], R = C_right[c];
+            if( L >= 0 )
+            {
+                require( L > c && R > c && L < C && R < C, "children must follow their parent in depth-first numbering" );
+                parent[(size_t)L] = (int32_t)c; parent[(size_t)R] = (int32_t)c;
+            }
+            else { leaves.push_back( (int32_t)c ); }
+        }
+        for( int64_t c = 0; c < C; ++c ) { if( parent[(size_t)c] >= 0 ) { depth[(size_t)c] = depth[(size_t)parent[(size_t)c]] + 1; } }
+        int maxd = 0;
+        for( auto d : depth ) { maxd = std::max( maxd, d ); }
+        require( maxd < 90, "cluster tree deeper than 90 levels" );
+        const int64_t L = (int64_t)leaves.size();
+        std::vector<int32_t> leaf_of_prim( (size_t)T, -1 );
+        int max_leaf = 0;
+        for( int64_t l = 0; l < L; ++l )
+        {
+            const int32_t c = leaves[(size_t)l];
+            max_leaf = std::max( max_leaf, C_end[c] - C_begin[c] );
+            for( int32_t i = C_begin[c]; i < C_end[c]; ++i ) { leaf_of_prim[(size_t)i] = (int32_t)l; }
+        }
+        for( auto x : leaf_of_prim ) { require( x >= 0, "leaves do not cover all primitives" ); }
+        for( int64_t i = 0; i < T; ++i ) { require( P_ordering[i] >= 0 && P_ordering[i] < T, "bad P_ordering" ); }
+        // interior clusters grouped by depth, deepest first
+        std::vector<int32_t> level_ptr( (size_t)maxd + 2, 0 );
+        for( int64_t c = 0; c < C; ++c ) { if( C_left[c] >= 0 ) { ++level_ptr[(size_t)(maxd - depth[(size_t)c]) + 1]; } }
+        for( int d = 0; d <= maxd; ++d ) { level_ptr[(size_t)d + 1] += level_ptr[(size_t)d]; }
+        std::vector<int32_t> level_nodes( (size_t)std::max<int32_t>( 1, level_ptr.back() ) );
+        {
+            std::vector<int32_t> pos( level_ptr.begin(), level_ptr.end() - 1 );
+            for( int64_t c = 0; c < C; ++c ) { if( C_left[c] >= 0 ) { level_nodes[(size_t)pos[(size_t)(maxd - depth[(size_t)c])]++] = (int32_t)c; } }
+        }
+        // vertex -> (tree primitive, slot) entries, increasing primitive
+        const int64_t V = ctx->V;
+        std::vector<int64_t> vptr( (size_t)V + 1, 0 );
+        for( int64_t e = 0; e < T * size; ++e )
+        {
+            require( pre_cols[e] >= 0 && pre_cols[e] < V, "bad pre_cols" );
+            ++vptr[(size_t)pre_cols[e] + 1];
+        }
+        for( int64_t v = 0; v < V; ++v ) { vptr[(size_t)v + 1] += vptr[(size_t)v]; }
+        std::vector<int32_t> ventry( (size_t)(T * size) );
+        {
+            std::vector<int64_t> pos( vptr.begin(), vptr.end() - 1 );
+            for( int64_t e = 0; e < T * size; ++e ) { ventry[(size_t)pos[(size_t)pre_cols[e]]++] = (int32_t)e; }
+        }
+        ctx->C = C; ctx->L = L; ctx->max_depth = maxd; ctx->max_leaf = max_leaf;
+        ctx->C_left.upload( C_left, (size_t)C, ctx->stream );
+        ctx->C_right.upload( C_right, (size_t)C, ctx->stream );
+        ctx->C_begin.upload( C_begin, (size_t)C, ctx->stream );
+        ctx->C_end.upload( C_end, (size_t)C, ctx->stream );
+        ctx->C_parent.upload( parent.data(), (size_t)C, ctx->stream );
+        ctx->P_ordering.upload( P_ordering, (size_t)T, ctx->stream );
+        ctx->leaf_of_prim.upload( leaf_of_prim.data(), (size_t)T, ctx->stream );
+        ctx->leaf_cluster.upload( leaves.data(), (size_t)L, ctx->stream );
+        ctx->pre_cols.upload( pre_cols, (size_t)(T * size), ctx->stream );
+        ctx->pre_vals.upload( pre_vals, (size_t)(T * 4 * size), ctx->stream );
+        ctx->vp_ptr.upload( vptr.data(), vptr.size(), ctx->stream );
+        ctx->vp_entry.upload( ventry.data(), ventry.size(), ctx->stream );
+        ctx->h_level_ptr = level_ptr;
+        ctx->level_nodes.upload( level_nodes.data(), level_nodes.size(), ctx->stream );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->have_tree = true;
+        ctx->have_near = ctx->have_far = ctx->have_data = ctx->have_diff = ctx->have_metric = false;
+    });
+}
+
+int rp_upload_near_tiles( rp_ctx * ctx, int32_t leaf_count,
+                          const int32_t * leaf_begin, const int32_t * leaf_end,
+                          const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask )
+{
+    return guarded( "rp_upload_near_tiles", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree, "upload the tree first", RP_ENOTREADY );
+        require( leaf_count == ctx->L, "leaf_count differs from the tree's leaf count" );
+        require( leaf_begin && leaf_end && col_ptr, "missing tile arrays" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t L = leaf_count;
+        std::vector<int64_t> val_ptr( (size_t)L + 1, 0 );
+        require( col_ptr[0] == 0, "col_ptr[0] must be 0" );
+        for( int64_t l = 0; l < L; ++l )
+        {
+            const int64_t n = leaf_end[l] - leaf_begin[l];
+            require( n >= 1 && n <= 32, "leaves must hold 1..32 primitives (split_threshold <= 32)", RP_EUNSUPPORTED );
+            require( leaf_begin[l] >= 0 && leaf_end[l] <= ctx->T, "bad leaf range" );
+            require( col_ptr[l + 1] >= col_ptr[l], "col_ptr must be non-decreasing" );
+            val_ptr[(size_t)l + 1] = val_ptr[(size_t)l] + n * ( col_ptr[l + 1] - col_ptr[l] );
+        }
+        const int64_t M = col_ptr[L];
+        require( M == 0 || (col_prim && col_mask), "missing column arrays" );
+        for( int64_t k = 0; k < M; ++k ) { require( col_prim[k] >= 0 && col_prim[k] < ctx->T, "column primitive out of range" ); }
+        ctx->leaf_begin.upload( leaf_begin, (size_t)L, ctx->stream );
+        ctx->leaf_end.upload( leaf_end, (size_t)L, ctx->stream );
+        ctx->col_ptr.upload( col_ptr, (size_t)L + 1, ctx->stream );
+        ctx->val_ptr.upload( val_ptr.data(), (size_t)L + 1, ctx->stream );
+        if( M > 0 )
+        {
+            ctx->col_prim.upload( col_prim, (size_t)M, ctx->stream );
+            ctx->col_mask.upload( col_mask, (size_t)M, ctx->stream );
+        }
+        ctx->nf_cols = M;
+        ctx->nf_slots = val_ptr.back();
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->have_near = true;
+        ctx->have_diff = ctx->have_metric = false;
+    });
+}
+
+int rp_upload_far_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp_, const int32_t * ci )
+{
+    return guarded( "rp_upload_far_pattern", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree, "upload the tree first", RP_ENOTREADY );
+        require( rows == ctx->C && rp_ != nullptr, "far pattern must have one row per cluster" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t nnz = rp_[rows];
+        require( rp_[0] == 0 && nnz >= 0, "bad row pointers" );
+        for( int64_t k = 0; k < nnz; ++k ) { require( ci[k] >= 0 && ci[k] < rows, "far column out of range" ); }
+        ctx->far_rp.upload( rp_, (size_t)rows + 1, ctx->stream );
+        if( nnz > 0 ) { ctx->far_ci.upload( ci, (size_t)nnz, ctx->stream ); }
+        ctx->far_nnz = nnz;
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->have_far = true;
+        ctx->have_diff = ctx->have_metric = false;
+    });
+}
+
+int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near, const double * C_far )
+{
+    return guarded( "rp_upload_data", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree, "upload the tree first", RP_ENOTREADY );
+        require( V_coords && P_near && C_far, "missing data arrays" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T, C = ctx->C, L = ctx->L;
+        for( int64_t k = 0; k < 3 * ctx->V; ++k ) { require( std::isfinite( V_coords[k] ), "vertex coordinates contain NaN or Inf" ); }
+        ctx->coords.upload( V_coords, (size_t)(3 * ctx->V), ctx->stream );
+        ctx->stage.upload( P_near, (size_t)(ctx->nd * T), ctx->stream );
+        ctx->p_a.require( (size_t)T ); ctx->p_x.require( (size_t)(3 * T) ); ctx->p_P.require( (size_t)(6 * T) );
+        launch_prep_prims( T, ctx->nd, ctx->size, ctx->stage.get(), ctx->p_a.get(), ctx->p_x.get(), ctx->p_P.get(), ctx->stream );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->stage.upload( C_far, (size_t)(10 * C), ctx->stream );
+        ctx->c_a.require( (size_t)C ); ctx->c_x.require( (size_t)(3 * C) ); ctx->c_P.require( (size_t)(6 * C) );
+        launch_prep_clusters( C, ctx->stage.get(), ctx->c_a.get(), ctx->c_x.get(), ctx->c_P.get(), ctx->stream );
+        ctx->leaf_origin.require( (size_t)(3 * L) );
+        launch_leaf_origin( L, C, ctx->leaf_cluster.get(), ctx->c_x.get(), ctx->leaf_origin.get(), ctx->stream );
+        RP_HIP( hipGetLastError() );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->have_data = true;
+        ctx->have_diff = ctx->have_metric = false;
+    });
+}
+
+int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precision, double * energy_out )
+{
+    return guarded( "rp_tp0_compute", [&]()
+    {
+        require( ctx != nullptr && ctx->have_data && ctx->have_near && ctx->have_far, "mesh, tree, patterns and data must be uploaded", RP_ENOTREADY );
+        require( (flags & (RP_ENERGY | RP_DIFF | RP_METRIC)) != 0, "no work requested" );
+        require( precision == RP_FP64 || precision == RP_MIXED, "unknown precision" );
+        require( q >= 0, "Tangent-point kernels not well-defined for q < 0." );
+        require( q == 0 || q >= 2, "Tangent-point kernels not implement for 0 < q < 2." );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int n = ctx->dom_dim;
+        const int64_t T = ctx->T, C = ctx->C, L = ctx->L;
+        const double s = (p - n) / q;
+        const double s_exp = -0.5 * ( 2.0 * (s - 1.0) + n );
+        const Expo ex = make_expo( q, p, n, s_exp );
+        const int kind = exponent_kind( q, p );
+        const bool mixed = ( precision == RP_MIXED );
+        const bool E = flags & RP_ENERGY, D = flags & RP_DIFF, M = flags & RP_METRIC;
+
+        ctx->partials.require( (size_t)(L + C) );
+        ctx->scalar.require( 4 );
+        if( D ) { ctx->nf_dx.require( (size_t)(10 * T) ); ctx->ff_dx.require( (size_t)(10 * C) ); }
+        if( M )
+        {
+            ctx->nf_vals.require( (size_t)std::max<int64_t>( 1, 2 * ctx->nf_slots ) );
+            ctx->nf_diag.require( (size_t)(2 * T) );
+            ctx->ff_vals.require( (size_t)std::max<int64_t>( 1, 2 * ctx->far_nnz ) );
+            ctx->ff_diag.require( (size_t)(2 * C) );
+        }
+        EventScope ev( ctx, 0 );
+        if( E ) { launch_fill( ctx->partials.get(), L + C, 0.0, ctx->stream ); }
+        const bool have_nf = ctx->nf_cols > 0;
+        const bool have_ff = ctx->far_nnz > 0;
+        if( !have_nf )
+        {
+            if( D ) { launch_fill( ctx->nf_dx.get(), 10 * T, 0.0, ctx->stream ); }
+            if( M ) { launch_fill( ctx->nf_diag.get(), 2 * T, 0.0, ctx->stream ); }
+        }
+        if( !have_ff )
+        {
+            if( D ) { launch_fill( ctx->ff_dx.get(), 10 * C, 0.0, ctx->stream ); }
+            if( M ) { launch_fill( ctx->ff_diag.get(), 2 * C, 0.0, ctx->stream ); }
+        }
+        RP_HIP( hipEventRecord( ctx->ev[4][0], ctx->stream ) );
+        if( have_nf )
+        {
+            NFArgs A{};
+            A.L = L; A.T = T;
+            A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
+            A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
+            A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.P = ctx->p_P.get(); A.origin = ctx->leaf_origin.get();
+            A.scale = 1.0 / ctx->size; A.size = ctx->size; A.ex = ex;
+            A.dx = D ? ctx->nf_dx.get() : nullptr;
+            A.epart = ctx->partials.get();
+            A.vals = M ? (void *)ctx->nf_vals.get() : nullptr;
+            A.diag = M ? ctx->nf_diag.get() : nullptr;
+            launch_nf( flags, kind, mixed, A, ctx->stream );
+            RP_HIP( hipGetLastError() );
+        }
+        RP_HIP( hipEventRecord( ctx->ev[4][1], ctx->stream ) );
+        RP_HIP( hipEventRecord( ctx->ev[5][0], ctx->stream ) );
+        if( have_ff )
+        {
+            FFArgs B{};
+            B.C = C; B.rp = ctx->far_rp.get(); B.ci = ctx->far_ci.get();
+            B.a = ctx->c_a.get(); B.x = ctx->c_x.get(); B.P = ctx->c_P.get(); B.ex = ex;
+            B.dx = D ? ctx->ff_dx.get() : nullptr;
+            B.epart = ctx->partials.get() + L;
+            B.vals = M ? (void *)ctx->ff_vals.get() : nullptr;
+            B.diag = M ? ctx->ff_diag.get() : nullptr;
+            launch_ff( flags, kind, mixed, B, ctx->stream );
+            RP_HIP( hipGetLastError() );
+        }
+        RP_HIP( hipEventRecord( ctx->ev[5][1], ctx->stream ) );
+        if( E ) { launch_reduce_sum( ctx->partials.get(), L + C, ctx->scalar.get(), ctx->stream ); }
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        double en = 0;
+        if( E ) { RP_HIP( hipMemcpyAsync( &en, ctx->scalar.get(), sizeof(double), hipMemcpyDeviceToHost, ctx->stream ) ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        finish_timing( ctx, 0 ); finish_timing( ctx, 4 ); finish_timing( ctx, 5 );
+        if( energy_out ) { *energy_out = E ? en : 0.0; }
+        if( D ) { ctx->have_diff = true; }
+        if( M ) { ctx->have_metric = true; ctx->metric_precision = precision; }
+    });
+}
+
+int rp_download_derivatives( rp_ctx * ctx, double * P_D_near, double * P_D_far )
+{
+    return guarded( "rp_download_derivatives", [&]()
+    {
+        require( ctx != nullptr && ctx->have_diff, "no differential computed", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T, L = ctx->L;
+        collect_far( ctx );
+        std::vector<double> dx( (size_t)(10 * T) ), lf( (size_t)(10 * L) );
+        std::vector<int32_t> ord( (size_t)T ), lop( (size_t)T );
+        ctx->nf_dx.download( dx.data(), dx.size(), ctx->stream );
+        ctx->leaf_far.download( lf.data(), lf.size(), ctx->stream );
+        ctx->P_ordering.download( ord.data(), ord.size(), ctx->stream );
+        ctx->leaf_of_prim.download( lop.data(), lop.size(), ctx->stream );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        const int nd = ctx->nd, size = ctx->size;
+        for( int64_t i = 0; i < T; ++i )
+        {
+            const int64_t t = ord[(size_t)i];
+            if( P_D_near )
+            {
+                double * o = P_D_near + nd * t;
+                o[0] = dx[(size_t)i];
+                for( int h = 0; h < size; ++h ) { for( int c = 0; c < 3; ++c ) { o[1 + 3 * h + c] = dx[(size_t)((1 + c) * T + i)]; } }
+                for( int k = 0; k < 6; ++k ) { o[1 + 3 * size + k] = dx[(size_t)((4 + k) * T + i)]; }
+            }
+            if( P_D_far )
+            {
+                const int64_t l = lop[(size_t)i];
+                for( int k = 0; k < 10; ++k ) { P_D_far[10 * t + k] = lf[(size_t)(k * L + l)]; }
+            }
+        }
+    });
+}
+
+int rp_assemble_vertex_gradient( rp_ctx * ctx, double alpha, double beta, double * Y, int64_t ldY, int ptr_flags )
+{
+    return guarded( "rp_assemble_vertex_gradient", [&]()
+    {
+        require( ctx != nullptr && ctx->have_diff, "no differential computed", RP_ENOTREADY );
+        require( Y != nullptr && ldY >= 3, "Y must be V x 3 with ldY >= 3" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T, V = ctx->V, L = ctx->L;
+        const bool dev = ptr_flags & RP_DEVICE_PTR;
+        std::vector<double> h, hy;
+        double * Yd = Y;
+        int64_t ld = ldY;
+        if( !dev )
+        {
+            ctx->Yd.require( (size_t)(3 * V) );
+            if( beta != 0.0 ) { pack_rows( Y, ldY, V, h ); ctx->Yd.upload( h.data(), h.size(), ctx->stream ); }
+            Yd = ctx->Yd.get(); ld = 3;
+        }
+        ctx->hull_grad.require( (size_t)(T * ctx->size * 3) );
+        EventScope ev( ctx, 1 );
+        collect_far( ctx );
+        launch_hulls( ctx->dom_dim, T, L, ctx->P_ordering.get(), ctx->leaf_of_prim.get(), ctx->simplices.get(), ctx->coords.get(),
+                      ctx->have_charges ? ctx->charges.get() : nullptr, ctx->nf_dx.get(), ctx->leaf_far.get(), ctx->hull_grad.get(), ctx->stream );
+        launch_assemble( V, ctx->vc_ptr.get(), ctx->vc_corner.get(), ctx->hull_grad.get(), alpha, beta, Yd, ld, ctx->stream );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        if( !dev )
+        {
+            hy.resize( (size_t)(3 * V) );
+            ctx->Yd.download( hy.data(), hy.size(), ctx->stream );
+        }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        finish_timing( ctx, 1 );
+        if( !dev ) { for( int64_t v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = hy[(size_t)(3 * v + k)]; } } }
+    });
+}
+
+int rp_download_metric( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * ff_vals, double * ff_diag )
+{
+    return guarded( "rp_download_metric", [&]()
+    {
+        require( ctx != nullptr && ctx->have_metric, "no metric computed", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const bool mixed = ctx->metric_precision == RP_MIXED;
+        auto get_vals = [&]( const DBuf<double> & buf, int64_t pairs, double * out )
+        {
+            if( !out || pairs == 0 ) { return; }
+            if( !mixed ) { buf.download( out, (size_t)(2 * pairs), ctx->stream ); return; }
+            std::vector<float> f( (size_t)(2 * pairs) );
+            RP_HIP( hipMemcpyAsync( f.data(), buf.get(), f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream ) );
+            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+            for( size_t k = 0; k < f.size(); ++k ) { out[k] = f[k]; }
+        };
+        get_vals( ctx->nf_vals, ctx->nf_cols > 0 ? ctx->nf_slots : 0, nf_vals );
+        get_vals( ctx->ff_vals, ctx->far_nnz, ff_vals );
+        if( nf_diag ) { ctx->nf_diag.download( nf_diag, (size_t)(2 * ctx->T), ctx->stream ); }
+        if( ff_diag ) { ctx->ff_diag.download( ff_diag, (size_t)(2 * ctx->C), ctx->stream ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+    });
+}
+
+int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ldX,
+                        double beta, double * Y, int64_t ldY, int nrhs,
+                        int vf_flag, int nf_flag, int ff_flag, int ptr_flags )
+{
+    return guarded( "rp_multiply_metric", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require( nrhs == 3, "nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
+        require( ctx->have_metric, "metric values not assembled (call rp_tp0_compute with RP_METRIC)", RP_ENOTREADY );
+        require( X != nullptr && Y != nullptr && ldX >= 3 && ldY >= 3, "X and Y must be V x 3 with ld >= 3" );
+        (void)vf_flag;
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
+        const bool dev = ptr_flags & RP_DEVICE_PTR;
+        const bool mixed = ctx->metric_precision == RP_MIXED;
+        std::vector<double> h, hx, hy;
+        const double * Xd = X;
+        int64_t ldx = ldX;
+        double * Yd = Y;
+        int64_t ldy = ldY;
+        if( !dev )
+        {
+            pack_rows( X, ldX, V, hx );
+            ctx->Xd.upload( hx.data(), hx.size(), ctx->stream );
+            Xd = ctx->Xd.get(); ldx = 3;
+            ctx->Yd.require( (size_t)(3 * V) );
+            if( beta != 0.0 ) { pack_rows( Y, ldY, V, h ); ctx->Yd.upload( h.data(), h.size(), ctx->stream ); }
+            Yd = ctx->Yd.get(); ldy = 3;
+        }
+        ctx->P_in.require( (size_t)(12 * T) ); ctx->P_out.require( (size_t)(12 * T) );
+        ctx->C_in.require( (size_t)(12 * C) ); ctx->C_out.require( (size_t)(12 * C) );
+        ctx->leaf_out.require( (size_t)(12 * L) );
+        const bool do_nf = nf_flag && ctx->nf_cols > 0;
+        const bool do_ff = ff_flag && ctx->far_nnz > 0;
+        EventScope ev( ctx, 2 );
+        launch_pre( ctx->size, T, ctx->pre_cols.get(), ctx->pre_vals.get(), Xd, ldx, ctx->P_in.get(), ctx->stream );
+        if( do_ff )
+        {
+            launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream );
+            for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
+            {
+                const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
+                launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream );
+            }
+        }
+        RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) );
+        if( do_nf )
+        {
+            launch_nf_spmm( mixed, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(), ctx->col_mask.get(),
+                            ctx->val_ptr.get(), ctx->nf_vals.get(), ctx->nf_diag.get(), ctx->P_in.get(), ctx->P_out.get(), ctx->stream );
+        }
+        else
+        {
+            launch_fill( ctx->P_out.get(), 12 * T, 0.0, ctx->stream );
+        }
+        RP_HIP( hipEventRecord( ctx->ev[6][1], ctx->stream ) );
+        if( do_ff )
+        {
+            launch_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->ff_vals.get(), ctx->ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
+            launch_leaf_out( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
+        }
+        launch_post( ctx->size, V, ctx->vp_ptr.get(), ctx->vp_entry.get(), ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->P_out.get(),
+                     ctx->leaf_out.get(), do_ff ? 1 : 0, alpha, beta, Yd, ldy, ctx->stream );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        if( !dev )
+        {
+            hy.resize( (size_t)(3 * V) );
+            ctx->Yd.download( hy.data(), hy.size(), ctx->stream );
+        }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        finish_timing( ctx, 2 ); finish_timing( ctx, 6 );
+        if( !dev ) { for( int64_t v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = hy[(size_t)(3 * v + k)]; } } }
+    });
+}
+
+int rp_last_timing( rp_ctx * ctx, int which, double * ms )
+{
+    return guarded( "rp_last_timing", [&]()
+    {
+        require( ctx != nullptr && ms != nullptr && which >= 0 && which < 8, "bad arguments" );
+        *ms = ctx->timings.ms[which];
+    });
+}
+
+} // extern "C"

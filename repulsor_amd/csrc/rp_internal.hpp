@@ -1,0 +1,194 @@
+#pragma once
+// Internal definitions shared by the HIP translation units of librepulsor_hip.so.
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/repulsor_hip.h"
+
+namespace rp
+{
+    void set_error( const std::string & msg );
+
+    struct HipError
+    {
+        int status;
+        std::string msg;
+    };
+
+    inline void check_hip( hipError_t e, const char * what )
+    {
+        if( e != hipSuccess )
+        {
+            throw HipError{ (e == hipErrorOutOfMemory) ? RP_ENOMEM : RP_EHIP,
+                            std::string(what) + ": " + hipGetErrorString(e) };
+        }
+    }
+
+#define RP_HIP(x) ::rp::check_hip( (x), #x )
+
+    // Device buffer owned by a context.
+    template<typename T>
+    struct DBuf
+    {
+        T * ptr = nullptr;
+        size_t n = 0;
+
+        DBuf() = default;
+        DBuf( const DBuf & ) = delete;
+        DBuf & operator=( const DBuf & ) = delete;
+        ~DBuf() { release(); }
+
+        void release()
+        {
+            if( ptr ) { (void)hipFree( ptr ); }
+            ptr = nullptr; n = 0;
+        }
+
+        // Grows (never shrinks) to at least `count` elements; contents are not preserved on growth.
+        void require( size_t count )
+        {
+            if( count <= n && ptr ) { return; }
+            release();
+            const size_t bytes = std::max<size_t>( 1, count ) * sizeof(T);
+            RP_HIP( hipMalloc( (void**)&ptr, bytes ) );
+            n = std::max<size_t>( 1, count );
+        }
+
+        void upload( const T * h, size_t count, hipStream_t s )
+        {
+            require( count );
+            if( count ) { RP_HIP( hipMemcpyAsync( ptr, h, count * sizeof(T), hipMemcpyHostToDevice, s ) ); }
+        }
+
+        void download( T * h, size_t count, hipStream_t s ) const
+        {
+            if( count ) { RP_HIP( hipMemcpyAsync( h, ptr, count * sizeof(T), hipMemcpyDeviceToHost, s ) ); }
+        }
+
+        T * get() const { return ptr; }
+    };
+
+    // Exponent description shared by host and device.
+    struct Expo
+    {
+        double q, p;
+        double q_half, p_half;
+        int    q_half_i, p_half_i;
+        int    q_int, p_int;
+        int    q_flag;            // 2: q/2 >= 1, 0: q == 0
+        // metric kernel exponent s_exp = -((s-1) + n/2), s = (p-n)/q, with a fast path when 3*s_exp is an integer
+        double s_exp;
+        int    s_mode;            // 0: integer k, 1: k/3 via cbrt, 2: generic pow
+        int    s_k;               // numerator
+    };
+
+    inline Expo make_expo( double q, double p, int n, double s_exp )
+    {
+        Expo e{};
+        e.q = q; e.p = p;
+        e.q_half = q / 2; e.p_half = p / 2;
+        e.q_half_i = (int)e.q_half; e.p_half_i = (int)e.p_half;
+        e.q_int = (e.q_half == (double)e.q_half_i);
+        e.p_int = (e.p_half == (double)e.p_half_i);
+        e.q_flag = (e.q_half >= 1) ? 2 : ((e.q_half == 0) ? 0 : -1);
+        e.s_exp = s_exp;
+        const double k3 = 3.0 * s_exp;
+        const double kr = std::nearbyint( k3 );
+        if( s_exp == std::nearbyint(s_exp) ) { e.s_mode = 0; e.s_k = (int)std::nearbyint(s_exp); }
+        else if( std::abs( k3 - kr ) < 1e-12 ) { e.s_mode = 1; e.s_k = (int)kr; }
+        else { e.s_mode = 2; e.s_k = 0; }
+        (void)n;
+        return e;
+    }
+
+    struct Timings
+    {
+        double ms[8] = {0,0,0,0,0,0,0,0};
+    };
+
+} // namespace rp
+
+struct rp_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8][2];
+
+    // sizes
+    int dom_dim = 0, size = 0, nd = 0;
+    int64_t V = 0, T = 0, C = 0, L = 0;
+    int max_depth = 0;
+    int max_leaf = 0;
+
+    // mesh (original order)
+    rp::DBuf<int32_t> simplices;
+    rp::DBuf<double>  charges;
+    bool have_charges = false;
+    rp::DBuf<double>  coords;
+    // vertex -> corner CSR (corner = t*(n+1)+j, increasing) for the derivative assembler
+    rp::DBuf<int64_t> vc_ptr;
+    rp::DBuf<int32_t> vc_corner;
+
+    // tree
+    rp::DBuf<int32_t> C_left, C_right, C_begin, C_end, C_parent, P_ordering;
+    rp::DBuf<int32_t> leaf_of_prim;      // tree position -> leaf cluster id
+    rp::DBuf<int32_t> leaf_cluster;      // leaf index -> cluster id
+    rp::DBuf<int32_t> pre_cols;          // T x (n+1)
+    rp::DBuf<double>  pre_vals;          // T x 4 x (n+1)
+    rp::DBuf<int64_t> vp_ptr;            // vertex -> (tree primitive, slot), sorted by primitive
+    rp::DBuf<int32_t> vp_entry;          // primitive * (n+1) + slot
+    std::vector<int32_t> h_level_ptr;    // clusters grouped by depth (deepest last)
+    rp::DBuf<int32_t> level_nodes;
+    bool have_tree = false;
+
+    // near tiles
+    rp::DBuf<int32_t>  leaf_begin, leaf_end;
+    rp::DBuf<int64_t>  col_ptr, val_ptr;
+    rp::DBuf<int32_t>  col_prim;
+    rp::DBuf<uint32_t> col_mask;
+    int64_t nf_slots = 0;
+    int64_t nf_cols = 0;
+    bool have_near = false;
+
+    // far pattern
+    rp::DBuf<int64_t> far_rp;
+    rp::DBuf<int32_t> far_ci;
+    int64_t far_nnz = 0;
+    bool have_far = false;
+
+    // data (tree order, SoA)
+    rp::DBuf<double> p_a, p_x, p_P;      // T, 3T, 6T
+    rp::DBuf<double> c_a, c_x, c_P;      // C, 3C, 6C
+    rp::DBuf<double> leaf_origin;        // 3L
+    rp::DBuf<double> stage;              // upload staging
+    bool have_data = false;
+
+    // energy/derivative outputs
+    rp::DBuf<double> nf_dx;              // 10 x T
+    rp::DBuf<double> ff_dx;              // 10 x C
+    rp::DBuf<double> leaf_far;           // 10 x L: percolated far derivatives per leaf
+    rp::DBuf<double> partials;           // L + C energy partials
+    rp::DBuf<double> scalar;             // device scalars
+    rp::DBuf<double> hull_grad;          // T x (n+1) x 3
+    bool have_diff = false;
+
+    // metric
+    rp::DBuf<double> nf_vals;            // 2 x nf_slots (fp64) or packed floats
+    rp::DBuf<double> nf_diag;            // 2T
+    rp::DBuf<double> ff_vals;            // 2 x far_nnz
+    rp::DBuf<double> ff_diag;            // 2C
+    bool have_metric = false;
+    int metric_precision = 0;
+
+    // multiply buffers
+    rp::DBuf<double> P_in, P_out, C_in, C_out, leaf_out;   // 12T, 12T, 12C, 12C, 12L
+    rp::DBuf<double> Xd, Yd;                                // 3V staging
+
+    rp::Timings timings;
+};

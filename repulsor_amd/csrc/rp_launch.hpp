@@ -1,0 +1,69 @@
+#pragma once
+// Kernel argument blocks and host launchers shared by the translation units.
+
+#include "rp_internal.hpp"
+
+namespace rp
+{
+    struct NFArgs
+    {
+        int64_t L;
+        int64_t T;
+        const int32_t  * leaf_begin;
+        const int32_t  * leaf_end;
+        const int64_t  * col_ptr;
+        const int32_t  * col_prim;
+        const uint32_t * col_mask;
+        const int64_t  * val_ptr;
+        const double   * a;       // T
+        const double   * x;       // 3 x T
+        const double   * P;       // 6 x T
+        const double   * origin;  // 3 x L (mixed precision local origins)
+        double           scale;   // 1/(n+1)
+        int              size;    // n+1
+        Expo             ex;
+        double         * dx;      // 10 x T
+        double         * epart;   // L
+        void           * vals;    // 2 x slots (double or float)
+        double         * diag;    // 2 x T
+    };
+
+    struct FFArgs
+    {
+        int64_t C;
+        const int64_t * rp;
+        const int32_t * ci;
+        const double  * a;        // C
+        const double  * x;        // 3 x C
+        const double  * P;        // 6 x C
+        Expo            ex;
+        double        * dx;       // 10 x C
+        double        * epart;    // C
+        void          * vals;     // 2 x nnz
+        double        * diag;     // 2 x C
+    };
+
+    void launch_nf( unsigned flags, int kind, bool mixed, const NFArgs & A, hipStream_t s );
+    void launch_ff( unsigned flags, int kind, bool mixed, const FFArgs & A, hipStream_t s );
+    void launch_reduce_sum( const double * part, int64_t n, double * out, hipStream_t s );
+
+    void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, hipStream_t s );
+    void launch_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P, hipStream_t s );
+    void launch_leaf_origin( int64_t L, int64_t C, const int32_t * leaf_cluster, const double * cx, double * origin, hipStream_t s );
+    void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * ff_dx, double * leaf_far, hipStream_t s );
+    void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s );
+    void launch_hulls( int n, int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim, const int32_t * S,
+                       const double * V, const double * charges, const double * nf_dx, const double * leaf_far, double * B, hipStream_t s );
+    void launch_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
+    void launch_pre( int size, int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
+    void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s );
+    void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s );
+    void launch_nf_spmm( bool mixed, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp, const int32_t * cpr, const uint32_t * cm,
+                         const int64_t * vp, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s );
+    void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s );
+    void launch_post( int size, int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_idx_of_prim,
+                      const double * pre_vals, const double * P_out, const double * leaf_out, int use_far,
+                      double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
+    void launch_fill( double * p, int64_t n, double v, hipStream_t s );
+
+} // namespace rp
