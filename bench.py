@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Benchmark of the tangent-point hot path on one MI355X.
+
+Metric: "energy+grad ms & metric-solve ms to 1e-6 residual, learned vs plain PC, 1 GPU".
+Workload (default): subdivided icosahedron level 8 (655,362 vertices, 1,310,720 triangles),
+radial spherical-harmonic noise l<=16 amplitude 0.03 (seed 5), q=6, p=12, far theta 0.5,
+near theta 10, leaf size 16.
+
+One step = TangentPointEnergy0 energy + differential (near and far kernels, derivative
+collection, chain rule, vertex assembly) with all inputs resident in HBM and the output in a
+device buffer, followed by the metric solve when enabled. Tree/BCT construction, uploads and
+metric assembly are timed separately and reported, never folded into the step.
+
+Prints one JSON line (see README contract): value = ms per step (lower is better).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+# Flop counts per upper-triangle pair, counted from the reference kernels with
+# add/sub/mul = 1, div = 1, integer powers as their multiplies (TP0_Kernel_NF.hpp:140-338 with
+# loadT's barycenter and writeT's 16-wide accumulation; TP0_Kernel_FF.hpp:130-319 with writeT's
+# 10-wide accumulation), for q=6, p=12 (integer exponents).
+NF_FLOPS_VALUE_DIFF = 263
+FF_FLOPS_VALUE_DIFF = 194
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (equal to the FP64 matrix peak)
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def _hip():
+    lib = C.CDLL("libamdhip64.so")
+    lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    lib.hipFree.argtypes = [C.c_void_p]
+    lib.hipDeviceSynchronize.argtypes = []
+    return lib
+
+
+def cpu_baseline(M, wl, target_pairs):
+    """Oracle restatement (-O3 -ffast-math build, 1 thread) on a bounded sample of the same workload:
+    the near-field rows [0, R) holding about `target_pairs` upper pairs, the complete far field.
+    Returns the extrapolated energy+grad ms for the whole workload."""
+    import oracle as O
+    f = O.fast_tp0()
+    b = M.build()
+    td = M.tree_data()
+    # choose R from the per-row counts of the tile structure: upper pairs of row i ~ half its row
+    T = b["T"]
+    near_upper_total = b["near_nnz"] // 2
+    R = max(1, int(T * min(1.0, target_pairs / max(1, near_upper_total))))
+    nrp, nci, ndg = M.pattern("near", row_end=R)
+    sample_pairs = int(np.sum(nrp[1:R + 1] - ndg[:R]))
+    frp, fci, fdg = M.pattern("far")
+    P_near, C_far = O.c64(td["P_near"]), O.c64(td["C_far"])
+    Cn = C_far.shape[0]
+    nd = P_near.shape[1]
+    PDn = np.zeros((T, nd)); CDf = np.zeros((Cn, 10))
+    d = C.POINTER(C.c_double); i32 = C.POINTER(C.c_int32); i64 = C.POINTER(C.c_int64)
+    zf = np.zeros((1, 2))
+
+    def run(np_rp, np_ci, np_dg, fp_rp, fp_ci, fp_dg):
+        t0 = time.perf_counter()
+        f(M.dom_dim, T, Cn, P_near.ctypes.data_as(d), C_far.ctypes.data_as(d),
+          np_rp.ctypes.data_as(i64), np_ci.ctypes.data_as(i32), np_dg.ctypes.data_as(i64),
+          fp_rp.ctypes.data_as(i64), fp_ci.ctypes.data_as(i32), fp_dg.ctypes.data_as(i64),
+          wl.q, wl.p, 3, PDn.ctypes.data_as(d), CDf.ctypes.data_as(d), zf.ctypes.data_as(d), zf.ctypes.data_as(d),
+          zf.ctypes.data_as(d), zf.ctypes.data_as(d))
+        return time.perf_counter() - t0
+
+    empty_rp = np.zeros(Cn + 1, dtype=np.int64); empty_ci = np.zeros(1, dtype=np.int32); empty_dg = np.zeros(Cn, dtype=np.int64)
+    t_near = run(O.c64i(nrp), O.c32(nci), O.c64i(ndg), empty_rp, empty_ci, empty_dg)
+    nrp0 = np.zeros(T + 1, dtype=np.int64); ndg0 = np.zeros(T, dtype=np.int64)
+    t_far = run(nrp0, empty_ci, ndg0, O.c64i(frp), O.c32(fci), O.c64i(fdg))
+    # collection + chain rule + assembly (complete)
+    tr = M.tree()
+    t0 = time.perf_counter()
+    O.differential(M.dom_dim, M.VertexCoordinates(), M.simplices, tr, PDn, CDf)
+    t_chain = time.perf_counter() - t0
+    total_s = t_near * (near_upper_total / max(1, sample_pairs)) + t_far + t_chain
+    sampled_s = t_near + t_far + t_chain
+    return {
+        "value": round(total_s * 1e3, 1), "unit": "ms (energy+grad, extrapolated)", "cores": 1, "kind": "port",
+        "sample": (f"near-field rows [0,{R}) = {sample_pairs} of {near_upper_total} upper pairs timed and scaled; "
+                   f"far field ({b['far_nnz'] // 2} pairs), derivative collection, chain rule and assembly complete; "
+                   f"{sampled_s:.1f} s of CPU work; oracle -O3 -ffast-math build"),
+        "pairs_per_s": round(sample_pairs / t_near, 1),
+    }
+
+
+def calibrate_copy(mb):
+    lib = C.CDLL(os.path.join(ROOT, "repulsor_amd", "lib", "librepulsor_calib.so"))
+    lib.rpc_copy.argtypes = [C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    ms = C.c_double(0)
+    st = lib.rpc_copy(int(mb) * (1 << 20), 5, C.byref(ms))
+    print(json.dumps({"calibration_copy_bytes": int(mb) * (1 << 20), "ms": ms.value, "status": st,
+                      "GBs": 2 * int(mb) * (1 << 20) / (ms.value * 1e-3) / 1e9}), flush=True)
+    return st
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="sphere_1310720")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-pairs", type=float, default=1.2e8)
+    ap.add_argument("--calibrate-copy", type=int, default=0, help="run only the copy calibration kernel (MiB)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design note, not an implemented path).", flush=True)
+        sys.exit(2)
+    if args.calibrate_copy:
+        sys.exit(calibrate_copy(args.calibrate_copy))
+
+    import repulsor_amd as R
+    from repulsor_amd.workloads import WORKLOADS
+
+    wl = WORKLOADS[args.workload]
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    M = wl.mesh(threads=threads)
+    b = M.build()
+    t_build = time.perf_counter() - t0
+    print(f"# workload {wl.name}: T={b['T']} V={b['V']} C={b['C']} leaves={b['L']} near nnz={b['near_nnz']} "
+          f"far nnz={b['far_nnz']} very near={b['verynear_nnz']} build {t_build:.2f}s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    M.upload()
+    t_upload = time.perf_counter() - t0
+
+    E = R.TangentPointEnergy0(wl.q, wl.p, precision=args.precision)
+    A = R.TangentPointMetric0(wl.q, wl.p, precision=args.precision)
+
+    hip = _hip()
+    Yd = C.c_void_p()
+    assert hip.hipMalloc(C.byref(Yd), M.V * 3 * 8) == 0
+    en = np.zeros(1)
+    from repulsor_amd import H, _p, _check  # noqa: E402
+
+    def step():
+        _check(H.rpm_energy_differential(E._h, M._h, 1.0, 0.0, C.cast(Yd, C.POINTER(C.c_double)), 3, _p(en), 1))
+
+    for _ in range(args.warmup):
+        step()
+    nf_ms, ff_ms, eg_ms = [], [], []
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        nf_ms.append(M.last_timing(4)); ff_ms.append(M.last_timing(5))
+        eg_ms.append(M.last_timing(0) + M.last_timing(1))
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    print(f"# energy {en[0]:.15e}  step {ms_per_step:.3f} ms (device {np.mean(eg_ms):.3f} ms: near {np.mean(nf_ms):.3f}, far {np.mean(ff_ms):.3f})",
+          file=sys.stderr, flush=True)
+
+    # metric assembly and MultiplyMetric (reported, not part of the step until the solve lands)
+    t0 = time.perf_counter()
+    A.assemble(M)
+    t_assemble = time.perf_counter() - t0
+    X = np.random.default_rng(11).standard_normal((M.V, 3))
+    Y = np.zeros_like(X)
+    A.MultiplyMetric(M, 1.0, X, 0.0, Y)
+    mm_ms, spmm_ms = [], []
+    for _ in range(max(3, args.steps)):
+        A.MultiplyMetric(M, 1.0, X, 0.0, Y)
+        mm_ms.append(M.last_timing(2)); spmm_ms.append(M.last_timing(6))
+
+    pairs_nf = b["near_nnz"] // 2
+    pairs_ff = b["far_nnz"] // 2
+    nf_flops = pairs_nf * NF_FLOPS_VALUE_DIFF
+    nf_avg_s = float(np.mean(nf_ms)) * 1e-3
+    achieved = nf_flops / nf_avg_s / 1e12
+    peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
+    spmm_bytes = 20 * b["near_nnz"] + 2 * 96 * b["T"] + 16 * b["T"]
+    result = {
+        "metric": "energy+grad ms & metric-solve ms to 1e-6 residual, learned vs plain PC, 1 GPU",
+        "value": round(ms_per_step, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pairs/f64-accumulate",
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name, "triangles": b["T"], "vertices": b["V"], "q": wl.q, "p": wl.p,
+            "theta_far": wl.far_theta, "theta_near": wl.near_theta, "leaf_size": wl.leaf,
+            "near_nnz": b["near_nnz"], "far_nnz": b["far_nnz"], "very_near_nnz": b["verynear_nnz"],
+            "precision": args.precision,
+            "step": "energy + differential (device-resident); metric solve not yet in the step",
+        },
+        "energy": float(en[0]),
+        "energy_grad_ms": round(float(np.mean(eg_ms)), 3),
+        "near_kernel_ms": round(float(np.mean(nf_ms)), 3),
+        "far_kernel_ms": round(float(np.mean(ff_ms)), 3),
+        "metric_assembly_ms": round(t_assemble * 1e3, 2),
+        "multiply_metric_ms": round(float(np.mean(mm_ms)), 3),
+        "near_spmm_ms": round(float(np.mean(spmm_ms)), 3),
+        "near_spmm_GBs_algorithmic": round(spmm_bytes / (float(np.mean(spmm_ms)) * 1e-3) / 1e9, 1),
+        "solve_ms": None,
+        "tree_bct_build_s": round(t_build, 2),
+        "upload_s": round(t_upload, 2),
+        "pairs_per_s": round((pairs_nf + pairs_ff) / (float(np.mean(eg_ms)) * 1e-3), 1),
+        "roofline": {
+            "kernel": "near-field energy+differential (k_nf_f64<Q6P12,1,1,0>)" if args.precision == "fp64"
+                      else "near-field energy+differential (k_nf_mixed<Q6P12,1,1,0>)",
+            "bound": "mfma", "compute_unit": "valu",
+            "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4),
+            "algorithmic_flops": nf_flops, "flops_per_pair": NF_FLOPS_VALUE_DIFF, "pairs": pairs_nf,
+            "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            result["cpu_baseline"] = cpu_baseline(M, wl, args.cpu_sample_pairs)
+        except Exception as e:  # reported, not fatal
+            result["cpu_baseline"] = {"error": str(e)}
+    hip.hipFree(Yd)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -361,16 +361,18 @@ namespace Repulsor
 
     public:
         // Reference-layout near pattern (BinaryMatrixCSR, full symmetric, sorted, no diagonal).
-        SymmetricCSR NearCSR() const
+        // With row_end >= 0 only rows [0, row_end) are filled (the others are empty).
+        SymmetricCSR NearCSR( const Int row_end = -1 ) const
         {
             const Int T = S.PrimitiveCount();
+            const Int R = (row_end < 0) ? T : std::min( row_end, T );
             SymmetricCSR A;
             A.rows = T;
             A.rp.assign( (size_t)T + 1, 0 );
             const NearTiles & Tl = tiles;
             for( Int l = 0; l < Tl.leaf_count; ++l )
             {
-                for( Int i = Tl.leaf_begin[(size_t)l]; i < Tl.leaf_end[(size_t)l]; ++i )
+                for( Int i = Tl.leaf_begin[(size_t)l]; i < std::min( R, Tl.leaf_end[(size_t)l] ); ++i )
                 {
                     LInt c = 0;
                     for( LInt k = Tl.col_ptr[(size_t)l]; k < Tl.col_ptr[(size_t)l+1]; ++k ) { c += (Tl.col_mask[(size_t)k] >> (i - Tl.leaf_begin[(size_t)l])) & 1u; }
@@ -379,10 +381,11 @@ namespace Repulsor
             }
             for( Int i = 0; i < T; ++i ) { A.rp[(size_t)i + 1] += A.rp[(size_t)i]; }
             A.ci.resize( (size_t)A.rp.back() );
-            A.diag.resize( (size_t)T );
+            A.diag.assign( (size_t)T, 0 );
+            for( Int i = R; i < T; ++i ) { A.diag[(size_t)i] = A.rp[(size_t)i]; }
             for( Int l = 0; l < Tl.leaf_count; ++l )
             {
-                for( Int i = Tl.leaf_begin[(size_t)l]; i < Tl.leaf_end[(size_t)l]; ++i )
+                for( Int i = Tl.leaf_begin[(size_t)l]; i < std::min( R, Tl.leaf_end[(size_t)l] ); ++i )
                 {
                     LInt pos = A.rp[(size_t)i];
                     A.diag[(size_t)i] = -1;

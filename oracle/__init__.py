@@ -23,6 +23,20 @@ def _load(name):
 
 
 _lib = _load("liboracle.so")
+# Release-flag build of the same restatement (-O3 -ffast-math), timed as the CPU baseline.
+_fast = None
+
+
+def fast_tp0():
+    """orc_tp0_compute from the -O3 -ffast-math build (CPU baseline timing only)."""
+    global _fast
+    if _fast is None:
+        lib = _load("liboracle_fast.so")
+        f = lib.orc_tp0_compute
+        f.restype = C.c_double
+        f.argtypes = _lib.orc_tp0_compute.argtypes
+        _fast = f
+    return _fast
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)

@@ -69,7 +69,7 @@ _host_sigs = {
     "rpm_mesh_build": (C.c_int, [_vp, _i64]),
     "rpm_mesh_tree": (C.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "rpm_mesh_tree_data": (C.c_int, [_vp, _d, _d, _d, _d, _d, _i32, _d]),
-    "rpm_mesh_pattern": (C.c_int, [_vp, C.c_int, _i64, _i32, _i64, _i64]),
+    "rpm_mesh_pattern": (C.c_int, [_vp, C.c_int, _i64, _i32, _i64, _i64, C.c_int32]),
     "rpm_mesh_leaf_blocks": (C.c_int, [_vp, _i32, _i32, _i32]),
     "rpm_mesh_semistatic_update": (C.c_int, [_vp, _d]),
     "rpm_mesh_coords": (C.c_int, [_vp, _d]),
@@ -234,17 +234,18 @@ class SimplicialMesh:
                                     _p(out["pre_vals"])))
         return out
 
-    def pattern(self, kind: str):
-        """Full symmetric CSR (rp, ci, diag) of 'near', 'far' or 'verynear' (reference BinaryMatrixCSR layout)."""
+    def pattern(self, kind: str, row_end: int = -1):
+        """Full symmetric CSR (rp, ci, diag) of 'near', 'far' or 'verynear' (reference BinaryMatrixCSR layout).
+        For 'near', row_end >= 0 keeps only the rows [0, row_end)."""
         k = {"near": 0, "far": 1, "verynear": 2}[kind]
         self.build()
         nnz = np.zeros(1, dtype=np.int64)
-        _check(H.rpm_mesh_pattern(self._h, k, None, None, None, _p(nnz, _i64)))
+        _check(H.rpm_mesh_pattern(self._h, k, None, None, None, _p(nnz, _i64), row_end))
         rows = self._built["C"] if k == 1 else self._built["T"]
         rp = np.zeros(rows + 1, dtype=np.int64)
         ci = np.zeros(max(1, int(nnz[0])), dtype=np.int32)
         dg = np.zeros(rows, dtype=np.int64)
-        _check(H.rpm_mesh_pattern(self._h, k, _p(rp, _i64), _p(ci, _i32), _p(dg, _i64), _p(nnz, _i64)))
+        _check(H.rpm_mesh_pattern(self._h, k, _p(rp, _i64), _p(ci, _i32), _p(dg, _i64), _p(nnz, _i64), row_end))
         return rp, ci[: int(nnz[0])], dg
 
     def leaf_blocks(self):

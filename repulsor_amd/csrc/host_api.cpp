@@ -160,13 +160,14 @@ int rpm_mesh_tree_data( void * h, double * P_near, double * P_far, double * C_fa
 }
 
 // kind 0: near (full symmetric, reference layout), 1: far, 2: very near. Pass NULL arrays to query nnz via *nnz.
-int rpm_mesh_pattern( void * h, int kind, int64_t * rp, int32_t * ci, int64_t * diag, int64_t * nnz )
+// For kind 0, row_end >= 0 keeps only the rows [0, row_end).
+int rpm_mesh_pattern( void * h, int kind, int64_t * rp, int32_t * ci, int64_t * diag, int64_t * nnz, int32_t row_end )
 {
     return call( (MeshHandle *)h, [&]( auto & M )
     {
         const auto & B = M.GetBlockClusterTree();
         SymmetricCSR A;
-        if( kind == 0 ) { A = B.NearCSR(); }
+        if( kind == 0 ) { A = B.NearCSR( row_end ); }
         else if( kind == 1 ) { A = B.far; }
         else { A = B.VeryNearCSR(); }
         if( nnz ) { *nnz = A.NonzeroCount(); }

@@ -1,0 +1,412 @@
+// Near-field (primitive pairs) kernels of TangentPointEnergy0 and TangentPointMetric0:
+// energy, moment derivatives, metric blocks (FP64 and FP32-mixed) and the near-field block SpMM.
+//
+// Work unit: one workgroup (one wavefront) per row leaf I (n_I <= 32 primitives). The leaf's
+// column list (the primitives of every leaf J sharing a non-admissible block with I) is processed
+// in chunks of 64 columns. Each chunk is loaded cooperatively (lane l loads column l of the chunk,
+// coalesced) into registers one chunk ahead and then stored into LDS, so the loads of chunk k+1
+// overlap the pair arithmetic of chunk k. Lane l computes the pairs of row ii = l mod n_I of the
+// leaf with the columns c, c + nS, ... of the chunk, c = l / n_I, nS = 64 / n_I. Per-lane partial
+// sums are combined in LDS in a fixed order, so results are bitwise reproducible.
+//
+// Both orientations of every pair are visited (the row-leaf column lists cover the full symmetric
+// pattern) and only row-side (S-side) derivatives are accumulated; they equal the reference's
+// DX + DY sums (TP0_Kernel_NF.hpp:245-272) without scatter or atomics. The energy counts each
+// unordered pair once (column > row), as the reference's upper-triangle traversal does
+// (FMM_Traversor.hpp:100-101). Metric blocks are stored per leaf as [column][row] (a0,a1) pairs,
+// so the lanes of a wave write consecutive slots.
+
+#include "k_pair.hpp"
+#include "rp_launch.hpp"
+
+#include <type_traits>
+
+namespace rp
+{
+    constexpr int NACC = 13;   // 0: d/da, 1..3: d/d(a hull) (scaled), 4..9: d/d(a P), 10,11: diag, 12: energy
+    constexpr int CH   = 64;   // columns per chunk
+
+    // Column data of one chunk: b, y (3), Q (6).
+    template<typename R>
+    struct Chunk
+    {
+        R       d[2][10][CH];
+        int32_t j[2][CH];
+        uint32_t m[2][CH];
+    };
+
+    // ------------------------------------------------------------------ energy / derivative / metric
+    // MIXED: pair arithmetic in FP32 on coordinates relative to the row leaf's origin o_I (the
+    // subtraction y - o_I is done in FP64 when the chunk is staged), FP32 partial sums over 8 pairs
+    // flushed into FP64 accumulators. The area derivative accumulated in the local frame is moved back
+    // with d/da = d/da' - sum_l d/d(a hull_l) . o_I.
+    template<int KIND, bool E, bool D, bool M, bool MIXED>
+    __global__ __launch_bounds__(64) void k_nf( NFArgs A )
+    {
+        using R = typename std::conditional<MIXED, float, double>::type;
+        union Smem { Chunk<R> ch; double red[64][NACC]; };
+        __shared__ Smem smem;
+        Chunk<R> & sh = smem.ch;
+        double (&red)[64][NACC] = smem.red;
+        const int lane = threadIdx.x;
+        const int64_t I = blockIdx.x;
+        const int ib = A.leaf_begin[I];
+        const int nI = A.leaf_end[I] - ib;
+        const int nS = 64 / nI;
+        const int ii = lane % nI;
+        const int c  = lane / nI;
+        const bool active = lane < nS * nI;
+        const int64_t i = ib + ii;
+        const int64_t T = A.T;
+
+        double o0 = 0, o1 = 0, o2 = 0;
+        if constexpr ( MIXED ) { o0 = A.origin[I]; o1 = A.origin[A.L + I]; o2 = A.origin[2 * A.L + I]; }
+
+        // row data
+        const double a_d = A.a[i];
+        const R a = (R)a_d;
+        R x[3], P[6];
+        if constexpr ( MIXED )
+        {
+            x[0] = (float)(A.x[i] - o0); x[1] = (float)(A.x[T + i] - o1); x[2] = (float)(A.x[2 * T + i] - o2);
+        }
+        else
+        {
+            x[0] = A.x[i]; x[1] = A.x[T + i]; x[2] = A.x[2 * T + i];
+        }
+        #pragma unroll
+        for( int k = 0; k < 6; ++k ) { P[k] = (R)A.P[k * T + i]; }
+        const R inv_a = R(1) / a;
+        const R sc = (R)A.scale;
+
+        const int64_t cb = A.col_ptr[I];
+        const int64_t M_ = A.col_ptr[I + 1] - cb;
+        const int64_t vb = M ? A.val_ptr[I] : 0;
+
+        double acc[NACC];
+        R part[NACC];
+        #pragma unroll
+        for( int k = 0; k < NACC; ++k ) { acc[k] = 0.0; part[k] = R(0); }
+        int cnt = 0;
+
+        // register staging of one column per lane
+        R rd[10];
+        int32_t rj = 0;
+        uint32_t rm = 0;
+        auto gload = [&]( int64_t base )
+        {
+            const int64_t m = base + lane;
+            if( m < M_ )
+            {
+                rj = A.col_prim[cb + m];
+                rm = A.col_mask[cb + m];
+                rd[0] = (R)A.a[rj];
+                if constexpr ( MIXED )
+                {
+                    rd[1] = (float)(A.x[rj] - o0); rd[2] = (float)(A.x[T + rj] - o1); rd[3] = (float)(A.x[2 * T + rj] - o2);
+                }
+                else
+                {
+                    rd[1] = A.x[rj]; rd[2] = A.x[T + rj]; rd[3] = A.x[2 * T + rj];
+                }
+                #pragma unroll
+                for( int k = 0; k < 6; ++k ) { rd[4 + k] = (R)A.P[k * T + rj]; }
+            }
+            else
+            {
+                rm = 0u;
+            }
+        };
+        auto lstore = [&]( int buf )
+        {
+            sh.j[buf][lane] = rj;
+            sh.m[buf][lane] = rm;
+            #pragma unroll
+            for( int k = 0; k < 10; ++k ) { sh.d[buf][k][lane] = rd[k]; }
+        };
+
+        gload( 0 );
+        lstore( 0 );
+        __syncthreads();
+        int buf = 0;
+        for( int64_t base = 0; base < M_; base += CH )
+        {
+            const bool more = base + CH < M_;
+            if( more ) { gload( base + CH ); }
+            const int cnt_c = (int)( (M_ - base < CH) ? (M_ - base) : CH );
+            if( active )
+            {
+                for( int mm = c; mm < cnt_c; mm += nS )
+                {
+                    const uint32_t mk = sh.m[buf][mm];
+                    if( (mk >> ii) & 1u )
+                    {
+                        const int32_t j = sh.j[buf][mm];
+                        const R b = sh.d[buf][0][mm];
+                        const R v[3] = { sh.d[buf][1][mm] - x[0], sh.d[buf][2][mm] - x[1], sh.d[buf][3][mm] - x[2] };
+                        R Q[6];
+                        #pragma unroll
+                        for( int k = 0; k < 6; ++k ) { Q[k] = sh.d[buf][4 + k][mm]; }
+                        PairTerms<R> t;
+                        pair_terms<R,KIND,E,D,M>( A.ex, v, P, Q, t );
+                        if constexpr ( E ) { if( j > i ) { part[12] += a * t.E * b; } }
+                        if constexpr ( D )
+                        {
+                            const R gx = t.g[0] * x[0] + t.g[1] * x[1] + t.g[2] * x[2];
+                            part[0] += b * ( t.E0 + gx );
+                            const R sb = sc * b;
+                            part[1] -= sb * t.g[0];
+                            part[2] -= sb * t.g[1];
+                            part[3] -= sb * t.g[2];
+                            const R bk = b * t.hK;
+                            #pragma unroll
+                            for( int k = 0; k < 6; ++k ) { part[4 + k] += bk * t.Vk[k]; }
+                        }
+                        if constexpr ( M )
+                        {
+                            const int64_t slot = vb + (base + mm) * nI + ii;
+                            if constexpr ( MIXED ) { reinterpret_cast<float2 *>(A.vals)[slot] = make_float2( t.a0, t.a1 ); }
+                            else                   { reinterpret_cast<double2 *>(A.vals)[slot] = make_double2( t.a0, t.a1 ); }
+                            const R boa = b * inv_a;
+                            part[10] -= boa * t.a0;
+                            part[11] -= boa * t.a1;
+                        }
+                        if constexpr ( MIXED )
+                        {
+                            if( ++cnt == 8 )
+                            {
+                                #pragma unroll
+                                for( int k = 0; k < NACC; ++k ) { acc[k] += (double)part[k]; part[k] = 0.0f; }
+                                cnt = 0;
+                            }
+                        }
+                    }
+                    else if constexpr ( M )
+                    {
+                        const int64_t slot = vb + (base + mm) * nI + ii;
+                        if constexpr ( MIXED ) { reinterpret_cast<float2 *>(A.vals)[slot] = make_float2( 0.0f, 0.0f ); }
+                        else                   { reinterpret_cast<double2 *>(A.vals)[slot] = make_double2( 0.0, 0.0 ); }
+                    }
+                }
+            }
+            if( more ) { lstore( buf ^ 1 ); }
+            __syncthreads();
+            buf ^= 1;
+        }
+        #pragma unroll
+        for( int k = 0; k < NACC; ++k ) { acc[k] += (double)part[k]; }
+        #pragma unroll
+        for( int k = 0; k < NACC; ++k ) { red[lane][k] = acc[k]; }
+        __syncthreads();
+        if( lane < nI )
+        {
+            double s[12];
+            #pragma unroll
+            for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
+            for( int cc = 0; cc < nS; ++cc )
+            {
+                #pragma unroll
+                for( int k = 0; k < 12; ++k ) { s[k] += red[lane + cc * nI][k]; }
+            }
+            if constexpr ( D )
+            {
+                if constexpr ( MIXED )
+                {
+                    // back to the global frame: every one of the (n+1) hull slots carries s[1..3]
+                    s[0] -= (double)A.size * ( s[1] * o0 + s[2] * o1 + s[3] * o2 );
+                }
+                #pragma unroll
+                for( int k = 0; k < 10; ++k ) { A.dx[k * T + i] = s[k]; }
+            }
+            if constexpr ( M )
+            {
+                A.diag[2 * i]     = s[10];
+                A.diag[2 * i + 1] = s[11];
+            }
+        }
+        if constexpr ( E )
+        {
+            if( lane == 0 )
+            {
+                double e = 0.0;
+                for( int l = 0; l < 64; ++l ) { e += red[l][12]; }
+                A.epart[I] = e;
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------ near-field block SpMM
+    // y_i = sum_j diag(a0, a1, a1, a1) x_j + diag_i x_i, x and y 4 x 3 per primitive (AoS, 12 doubles).
+    // Epilogue (post-operator, ClusterTree::Post): ytot_i = y_i + far-field leaf total of leaf(i), then
+    // S_out[i][s][c] = sum_r pre_vals[i][r][s] ytot_i[r][c] for the (n+1) vertex slots s.
+    template<typename VT, int SIZE>
+    __global__ __launch_bounds__(64) void k_nf_spmm( int64_t T, const int32_t * leaf_begin, const int32_t * leaf_end,
+        const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
+        const VT * vals, const double * diag, const double * xin,
+        const double * leaf_out, int use_far, const double * pre_vals, double * S_out )
+    {
+        union Smem { double sx[2][12][CH]; double red[64][12]; };
+        __shared__ Smem smem;
+        double (&sx)[2][12][CH] = smem.sx;
+        double (&red)[64][12] = smem.red;
+        __shared__ int32_t sj[2][CH];
+        __shared__ uint32_t sm[2][CH];
+        const int lane = threadIdx.x;
+        const int64_t I = blockIdx.x;
+        const int ib = leaf_begin[I];
+        const int nI = leaf_end[I] - ib;
+        const int nS = 64 / nI;
+        const int ii = lane % nI;
+        const int c  = lane / nI;
+        const bool active = lane < nS * nI;
+        const int64_t cb = col_ptr[I];
+        const int64_t M_ = col_ptr[I + 1] - cb;
+        const int64_t vb = val_ptr[I];
+        double acc[12];
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
+        double rx[12];
+        int32_t rj = 0;
+        uint32_t rm = 0;
+        auto gload = [&]( int64_t base )
+        {
+            const int64_t m = base + lane;
+            if( m < M_ )
+            {
+                rj = col_prim[cb + m];
+                rm = col_mask[cb + m];
+                const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj );
+                #pragma unroll
+                for( int h = 0; h < 6; ++h ) { const double2 t = xj[h]; rx[2 * h] = t.x; rx[2 * h + 1] = t.y; }
+            }
+            else { rm = 0u; }
+        };
+        auto lstore = [&]( int buf )
+        {
+            sj[buf][lane] = rj;
+            sm[buf][lane] = rm;
+            #pragma unroll
+            for( int k = 0; k < 12; ++k ) { sx[buf][k][lane] = rx[k]; }
+        };
+        gload( 0 );
+        lstore( 0 );
+        __syncthreads();
+        int buf = 0;
+        for( int64_t base = 0; base < M_; base += CH )
+        {
+            const bool more = base + CH < M_;
+            if( more ) { gload( base + CH ); }
+            const int cnt_c = (int)( (M_ - base < CH) ? (M_ - base) : CH );
+            if( active )
+            {
+                for( int mm = c; mm < cnt_c; mm += nS )
+                {
+                    if( (sm[buf][mm] >> ii) & 1u )
+                    {
+                        const VT av = vals[vb + (base + mm) * nI + ii];
+                        const double a0 = (double)av.x, a1 = (double)av.y;
+                        #pragma unroll
+                        for( int k = 0; k < 3; ++k ) { acc[k] += a0 * sx[buf][k][mm]; }
+                        #pragma unroll
+                        for( int k = 3; k < 12; ++k ) { acc[k] += a1 * sx[buf][k][mm]; }
+                    }
+                }
+            }
+            if( more ) { lstore( buf ^ 1 ); }
+            __syncthreads();
+            buf ^= 1;
+        }
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { red[lane][k] = acc[k]; }
+        __syncthreads();
+        if( lane < nI )
+        {
+            const int64_t i = ib + ii;
+            double s[12];
+            #pragma unroll
+            for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
+            for( int cc = 0; cc < nS; ++cc )
+            {
+                #pragma unroll
+                for( int k = 0; k < 12; ++k ) { s[k] += red[lane + cc * nI][k]; }
+            }
+            const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
+            const double * xi = xin + 12 * i;
+            #pragma unroll
+            for( int k = 0; k < 3; ++k )  { s[k] += d0 * xi[k]; }
+            #pragma unroll
+            for( int k = 3; k < 12; ++k ) { s[k] += d1 * xi[k]; }
+            if( use_far )
+            {
+                const double * lo = leaf_out + 12 * I;
+                #pragma unroll
+                for( int k = 0; k < 12; ++k ) { s[k] = s[k] + lo[k]; }
+            }
+            const double * pv = pre_vals + 4 * SIZE * i;
+            #pragma unroll
+            for( int sl = 0; sl < SIZE; ++sl )
+            {
+                #pragma unroll
+                for( int cc = 0; cc < 3; ++cc )
+                {
+                    double t = 0.0;
+                    #pragma unroll
+                    for( int r = 0; r < 4; ++r ) { t += pv[SIZE * r + sl] * s[3 * r + cc]; }
+                    S_out[(SIZE * i + sl) * 3 + cc] = t;
+                }
+            }
+        }
+    }
+
+    // ------------------------------------------------------------------ host launchers
+    template<bool E, bool D, bool M>
+    static void launch_nf_flags( int kind, bool mixed, const NFArgs & A, hipStream_t s )
+    {
+        if( A.L == 0 ) { return; }
+        const dim3 grid( (unsigned)A.L ), block( 64 );
+        if( !mixed )
+        {
+            if( kind == KIND_Q6P12 )      { hipLaunchKernelGGL( (k_nf<KIND_Q6P12,E,D,M,false>),  grid, block, 0, s, A ); }
+            else if( kind == KIND_Q2P4 )  { hipLaunchKernelGGL( (k_nf<KIND_Q2P4,E,D,M,false>),   grid, block, 0, s, A ); }
+            else                          { hipLaunchKernelGGL( (k_nf<KIND_GENERIC,E,D,M,false>), grid, block, 0, s, A ); }
+        }
+        else
+        {
+            if( kind == KIND_Q6P12 )      { hipLaunchKernelGGL( (k_nf<KIND_Q6P12,E,D,M,true>),  grid, block, 0, s, A ); }
+            else if( kind == KIND_Q2P4 )  { hipLaunchKernelGGL( (k_nf<KIND_Q2P4,E,D,M,true>),   grid, block, 0, s, A ); }
+            else                          { hipLaunchKernelGGL( (k_nf<KIND_GENERIC,E,D,M,true>), grid, block, 0, s, A ); }
+        }
+    }
+
+    void launch_nf( unsigned flags, int kind, bool mixed, const NFArgs & A, hipStream_t s )
+    {
+        const bool E = flags & RP_ENERGY, D = flags & RP_DIFF, M = flags & RP_METRIC;
+        if(  E &&  D &&  M ) { launch_nf_flags<true ,true ,true >( kind, mixed, A, s ); }
+        else if(  E &&  D && !M ) { launch_nf_flags<true ,true ,false>( kind, mixed, A, s ); }
+        else if(  E && !D &&  M ) { launch_nf_flags<true ,false,true >( kind, mixed, A, s ); }
+        else if(  E && !D && !M ) { launch_nf_flags<true ,false,false>( kind, mixed, A, s ); }
+        else if( !E &&  D &&  M ) { launch_nf_flags<false,true ,true >( kind, mixed, A, s ); }
+        else if( !E &&  D && !M ) { launch_nf_flags<false,true ,false>( kind, mixed, A, s ); }
+        else if( !E && !D &&  M ) { launch_nf_flags<false,false,true >( kind, mixed, A, s ); }
+    }
+
+    void launch_nf_spmm_post( bool mixed, int size, int64_t T, int64_t L, const int32_t * lb, const int32_t * le,
+                              const int64_t * cp, const int32_t * cpr, const uint32_t * cm, const int64_t * vp,
+                              const void * vals, const double * diag, const double * xin,
+                              const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s )
+    {
+        if( !L ) { return; }
+        const dim3 grid( (unsigned)L ), block( 64 );
+        if( size == 3 )
+        {
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,3>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_spmm<double2,3>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+        }
+        else
+        {
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,2>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_spmm<double2,2>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+        }
+    }
+
+} // namespace rp

@@ -8,8 +8,9 @@
 //   k_hulls                          DNearFarToHulls<2,3>/<1,3> (SimplicialMeshDetails.hpp:529-736, 1097-1410)
 //   k_assemble                       DerivativeAssembler().Dot<3> (SimplicialMesh.hpp:625-675)
 //   k_pre, k_up_leaves, k_up_level   ClusterTree::Pre: mi_pre.Dot, PrimitivesToClusters, PercolateUp (ClusterTree.hpp:770-857)
-//   k_nf_spmm, k_ff_spmm             TP0_Traversor::NF/FF_MultiplyMetric + DiagonalKernelMatrix (TP0_Traversor.hpp:426-505)
-//   k_down_leaves, k_post            ClusterTree::Post: PercolateDown, ClustersToPrimitives(add), mi_post.Dot (ClusterTree.hpp:860-943)
+//   k_ff_spmm                        TP0_Traversor::FF_MultiplyMetric + DiagonalKernelMatrix (TP0_Traversor.hpp:480-505)
+//   k_leaf_path_sum, k_post_*        ClusterTree::Post: PercolateDown, ClustersToPrimitives(add), mi_post.Dot (ClusterTree.hpp:860-943)
+// (the near-field SpMM with the fused post epilogue lives in k_nf.hip)
 
 #include "rp_launch.hpp"
 
@@ -245,75 +246,6 @@ namespace rp
         C_in[12 * (int64_t)c + h] = C_in[12 * (int64_t)C_left[c] + h] + C_in[12 * (int64_t)C_right[c] + h];
     }
 
-    // ---------------------------------------------------------------- MultiplyMetric: near SpMM
-    // y_i = sum_j diag(a0, a1, a1, a1) x_j + diag_i x_i over the leaf tiles; x, y are 4 x 3 per primitive.
-    template<typename VT>
-    __global__ __launch_bounds__(256) void k_nf_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
-        const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
-        const VT * vals, const double * diag, const double * xin, double * yout )
-    {
-        __shared__ double red[4][64][12];
-        const int w = threadIdx.x >> 6;
-        const int lane = threadIdx.x & 63;
-        const int64_t I = (int64_t)blockIdx.x * 4 + w;
-        const bool wave_ok = I < L;
-        double acc[12];
-        #pragma unroll
-        for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
-        int nI = 1, nS = 64, ii = 0;
-        int64_t i = 0;
-        if( wave_ok )
-        {
-            const int ib = leaf_begin[I];
-            nI = leaf_end[I] - ib;
-            nS = 64 / nI;
-            ii = lane % nI;
-            const int c = lane / nI;
-            i = ib + ii;
-            if( lane < nS * nI )
-            {
-                const int64_t cb = col_ptr[I];
-                const int64_t M_ = col_ptr[I + 1] - cb;
-                const int64_t vb = val_ptr[I];
-                for( int64_t m = c; m < M_; m += nS )
-                {
-                    if( (col_mask[cb + m] >> ii) & 1u )
-                    {
-                        const int32_t j = col_prim[cb + m];
-                        const VT av = vals[vb + m * nI + ii];
-                        const double a0 = (double)av.x, a1 = (double)av.y;
-                        const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)j );
-                        const double2 x01 = xj[0], x23 = xj[1], x45 = xj[2], x67 = xj[3], x89 = xj[4], xab = xj[5];
-                        acc[0] += a0 * x01.x; acc[1] += a0 * x01.y; acc[2] += a0 * x23.x;
-                        acc[3] += a1 * x23.y; acc[4] += a1 * x45.x; acc[5] += a1 * x45.y;
-                        acc[6] += a1 * x67.x; acc[7] += a1 * x67.y; acc[8] += a1 * x89.x;
-                        acc[9] += a1 * x89.y; acc[10] += a1 * xab.x; acc[11] += a1 * xab.y;
-                    }
-                }
-            }
-        }
-        #pragma unroll
-        for( int k = 0; k < 12; ++k ) { red[w][lane][k] = acc[k]; }
-        __syncthreads();
-        if( wave_ok && lane < nI )
-        {
-            double s[12];
-            #pragma unroll
-            for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
-            for( int c = 0; c < nS; ++c )
-            {
-                #pragma unroll
-                for( int k = 0; k < 12; ++k ) { s[k] += red[w][lane + c * nI][k]; }
-            }
-            const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
-            const double * xi = xin + 12 * i;
-            for( int k = 0; k < 3; ++k )  { s[k] += d0 * xi[k]; }
-            for( int k = 3; k < 12; ++k ) { s[k] += d1 * xi[k]; }
-            double2 * o = reinterpret_cast<double2 *>( yout + 12 * i );
-            for( int h = 0; h < 6; ++h ) { o[h] = make_double2( s[2 * h], s[2 * h + 1] ); }
-        }
-    }
-
     template<typename T>
     __device__ __forceinline__ T wave_sum64( T v )
     {
@@ -357,35 +289,43 @@ namespace rp
     }
 
     // ---------------------------------------------------------------- MultiplyMetric: Post
-    // Y[v] = alpha * sum_{(i,slot) of v, increasing i} sum_r pre_vals[i][r][slot] (P_out[i][r] + leaf_out[leaf(i)][r]) + beta Y[v]
+    // Primitive side of the post-operator when the near-field SpMM is skipped: ytot_i = far-field
+    // leaf total of leaf(i) (or 0), S_out[i][s][c] = sum_r pre_vals[i][r][s] ytot_i[r][c].
     template<int SIZE>
-    __global__ void k_post( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_idx_of_prim,
-                            const double * pre_vals, const double * P_out, const double * leaf_out, int use_far,
-                            double alpha, double beta, double * Y, int64_t ldY )
+    __global__ void k_post_prim( int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals,
+                                 const double * leaf_out, int use_far, double * S_out )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        double s[12];
+        for( int k = 0; k < 12; ++k ) { s[k] = use_far ? leaf_out[12 * (int64_t)leaf_idx_of_prim[i] + k] : 0.0; }
+        const double * pv = pre_vals + 4 * SIZE * i;
+        for( int sl = 0; sl < SIZE; ++sl )
+        {
+            for( int cc = 0; cc < 3; ++cc )
+            {
+                double t = 0.0;
+                for( int r = 0; r < 4; ++r ) { t += pv[SIZE * r + sl] * s[3 * r + cc]; }
+                S_out[(SIZE * i + sl) * 3 + cc] = t;
+            }
+        }
+    }
+
+    // Y[v] = alpha * sum_{(i,slot) of v, increasing i} S_out[i][slot] + beta * Y[v]   (mi_post.Dot)
+    __global__ void k_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out,
+                                 double alpha, double beta, double * Y, int64_t ldY )
     {
         const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if( v >= V ) { return; }
-        double s[3] = { 0, 0, 0 };
+        double s0 = 0, s1 = 0, s2 = 0;
         for( int64_t k = vp_ptr[v]; k < vp_ptr[v + 1]; ++k )
         {
-            const int64_t e = vp_entry[k];
-            const int64_t i = e / SIZE;
-            const int slot = (int)(e % SIZE);
-            const double * po = P_out + 12 * i;
-            const double * lo = use_far ? leaf_out + 12 * (int64_t)leaf_idx_of_prim[i] : nullptr;
-            for( int r = 0; r < 4; ++r )
-            {
-                const double wgt = pre_vals[4 * SIZE * i + SIZE * r + slot];
-                for( int c = 0; c < 3; ++c )
-                {
-                    const double pval = use_far ? po[3 * r + c] + lo[3 * r + c] : po[3 * r + c];
-                    s[c] += wgt * pval;
-                }
-            }
+            const double * so = S_out + 3 * (int64_t)vp_entry[k];
+            s0 += so[0]; s1 += so[1]; s2 += so[2];
         }
         double * y = Y + ldY * v;
-        if( beta == 0.0 ) { for( int c = 0; c < 3; ++c ) { y[c] = alpha * s[c]; } }
-        else { for( int c = 0; c < 3; ++c ) { y[c] = alpha * s[c] + beta * y[c]; } }
+        if( beta == 0.0 ) { y[0] = alpha * s0; y[1] = alpha * s1; y[2] = alpha * s2; }
+        else { y[0] = alpha * s0 + beta * y[0]; y[1] = alpha * s1 + beta * y[1]; y[2] = alpha * s2 + beta * y[2]; }
     }
 
     __global__ void k_fill( double * p, int64_t n, double v )
@@ -436,15 +376,6 @@ namespace rp
     void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s )
     { if( n ) { hipLaunchKernelGGL( k_up_level, grid1(12 * n), dim3(256), 0, s, n, nodes, C_left, C_right, C_in ); } }
 
-    void launch_nf_spmm( bool mixed, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp, const int32_t * cpr, const uint32_t * cm,
-                         const int64_t * vp, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
-    {
-        if( !L ) { return; }
-        const dim3 grid( (unsigned)((L + 3) / 4) );
-        if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2>), grid, dim3(256), 0, s, L, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, yout ); }
-        else        { hipLaunchKernelGGL( (k_nf_spmm<double2>), grid, dim3(256), 0, s, L, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, yout ); }
-    }
-
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {
         if( !C ) { return; }
@@ -453,14 +384,15 @@ namespace rp
         else        { hipLaunchKernelGGL( (k_ff_spmm<double2>), grid, dim3(256), 0, s, C, rp, ci, (const double2 *)vals, diag, xin, yout ); }
     }
 
-    void launch_post( int size, int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_idx_of_prim,
-                      const double * pre_vals, const double * P_out, const double * leaf_out, int use_far,
-                      double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )
+    void launch_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s )
     {
-        if( !V ) { return; }
-        if( size == 3 ) { hipLaunchKernelGGL( (k_post<3>), grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, leaf_idx_of_prim, pre_vals, P_out, leaf_out, use_far, alpha, beta, Y, ldY ); }
-        else            { hipLaunchKernelGGL( (k_post<2>), grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, leaf_idx_of_prim, pre_vals, P_out, leaf_out, use_far, alpha, beta, Y, ldY ); }
+        if( !T ) { return; }
+        if( size == 3 ) { hipLaunchKernelGGL( (k_post_prim<3>), grid1(T), dim3(256), 0, s, T, leaf_idx_of_prim, pre_vals, leaf_out, use_far, S_out ); }
+        else            { hipLaunchKernelGGL( (k_post_prim<2>), grid1(T), dim3(256), 0, s, T, leaf_idx_of_prim, pre_vals, leaf_out, use_far, S_out ); }
     }
+
+    void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )
+    { if( V ) { hipLaunchKernelGGL( k_post_vert, grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, S_out, alpha, beta, Y, ldY ); } }
 
     void launch_fill( double * p, int64_t n, double v, hipStream_t s )
     { if( n ) { hipLaunchKernelGGL( k_fill, grid1(n), dim3(256), 0, s, p, n, v ); } }

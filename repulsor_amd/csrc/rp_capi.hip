@@ -340,7 +340,7 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         const bool E = flags & RP_ENERGY, D = flags & RP_DIFF, M = flags & RP_METRIC;
 
         ctx->partials.require( (size_t)(L + C) );
-        ctx->scalar.require( 4 );
+        ctx->scalar.require( (size_t)(1 + REDUCE_BLOCKS) );
         if( D ) { ctx->nf_dx.require( (size_t)(10 * T) ); ctx->ff_dx.require( (size_t)(10 * C) ); }
         if( M )
         {
@@ -548,24 +548,27 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
                 launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream );
             }
         }
-        RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) );
-        if( do_nf )
-        {
-            launch_nf_spmm( mixed, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(), ctx->col_mask.get(),
-                            ctx->val_ptr.get(), ctx->nf_vals.get(), ctx->nf_diag.get(), ctx->P_in.get(), ctx->P_out.get(), ctx->stream );
-        }
-        else
-        {
-            launch_fill( ctx->P_out.get(), 12 * T, 0.0, ctx->stream );
-        }
-        RP_HIP( hipEventRecord( ctx->ev[6][1], ctx->stream ) );
+        // Far field first, so that the near-field SpMM epilogue can add the percolated cluster
+        // output of its leaf and apply the post-operator (S_out = per-vertex-slot contributions).
         if( do_ff )
         {
             launch_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->ff_vals.get(), ctx->ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
             launch_leaf_out( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
         }
-        launch_post( ctx->size, V, ctx->vp_ptr.get(), ctx->vp_entry.get(), ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->P_out.get(),
-                     ctx->leaf_out.get(), do_ff ? 1 : 0, alpha, beta, Yd, ldy, ctx->stream );
+        double * S_out = ctx->P_out.get();   // T x (n+1) x 3 <= 12 T
+        RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) );
+        if( do_nf )
+        {
+            launch_nf_spmm_post( mixed, ctx->size, T, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
+                                 ctx->col_mask.get(), ctx->val_ptr.get(), ctx->nf_vals.get(), ctx->nf_diag.get(), ctx->P_in.get(),
+                                 ctx->leaf_out.get(), do_ff ? 1 : 0, ctx->pre_vals.get(), S_out, ctx->stream );
+        }
+        else
+        {
+            launch_post_prim( ctx->size, T, ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->leaf_out.get(), do_ff ? 1 : 0, S_out, ctx->stream );
+        }
+        RP_HIP( hipEventRecord( ctx->ev[6][1], ctx->stream ) );
+        launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
         ev.stop();
         RP_HIP( hipGetLastError() );
         if( !dev )

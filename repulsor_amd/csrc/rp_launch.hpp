@@ -45,7 +45,8 @@ namespace rp
 
     void launch_nf( unsigned flags, int kind, bool mixed, const NFArgs & A, hipStream_t s );
     void launch_ff( unsigned flags, int kind, bool mixed, const FFArgs & A, hipStream_t s );
-    void launch_reduce_sum( const double * part, int64_t n, double * out, hipStream_t s );
+    constexpr int64_t REDUCE_BLOCKS = 1024;
+    void launch_reduce_sum( const double * part, int64_t n, double * out, hipStream_t s );   // out: 1 + REDUCE_BLOCKS
 
     void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, hipStream_t s );
     void launch_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P, hipStream_t s );
@@ -58,12 +59,13 @@ namespace rp
     void launch_pre( int size, int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
     void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s );
     void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s );
-    void launch_nf_spmm( bool mixed, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp, const int32_t * cpr, const uint32_t * cm,
-                         const int64_t * vp, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s );
+    void launch_nf_spmm_post( bool mixed, int size, int64_t T, int64_t L, const int32_t * lb, const int32_t * le,
+                              const int64_t * cp, const int32_t * cpr, const uint32_t * cm, const int64_t * vp,
+                              const void * vals, const double * diag, const double * xin,
+                              const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s );
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s );
-    void launch_post( int size, int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_idx_of_prim,
-                      const double * pre_vals, const double * P_out, const double * leaf_out, int use_far,
-                      double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
+    void launch_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s );
+    void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
     void launch_fill( double * p, int64_t n, double v, hipStream_t s );
 
 } // namespace rp
