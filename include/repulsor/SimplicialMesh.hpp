@@ -10,6 +10,8 @@
 #include "SimplexData.hpp"
 #include "ClusterTree.hpp"
 #include "BlockClusterTree.hpp"
+#include "H1Metric.hpp"
+#include "SparseCholesky.hpp"
 #include "../repulsor_hip.h"
 
 #include <map>
@@ -181,6 +183,49 @@ namespace Repulsor
             scalar_cache.clear();
             vector_cache.clear();
             device_state.clear();
+            h1.reset();
+            h1_factor.reset();
+        }
+
+        // ---------------------------------------------------------------- H^1 metric and its factor
+        // Weights of the H^1 metric (src/SimplicialMesh/H1Metric.hpp:4-5, setters :13-27).
+    protected:
+        double h1_stiffness_weight = 1.0;
+        double h1_mass_weight      = 1.0;
+        mutable std::unique_ptr<CSRMatrix> h1;
+        mutable std::unique_ptr<SupernodalFactor> h1_factor;
+        mutable std::unique_ptr<std::vector<Int>> nd_ordering;   // persistent (depends on the tree only)
+
+    public:
+        double H1StiffnessWeight() const { return h1_stiffness_weight; }
+        double H1MassWeight()      const { return h1_mass_weight; }
+        void SetH1StiffnessWeight( double w ) { h1_stiffness_weight = w; h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
+        void SetH1MassWeight( double w )      { h1_mass_weight = w;      h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
+
+        const CSRMatrix & H1Metric() const
+        {
+            if( !h1 )
+            {
+                h1 = std::make_unique<CSRMatrix>( AssembleH1Metric<DOM_DIM>( V_coords.data(), VertexCount(), simplices.data(), SimplexCount(),
+                                                                             h1_stiffness_weight, h1_mass_weight ) );
+            }
+            return *h1;
+        }
+
+        const std::vector<Int> & NestedDissection() const
+        {
+            if( !nd_ordering ) { nd_ordering = std::make_unique<std::vector<Int>>( NestedDissectionOrdering<DOM_DIM>( GetClusterTree(), VertexCount() ) ); }
+            return *nd_ordering;
+        }
+
+        const SupernodalFactor & H1Solver() const
+        {
+            if( !h1_factor )
+            {
+                const CSRMatrix & H = H1Metric();
+                h1_factor = std::make_unique<SupernodalFactor>( SparseCholeskyFactor( H.n, H.rp, H.ci, H.v, NestedDissection(), threads ) );
+            }
+            return *h1_factor;
         }
 
         // ---------------------------------------------------------------- device
@@ -222,9 +267,27 @@ namespace Repulsor
 
         bool DeviceReady() const { return dev != nullptr; }
 
-        // Converts near metric values from the device tile layout to the reference's CSR order
-        // (full symmetric pattern of NearCSR()).
-        void NearTileValuesToCSR( const double * tile_vals, const SymmetricCSR & A, double * csr_vals ) const
+        // Device context holding the H^1 factor of the current state (factorised on the host once
+        // per state, like the reference's cached H1Solver).
+        rp_ctx * DeviceH1() const
+        {
+            rp_ctx * c = Device();
+            if( !device_state.count( "h1" ) )
+            {
+                const SupernodalFactor & F = H1Solver();
+                std::vector<int32_t> perm( F.perm.begin(), F.perm.end() );
+                auto st = rp_upload_h1_factor( c, F.n, F.ns, perm.data(), F.sn_begin.data(), F.row_ptr.data(), F.rows.data(),
+                                               F.val_ptr.data(), F.vals.data(), F.sn_parent.data() );
+                if( st != RP_OK ) { throw std::runtime_error( std::string("rp_upload_h1_factor: ") + rp_last_error() ); }
+                device_state["h1"] = 1;
+            }
+            return c;
+        }
+
+        // Converts near metric values (width values per slot: 2 for the tangent-point metric, 1 for
+        // the pseudo-Laplacian) from the device tile layout to the reference's CSR order (full
+        // symmetric pattern of NearCSR()).
+        void NearTileValuesToCSR( const double * tile_vals, const SymmetricCSR & A, double * csr_vals, const int width = 2 ) const
         {
             const NearTiles & Tl = GetBlockClusterTree().tiles;
             for( Int l = 0; l < Tl.leaf_count; ++l )
@@ -240,8 +303,7 @@ namespace Repulsor
                         if( (Tl.col_mask[(size_t)k] >> (i - ib)) & 1u )
                         {
                             const LInt slot = Tl.val_ptr[(size_t)l] + m * nI + (i - ib);
-                            csr_vals[2 * pos]     = tile_vals[2 * slot];
-                            csr_vals[2 * pos + 1] = tile_vals[2 * slot + 1];
+                            for( int w = 0; w < width; ++w ) { csr_vals[width * pos + w] = tile_vals[width * slot + w]; }
                             ++pos;
                         }
                     }

@@ -96,10 +96,110 @@ namespace Repulsor
         // sqrt(<X, A X>) (MetricBase.hpp:81-100).
         double MetricNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs ) const
         {
+            return QuadraticNorm( M, X, ldX, nrhs, false );
+        }
+
+        // sqrt(<X, P X>) (MetricBase.hpp:102-121).
+        double PreconditionerNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs ) const
+        {
+            return QuadraticNorm( M, X, ldX, nrhs, true );
+        }
+
+        // Pseudo-Laplacian L_{2-s} of the preconditioner (the reference's member
+        // PseudoLaplacian<Mesh,false> pseudo_lap(2 - s), TangentPointMetric0.hpp:42,53), assembled on the
+        // device and cached until SemiStaticUpdate.
+        bool PseudoLaplacianValues( const Mesh_T & M ) const
+        {
+            try
+            {
+                rp_ctx * c = M.Device();
+                char key[96];
+                std::snprintf( key, sizeof(key), "pl:%.17g:%d", 2.0 - s, (int)precision );
+                if( M.device_state.count( key ) ) { return true; }
+                ScopedTimer timer( "PseudoLaplacian::MetricValues" );
+                if( !Check( rp_pl_compute( c, 2.0 - s, (int)precision ), ClassName() + "::PseudoLaplacianValues" ) ) { return false; }
+                for( auto k = M.device_state.begin(); k != M.device_state.end(); )
+                {
+                    if( k->first.rfind( "pl:", 0 ) == 0 ) { k = M.device_state.erase(k); } else { ++k; }
+                }
+                M.device_state[key] = 1;
+                return true;
+            }
+            catch( const std::exception & e ) { eprint( ClassName() + "::PseudoLaplacianValues: " + e.what() ); return false; }
+        }
+
+        // Y = alpha * H^-1 L_{2-s} H^-1 X + beta * Y (TangentPointMetric0::multiply_preconditioner,
+        // TangentPointMetric0.hpp:88-126; nrhs check MetricDimRestricted.hpp:173-195).
+        bool MultiplyPreconditioner( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,
+                                     const double beta, double * Y, const LInt ldY, const int nrhs,
+                                     const bool device_ptrs = false ) const
+        {
+            if( nrhs != AMB_DIM )
+            {
+                eprint( ClassName() + "::MultiplyPreconditioner: nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
+                return false;
+            }
+            if( !PreparePreconditioner( M ) ) { return false; }
+            ScopedTimer timer( "TangentPointMetric0::MultiplyPreconditioner" );
+            return Check( rp_multiply_preconditioner( M.Device(), alpha, X, ldX, beta, Y, ldY, nrhs, device_ptrs ? RP_DEVICE_PTR : 0 ),
+                          ClassName() + "::MultiplyPreconditioner" );
+        }
+
+        // X = alpha * A^+ B + beta * X with P0 on the GPU (MetricDimRestricted::Solve,
+        // MetricDimRestricted.hpp:232-289). solver_flags (RP_SOLVE_*): default preconditioned conjugate
+        // gradients with the l2 residual; RP_SOLVE_PC_NORM measures in the preconditioner norm;
+        // RP_SOLVE_GMRES runs the reference's left-preconditioned GMRES. The iteration count and the
+        // per-column relative residuals are kept for Solver_IterationCount / Solver_RelativeResiduals.
+        bool Solve( const Mesh_T & M, const double alpha, const double * B, const LInt ldB,
+                    const double beta, double * X, const LInt ldX, const int nrhs,
+                    const int max_iter, const double tolerance, const bool device_ptrs = false,
+                    const int solver_flags = 0 ) const
+        {
+            const bool use_preconditioner = !( solver_flags & RP_SOLVE_NO_PRECONDITIONER );
+            if( nrhs != AMB_DIM )
+            {
+                eprint( ClassName() + "::Solve: nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
+                return false;
+            }
+            if( !MetricValues( M ) ) { return false; }
+            if( use_preconditioner && !PreparePreconditioner( M ) ) { return false; }
+            ScopedTimer timer( "TangentPointMetric0::Solve" );
+            int it = 0;
+            double rel[3] = { 0, 0, 0 };
+            const int st = rp_solve( M.Device(), alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance,
+                                     solver_flags, device_ptrs ? RP_DEVICE_PTR : 0, &it, rel );
+            iter = it;
+            rel_residuals.assign( rel, rel + 3 );
+            if( st == RP_ESTAGNATION )
+            {
+                wprint( ClassName() + "::Solve: " + rp_last_error() );
+                return true;
+            }
+            return Check( st, ClassName() + "::Solve" );
+        }
+
+        int Solver_IterationCount() const { return iter; }
+        std::vector<double> Solver_RelativeResiduals() const { return rel_residuals; }
+
+    protected:
+        mutable int iter = 0;
+        mutable std::vector<double> rel_residuals;
+
+        bool PreparePreconditioner( const Mesh_T & M ) const
+        {
+            try { M.DeviceH1(); }
+            catch( const std::exception & e ) { eprint( ClassName() + "::MultiplyPreconditioner: " + e.what() ); return false; }
+            return PseudoLaplacianValues( M );
+        }
+
+        double QuadraticNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs, const bool pc ) const
+        {
             const LInt V = M.VertexCount();
             std::vector<double> Xc( (size_t)(3 * V) ), Yc( (size_t)(3 * V), 0.0 );
             for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { Xc[(size_t)(3 * v + k)] = X[ldX * v + k]; } }
-            if( !MultiplyMetric( M, 1.0, Xc.data(), 3, 0.0, Yc.data(), 3, nrhs ) ) { return 0; }
+            const bool ok = pc ? MultiplyPreconditioner( M, 1.0, Xc.data(), 3, 0.0, Yc.data(), 3, nrhs )
+                               : MultiplyMetric( M, 1.0, Xc.data(), 3, 0.0, Yc.data(), 3, nrhs );
+            if( !ok ) { return 0; }
             double s2 = 0;
             for( size_t k = 0; k < Xc.size(); ++k ) { s2 += Xc[k] * Yc[k]; }
             return std::sqrt( s2 );

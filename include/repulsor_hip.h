@@ -110,9 +110,65 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
                         double beta, double * Y, int64_t ldY, int nrhs,
                         int vf_flag, int nf_flag, int ff_flag, int ptr_flags );
 
+/* Pseudo-Laplacian L_s (scalar blocks -(r^2)^(-n/2-s) on the same near/far patterns, diagonal
+ * -sum_j (a_j/a_i) val_ij), assembled on the device from the uploaded data.
+ * Replaces: PseudoLaplacian<Mesh,false>::compute_metric (reference src/Metrics/PseudoLaplacian.hpp:54-64,
+ * kernels src/Metrics/PseudoLaplacian/PseudoLaplacian_Kernel_NF.hpp:124-146, _FF.hpp:101-146). */
+int rp_pl_compute( rp_ctx * ctx, double s, int precision );
+
+/* Read-out of the pseudo-Laplacian values (near values in tile order, far values in CSR order).
+ * Any pointer may be NULL. Replaces: the PseudoLaplacian MetricValues container read-out. */
+int rp_download_pl( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * ff_vals, double * ff_diag );
+
+/* Y = alpha * L X + beta * Y with the LowOrder pre/post operator (nrhs must be 3).
+ * Replaces: PseudoLaplacian's MultiplyMetric (reference src/Metrics/MetricDimRestricted.hpp:109-139 with
+ * src/Metrics/PseudoLaplacian.hpp:66-75). */
+int rp_pl_multiply( rp_ctx * ctx, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
+                    int nrhs, int nf_flag, int ff_flag, int ptr_flags );
+
+/* Supernodal Cholesky factor H = L L^T of the H^1 metric, computed on the host: perm (new -> old),
+ * supernode column ranges sn_begin (ns+1), sorted row sets rows[row_ptr[s]..row_ptr[s+1]) starting with
+ * the supernode's own columns, column-major r x w value blocks at val_ptr[s] whose top w x w block holds
+ * the inverse of the diagonal block, and the supernodal elimination-tree parents (postorder, -1 = root).
+ * Replaces: the CholeskyDecomposition held by SimplicialMesh::H1Solver (reference
+ * src/SimplicialMesh/H1Metric.hpp:208-270). */
+int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * perm, const int32_t * sn_begin,
+                         const int64_t * row_ptr, const int32_t * rows, const int64_t * val_ptr, const double * vals,
+                         const int32_t * sn_parent );
+
+/* X = alpha * H^-1 B + beta * X (nrhs must be 3).
+ * Replaces: H1Solver().Solve<3>(alpha, B, ldB, beta, X, ldX, nrhs) (reference src/Metrics/TangentPointMetric0.hpp:104-124). */
+int rp_h1_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
+                 int nrhs, int ptr_flags );
+
+/* Y = alpha * H^-1 L_{2-s} H^-1 X + beta * Y (needs rp_upload_h1_factor and rp_pl_compute(2 - s)).
+ * Replaces: TangentPointMetric0::multiply_preconditioner (reference src/Metrics/TangentPointMetric0.hpp:88-126). */
+int rp_multiply_preconditioner( rp_ctx * ctx, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
+                                int nrhs, int ptr_flags );
+
+/* rp_solve flags */
+#define RP_SOLVE_NO_PRECONDITIONER 1
+#define RP_SOLVE_PC_NORM           2
+#define RP_SOLVE_GMRES             4
+
+/* X = alpha * A^+ B + beta * X on the three columns of B (zero initial guess).
+ * Default: preconditioned conjugate gradients, stopping when every column has
+ * |B_k - A Z_k|_2 / |B_k|_2 <= tolerance (recursive residual) or, with RP_SOLVE_PC_NORM, the same ratio
+ * in the preconditioner norm |r|_P = sqrt(<r, P r>).
+ * RP_SOLVE_GMRES: left-preconditioned restarted GMRES(100), stopping when |P (B_k - A Z_k)|_2 / |P B_k|_2
+ * <= tolerance (the reference's algorithm and measure).
+ * After max_iter iterations without convergence: RP_ESTAGNATION, X still written. iterations and
+ * rel_residuals (3 values, in the chosen measure) may be NULL.
+ * Replaces: MetricDimRestricted::Solve (reference src/Metrics/MetricDimRestricted.hpp:232-289; the
+ * reference runs left-preconditioned GMRES there with the same operator and preconditioner) and
+ * Solver_IterationCount / Solver_RelativeResiduals (src/Metrics/MetricBase.hpp:126-134). */
+int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
+              int nrhs, int max_iter, double tolerance, int solver_flags, int ptr_flags, int * iterations, double * rel_residuals );
+
 /* Device timing of the last call of each entry point, in milliseconds (HIP events on the
  * context's stream). which: 0 = tp0_compute, 1 = assemble_vertex_gradient, 2 = multiply_metric,
- * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM. */
+ * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM, 7 = multiply_preconditioner,
+ * 8 = h1_solve, 9 = pl_compute. */
 int rp_last_timing( rp_ctx * ctx, int which, double * ms );
 
 #ifdef __cplusplus

@@ -153,6 +153,7 @@ def multiply_metric(n, nV, tree, pre_cols, pre_vals, near_pat, nf_vals, nf_diag,
     T = len(tree["P_ordering"])
     Cn = len(tree["C_left"])
     X = c64(X)
+    assert Y.dtype == np.float64 and Y.flags.c_contiguous
     z = np.zeros(1, np.int32)
     nci = c32(near_pat[1]); fci = c32(far_pat[1])
     _mm(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
@@ -174,3 +175,41 @@ def allpairs_metric(n, P_far, q, p):
     out = np.zeros((T, T, 2))
     _allpairs_metric(n, T, _p(P_far), q, p, _p(out))
     return out
+
+
+_pl = _f("orc_pl_compute", None, [C.c_int, C.c_int64, C.c_int64, _d, _d, _i64, _i32, _i64, _i64, _i32, _i64,
+                                  C.c_double, _d, _d, _d, _d])
+_plm = _f("orc_pl_multiply", None, [C.c_int, C.c_int64, C.c_int64, C.c_int64, _i32, _i32, _i32, _i32, _i32, _d,
+                                    _i64, _i32, _d, _d, _i64, _i32, _d, _d,
+                                    C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int])
+
+
+def pl_compute(n, P_near, C_far, near_pat, far_pat, s):
+    """PseudoLaplacian (LowOrder) values restatement: NF (N_nf), FF (N_ff), NF_diag (T), FF_diag (C)."""
+    P_near, C_far = c64(P_near), c64(C_far)
+    T, Cn = P_near.shape[0], C_far.shape[0]
+    nrp, nci, ndg = c64i(near_pat[0]), c32(near_pat[1]), c64i(near_pat[2])
+    frp, fci, fdg = c64i(far_pat[0]), c32(far_pat[1]), c64i(far_pat[2])
+    z = np.zeros(1, np.int32)
+    nfv = np.zeros(max(1, len(nci))); ffv = np.zeros(max(1, len(fci)))
+    nfd = np.zeros(T); ffd = np.zeros(Cn)
+    _pl(n, T, Cn, _p(P_near), _p(C_far), _p(nrp, _i64), _p(nci if len(nci) else z, _i32), _p(ndg, _i64),
+        _p(frp, _i64), _p(fci if len(fci) else z, _i32), _p(fdg, _i64), s, _p(nfv), _p(ffv), _p(nfd), _p(ffd))
+    return {"NF": nfv[: len(nci)], "FF": ffv[: len(fci)], "NF_diag": nfd, "FF_diag": ffd}
+
+
+def pl_multiply(n, nV, tree, pre_cols, pre_vals, near_pat, pl, far_pat, alpha, X, beta, Y, nf=True, ff=True):
+    T = len(tree["P_ordering"])
+    Cn = len(tree["C_left"])
+    X = c64(X)
+    assert Y.dtype == np.float64 and Y.flags.c_contiguous
+    z = np.zeros(1, np.int32)
+    nci = c32(near_pat[1]); fci = c32(far_pat[1])
+    _plm(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
+         _p(c32(tree["C_end"]), _i32), _p(c32(pre_cols), _i32), _p(c64(pre_vals)),
+         _p(c64i(near_pat[0]), _i64), _p(nci if len(nci) else z, _i32), _p(c64(pl["NF"]) if len(nci) else np.zeros(1)),
+         _p(c64(pl["NF_diag"])),
+         _p(c64i(far_pat[0]), _i64), _p(fci if len(fci) else z, _i32), _p(c64(pl["FF"]) if len(fci) else np.zeros(1)),
+         _p(c64(pl["FF_diag"])),
+         alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], int(nf), int(ff))
+    return Y

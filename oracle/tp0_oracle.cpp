@@ -27,6 +27,12 @@
 //                             TP0_Kernel_MultiplyMetric_Tiny::ApplyBlock .../TP0_Kernel_MultiplyMetric_Tiny.hpp:89-138
 //   orc_allpairs_energy       TangentPointEnergy_AllPairs idea (src/Energies/TangentPointEnergy/TP_AllPairs_Traversor.hpp):
 //                             sum over all unordered pairs of a_i a_j E_ij, no tree.
+//   orc_pl_compute            PseudoLaplacian_Traversor::Compute (LowOrder)
+//                             src/Metrics/PseudoLaplacian/PseudoLaplacian_Traversor.hpp:143-300,
+//                             PseudoLaplacian_Kernel_NF.hpp:124-146, PseudoLaplacian_Kernel_FF.hpp:101-146
+//   orc_pl_multiply           MetricDimRestricted::MultiplyMetric with OperatorType::LowOrder
+//                             src/Metrics/MetricDimRestricted.hpp:109-139, ClusterTree::Pre/Post (lo_pre, lo_post)
+//                             src/ClusterTree.hpp:458-500,770-943
 //
 // The `Power` of the Tensors library (absent from the reference checkout) is restated as binary
 // powering for integer exponents and std::pow for real exponents.
@@ -657,6 +663,154 @@ void orc_allpairs_metric( int n, LInt T, const double * P_far, double q, double 
             const PairOut o = PairCompute( ex, n, false, false, true, X[0], X + 1, X + 4, Y[0], Y + 1, Y + 4, DX, DY, 1, 1, 1.0, 1.0 );
             vals[2 * (T * i + j)] = o.a0; vals[2 * (T * i + j) + 1] = o.a1;
         }
+    }
+}
+
+// PseudoLaplacian_Traversor::Compute (LowOrder) for a symmetric BCT: on the upper part of each
+// pattern, val = -Power(r2, s_exp) with s_exp = -n/2 - s (PseudoLaplacian_Kernel_NF.hpp:124-146,
+// _FF.hpp:101-146), row diagonal -= (b/a) val, column diagonal += -(a/b) val; the lower part is
+// filled from the upper part. Near rows use the barycenter of the hull (FMM_Kernel_NF::loadS),
+// far rows the cluster center. nf_vals N_nf, ff_vals N_ff, nf_diag T, ff_diag C.
+void orc_pl_compute( int n, LInt T, LInt C, const double * P_near, const double * C_far,
+    const LInt * nrp, const Int * nci, const LInt * ndiag,
+    const LInt * frp, const Int * fci, const LInt * fdiag,
+    double s, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag )
+{
+    const int SIZE = n + 1;
+    const int ND = 1 + 3 * SIZE + 6;
+    const double scale = 1.0 / SIZE;
+    const double s_exp = -0.5 * n - s;
+    auto power = [&]( double r2 )
+    {
+        const double k = std::nearbyint( s_exp );
+        return ( k == s_exp ) ? IntPow( r2, (LInt)k ) : std::pow( r2, s_exp );
+    };
+    auto run = [&]( LInt rows, const LInt * rp, const Int * ci, const LInt * dg, double * vals, double * diag,
+                    auto && load )
+    {
+        std::memset( diag, 0, sizeof(double) * (size_t)rows );
+        for( LInt i = 0; i < rows; ++i )
+        {
+            double a, x[3];
+            load( i, a, x );
+            double ii_block = 0;
+            for( LInt k = dg[i]; k < rp[i + 1]; ++k )
+            {
+                const Int j = ci[k];
+                double b, y[3];
+                load( j, b, y );
+                double r2 = 0;
+                for( int l = 0; l < 3; ++l ) { const double v = y[l] - x[l]; r2 += v * v; }
+                const double val = -power( r2 );
+                vals[k] = val;
+                ii_block -= (b / a) * val;
+                diag[j] += -(a / b) * val;
+            }
+            diag[i] += ii_block;
+        }
+        for( LInt i = 0; i < rows; ++i )
+        {
+            for( LInt k = rp[i]; k < dg[i]; ++k )
+            {
+                const Int j = ci[k];
+                const Int * f = std::lower_bound( ci + dg[j], ci + rp[j + 1], (Int)i );
+                vals[k] = vals[f - ci];
+            }
+        }
+    };
+    run( T, nrp, nci, ndiag, nf_vals, nf_diag, [&]( LInt i, double & a, double * x )
+    {
+        const double * X = P_near + ND * i;
+        a = X[0];
+        for( int k = 0; k < 3; ++k ) { x[k] = scale * X[1 + k]; for( int kk = 1; kk < SIZE; ++kk ) { x[k] += scale * X[1 + 3 * kk + k]; } }
+    });
+    run( C, frp, fci, fdiag, ff_vals, ff_diag, [&]( LInt i, double & a, double * x )
+    {
+        const double * X = C_far + 10 * i;
+        a = X[0];
+        for( int k = 0; k < 3; ++k ) { x[k] = X[1 + k]; }
+    });
+}
+
+// MultiplyMetric of the pseudo-Laplacian: LowOrder Pre (row 0 of the pre-operator: a/(n+1) per
+// vertex, ClusterTree.hpp:458-500) -> scalar NF/FF block products plus diagonals -> Post
+// (PercolateDown, ClustersToPrimitives, lo_post = lo_pre^T), for 3 right-hand sides.
+void orc_pl_multiply( int n, LInt V, LInt T, LInt C,
+    const Int * C_left, const Int * C_right, const Int * C_begin, const Int * C_end,
+    const Int * pre_cols, const double * pre_vals,
+    const LInt * nrp, const Int * nci, const double * nf_vals, const double * nf_diag,
+    const LInt * frp, const Int * fci, const double * ff_vals, const double * ff_diag,
+    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag )
+{
+    const int SIZE = n + 1;
+    std::vector<double> P_in( (size_t)(3 * T), 0.0 ), P_out( (size_t)(3 * T), 0.0 );
+    std::vector<double> C_in( (size_t)(3 * C), 0.0 ), C_out( (size_t)(3 * C), 0.0 );
+    for( LInt i = 0; i < T; ++i )
+    {
+        for( int k = 0; k < 3; ++k )
+        {
+            double s = 0;
+            for( int l = 0; l < SIZE; ++l ) { s += pre_vals[4 * SIZE * i + l] * X[ldX * pre_cols[SIZE * i + l] + k]; }
+            P_in[3 * i + k] = s;
+        }
+    }
+    for( LInt c = C; c-- > 0; )
+    {
+        if( C_left[c] < 0 )
+        {
+            for( int h = 0; h < 3; ++h ) { double s = 0; for( Int i = C_begin[c]; i < C_end[c]; ++i ) { s += P_in[3 * (size_t)i + h]; } C_in[3 * c + h] = s; }
+        }
+        else
+        {
+            for( int h = 0; h < 3; ++h ) { C_in[3 * c + h] = C_in[3 * C_left[c] + h] + C_in[3 * C_right[c] + h]; }
+        }
+    }
+    auto apply = [&]( const LInt rows, const LInt * rp, const Int * ci, const double * vals, const double * diag,
+                      const std::vector<double> & xin, std::vector<double> & yout )
+    {
+        for( LInt i = 0; i < rows; ++i )
+        {
+            double y[3] = {0, 0, 0};
+            for( LInt k = rp[i]; k < rp[i + 1]; ++k )
+            {
+                const double * x = &xin[3 * (size_t)ci[k]];
+                for( int h = 0; h < 3; ++h ) { y[h] += vals[k] * x[h]; }
+            }
+            for( int h = 0; h < 3; ++h ) { yout[3 * (size_t)i + h] = y[h] + diag[i] * xin[3 * (size_t)i + h]; }
+        }
+    };
+    if( nf_flag && nrp[T] > 0 ) { apply( T, nrp, nci, nf_vals, nf_diag, P_in, P_out ); }
+    if( ff_flag && frp[C] > 0 ) { apply( C, frp, fci, ff_vals, ff_diag, C_in, C_out ); }
+    for( LInt c = 0; c < C; ++c )
+    {
+        const Int L = C_left[c];
+        if( L >= 0 )
+        {
+            const Int R = C_right[c];
+            for( int h = 0; h < 3; ++h ) { C_out[3 * L + h] = C_out[3 * c + h] + C_out[3 * L + h]; }
+            for( int h = 0; h < 3; ++h ) { C_out[3 * R + h] = C_out[3 * c + h] + C_out[3 * R + h]; }
+        }
+    }
+    for( LInt c = 0; c < C; ++c )
+    {
+        if( C_left[c] < 0 )
+        {
+            for( Int i = C_begin[c]; i < C_end[c]; ++i ) { for( int h = 0; h < 3; ++h ) { P_out[3 * (size_t)i + h] += C_out[3 * c + h]; } }
+        }
+    }
+    std::vector<double> acc( (size_t)(3 * V), 0.0 );
+    for( LInt i = 0; i < T; ++i )
+    {
+        for( int l = 0; l < SIZE; ++l )
+        {
+            const Int v = pre_cols[SIZE * i + l];
+            const double w = pre_vals[4 * SIZE * i + l];
+            for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += w * P_out[3 * i + k]; }
+        }
+    }
+    for( LInt v = 0; v < V; ++v )
+    {
+        for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = alpha * acc[3 * v + k] + (beta == 0.0 ? 0.0 : beta * Y[ldY * v + k]); }
     }
 }
 

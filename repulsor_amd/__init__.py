@@ -20,6 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 
 RP_OK, RP_EINVAL, RP_EHIP, RP_ENOMEM, RP_ESTAGNATION, RP_EUNSUPPORTED, RP_ENOTREADY = range(7)
+RP_SOLVE_NO_PRECONDITIONER, RP_SOLVE_PC_NORM, RP_SOLVE_GMRES = 1, 2, 4
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
@@ -86,6 +87,16 @@ _host_sigs = {
                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "rpm_metric_assemble": (C.c_int, [_vp, _vp]),
     "rpm_timer_report": (C.c_int, [C.c_char_p, C.c_int64]),
+    "rpm_h1_solve_host": (C.c_int, [_vp, _d, _d, _d]),
+    "rpm_h1_multiply_host": (C.c_int, [_vp, _d, _d]),
+    "rpm_nd_ordering": (C.c_int, [_vp, _i32]),
+    "rpm_h1_csr": (C.c_int, [_vp, _i64, _i32, _d, _i64]),
+    "rpm_metric_precondition": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int]),
+    "rpm_metric_solve": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int,
+                                   C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _d]),
+    "rpm_pl_values": (C.c_int, [_vp, _vp, _d, _d, _d, _d]),
+    "rpm_pl_multiply": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
+    "rpm_h1_solve_device": (C.c_int, [_vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
 }
 H = type("H", (), {})()
 for _n, (_r, _a) in _host_sigs.items():
@@ -266,6 +277,47 @@ class SimplicialMesh:
         _check(H.rpm_mesh_coords(self._h, _p(X)))
         return X
 
+    # ---------------------------------------------------------------- H^1 metric (host side)
+    def h1_csr(self):
+        self.build()
+        nnz = np.zeros(1, dtype=np.int64)
+        _check(H.rpm_h1_csr(self._h, None, None, None, _p(nnz, _i64)))
+        rp = np.zeros(self.V + 1, dtype=np.int64); ci = np.zeros(int(nnz[0]), dtype=np.int32); v = np.zeros(int(nnz[0]))
+        _check(H.rpm_h1_csr(self._h, _p(rp, _i64), _p(ci, _i32), _p(v), _p(nnz, _i64)))
+        return rp, ci, v
+
+    def nd_ordering(self):
+        self.build()
+        p = np.zeros(self.V, dtype=np.int32)
+        _check(H.rpm_nd_ordering(self._h, _p(p, _i32)))
+        return p
+
+    def h1_solve_host(self, B):
+        """X = H^{-1} B with the host supernodal factor (B: V x 3); returns (X, stats)."""
+        self.build()
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        X = np.zeros_like(B)
+        st = np.zeros(5)
+        _check(H.rpm_h1_solve_host(self._h, _p(B), _p(X), _p(st)))
+        keys = ["nnz_H", "nnz_L", "supernodes", "levels", "flops"]
+        return X, dict(zip(keys, st.tolist()))
+
+    def h1_solve(self, B, alpha=1.0, beta=0.0, X=None):
+        """X = alpha H^{-1} B + beta X with the factor's triangular solves on the GPU (B: V x 3)."""
+        self.build()
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        if X is None:
+            X = np.zeros_like(B)
+        _check(H.rpm_h1_solve_device(self._h, alpha, _p(B), B.shape[1], beta, _p(X), X.shape[1], 0))
+        return X
+
+    def h1_multiply_host(self, X):
+        self.build()
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.zeros_like(X)
+        _check(H.rpm_h1_multiply_host(self._h, _p(X), _p(Y)))
+        return Y
+
     def upload(self):
         """Builds the hierarchy and makes the device copies current (no compute)."""
         self.build()
@@ -347,3 +399,62 @@ class TangentPointMetric0(_Op):
         Y = np.zeros_like(X)
         self.MultiplyMetric(M, 1.0, X, 0.0, Y)
         return float(np.sqrt(np.sum(X * Y)))
+
+    # ------------------------------------------------------------------ preconditioner and solve
+    def PseudoLaplacianValues(self, M: SimplicialMesh):
+        """Values of L_{2-s} (near CSR order, scalar per nnz), its diagonals and the far values."""
+        b = M.build()
+        out = {"NF": np.zeros(b["near_nnz"]), "NF_diag": np.zeros(b["T"]),
+               "FF": np.zeros(b["far_nnz"]), "FF_diag": np.zeros(b["C"])}
+        _check(H.rpm_pl_values(self._h, M._h, _p(out["NF"]), _p(out["NF_diag"]), _p(out["FF"]), _p(out["FF_diag"])))
+        return out
+
+    def MultiplyPseudoLaplacian(self, M: SimplicialMesh, alpha, X, beta, Y):
+        """Y = alpha L_{2-s} X + beta Y (V x 3)."""
+        M.build()
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        assert Y.dtype == np.float64 and Y.flags.c_contiguous
+        _check(H.rpm_pl_multiply(self._h, M._h, alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], 0))
+        return Y
+
+    def MultiplyPreconditioner(self, M: SimplicialMesh, alpha, X, beta, Y, nrhs=3):
+        """Y = alpha H^-1 L_{2-s} H^-1 X + beta Y (reference TangentPointMetric0.hpp:88-126)."""
+        M.build()
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        assert Y.dtype == np.float64 and Y.flags.c_contiguous
+        _check(H.rpm_metric_precondition(self._h, M._h, alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], nrhs, 0))
+        return Y
+
+    def PreconditionerNorm(self, M: SimplicialMesh, X):
+        Y = np.zeros_like(X)
+        self.MultiplyPreconditioner(M, 1.0, X, 0.0, Y)
+        return float(np.sqrt(np.sum(X * Y)))
+
+    def Solve(self, M: SimplicialMesh, alpha, B, beta, X, max_iter=1000, tolerance=1e-6, nrhs=3,
+              preconditioner=True, measure="l2", method="pcg", raise_on_stagnation=True):
+        """X = alpha A^+ B + beta X on the GPU; the record is kept for Solver_IterationCount and
+        Solver_RelativeResiduals (reference MetricDimRestricted.hpp:232-289, MetricBase.hpp:126-134).
+        method "pcg" with measure "l2" (|B - A X|_2 / |B|_2 <= tolerance per column) or "pc" (the same
+        ratio in the preconditioner norm sqrt(<r, P r>)); method "gmres": the reference's
+        left-preconditioned GMRES, measure |P (B - A X)|_2 / |P B|_2."""
+        M.build()
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        assert X.dtype == np.float64 and X.flags.c_contiguous
+        assert measure in ("l2", "pc") and method in ("pcg", "gmres")
+        it = (C.c_int * 1)()
+        rel = np.zeros(3)
+        flags = (0 if preconditioner else RP_SOLVE_NO_PRECONDITIONER) | (RP_SOLVE_PC_NORM if measure == "pc" else 0)
+        flags |= RP_SOLVE_GMRES if method == "gmres" else 0
+        st = H.rpm_metric_solve(self._h, M._h, alpha, _p(B), B.shape[1], beta, _p(X), X.shape[1], nrhs,
+                                int(max_iter), float(tolerance), flags, 0, it, _p(rel))
+        self._iter, self._relres = int(it[0]), rel
+        if st == RP_ESTAGNATION and not raise_on_stagnation:
+            return X
+        _check(st)
+        return X
+
+    def Solver_IterationCount(self):
+        return getattr(self, "_iter", 0)
+
+    def Solver_RelativeResiduals(self):
+        return getattr(self, "_relres", np.zeros(3))

@@ -294,6 +294,122 @@ int rpm_metric_assemble( void * op, void * h )
     });
 }
 
+int rpm_metric_precondition( void * op, void * h, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
+                             int nrhs, int device_ptr )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        const bool ok = A.MultiplyPreconditioner( M, alpha, X, ldX, beta, Y, ldY, nrhs, device_ptr );
+        if( !ok && nrhs != 3 ) { g_err = "nrhs must be 3"; }
+        return ok ? RP_OK : RP_EINVAL;
+    });
+}
+
+// X = alpha A^+ B + beta X; iters and relres (3) receive the solver's record. Returns RP_ESTAGNATION
+// when max_iter was reached before the tolerance.
+int rpm_metric_solve( void * op, void * h, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
+                      int nrhs, int max_iter, double tol, int solver_flags, int device_ptr, int * iters, double * relres )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        const bool ok = A.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tol, device_ptr, solver_flags );
+        if( !ok ) { if( nrhs != 3 ) { g_err = "nrhs must be 3"; } return RP_EINVAL; }
+        const auto rr = A.Solver_RelativeResiduals();
+        if( iters ) { *iters = A.Solver_IterationCount(); }
+        bool conv = true;
+        for( size_t k = 0; k < rr.size(); ++k ) { if( relres ) { relres[k] = rr[k]; } if( !( rr[k] <= tol ) ) { conv = false; } }
+        return conv ? RP_OK : RP_ESTAGNATION;
+    });
+}
+
+// Pseudo-Laplacian L_{2-s} of the metric's preconditioner: values in the reference's CSR order.
+int rpm_pl_values( void * op, void * h, double * nf_csr, double * nf_diag, double * ff_vals, double * ff_diag )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        if( !A.PseudoLaplacianValues( M ) ) { return RP_EINVAL; }
+        const auto & B = M.GetBlockClusterTree();
+        std::vector<double> tile( (size_t)std::max<LInt>( 1, B.tiles.val_ptr.back() ) );
+        const int st = rp_download_pl( M.Device(), nf_csr ? tile.data() : nullptr, nf_diag, ff_vals, ff_diag );
+        if( st != RP_OK ) { g_err = rp_last_error(); return st; }
+        if( nf_csr ) { M.NearTileValuesToCSR( tile.data(), B.NearCSR(), nf_csr, 1 ); }
+        return RP_OK;
+    });
+}
+
+int rpm_pl_multiply( void * op, void * h, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY, int device_ptr )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        if( !A.PseudoLaplacianValues( M ) ) { return RP_EINVAL; }
+        const int st = rp_pl_multiply( M.Device(), alpha, X, ldX, beta, Y, ldY, 3, 1, 1, device_ptr ? RP_DEVICE_PTR : 0 );
+        if( st != RP_OK ) { g_err = rp_last_error(); }
+        return st;
+    });
+}
+
+// X = alpha H^-1 B + beta X with the H^1 factor on the device.
+int rpm_h1_solve_device( void * h, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX, int device_ptr )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        rp_ctx * c = M.DeviceH1();
+        const int st = rp_h1_solve( c, alpha, B, ldB, beta, X, ldX, 3, device_ptr ? RP_DEVICE_PTR : 0 );
+        if( st != RP_OK ) { g_err = rp_last_error(); }
+        return st;
+    });
+}
+
+// H^1 metric factor on the host: x = H^{-1} b (V x 3); stats: nnz(H), nnz(L), supernodes, levels, flops.
+int rpm_h1_solve_host( void * h, const double * b, double * x, double * stats )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        const auto & F = M.H1Solver();
+        if( b && x ) { F.Solve( b, x ); }
+        if( stats )
+        {
+            stats[0] = (double)M.H1Metric().ci.size(); stats[1] = (double)F.nnz_L; stats[2] = F.ns;
+            stats[3] = F.levels; stats[4] = F.flops;
+        }
+        return RP_OK;
+    });
+}
+
+// y = H x (V x 3), the assembled H^1 metric.
+int rpm_h1_multiply_host( void * h, const double * x, double * y )
+{
+    return call( (MeshHandle *)h, [&]( auto & M ) { M.H1Metric().Multiply3( x, y ); return RP_OK; } );
+}
+
+// The nested-dissection ordering (new -> old).
+int rpm_nd_ordering( void * h, int32_t * perm )
+{
+    return call( (MeshHandle *)h, [&]( auto & M ) { const auto & p = M.NestedDissection(); std::copy( p.begin(), p.end(), perm ); return RP_OK; } );
+}
+
+// H^1 metric in CSR (pass NULL arrays to query nnz).
+int rpm_h1_csr( void * h, int64_t * rp, int32_t * ci, double * v, int64_t * nnz )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        const auto & H = M.H1Metric();
+        if( nnz ) { *nnz = (int64_t)H.ci.size(); }
+        if( rp ) { std::copy( H.rp.begin(), H.rp.end(), rp ); }
+        if( ci ) { std::copy( H.ci.begin(), H.ci.end(), ci ); }
+        if( v )  { std::copy( H.v.begin(), H.v.end(), v ); }
+        return RP_OK;
+    });
+}
+
 // Host timers of the header-only API (name \t seconds lines).
 int rpm_timer_report( char * buf, int64_t cap )
 {

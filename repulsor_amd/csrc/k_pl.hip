@@ -1,0 +1,326 @@
+// Pseudo-Laplacian L_{s'} of the reference preconditioner (PseudoLaplacian<Mesh,false>, s' = 2 - s):
+// scalar metric blocks val_ij = -(r_ij^2)^{s_exp}, s_exp = -n/2 - s', on the near field (primitive
+// barycenters) and the far field (cluster centers), diagonal diag_i = -sum_j (a_j/a_i) val_ij, and
+// the block product on 3 right-hand sides with the low-order pre/post operator
+// (P_in_i = a_i mean(X over simplex i)).
+//
+// Reference: src/Metrics/PseudoLaplacian/PseudoLaplacian_Kernel_NF.hpp:124-146,
+// PseudoLaplacian_Kernel_FF.hpp:101-146, PseudoLaplacian_Traversor.hpp:143-300,
+// src/Metrics/PseudoLaplacian.hpp:16-107 (LowOrder pre/post, ClusterTree.hpp:770-943).
+// Same leaf-tile traversal and chunk staging as the tangent-point kernels (k_nf.hip).
+
+#include "k_pair.hpp"
+#include "rp_launch.hpp"
+
+#include <type_traits>
+
+namespace rp
+{
+    constexpr int PL_CH = 64;
+
+    // ------------------------------------------------------------------ near-field assembly
+    template<bool MIXED>
+    __global__ __launch_bounds__(64) void k_nf_pl_assemble( NFArgs A )
+    {
+        using R = typename std::conditional<MIXED, float, double>::type;
+        __shared__ double sy[2][4][PL_CH];   // b, y0, y1, y2
+        __shared__ uint32_t sm[2][PL_CH];
+        __shared__ double red[64];
+        const int lane = threadIdx.x;
+        const int64_t I = blockIdx.x;
+        const int ib = A.leaf_begin[I];
+        const int nI = A.leaf_end[I] - ib;
+        const int nS = 64 / nI;
+        const int ii = lane % nI;
+        const int c  = lane / nI;
+        const bool active = lane < nS * nI;
+        const int64_t i = ib + ii;
+        const int64_t T = A.T;
+        const double a = A.a[i];
+        const double x0 = A.x[i], x1 = A.x[T + i], x2 = A.x[2 * T + i];
+        const double inv_a = 1.0 / a;
+        const int64_t cb = A.col_ptr[I];
+        const int64_t M_ = A.col_ptr[I + 1] - cb;
+        const int64_t vb = A.val_ptr[I];
+        double dacc = 0.0;
+        double rd[4];
+        uint32_t rm = 0;
+        auto gload = [&]( int64_t base )
+        {
+            const int64_t m = base + lane;
+            if( m < M_ )
+            {
+                const int32_t j = A.col_prim[cb + m];
+                rm = A.col_mask[cb + m];
+                rd[0] = A.a[j]; rd[1] = A.x[j]; rd[2] = A.x[T + j]; rd[3] = A.x[2 * T + j];
+            }
+            else { rm = 0u; }
+        };
+        auto lstore = [&]( int buf ) { sm[buf][lane] = rm; for( int k = 0; k < 4; ++k ) { sy[buf][k][lane] = rd[k]; } };
+        gload( 0 );
+        lstore( 0 );
+        __syncthreads();
+        int buf = 0;
+        for( int64_t base = 0; base < M_; base += PL_CH )
+        {
+            const bool more = base + PL_CH < M_;
+            if( more ) { gload( base + PL_CH ); }
+            const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
+            if( active )
+            {
+                for( int mm = c; mm < cnt_c; mm += nS )
+                {
+                    const int64_t slot = vb + (base + mm) * nI + ii;
+                    if( (sm[buf][mm] >> ii) & 1u )
+                    {
+                        const double b = sy[buf][0][mm];
+                        const R v0 = (R)(sy[buf][1][mm] - x0), v1 = (R)(sy[buf][2][mm] - x1), v2 = (R)(sy[buf][3][mm] - x2);
+                        const R r2 = v0 * v0 + v1 * v1 + v2 * v2;
+                        const R val = -metric_power<R>( A.ex, r2 );
+                        if constexpr ( MIXED ) { reinterpret_cast<float *>(A.vals)[slot] = val; }
+                        else                   { reinterpret_cast<double *>(A.vals)[slot] = val; }
+                        dacc -= ( b * inv_a ) * (double)val;
+                    }
+                    else
+                    {
+                        if constexpr ( MIXED ) { reinterpret_cast<float *>(A.vals)[slot] = 0.0f; }
+                        else                   { reinterpret_cast<double *>(A.vals)[slot] = 0.0; }
+                    }
+                }
+            }
+            if( more ) { lstore( buf ^ 1 ); }
+            __syncthreads();
+            buf ^= 1;
+        }
+        red[lane] = dacc;
+        __syncthreads();
+        if( lane < nI )
+        {
+            double s = 0.0;
+            for( int cc = 0; cc < nS; ++cc ) { s += red[lane + cc * nI]; }
+            A.diag[i] = s;
+        }
+    }
+
+    // ------------------------------------------------------------------ far-field assembly
+    template<bool MIXED>
+    __global__ __launch_bounds__(256) void k_ff_pl_assemble( FFArgs A )
+    {
+        using R = typename std::conditional<MIXED, float, double>::type;
+        const int w = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        const int64_t i = (int64_t)blockIdx.x * 4 + w;
+        if( i >= A.C ) { return; }
+        const double inv_a = 1.0 / A.a[i];
+        const double x0 = A.x[i], x1 = A.x[A.C + i], x2 = A.x[2 * A.C + i];
+        double dacc = 0.0;
+        for( int64_t k = A.rp[i] + lane; k < A.rp[i + 1]; k += 64 )
+        {
+            const int32_t j = A.ci[k];
+            const R v0 = (R)(A.x[j] - x0), v1 = (R)(A.x[A.C + j] - x1), v2 = (R)(A.x[2 * A.C + j] - x2);
+            const R r2 = v0 * v0 + v1 * v1 + v2 * v2;
+            const R val = -metric_power<R>( A.ex, r2 );
+            if constexpr ( MIXED ) { reinterpret_cast<float *>(A.vals)[k] = val; }
+            else                   { reinterpret_cast<double *>(A.vals)[k] = val; }
+            dacc -= ( A.a[j] * inv_a ) * (double)val;
+        }
+        #pragma unroll
+        for( int off = 32; off > 0; off >>= 1 ) { dacc += __shfl_xor( dacc, off, 64 ); }
+        if( lane == 0 ) { A.diag[i] = dacc; }
+    }
+
+    // ------------------------------------------------------------------ low-order pre-operator
+    template<int SIZE>
+    __global__ void k_pl_pre( int64_t T, const int32_t * cols, const double * pre_vals, const double * X, int64_t ldX, double * P_in )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        double s[3] = { 0, 0, 0 };
+        for( int l = 0; l < SIZE; ++l )
+        {
+            const double wgt = pre_vals[4 * SIZE * i + l];
+            const double * x = X + ldX * (int64_t)cols[SIZE * i + l];
+            for( int k = 0; k < 3; ++k ) { s[k] += wgt * x[k]; }
+        }
+        for( int k = 0; k < 3; ++k ) { P_in[3 * i + k] = s[k]; }
+    }
+
+    // ------------------------------------------------------------------ near-field product + post epilogue
+    template<typename VT, int SIZE>
+    __global__ __launch_bounds__(64) void k_nf_pl_spmm( const int32_t * leaf_begin, const int32_t * leaf_end,
+        const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
+        const VT * vals, const double * diag, const double * xin, const double * leaf_out, int use_far,
+        const double * pre_vals, double * S_out )
+    {
+        __shared__ double sx[2][3][PL_CH];
+        __shared__ uint32_t sm[2][PL_CH];
+        __shared__ double red[64][3];
+        const int lane = threadIdx.x;
+        const int64_t I = blockIdx.x;
+        const int ib = leaf_begin[I];
+        const int nI = leaf_end[I] - ib;
+        const int nS = 64 / nI;
+        const int ii = lane % nI;
+        const int c  = lane / nI;
+        const bool active = lane < nS * nI;
+        const int64_t cb = col_ptr[I];
+        const int64_t M_ = col_ptr[I + 1] - cb;
+        const int64_t vb = val_ptr[I];
+        double acc[3] = { 0, 0, 0 };
+        double rx[3];
+        uint32_t rm = 0;
+        auto gload = [&]( int64_t base )
+        {
+            const int64_t m = base + lane;
+            if( m < M_ )
+            {
+                const int32_t j = col_prim[cb + m];
+                rm = col_mask[cb + m];
+                for( int k = 0; k < 3; ++k ) { rx[k] = xin[3 * (int64_t)j + k]; }
+            }
+            else { rm = 0u; }
+        };
+        auto lstore = [&]( int buf ) { sm[buf][lane] = rm; for( int k = 0; k < 3; ++k ) { sx[buf][k][lane] = rx[k]; } };
+        gload( 0 );
+        lstore( 0 );
+        __syncthreads();
+        int buf = 0;
+        for( int64_t base = 0; base < M_; base += PL_CH )
+        {
+            const bool more = base + PL_CH < M_;
+            if( more ) { gload( base + PL_CH ); }
+            const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
+            if( active )
+            {
+                for( int mm = c; mm < cnt_c; mm += nS )
+                {
+                    if( (sm[buf][mm] >> ii) & 1u )
+                    {
+                        const double v = (double)vals[vb + (base + mm) * nI + ii];
+                        for( int k = 0; k < 3; ++k ) { acc[k] += v * sx[buf][k][mm]; }
+                    }
+                }
+            }
+            if( more ) { lstore( buf ^ 1 ); }
+            __syncthreads();
+            buf ^= 1;
+        }
+        for( int k = 0; k < 3; ++k ) { red[lane][k] = acc[k]; }
+        __syncthreads();
+        if( lane < nI )
+        {
+            const int64_t i = ib + ii;
+            double s[3] = { 0, 0, 0 };
+            for( int cc = 0; cc < nS; ++cc ) { for( int k = 0; k < 3; ++k ) { s[k] += red[lane + cc * nI][k]; } }
+            const double d = diag[i];
+            for( int k = 0; k < 3; ++k ) { s[k] += d * xin[3 * i + k]; }
+            if( use_far ) { for( int k = 0; k < 3; ++k ) { s[k] = s[k] + leaf_out[3 * I + k]; } }
+            for( int sl = 0; sl < SIZE; ++sl )
+            {
+                const double wgt = pre_vals[4 * SIZE * i + sl];
+                for( int k = 0; k < 3; ++k ) { S_out[(SIZE * i + sl) * 3 + k] = wgt * s[k]; }
+            }
+        }
+    }
+
+    template<int SIZE>
+    __global__ void k_pl_post_prim( int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals,
+                                    const double * leaf_out, int use_far, double * S_out )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        double s[3];
+        for( int k = 0; k < 3; ++k ) { s[k] = use_far ? leaf_out[3 * (int64_t)leaf_idx_of_prim[i] + k] : 0.0; }
+        for( int sl = 0; sl < SIZE; ++sl )
+        {
+            const double wgt = pre_vals[4 * SIZE * i + sl];
+            for( int k = 0; k < 3; ++k ) { S_out[(SIZE * i + sl) * 3 + k] = wgt * s[k]; }
+        }
+    }
+
+    template<typename VT>
+    __global__ __launch_bounds__(256) void k_ff_pl_spmm( int64_t C, const int64_t * rp, const int32_t * ci,
+        const VT * vals, const double * diag, const double * xin, double * yout )
+    {
+        const int w = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        const int64_t i = (int64_t)blockIdx.x * 4 + w;
+        if( i >= C ) { return; }
+        double acc[3] = { 0, 0, 0 };
+        for( int64_t k = rp[i] + lane; k < rp[i + 1]; k += 64 )
+        {
+            const double v = (double)vals[k];
+            const double * xj = xin + 3 * (int64_t)ci[k];
+            for( int h = 0; h < 3; ++h ) { acc[h] += v * xj[h]; }
+        }
+        for( int h = 0; h < 3; ++h )
+        {
+            #pragma unroll
+            for( int off = 32; off > 0; off >>= 1 ) { acc[h] += __shfl_xor( acc[h], off, 64 ); }
+        }
+        if( lane == 0 )
+        {
+            const double d = diag[i];
+            for( int h = 0; h < 3; ++h ) { yout[3 * i + h] = acc[h] + d * xin[3 * i + h]; }
+        }
+    }
+
+    // ------------------------------------------------------------------ launchers
+    void launch_pl_assemble( bool mixed, const NFArgs & A, const FFArgs & B, bool do_nf, bool do_ff, hipStream_t s )
+    {
+        if( do_nf && A.L )
+        {
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_assemble<true>), dim3((unsigned)A.L), dim3(64), 0, s, A ); }
+            else        { hipLaunchKernelGGL( (k_nf_pl_assemble<false>), dim3((unsigned)A.L), dim3(64), 0, s, A ); }
+        }
+        if( do_ff && B.C )
+        {
+            const dim3 grid( (unsigned)((B.C + 3) / 4) );
+            if( mixed ) { hipLaunchKernelGGL( (k_ff_pl_assemble<true>), grid, dim3(256), 0, s, B ); }
+            else        { hipLaunchKernelGGL( (k_ff_pl_assemble<false>), grid, dim3(256), 0, s, B ); }
+        }
+    }
+
+    void launch_pl_pre( int size, int64_t T, const int32_t * cols, const double * pre_vals, const double * X, int64_t ldX, double * P_in, hipStream_t s )
+    {
+        if( !T ) { return; }
+        const dim3 grid( (unsigned)((T + 255) / 256) );
+        if( size == 3 ) { hipLaunchKernelGGL( (k_pl_pre<3>), grid, dim3(256), 0, s, T, cols, pre_vals, X, ldX, P_in ); }
+        else            { hipLaunchKernelGGL( (k_pl_pre<2>), grid, dim3(256), 0, s, T, cols, pre_vals, X, ldX, P_in ); }
+    }
+
+    void launch_pl_nf_spmm( bool mixed, int size, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp,
+                            const int32_t * cpr, const uint32_t * cm, const int64_t * vp, const void * vals, const double * diag,
+                            const double * xin, const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s )
+    {
+        if( !L ) { return; }
+        const dim3 grid( (unsigned)L ), block( 64 );
+        if( size == 3 )
+        {
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,3>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,3>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+        }
+        else
+        {
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,2>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,2>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+        }
+    }
+
+    void launch_pl_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s )
+    {
+        if( !T ) { return; }
+        const dim3 grid( (unsigned)((T + 255) / 256) );
+        if( size == 3 ) { hipLaunchKernelGGL( (k_pl_post_prim<3>), grid, dim3(256), 0, s, T, leaf_idx_of_prim, pre_vals, leaf_out, use_far, S_out ); }
+        else            { hipLaunchKernelGGL( (k_pl_post_prim<2>), grid, dim3(256), 0, s, T, leaf_idx_of_prim, pre_vals, leaf_out, use_far, S_out ); }
+    }
+
+    void launch_pl_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
+    {
+        if( !C ) { return; }
+        const dim3 grid( (unsigned)((C + 3) / 4) );
+        if( mixed ) { hipLaunchKernelGGL( (k_ff_pl_spmm<float>), grid, dim3(256), 0, s, C, rp, ci, (const float *)vals, diag, xin, yout ); }
+        else        { hipLaunchKernelGGL( (k_ff_pl_spmm<double>), grid, dim3(256), 0, s, C, rp, ci, (const double *)vals, diag, xin, yout ); }
+    }
+
+} // namespace rp

@@ -221,29 +221,29 @@ namespace rp
         for( int h = 0; h < 6; ++h ) { o[h] = make_double2( out[2 * h], out[2 * h + 1] ); }
     }
 
-    // Leaf sums of P_in (PrimitivesToClusters).
-    __global__ void k_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end,
+    // Leaf sums of P_in (PrimitivesToClusters), W values per primitive.
+    __global__ void k_up_leaves( int64_t L, int W, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end,
                                  const double * P_in, double * C_in )
     {
         const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        const int64_t l = g / 12;
-        const int h = (int)(g % 12);
+        const int64_t l = g / W;
+        const int h = (int)(g % W);
         if( l >= L ) { return; }
         const int32_t c = leaf_cluster[l];
         double s = 0;
-        for( int32_t i = C_begin[c]; i < C_end[c]; ++i ) { s += P_in[12 * (int64_t)i + h]; }
-        C_in[12 * (int64_t)c + h] = s;
+        for( int32_t i = C_begin[c]; i < C_end[c]; ++i ) { s += P_in[W * (int64_t)i + h]; }
+        C_in[W * (int64_t)c + h] = s;
     }
 
     // One depth level of PercolateUp: C_in[c] = C_in[left] + C_in[right].
-    __global__ void k_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in )
+    __global__ void k_up_level( int64_t n, int W, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in )
     {
         const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        const int64_t q = g / 12;
-        const int h = (int)(g % 12);
+        const int64_t q = g / W;
+        const int h = (int)(g % W);
         if( q >= n ) { return; }
         const int32_t c = nodes[q];
-        C_in[12 * (int64_t)c + h] = C_in[12 * (int64_t)C_left[c] + h] + C_in[12 * (int64_t)C_right[c] + h];
+        C_in[W * (int64_t)c + h] = C_in[W * (int64_t)C_left[c] + h] + C_in[W * (int64_t)C_right[c] + h];
     }
 
     template<typename T>
@@ -370,11 +370,14 @@ namespace rp
         else            { hipLaunchKernelGGL( (k_pre<2>), grid1(T), dim3(256), 0, s, T, cols, vals, X, ldX, P_in ); }
     }
 
-    void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s )
-    { if( L ) { hipLaunchKernelGGL( k_up_leaves, grid1(12 * L), dim3(256), 0, s, L, leaf_cluster, C_begin, C_end, P_in, C_in ); } }
+    void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s, int W )
+    { if( L ) { hipLaunchKernelGGL( k_up_leaves, grid1(W * L), dim3(256), 0, s, L, W, leaf_cluster, C_begin, C_end, P_in, C_in ); } }
 
-    void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s )
-    { if( n ) { hipLaunchKernelGGL( k_up_level, grid1(12 * n), dim3(256), 0, s, n, nodes, C_left, C_right, C_in ); } }
+    void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s, int W )
+    { if( n ) { hipLaunchKernelGGL( k_up_level, grid1(W * n), dim3(256), 0, s, n, W, nodes, C_left, C_right, C_in ); } }
+
+    void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<3>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
 
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {

@@ -1,52 +1,15 @@
 // C-ABI entry points of librepulsor_hip.so (declared in include/repulsor_hip.h).
 
-#include "rp_internal.hpp"
+#include "rp_capi_util.hpp"
 #include "k_pair.hpp"
 #include "rp_launch.hpp"
 
 #include <cstring>
-#include <exception>
-#include <new>
 
 namespace rp
 {
     static thread_local std::string g_last_error;
     void set_error( const std::string & msg ) { g_last_error = msg; }
-
-    struct InvalidArg { std::string msg; int status; };
-
-    static void require( bool cond, const std::string & msg, int status = RP_EINVAL )
-    {
-        if( !cond ) { throw InvalidArg{ msg, status }; }
-    }
-
-    template<typename F>
-    static int guarded( const char * name, F && f )
-    {
-        try
-        {
-            f();
-            return RP_OK;
-        }
-        catch( const InvalidArg & e ) { set_error( std::string(name) + ": " + e.msg ); return e.status; }
-        catch( const HipError & e )   { set_error( std::string(name) + ": " + e.msg ); return e.status; }
-        catch( const std::bad_alloc & ) { set_error( std::string(name) + ": host allocation failed" ); return RP_ENOMEM; }
-        catch( const std::exception & e ) { set_error( std::string(name) + ": " + e.what() ); return RP_EINVAL; }
-    }
-
-    struct EventScope
-    {
-        rp_ctx * ctx; int which;
-        EventScope( rp_ctx * c, int w ) : ctx(c), which(w) { RP_HIP( hipEventRecord( ctx->ev[which][0], ctx->stream ) ); }
-        void stop() { RP_HIP( hipEventRecord( ctx->ev[which][1], ctx->stream ) ); }
-    };
-
-    static void finish_timing( rp_ctx * ctx, int which )
-    {
-        float ms = 0;
-        RP_HIP( hipEventElapsedTime( &ms, ctx->ev[which][0], ctx->ev[which][1] ) );
-        ctx->timings.ms[which] = ms;
-    }
 
     static int exponent_kind( double q, double p )
     {
@@ -55,10 +18,47 @@ namespace rp
         return KIND_GENERIC;
     }
 
-    static void pack_rows( const double * src, int64_t ld, int64_t n, std::vector<double> & dst )
+    // Y = alpha A X + beta Y on device vectors (V x 3, leading dimensions ldx, ldy): pre-operator,
+    // cluster sums, far-field SpMM and down-sweep, near-field SpMM with fused post epilogue, vertex gather.
+    // Reference: MetricDimRestricted::MultiplyMetric (src/Metrics/MetricDimRestricted.hpp:109-139).
+    void metric_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy,
+                       bool nf_flag, bool ff_flag, bool time_nf )
     {
-        dst.resize( (size_t)(3 * n) );
-        for( int64_t v = 0; v < n; ++v ) { for( int k = 0; k < 3; ++k ) { dst[3 * v + k] = src[ld * v + k]; } }
+        const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
+        const bool mixed = ctx->metric_precision == RP_MIXED;
+        ctx->P_in.require( (size_t)(12 * T) ); ctx->P_out.require( (size_t)(12 * T) );
+        ctx->C_in.require( (size_t)(12 * C) ); ctx->C_out.require( (size_t)(12 * C) );
+        ctx->leaf_out.require( (size_t)(12 * L) );
+        const bool do_nf = nf_flag && ctx->nf_cols > 0;
+        const bool do_ff = ff_flag && ctx->far_nnz > 0;
+        launch_pre( ctx->size, T, ctx->pre_cols.get(), ctx->pre_vals.get(), Xd, ldx, ctx->P_in.get(), ctx->stream );
+        if( do_ff )
+        {
+            launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream );
+            for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
+            {
+                const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
+                launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream );
+            }
+            // Far field first, so that the near-field SpMM epilogue can add the percolated cluster
+            // output of its leaf and apply the post-operator (S_out = per-vertex-slot contributions).
+            launch_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->ff_vals.get(), ctx->ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
+            launch_leaf_out( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
+        }
+        double * S_out = ctx->P_out.get();   // T x (n+1) x 3 <= 12 T
+        if( time_nf ) { RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) ); }
+        if( do_nf )
+        {
+            launch_nf_spmm_post( mixed, ctx->size, T, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
+                                 ctx->col_mask.get(), ctx->val_ptr.get(), ctx->nf_vals.get(), ctx->nf_diag.get(), ctx->P_in.get(),
+                                 ctx->leaf_out.get(), do_ff ? 1 : 0, ctx->pre_vals.get(), S_out, ctx->stream );
+        }
+        else
+        {
+            launch_post_prim( ctx->size, T, ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->leaf_out.get(), do_ff ? 1 : 0, S_out, ctx->stream );
+        }
+        if( time_nf ) { RP_HIP( hipEventRecord( ctx->ev[6][1], ctx->stream ) ); }
+        launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
     }
 
     // Leaf-path sums of the far-field derivatives (needed by both read-out paths).
@@ -93,7 +93,7 @@ int rp_ctx_create( int device, rp_ctx ** out )
         try
         {
             RP_HIP( hipStreamCreateWithFlags( &c->stream, hipStreamNonBlocking ) );
-            for( int k = 0; k < 8; ++k ) { RP_HIP( hipEventCreate( &c->ev[k][0] ) ); RP_HIP( hipEventCreate( &c->ev[k][1] ) ); }
+            for( int k = 0; k < 12; ++k ) { RP_HIP( hipEventCreate( &c->ev[k][0] ) ); RP_HIP( hipEventCreate( &c->ev[k][1] ) ); }
         }
         catch( ... ) { delete c; throw; }
         *out = c;
@@ -105,7 +105,7 @@ int rp_ctx_destroy( rp_ctx * ctx )
     if( !ctx ) { return RP_OK; }
     (void)hipSetDevice( ctx->device );
     if( ctx->stream ) { (void)hipStreamSynchronize( ctx->stream ); }
-    for( int k = 0; k < 8; ++k ) { (void)hipEventDestroy( ctx->ev[k][0] ); (void)hipEventDestroy( ctx->ev[k][1] ); }
+    for( int k = 0; k < 12; ++k ) { (void)hipEventDestroy( ctx->ev[k][0] ); (void)hipEventDestroy( ctx->ev[k][1] ); }
     if( ctx->stream ) { (void)hipStreamDestroy( ctx->stream ); }
     delete ctx;
     return RP_OK;
@@ -145,7 +145,7 @@ int rp_upload_mesh( rp_ctx * ctx, int dom_dim, int64_t vertex_count, int64_t sim
         ctx->vc_corner.upload( corner.data(), corner.size(), ctx->stream );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_tree = ctx->have_near = ctx->have_far = ctx->have_data = false;
-        ctx->have_diff = ctx->have_metric = false;
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = false;
     });
 }
 
@@ -230,7 +230,7 @@ int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
         ctx->level_nodes.upload( level_nodes.data(), level_nodes.size(), ctx->stream );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_tree = true;
-        ctx->have_near = ctx->have_far = ctx->have_data = ctx->have_diff = ctx->have_metric = false;
+        ctx->have_near = ctx->have_far = ctx->have_data = ctx->have_diff = ctx->have_metric = ctx->have_pl = false;
     });
 }
 
@@ -271,7 +271,7 @@ int rp_upload_near_tiles( rp_ctx * ctx, int32_t leaf_count,
         ctx->nf_slots = val_ptr.back();
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_near = true;
-        ctx->have_diff = ctx->have_metric = false;
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = false;
     });
 }
 
@@ -290,7 +290,7 @@ int rp_upload_far_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp_, cons
         ctx->far_nnz = nnz;
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_far = true;
-        ctx->have_diff = ctx->have_metric = false;
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = false;
     });
 }
 
@@ -316,7 +316,7 @@ int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near
         RP_HIP( hipGetLastError() );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_data = true;
-        ctx->have_diff = ctx->have_metric = false;
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = false;
     });
 }
 
@@ -515,9 +515,8 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
         require( X != nullptr && Y != nullptr && ldX >= 3 && ldY >= 3, "X and Y must be V x 3 with ld >= 3" );
         (void)vf_flag;
         RP_HIP( hipSetDevice( ctx->device ) );
-        const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
+        const int64_t V = ctx->V;
         const bool dev = ptr_flags & RP_DEVICE_PTR;
-        const bool mixed = ctx->metric_precision == RP_MIXED;
         std::vector<double> h, hx, hy;
         const double * Xd = X;
         int64_t ldx = ldX;
@@ -532,43 +531,8 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
             if( beta != 0.0 ) { pack_rows( Y, ldY, V, h ); ctx->Yd.upload( h.data(), h.size(), ctx->stream ); }
             Yd = ctx->Yd.get(); ldy = 3;
         }
-        ctx->P_in.require( (size_t)(12 * T) ); ctx->P_out.require( (size_t)(12 * T) );
-        ctx->C_in.require( (size_t)(12 * C) ); ctx->C_out.require( (size_t)(12 * C) );
-        ctx->leaf_out.require( (size_t)(12 * L) );
-        const bool do_nf = nf_flag && ctx->nf_cols > 0;
-        const bool do_ff = ff_flag && ctx->far_nnz > 0;
         EventScope ev( ctx, 2 );
-        launch_pre( ctx->size, T, ctx->pre_cols.get(), ctx->pre_vals.get(), Xd, ldx, ctx->P_in.get(), ctx->stream );
-        if( do_ff )
-        {
-            launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream );
-            for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
-            {
-                const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
-                launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream );
-            }
-        }
-        // Far field first, so that the near-field SpMM epilogue can add the percolated cluster
-        // output of its leaf and apply the post-operator (S_out = per-vertex-slot contributions).
-        if( do_ff )
-        {
-            launch_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->ff_vals.get(), ctx->ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
-            launch_leaf_out( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
-        }
-        double * S_out = ctx->P_out.get();   // T x (n+1) x 3 <= 12 T
-        RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) );
-        if( do_nf )
-        {
-            launch_nf_spmm_post( mixed, ctx->size, T, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
-                                 ctx->col_mask.get(), ctx->val_ptr.get(), ctx->nf_vals.get(), ctx->nf_diag.get(), ctx->P_in.get(),
-                                 ctx->leaf_out.get(), do_ff ? 1 : 0, ctx->pre_vals.get(), S_out, ctx->stream );
-        }
-        else
-        {
-            launch_post_prim( ctx->size, T, ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->leaf_out.get(), do_ff ? 1 : 0, S_out, ctx->stream );
-        }
-        RP_HIP( hipEventRecord( ctx->ev[6][1], ctx->stream ) );
-        launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
+        metric_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy, nf_flag != 0, ff_flag != 0, true );
         ev.stop();
         RP_HIP( hipGetLastError() );
         if( !dev )
@@ -586,7 +550,7 @@ int rp_last_timing( rp_ctx * ctx, int which, double * ms )
 {
     return guarded( "rp_last_timing", [&]()
     {
-        require( ctx != nullptr && ms != nullptr && which >= 0 && which < 8, "bad arguments" );
+        require( ctx != nullptr && ms != nullptr && which >= 0 && which < 12, "bad arguments" );
         *ms = ctx->timings.ms[which];
     });
 }

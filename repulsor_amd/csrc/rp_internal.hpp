@@ -109,7 +109,7 @@ namespace rp
 
     struct Timings
     {
-        double ms[8] = {0,0,0,0,0,0,0,0};
+        double ms[12] = {0,0,0,0,0,0,0,0,0,0,0,0};
     };
 
 } // namespace rp
@@ -118,7 +118,7 @@ struct rp_ctx
 {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8][2];
+    hipEvent_t ev[12][2];
 
     // sizes
     int dom_dim = 0, size = 0, nd = 0;
@@ -189,6 +189,28 @@ struct rp_ctx
     // multiply buffers
     rp::DBuf<double> P_in, P_out, C_in, C_out, leaf_out;   // 12T, 12T, 12C, 12C, 12L
     rp::DBuf<double> Xd, Yd;                                // 3V staging
+
+    // pseudo-Laplacian metric values (scalar blocks)
+    rp::DBuf<double> pl_nf_vals, pl_nf_diag, pl_ff_vals, pl_ff_diag;
+    bool have_pl = false;
+    int pl_precision = 0;
+    double pl_s = 0.0;
+
+    // supernodal Cholesky factor of the H^1 metric
+    rp::DBuf<int32_t> h1_sn_begin, h1_rows, h1_kid_ptr, h1_kids, h1_cmap, h1_perm, h1_level_nodes;
+    rp::DBuf<int64_t> h1_row_ptr, h1_val_ptr;
+    rp::DBuf<double>  h1_vals, h1_z, h1_xp;
+    std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
+    std::vector<int>     h1_level_wmax;
+    std::vector<int>     h1_level_rmax;
+    int64_t h1_n = 0, h1_ns = 0;
+    bool have_h1 = false;
+
+    // Krylov solver state
+    rp::DBuf<double> cg_x, cg_r, cg_p, cg_Ap, cg_z, cg_b, pc_tmp, cg_scal, cg_part;
+    rp::DBuf<double> gm_V, gm_w, gm_t, gm_small, gm_part;   // GMRES basis (m+1) x 3V and work
+    int solve_iters = 0;
+    std::vector<double> solve_relres;
 
     rp::Timings timings;
 };

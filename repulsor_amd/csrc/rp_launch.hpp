@@ -57,8 +57,9 @@ namespace rp
                        const double * V, const double * charges, const double * nf_dx, const double * leaf_far, double * B, hipStream_t s );
     void launch_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
     void launch_pre( int size, int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
-    void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s );
-    void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s );
+    void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s, int W = 12 );
+    void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s, int W = 12 );
+    void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s );
     void launch_nf_spmm_post( bool mixed, int size, int64_t T, int64_t L, const int32_t * lb, const int32_t * le,
                               const int64_t * cp, const int32_t * cpr, const uint32_t * cm, const int64_t * vp,
                               const void * vals, const double * diag, const double * xin,
@@ -67,5 +68,47 @@ namespace rp
     void launch_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s );
     void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
     void launch_fill( double * p, int64_t n, double v, hipStream_t s );
+
+    // pseudo-Laplacian (k_pl.hip)
+    void launch_pl_assemble( bool mixed, const NFArgs & A, const FFArgs & B, bool do_nf, bool do_ff, hipStream_t s );
+    void launch_pl_pre( int size, int64_t T, const int32_t * cols, const double * pre_vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
+    void launch_pl_nf_spmm( bool mixed, int size, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp,
+                            const int32_t * cpr, const uint32_t * cm, const int64_t * vp, const void * vals, const double * diag,
+                            const double * xin, const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s );
+    void launch_pl_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s );
+    void launch_pl_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s );
+
+    // supernodal triangular solves (k_chol.hip)
+    struct CholDev
+    {
+        const int32_t * sn_begin;   // ns + 1
+        const int64_t * row_ptr;    // ns + 1
+        const int32_t * rows;       // row sets, sorted, own columns first
+        const int64_t * val_ptr;    // ns + 1
+        const double  * vals;       // column-major r x w blocks
+        const int32_t * kid_ptr;    // ns + 1
+        const int32_t * kids;       // children in increasing order
+        const int32_t * cmap;       // per row entry: position in the parent's row list
+        const int32_t * perm;       // new -> old
+        double        * z;          // 3 x rows: front vectors
+        double        * xp;         // 3 x n: solution in the new numbering
+    };
+    void launch_chol_fwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big,
+                                double alpha, const double * B, int64_t ldB, hipStream_t s );
+    void launch_chol_bwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big, hipStream_t s );
+    void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
+
+    // Krylov vector kernels (k_solve.hip)
+    void launch_dot3( int64_t n, const double * u, const double * v, double * part, double * out, hipStream_t s );   // part: 3 x 1024
+    void launch_cg_update( int64_t n, double * x, double * r, const double * p, const double * Ap, const double * rho,
+                           const double * pAp, const double * active, double * part, double * rr, hipStream_t s );
+    void launch_cg_dir( int64_t n, double * p, const double * z, const double * rho_new, const double * rho_old, const double * active, hipStream_t s );
+    void launch_cg_check( const double * rr, const double * bb, double tol2, double * active, hipStream_t s );
+    void launch_copy3( int64_t n, const double * src, int64_t lds, double * dst, int64_t ldd, hipStream_t s );
+    void launch_axpby3( int64_t n, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY, hipStream_t s );
+    constexpr int MDOT_PART = 256 * 3;   // partial-sum doubles per basis vector of launch_mdot
+    void launch_mdot( int64_t n, const double * Vb, int64_t stride, int nv, const double * w, double * part, double * out, int accumulate, hipStream_t s );
+    void launch_maxpy( int64_t n, const double * Vb, int64_t stride, int nv, const double * coef, double sign, double * w, hipStream_t s );
+    void launch_normalize3( int64_t n, const double * w, const double * nrm2, double * out, hipStream_t s );
 
 } // namespace rp

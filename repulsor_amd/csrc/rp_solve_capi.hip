@@ -1,0 +1,598 @@
+// C-ABI entry points of the metric solve (declared in include/repulsor_hip.h): the pseudo-Laplacian
+// L_{s'}, the H^1 factor and its triangular solves, the preconditioner P0 = H^-1 L_{2-s} H^-1 and the
+// preconditioned conjugate-gradient solve of the tangent-point metric.
+
+#include "rp_capi_util.hpp"
+#include "k_pair.hpp"
+#include "rp_launch.hpp"
+
+#include <algorithm>
+
+namespace rp
+{
+    // Y = alpha L X + beta Y (device vectors). Reference: MetricDimRestricted::MultiplyMetric with the
+    // LowOrder pre/post (src/Metrics/MetricDimRestricted.hpp:109-139) around
+    // PseudoLaplacian_Traversor::MultiplyMetric (src/Metrics/PseudoLaplacian/PseudoLaplacian_Traversor.hpp:143-300).
+    static void pl_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy, bool nf_flag, bool ff_flag )
+    {
+        const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
+        const bool mixed = ctx->pl_precision == RP_MIXED;
+        ctx->P_in.require( (size_t)(12 * T) ); ctx->P_out.require( (size_t)(12 * T) );
+        ctx->C_in.require( (size_t)(12 * C) ); ctx->C_out.require( (size_t)(12 * C) );
+        ctx->leaf_out.require( (size_t)(12 * L) );
+        const bool do_nf = nf_flag && ctx->nf_cols > 0;
+        const bool do_ff = ff_flag && ctx->far_nnz > 0;
+        launch_pl_pre( ctx->size, T, ctx->pre_cols.get(), ctx->pre_vals.get(), Xd, ldx, ctx->P_in.get(), ctx->stream );
+        if( do_ff )
+        {
+            launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream, 3 );
+            for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
+            {
+                const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
+                launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream, 3 );
+            }
+            launch_pl_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->pl_ff_vals.get(), ctx->pl_ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
+            launch_leaf_out3( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
+        }
+        double * S_out = ctx->P_out.get();   // T x (n+1) x 3
+        if( do_nf )
+        {
+            launch_pl_nf_spmm( mixed, ctx->size, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
+                               ctx->col_mask.get(), ctx->val_ptr.get(), ctx->pl_nf_vals.get(), ctx->pl_nf_diag.get(), ctx->P_in.get(),
+                               ctx->leaf_out.get(), do_ff ? 1 : 0, ctx->pre_vals.get(), S_out, ctx->stream );
+        }
+        else
+        {
+            launch_pl_post_prim( ctx->size, T, ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->leaf_out.get(), do_ff ? 1 : 0, S_out, ctx->stream );
+        }
+        launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
+    }
+
+    static CholDev chol_dev( rp_ctx * ctx )
+    {
+        CholDev F{};
+        F.sn_begin = ctx->h1_sn_begin.get(); F.row_ptr = ctx->h1_row_ptr.get(); F.rows = ctx->h1_rows.get();
+        F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get();
+        F.kid_ptr = ctx->h1_kid_ptr.get(); F.kids = ctx->h1_kids.get(); F.cmap = ctx->h1_cmap.get();
+        F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get();
+        return F;
+    }
+
+    // X = alpha H^-1 B + beta X (device vectors, V x 3).
+    static void h1_solve( rp_ctx * ctx, double alpha, const double * Bd, int64_t ldb, double beta, double * Xd, int64_t ldx )
+    {
+        const CholDev F = chol_dev( ctx );
+        const size_t nl = ctx->h1_level_wmax.size();
+        for( size_t l = 0; l < nl; ++l )
+        {
+            const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
+            launch_chol_fwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], ctx->h1_level_rmax[l] > 96,
+                                   alpha, Bd, ldb, ctx->stream );
+        }
+        for( size_t l = nl; l-- > 0; )
+        {
+            const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
+            launch_chol_bwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], ctx->h1_level_rmax[l] > 96, ctx->stream );
+        }
+        launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream );
+    }
+
+    // Y = alpha P0 X + beta Y with P0 = H^-1 L_{2-s} H^-1.
+    // Reference: TangentPointMetric0::multiply_preconditioner (src/Metrics/TangentPointMetric0.hpp:88-126).
+    static void pc_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    {
+        const int64_t V = ctx->V;
+        ctx->pc_tmp.require( (size_t)(3 * V) );
+        double * Z = ctx->pc_tmp.get();
+        h1_solve( ctx, alpha, Xd, ldx, 0.0, Z, 3 );
+        if( alpha != 0.0 ) { pl_apply( ctx, 1.0, Z, 3, 0.0, Z, 3, true, true ); }
+        h1_solve( ctx, 1.0, Z, 3, beta, Yd, ldy );
+    }
+
+    // Host staging of a V x 3 input/output pair for calls with host pointers.
+    struct Staged
+    {
+        rp_ctx * ctx;
+        bool dev;
+        const double * X; int64_t ldx;
+        double * Y; int64_t ldy;
+        double * Yh; int64_t ldYh;
+        std::vector<double> hx, hy;
+
+        Staged( rp_ctx * c, bool device, const double * X_, int64_t ldX, double beta, double * Y_, int64_t ldY )
+        : ctx(c), dev(device), X(X_), ldx(ldX), Y(Y_), ldy(ldY), Yh(Y_), ldYh(ldY)
+        {
+            if( dev ) { return; }
+            const int64_t V = ctx->V;
+            pack_rows( X_, ldX, V, hx );
+            ctx->Xd.upload( hx.data(), hx.size(), ctx->stream );
+            X = ctx->Xd.get(); ldx = 3;
+            ctx->Yd.require( (size_t)(3 * V) );
+            if( beta != 0.0 ) { pack_rows( Y_, ldY, V, hy ); ctx->Yd.upload( hy.data(), hy.size(), ctx->stream ); }
+            Y = ctx->Yd.get(); ldy = 3;
+        }
+
+        // Downloads the result (host pointers) after the stream has finished.
+        void finish()
+        {
+            if( dev ) { RP_HIP( hipStreamSynchronize( ctx->stream ) ); return; }
+            hy.resize( (size_t)(3 * ctx->V) );
+            ctx->Yd.download( hy.data(), hy.size(), ctx->stream );
+            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+            unpack_rows( hy, ctx->V, Yh, ldYh );
+        }
+    };
+
+    static void require_nrhs( int nrhs )
+    {
+        require( nrhs == 3, "nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
+    }
+
+    // Preconditioned conjugate gradients on the three columns (independent scalars per column, zero
+    // start). Stopping measure: |r|_2 / |b|_2, or with pc_norm the preconditioner norm
+    // sqrt(<r, P r>) / sqrt(<b, P b>). b, x: device V x 3 (ld 3). Returns the iteration count.
+    static int pcg( rp_ctx * ctx, const double * b, double * x, int max_iter, double tol, bool use_pc, bool pc_norm,
+                    std::vector<double> & rel, bool & done )
+    {
+        const int64_t V = ctx->V;
+        const size_t n3 = (size_t)(3 * V);
+        hipStream_t s = ctx->stream;
+        ctx->cg_r.require( n3 ); ctx->cg_p.require( n3 ); ctx->cg_Ap.require( n3 ); ctx->cg_z.require( n3 );
+        ctx->cg_scal.require( 32 ); ctx->cg_part.require( 3 * 1024 );
+        double * r = ctx->cg_r.get(); double * p = ctx->cg_p.get();
+        double * Ap = ctx->cg_Ap.get(); double * z = ctx->cg_z.get();
+        double * sc = ctx->cg_scal.get(); double * part = ctx->cg_part.get();
+        double * rho[2] = { sc + 0, sc + 3 };
+        double * pAp = sc + 6; double * rr = sc + 9; double * bb = sc + 12; double * active = sc + 15;
+        double * rho0 = sc + 18;
+        const double tol2 = tol * tol;
+        auto precondition = [&]( const double * in, double * out )
+        {
+            if( use_pc ) { pc_apply( ctx, 1.0, in, 3, 0.0, out, 3 ); }
+            else         { launch_copy3( V, in, 3, out, 3, s ); }
+        };
+        // x = 0, r = b, z = P r, p = z, rho = <r, z>
+        launch_fill( x, (int64_t)n3, 0.0, s );
+        launch_copy3( V, b, 3, r, 3, s );
+        launch_dot3( V, r, r, part, bb, s );
+        precondition( r, z );
+        launch_copy3( V, z, 3, p, 3, s );
+        launch_dot3( V, r, z, part, rho[0], s );
+        launch_copy3( 1, rho[0], 3, rho0, 3, s );
+        const double * ref = pc_norm ? rho0 : bb;
+        launch_cg_check( ref, ref, 0.0, active, s );
+        double h_m[3], h_ref[3];
+        RP_HIP( hipMemcpyAsync( h_ref, ref, sizeof(h_ref), hipMemcpyDeviceToHost, s ) );
+        RP_HIP( hipStreamSynchronize( s ) );
+        rel.assign( 3, 0.0 );
+        auto all_done = [&]( const double * m )
+        {
+            bool ok = true;
+            for( int k = 0; k < 3; ++k )
+            {
+                rel[(size_t)k] = h_ref[k] > 0 ? std::sqrt( std::max( 0.0, m[k] ) / h_ref[k] ) : 0.0;
+                if( m[k] > tol2 * h_ref[k] ) { ok = false; }
+            }
+            return ok;
+        };
+        int it = 0;
+        done = true;   // nothing to do when every column of B vanishes
+        for( int k = 0; k < 3; ++k ) { if( h_ref[k] > 0 ) { done = false; } }
+        int cur = 0;
+        while( !done && it < max_iter )
+        {
+            metric_apply( ctx, 1.0, p, 3, 0.0, Ap, 3, true, true, false );
+            launch_dot3( V, p, Ap, part, pAp, s );
+            launch_cg_update( V, x, r, p, Ap, rho[cur], pAp, active, part, rr, s );
+            if( !pc_norm )
+            {
+                RP_HIP( hipMemcpyAsync( h_m, rr, sizeof(h_m), hipMemcpyDeviceToHost, s ) );
+                launch_cg_check( rr, bb, tol2, active, s );
+                RP_HIP( hipStreamSynchronize( s ) );
+                ++it;
+                done = all_done( h_m );
+                if( done || it >= max_iter ) { break; }
+                precondition( r, z );
+                launch_dot3( V, r, z, part, rho[cur ^ 1], s );
+            }
+            else
+            {
+                precondition( r, z );
+                launch_dot3( V, r, z, part, rho[cur ^ 1], s );
+                RP_HIP( hipMemcpyAsync( h_m, rho[cur ^ 1], sizeof(h_m), hipMemcpyDeviceToHost, s ) );
+                launch_cg_check( rho[cur ^ 1], rho0, tol2, active, s );
+                RP_HIP( hipStreamSynchronize( s ) );
+                ++it;
+                done = all_done( h_m );
+                if( done || it >= max_iter ) { break; }
+            }
+            launch_cg_dir( V, p, z, rho[cur ^ 1], rho[cur], active, s );
+            cur ^= 1;
+        }
+        return it;
+    }
+
+    constexpr int GMRES_RESTART = 100;
+
+    // Restarted GMRES(m), left preconditioned (Saad & Schultz 1986): minimises |P (b - A x)|_2 over
+    // x in x0 + K_m(P A, P r0) for each of the three columns, Arnoldi with classical Gram-Schmidt
+    // applied twice (each pass is one batched inner-product kernel and one batched update), Givens
+    // rotations on the host. b: device V x 3 (ld 3), x: device V x 3 (ld 3, overwritten, zero start).
+    // Returns the number of Arnoldi steps; rel receives |P r|_2 / |P b|_2 per column.
+    // Reference: MetricDimRestricted::Solve with GMRES<3,Real,Int,Side::Left>
+    // (src/Metrics/MetricDimRestricted.hpp:232-289).
+    static int gmres( rp_ctx * ctx, const double * b, double * x, int max_iter, double tol, bool use_pc, std::vector<double> & rel, bool & converged )
+    {
+        const int64_t V = ctx->V;
+        const int64_t n3 = 3 * V;
+        const int m = std::max( 1, std::min( max_iter, GMRES_RESTART ) );
+        ctx->gm_V.require( (size_t)( (m + 1) * n3 ) );
+        ctx->gm_w.require( (size_t)n3 );
+        ctx->gm_t.require( (size_t)n3 );
+        ctx->gm_small.require( (size_t)( 4 * 3 * (m + 2) ) );
+        ctx->gm_part.require( (size_t)( (m + 1) * MDOT_PART ) );
+        double * Vb = ctx->gm_V.get(); double * w = ctx->gm_w.get(); double * t = ctx->gm_t.get();
+        double * h1 = ctx->gm_small.get(); double * h2 = h1 + 3 * (m + 2); double * nrm = h2 + 3 * (m + 2); double * y = nrm + 3 * (m + 2);
+        double * part = ctx->gm_part.get();
+        hipStream_t s = ctx->stream;
+        auto precondition = [&]( const double * in, double * out )
+        {
+            if( use_pc ) { pc_apply( ctx, 1.0, in, 3, 0.0, out, 3 ); }
+            else         { launch_copy3( V, in, 3, out, 3, s ); }
+        };
+        launch_fill( x, n3, 0.0, s );
+        precondition( b, w );
+        launch_dot3( V, w, w, part, nrm, s );
+        double hn[3];
+        RP_HIP( hipMemcpyAsync( hn, nrm, sizeof(hn), hipMemcpyDeviceToHost, s ) );
+        RP_HIP( hipStreamSynchronize( s ) );
+        double beta0[3];
+        for( int k = 0; k < 3; ++k ) { beta0[k] = std::sqrt( hn[k] ); }
+        rel.assign( 3, 0.0 );
+        converged = ( beta0[0] == 0 && beta0[1] == 0 && beta0[2] == 0 );
+        int it = 0;
+        std::vector<double> R( (size_t)(3 * m * (m + 1)) ), g( (size_t)(3 * (m + 1)) ), cs( (size_t)(3 * m) ), sn( (size_t)(3 * m) );
+        std::vector<double> hh( (size_t)(3 * (m + 2)) ), hh2( (size_t)(3 * (m + 2)) ), yy( (size_t)(3 * (m + 1)) );
+        double beta[3] = { beta0[0], beta0[1], beta0[2] };
+        while( !converged && it < max_iter )
+        {
+            launch_normalize3( V, w, nrm, Vb, s );
+            std::fill( g.begin(), g.end(), 0.0 );
+            for( int k = 0; k < 3; ++k ) { g[(size_t)k] = beta[k]; }
+            int j = 0;
+            for( ; j < m && it < max_iter; ++j )
+            {
+                double * vj = Vb + n3 * j;
+                double * vn = Vb + n3 * (j + 1);
+                metric_apply( ctx, 1.0, vj, 3, 0.0, t, 3, true, true, false );
+                precondition( t, w );
+                launch_mdot( V, Vb, n3, j + 1, w, part, h1, 0, s );
+                launch_maxpy( V, Vb, n3, j + 1, h1, -1.0, w, s );
+                launch_mdot( V, Vb, n3, j + 1, w, part, h2, 0, s );
+                launch_maxpy( V, Vb, n3, j + 1, h2, -1.0, w, s );
+                launch_dot3( V, w, w, part, nrm, s );
+                launch_normalize3( V, w, nrm, vn, s );
+                RP_HIP( hipMemcpyAsync( hh.data(), h1, sizeof(double) * 3 * (j + 1), hipMemcpyDeviceToHost, s ) );
+                RP_HIP( hipMemcpyAsync( hh2.data(), h2, sizeof(double) * 3 * (j + 1), hipMemcpyDeviceToHost, s ) );
+                RP_HIP( hipMemcpyAsync( hn, nrm, sizeof(hn), hipMemcpyDeviceToHost, s ) );
+                RP_HIP( hipStreamSynchronize( s ) );
+                ++it;
+                bool all = true;
+                for( int k = 0; k < 3; ++k )
+                {
+                    double * Hc = &R[(size_t)(k * m * (m + 1) + j * (m + 1))];   // column j of column k's Hessenberg
+                    for( int i = 0; i <= j; ++i ) { Hc[i] = hh[(size_t)(3 * i + k)] + hh2[(size_t)(3 * i + k)]; }
+                    Hc[j + 1] = std::sqrt( hn[k] );
+                    for( int i = 0; i < j; ++i )
+                    {
+                        const double c_ = cs[(size_t)(k * m + i)], s_ = sn[(size_t)(k * m + i)];
+                        const double a = Hc[i], bq = Hc[i + 1];
+                        Hc[i] = c_ * a + s_ * bq;
+                        Hc[i + 1] = -s_ * a + c_ * bq;
+                    }
+                    const double d = std::hypot( Hc[j], Hc[j + 1] );
+                    const double c_ = d > 0 ? Hc[j] / d : 1.0, s_ = d > 0 ? Hc[j + 1] / d : 0.0;
+                    cs[(size_t)(k * m + j)] = c_; sn[(size_t)(k * m + j)] = s_;
+                    Hc[j] = d; Hc[j + 1] = 0.0;
+                    const double gj = g[(size_t)(3 * j + k)];   // g[3 i + k]: rotated right-hand side
+                    g[(size_t)(3 * (j + 1) + k)] = -s_ * gj;
+                    g[(size_t)(3 * j + k)] = c_ * gj;
+                    rel[(size_t)k] = beta0[k] > 0 ? std::abs( g[(size_t)(3 * (j + 1) + k)] ) / beta0[k] : 0.0;
+                    if( rel[(size_t)k] > tol ) { all = false; }
+                }
+                if( all ) { converged = true; ++j; break; }
+            }
+            // x += V y with R y = g (upper triangular, per column)
+            const int nj = j;
+            for( int k = 0; k < 3; ++k )
+            {
+                for( int i = nj - 1; i >= 0; --i )
+                {
+                    double acc = g[(size_t)(3 * i + k)];
+                    for( int l = i + 1; l < nj; ++l ) { acc -= R[(size_t)(k * m * (m + 1) + l * (m + 1) + i)] * yy[(size_t)(3 * l + k)]; }
+                    const double rii = R[(size_t)(k * m * (m + 1) + i * (m + 1) + i)];
+                    yy[(size_t)(3 * i + k)] = rii != 0.0 ? acc / rii : 0.0;
+                }
+            }
+            RP_HIP( hipMemcpyAsync( y, yy.data(), sizeof(double) * 3 * nj, hipMemcpyHostToDevice, s ) );
+            launch_maxpy( V, Vb, n3, nj, y, 1.0, x, s );
+            if( converged || it >= max_iter ) { break; }
+            // restart: w = P (b - A x)
+            metric_apply( ctx, 1.0, x, 3, 0.0, t, 3, true, true, false );
+            launch_axpby3( V, 1.0, b, 3, -1.0, t, 3, s );
+            precondition( t, w );
+            launch_dot3( V, w, w, part, nrm, s );
+            RP_HIP( hipMemcpyAsync( hn, nrm, sizeof(hn), hipMemcpyDeviceToHost, s ) );
+            RP_HIP( hipStreamSynchronize( s ) );
+            for( int k = 0; k < 3; ++k ) { beta[k] = std::sqrt( hn[k] ); }
+        }
+        RP_HIP( hipStreamSynchronize( s ) );
+        return it;
+    }
+}
+
+using namespace rp;
+
+extern "C"
+{
+
+int rp_pl_compute( rp_ctx * ctx, double s, int precision )
+{
+    return guarded( "rp_pl_compute", [&]()
+    {
+        require( ctx != nullptr && ctx->have_data && ctx->have_near && ctx->have_far, "mesh, tree, patterns and data must be uploaded", RP_ENOTREADY );
+        require( precision == RP_FP64 || precision == RP_MIXED, "unknown precision" );
+        require( std::isfinite( s ), "s must be finite" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int n = ctx->dom_dim;
+        const int64_t T = ctx->T, C = ctx->C, L = ctx->L;
+        const double s_exp = -0.5 * n - s;
+        const Expo ex = make_expo( 0.0, 0.0, n, s_exp );
+        const bool mixed = precision == RP_MIXED;
+        ctx->pl_nf_vals.require( (size_t)std::max<int64_t>( 1, ctx->nf_slots ) );
+        ctx->pl_nf_diag.require( (size_t)T );
+        ctx->pl_ff_vals.require( (size_t)std::max<int64_t>( 1, ctx->far_nnz ) );
+        ctx->pl_ff_diag.require( (size_t)C );
+        const bool have_nf = ctx->nf_cols > 0, have_ff = ctx->far_nnz > 0;
+        EventScope ev( ctx, 9 );
+        if( !have_nf ) { launch_fill( ctx->pl_nf_diag.get(), T, 0.0, ctx->stream ); }
+        if( !have_ff ) { launch_fill( ctx->pl_ff_diag.get(), C, 0.0, ctx->stream ); }
+        NFArgs A{};
+        A.L = L; A.T = T;
+        A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
+        A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
+        A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.ex = ex;
+        A.vals = ctx->pl_nf_vals.get(); A.diag = ctx->pl_nf_diag.get();
+        FFArgs B{};
+        B.C = C; B.rp = ctx->far_rp.get(); B.ci = ctx->far_ci.get();
+        B.a = ctx->c_a.get(); B.x = ctx->c_x.get(); B.ex = ex;
+        B.vals = ctx->pl_ff_vals.get(); B.diag = ctx->pl_ff_diag.get();
+        launch_pl_assemble( mixed, A, B, have_nf, have_ff, ctx->stream );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        finish_timing( ctx, 9 );
+        ctx->have_pl = true;
+        ctx->pl_precision = precision;
+        ctx->pl_s = s;
+    });
+}
+
+int rp_download_pl( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * ff_vals, double * ff_diag )
+{
+    return guarded( "rp_download_pl", [&]()
+    {
+        require( ctx != nullptr && ctx->have_pl, "no pseudo-Laplacian computed", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const bool mixed = ctx->pl_precision == RP_MIXED;
+        auto get_vals = [&]( const DBuf<double> & buf, int64_t cnt, double * out )
+        {
+            if( !out || cnt == 0 ) { return; }
+            if( !mixed ) { buf.download( out, (size_t)cnt, ctx->stream ); RP_HIP( hipStreamSynchronize( ctx->stream ) ); return; }
+            std::vector<float> f( (size_t)cnt );
+            RP_HIP( hipMemcpyAsync( f.data(), buf.get(), f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream ) );
+            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+            for( size_t k = 0; k < f.size(); ++k ) { out[k] = f[k]; }
+        };
+        get_vals( ctx->pl_nf_vals, ctx->nf_cols > 0 ? ctx->nf_slots : 0, nf_vals );
+        get_vals( ctx->pl_ff_vals, ctx->far_nnz, ff_vals );
+        if( nf_diag ) { ctx->pl_nf_diag.download( nf_diag, (size_t)ctx->T, ctx->stream ); }
+        if( ff_diag ) { ctx->pl_ff_diag.download( ff_diag, (size_t)ctx->C, ctx->stream ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+    });
+}
+
+int rp_pl_multiply( rp_ctx * ctx, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
+                    int nrhs, int nf_flag, int ff_flag, int ptr_flags )
+{
+    return guarded( "rp_pl_multiply", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require_nrhs( nrhs );
+        require( ctx->have_pl, "pseudo-Laplacian not assembled (call rp_pl_compute)", RP_ENOTREADY );
+        require( X != nullptr && Y != nullptr && ldX >= 3 && ldY >= 3, "X and Y must be V x 3 with ld >= 3" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        Staged st( ctx, ptr_flags & RP_DEVICE_PTR, X, ldX, beta, Y, ldY );
+        pl_apply( ctx, alpha, st.X, st.ldx, beta, st.Y, st.ldy, nf_flag != 0, ff_flag != 0 );
+        RP_HIP( hipGetLastError() );
+        st.finish();
+    });
+}
+
+int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * perm, const int32_t * sn_begin,
+                         const int64_t * row_ptr, const int32_t * rows, const int64_t * val_ptr, const double * vals,
+                         const int32_t * sn_parent )
+{
+    return guarded( "rp_upload_h1_factor", [&]()
+    {
+        require( ctx != nullptr && ctx->V > 0, "upload the mesh first", RP_ENOTREADY );
+        require( n == ctx->V, "factor size differs from the vertex count" );
+        require( ns > 0 && perm && sn_begin && row_ptr && rows && val_ptr && vals && sn_parent, "missing factor arrays" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        require( sn_begin[0] == 0 && sn_begin[ns] == n && row_ptr[0] == 0 && val_ptr[0] == 0, "bad supernode pointers" );
+        {
+            std::vector<char> seen( (size_t)n, 0 );
+            for( int32_t j = 0; j < n; ++j )
+            {
+                require( perm[j] >= 0 && perm[j] < n && !seen[(size_t)perm[j]], "perm is not a permutation" );
+                seen[(size_t)perm[j]] = 1;
+            }
+        }
+        std::vector<int32_t> height( (size_t)ns, 0 );
+        std::vector<int32_t> kid_ptr( (size_t)ns + 1, 0 );
+        for( int32_t s = 0; s < ns; ++s )
+        {
+            const int32_t w = sn_begin[s + 1] - sn_begin[s];
+            const int64_t r = row_ptr[s + 1] - row_ptr[s];
+            require( w >= 1 && w <= 512, "supernode widths must lie in 1..512", RP_EUNSUPPORTED );
+            require( r >= w && r <= n, "bad supernode row count" );
+            require( val_ptr[s + 1] - val_ptr[s] == r * w, "val_ptr must hold r x w blocks" );
+            const int32_t * R = rows + row_ptr[s];
+            for( int32_t a = 0; a < w; ++a ) { require( R[a] == sn_begin[s] + a, "row sets must start with the supernode's columns" ); }
+            for( int64_t q = 1; q < r; ++q ) { require( R[q] > R[q - 1] && R[q] < n, "row sets must be sorted and in range" ); }
+            const int32_t p = sn_parent[s];
+            require( p == -1 || (p > s && p < ns), "supernodes must be in postorder (parent after child)" );
+            if( p >= 0 ) { ++kid_ptr[(size_t)p + 1]; height[(size_t)p] = std::max( height[(size_t)p], height[(size_t)s] + 1 ); }
+            else { require( r == w, "a root supernode cannot have update rows" ); }
+        }
+        for( int32_t s = 0; s < ns; ++s ) { kid_ptr[(size_t)s + 1] += kid_ptr[(size_t)s]; }
+        std::vector<int32_t> kids( (size_t)std::max( 1, kid_ptr.back() ) );
+        {
+            std::vector<int32_t> pos( kid_ptr.begin(), kid_ptr.end() - 1 );
+            for( int32_t s = 0; s < ns; ++s ) { if( sn_parent[s] >= 0 ) { kids[(size_t)pos[(size_t)sn_parent[s]]++] = s; } }
+        }
+        // cmap: position of each update row of a child in its parent's (sorted) row list
+        const int64_t nrows = row_ptr[ns];
+        std::vector<int32_t> cmap( (size_t)nrows, -1 );
+        for( int32_t s = 0; s < ns; ++s )
+        {
+            const int32_t p = sn_parent[s];
+            if( p < 0 ) { continue; }
+            const int32_t w = sn_begin[s + 1] - sn_begin[s];
+            const int32_t * Rp = rows + row_ptr[p];
+            const int64_t rpn = row_ptr[p + 1] - row_ptr[p];
+            for( int64_t q = row_ptr[s] + w; q < row_ptr[s + 1]; ++q )
+            {
+                const int32_t * it = std::lower_bound( Rp, Rp + rpn, rows[q] );
+                require( it != Rp + rpn && *it == rows[q], "a child's update row is missing from its parent's row set" );
+                cmap[(size_t)q] = (int32_t)( it - Rp );
+            }
+        }
+        int32_t nl = 0;
+        for( auto h : height ) { nl = std::max( nl, h + 1 ); }
+        std::vector<int64_t> lptr( (size_t)nl + 1, 0 );
+        for( auto h : height ) { ++lptr[(size_t)h + 1]; }
+        for( int32_t l = 0; l < nl; ++l ) { lptr[(size_t)l + 1] += lptr[(size_t)l]; }
+        std::vector<int32_t> lnodes( (size_t)ns );
+        std::vector<int> wmax( (size_t)nl, 1 ), rmax( (size_t)nl, 1 );
+        {
+            std::vector<int64_t> pos( lptr.begin(), lptr.end() - 1 );
+            for( int32_t s = 0; s < ns; ++s )
+            {
+                const int32_t h = height[(size_t)s];
+                lnodes[(size_t)pos[(size_t)h]++] = s;
+                wmax[(size_t)h] = std::max( wmax[(size_t)h], sn_begin[s + 1] - sn_begin[s] );
+                rmax[(size_t)h] = std::max( rmax[(size_t)h], (int)( row_ptr[s + 1] - row_ptr[s] ) );
+            }
+        }
+        ctx->h1_sn_begin.upload( sn_begin, (size_t)ns + 1, ctx->stream );
+        ctx->h1_row_ptr.upload( row_ptr, (size_t)ns + 1, ctx->stream );
+        ctx->h1_rows.upload( rows, (size_t)nrows, ctx->stream );
+        ctx->h1_val_ptr.upload( val_ptr, (size_t)ns + 1, ctx->stream );
+        ctx->h1_vals.upload( vals, (size_t)val_ptr[ns], ctx->stream );
+        ctx->h1_kid_ptr.upload( kid_ptr.data(), kid_ptr.size(), ctx->stream );
+        ctx->h1_kids.upload( kids.data(), kids.size(), ctx->stream );
+        ctx->h1_cmap.upload( cmap.data(), cmap.size(), ctx->stream );
+        ctx->h1_perm.upload( perm, (size_t)n, ctx->stream );
+        ctx->h1_level_nodes.upload( lnodes.data(), lnodes.size(), ctx->stream );
+        ctx->h1_z.require( (size_t)(3 * nrows) );
+        ctx->h1_xp.require( (size_t)(3 * (int64_t)n) );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->h1_level_ptr = lptr;
+        ctx->h1_level_wmax = wmax;
+        ctx->h1_level_rmax = rmax;
+        ctx->h1_n = n;
+        ctx->h1_ns = ns;
+        ctx->have_h1 = true;
+    });
+}
+
+int rp_h1_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
+                 int nrhs, int ptr_flags )
+{
+    return guarded( "rp_h1_solve", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require_nrhs( nrhs );
+        require( ctx->have_h1, "no H1 factor uploaded (call rp_upload_h1_factor)", RP_ENOTREADY );
+        require( B != nullptr && X != nullptr && ldB >= 3 && ldX >= 3, "B and X must be V x 3 with ld >= 3" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        Staged st( ctx, ptr_flags & RP_DEVICE_PTR, B, ldB, beta, X, ldX );
+        EventScope ev( ctx, 8 );
+        h1_solve( ctx, alpha, st.X, st.ldx, beta, st.Y, st.ldy );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        st.finish();
+        finish_timing( ctx, 8 );
+    });
+}
+
+int rp_multiply_preconditioner( rp_ctx * ctx, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
+                                int nrhs, int ptr_flags )
+{
+    return guarded( "rp_multiply_preconditioner", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require_nrhs( nrhs );
+        require( ctx->have_h1 && ctx->have_pl, "the preconditioner needs the H1 factor and the pseudo-Laplacian", RP_ENOTREADY );
+        require( X != nullptr && Y != nullptr && ldX >= 3 && ldY >= 3, "X and Y must be V x 3 with ld >= 3" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        Staged st( ctx, ptr_flags & RP_DEVICE_PTR, X, ldX, beta, Y, ldY );
+        EventScope ev( ctx, 7 );
+        pc_apply( ctx, alpha, st.X, st.ldx, beta, st.Y, st.ldy );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        st.finish();
+        finish_timing( ctx, 7 );
+    });
+}
+
+int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
+              int nrhs, int max_iter, double tolerance, int solver_flags, int ptr_flags, int * iterations, double * rel_residuals )
+{
+    return guarded( "rp_solve", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require_nrhs( nrhs );
+        require( ctx->have_metric, "metric values not assembled (call rp_tp0_compute with RP_METRIC)", RP_ENOTREADY );
+        const bool use_pc = !( solver_flags & RP_SOLVE_NO_PRECONDITIONER );
+        require( !use_pc || (ctx->have_h1 && ctx->have_pl), "the preconditioner needs the H1 factor and the pseudo-Laplacian", RP_ENOTREADY );
+        require( B != nullptr && X != nullptr && ldB >= 3 && ldX >= 3, "B and X must be V x 3 with ld >= 3" );
+        require( max_iter >= 0 && tolerance >= 0, "max_iter and tolerance must be non-negative" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t V = ctx->V;
+        const size_t n3 = (size_t)(3 * V);
+        Staged st( ctx, ptr_flags & RP_DEVICE_PTR, B, ldB, beta, X, ldX );
+        ctx->cg_x.require( n3 ); ctx->cg_b.require( n3 );
+        double * x = ctx->cg_x.get(); double * b = ctx->cg_b.get();
+        EventScope ev( ctx, 3 );
+        launch_copy3( V, st.X, st.ldx, b, 3, ctx->stream );
+        std::vector<double> rel( 3, 0.0 );
+        bool done = false;
+        const int it = ( solver_flags & RP_SOLVE_GMRES )
+                     ? gmres( ctx, b, x, max_iter, tolerance, use_pc, rel, done )
+                     : pcg( ctx, b, x, max_iter, tolerance, use_pc, use_pc && ( solver_flags & RP_SOLVE_PC_NORM ), rel, done );
+        launch_axpby3( V, alpha, x, 3, beta, st.Y, st.ldy, ctx->stream );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        st.finish();
+        finish_timing( ctx, 3 );
+        ctx->solve_iters = it;
+        ctx->solve_relres = rel;
+        if( iterations ) { *iterations = it; }
+        if( rel_residuals ) { for( int k = 0; k < 3; ++k ) { rel_residuals[k] = rel[(size_t)k]; } }
+        if( !done ) { throw InvalidArg{ "no convergence to the requested tolerance within max_iter iterations", RP_ESTAGNATION }; }
+    });
+}
+
+} // extern "C"

@@ -1,6 +1,8 @@
 """Shared helpers of the test suite: run the oracle on the same tree and patterns as the product."""
 
 import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 import oracle as O
 
@@ -44,3 +46,95 @@ def rel(a, b):
     b = np.asarray(b, dtype=np.float64)
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+def h1_matrix(M):
+    """The host-assembled H^1 metric as a scipy CSR matrix."""
+    rp, ci, v = M.h1_csr()
+    return sp.csr_matrix((v, ci, rp), shape=(M.V, M.V))
+
+
+class OracleSolve:
+    """A (oracle metric), P0 = H^-1 L H^-1 (scipy LU of H + oracle pseudo-Laplacian)."""
+
+    def __init__(self, M, q, p):
+        self.M = M
+        n = M.dom_dim
+        s = (p - n) / q
+        self.o, self.tr, self.td, self.npat, self.fpat = oracle_metric(M, q, p)
+        self.pl = O.pl_compute(n, self.td["P_near"], self.td["C_far"], self.npat, self.fpat, 2.0 - s)
+        self.lu = spla.splu(h1_matrix(M).tocsc())
+
+    def A(self, X):
+        M, td = self.M, self.td
+        Y = np.zeros_like(X)
+        return O.multiply_metric(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.npat, self.o["NF"], self.o["NF_diag"],
+                                 self.fpat, self.o["FF"], self.o["FF_diag"], 1.0, X, 0.0, Y)
+
+    def P(self, X):
+        M, td = self.M, self.td
+        Z = np.ascontiguousarray(self.lu.solve(np.ascontiguousarray(X)))
+        Y = np.zeros(Z.shape)
+        O.pl_multiply(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.npat, self.pl, self.fpat, 1.0, Z, 0.0, Y)
+        return np.ascontiguousarray(self.lu.solve(Y))
+
+    def pcg(self, B, tol=1e-6, max_iter=500, measure="l2"):
+        """Textbook PCG per column (all columns iterate together); measure "l2" stops on
+        |r|_2 <= tol |b|_2, "pc" on <r, P r> <= tol^2 <b, P b>."""
+        X = np.zeros_like(B); Rr = B.copy(); Z = self.P(Rr); Pd = Z.copy()
+        rho = np.sum(Rr * Z, axis=0); bb = np.sum(B * B, axis=0); rho0 = rho.copy()
+        for it in range(1, max_iter + 1):
+            AP = self.A(Pd)
+            al = rho / np.sum(Pd * AP, axis=0)
+            X += al * Pd; Rr -= al * AP
+            if measure == "l2" and np.all(np.sum(Rr * Rr, axis=0) <= tol * tol * bb):
+                return X, it
+            Z = self.P(Rr)
+            rn = np.sum(Rr * Z, axis=0)
+            if measure == "pc" and np.all(rn <= tol * tol * rho0):
+                return X, it
+            Pd = Z + (rn / rho) * Pd
+            rho = rn
+        return X, max_iter
+
+    def gmres(self, B, tol=1e-6, max_iter=500, restart=100):
+        """Textbook left-preconditioned restarted GMRES (modified Gram-Schmidt, Givens rotations), one
+        column at a time; stops on |P r|_2 <= tol |P b|_2. Returns X and the largest step count."""
+        X = np.zeros_like(B)
+        its = 0
+        for k in range(B.shape[1]):
+            b = B[:, k]
+            def op(v):
+                Z = np.zeros_like(B); Z[:, k] = v
+                return self.P(self.A(Z))[:, k]
+            def pc(v):
+                Z = np.zeros_like(B); Z[:, k] = v
+                return self.P(Z)[:, k]
+            x = np.zeros(len(b)); r = pc(b); beta0 = np.linalg.norm(r); it = 0
+            while it < max_iter and beta0 > 0:
+                beta = np.linalg.norm(r)
+                Vb = [r / beta]; Hm = np.zeros((restart + 1, restart)); g = np.zeros(restart + 1); g[0] = beta
+                cs = np.zeros(restart); sn = np.zeros(restart); j = 0; done = False
+                while j < restart and it < max_iter:
+                    w = op(Vb[j])
+                    for i in range(j + 1):
+                        Hm[i, j] = Vb[i] @ w; w = w - Hm[i, j] * Vb[i]
+                    Hm[j + 1, j] = np.linalg.norm(w); Vb.append(w / Hm[j + 1, j] if Hm[j + 1, j] > 0 else 0 * w)
+                    for i in range(j):
+                        a, c = Hm[i, j], Hm[i + 1, j]
+                        Hm[i, j], Hm[i + 1, j] = cs[i] * a + sn[i] * c, -sn[i] * a + cs[i] * c
+                    d = np.hypot(Hm[j, j], Hm[j + 1, j]); cs[j], sn[j] = Hm[j, j] / d, Hm[j + 1, j] / d
+                    Hm[j, j] = d; Hm[j + 1, j] = 0; g[j + 1] = -sn[j] * g[j]; g[j] = cs[j] * g[j]
+                    j += 1; it += 1
+                    if abs(g[j]) <= tol * beta0:
+                        done = True
+                        break
+                y = np.linalg.solve(np.triu(Hm[:j, :j]), g[:j])
+                x = x + np.array(Vb[:j]).T @ y
+                if done:
+                    break
+                Z = np.zeros_like(B); Z[:, k] = x
+                r = pc(b - self.A(Z)[:, k])
+            X[:, k] = x
+            its = max(its, it)
+        return X, its
