@@ -8,10 +8,13 @@ near theta 10, leaf size 16.
 
 One step = TangentPointEnergy0 energy + differential (near and far kernels, derivative
 collection, chain rule, vertex assembly) with all inputs resident in HBM and the output in a
-device buffer, followed by the metric solve when enabled. Tree/BCT construction, uploads and
-metric assembly are timed separately and reported, never folded into the step.
+device buffer, followed by the metric solve A X = dE (TangentPointMetric0::Solve, three
+columns, preconditioner P0 = H^-1 L_{2-s} H^-1, stopped at |dE - A X|_2 / |dE|_2 <= 1e-6 per
+column) from and into device buffers. Tree/BCT construction, uploads, the per-state metric and
+pseudo-Laplacian assembly (device) and the H^1 factorisation (host) are timed separately and
+reported, never folded into the step.
 
-Prints one JSON line (see README contract): value = ms per step (lower is better).
+Prints one JSON line: value = ms per step (lower is better).
 """
 
 from __future__ import annotations
@@ -118,6 +121,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-pairs", type=float, default=1.2e8)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run only the copy calibration kernel (MiB)")
+    ap.add_argument("--solver", default="pcg", choices=["pcg", "gmres"])
+    ap.add_argument("--max-iter", type=int, default=3000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-solve", action="store_true", help="time energy+grad only")
+    ap.add_argument("--pc-weight", type=float, default=-1.0,
+                    help="diagonal weight w of P = P0 + w diag(A)^-1 (0: reference P0, <0: estimated per state)")
+    ap.add_argument("--plain-pc-max-iter", type=int, default=4000,
+                    help="iteration cap of the one reported solve with the reference P0 (0 skips it)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -144,42 +155,91 @@ def main():
 
     E = R.TangentPointEnergy0(wl.q, wl.p, precision=args.precision)
     A = R.TangentPointMetric0(wl.q, wl.p, precision=args.precision)
+    A.SetPreconditionerDiagonalWeight(args.pc_weight)
 
     hip = _hip()
     Yd = C.c_void_p()
     assert hip.hipMalloc(C.byref(Yd), M.V * 3 * 8) == 0
+    Xd = C.c_void_p()
+    assert hip.hipMalloc(C.byref(Xd), M.V * 3 * 8) == 0
     en = np.zeros(1)
-    from repulsor_amd import H, _p, _check  # noqa: E402
+    from repulsor_amd import H, _p, _check, RP_ESTAGNATION  # noqa: E402
 
-    def step():
-        _check(H.rpm_energy_differential(E._h, M._h, 1.0, 0.0, C.cast(Yd, C.POINTER(C.c_double)), 3, _p(en), 1))
-
-    for _ in range(args.warmup):
-        step()
-    nf_ms, ff_ms, eg_ms = [], [], []
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        nf_ms.append(M.last_timing(4)); ff_ms.append(M.last_timing(5))
-        eg_ms.append(M.last_timing(0) + M.last_timing(1))
-    hip.hipDeviceSynchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    print(f"# energy {en[0]:.15e}  step {ms_per_step:.3f} ms (device {np.mean(eg_ms):.3f} ms: near {np.mean(nf_ms):.3f}, far {np.mean(ff_ms):.3f})",
-          file=sys.stderr, flush=True)
-
-    # metric assembly and MultiplyMetric (reported, not part of the step until the solve lands)
+    # per-state preparation, timed separately: metric blocks and pseudo-Laplacian on the device,
+    # H^1 factorisation on the host (+ upload)
     t0 = time.perf_counter()
     A.assemble(M)
     t_assemble = time.perf_counter() - t0
+    metric_assembly_dev_ms = M.last_timing(0)
+    t0 = time.perf_counter()
+    Z0 = np.zeros((M.V, 3))
+    A.MultiplyPreconditioner(M, 1.0, np.ones((M.V, 3)), 0.0, Z0)
+    t_pc_prep = time.perf_counter() - t0
+    pl_assembly_dev_ms = M.last_timing(9)
+    pc_setup_dev_ms = M.last_timing(10)
+    pc_weight = A.PreconditionerWeightInUse(M)
+    print(f"# metric assembly {metric_assembly_dev_ms:.2f} ms (device), pseudo-Laplacian {pl_assembly_dev_ms:.2f} ms (device), "
+          f"preconditioner setup {pc_setup_dev_ms:.2f} ms (device, weight {pc_weight:.4g}), "
+          f"H1 assembly+factor+upload+setup {t_pc_prep:.2f} s (wall)", file=sys.stderr, flush=True)
+
+    it_c = (C.c_int * 1)()
+    rel = np.zeros(3)
+    solve_flags = 0 if args.solver == "pcg" else R.RP_SOLVE_GMRES
+    dY = C.cast(Yd, C.POINTER(C.c_double))
+    dX = C.cast(Xd, C.POINTER(C.c_double))
+
+    def step():
+        _check(H.rpm_energy_differential(E._h, M._h, 1.0, 0.0, dY, 3, _p(en), 1))
+        t_eg = M.last_timing(0) + M.last_timing(1)
+        tn, tf = M.last_timing(4), M.last_timing(5)
+        if args.no_solve:
+            return t_eg, tn, tf, 0.0
+        st = H.rpm_metric_solve(A._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, args.max_iter, args.tol, solve_flags, 1, it_c, _p(rel))
+        if st not in (0, RP_ESTAGNATION):
+            _check(st)
+        return t_eg, tn, tf, M.last_timing(3)
+
+    for _ in range(args.warmup):
+        step()
+    nf_ms, ff_ms, eg_ms, sv_ms = [], [], [], []
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t_eg, tn, tf, ts = step()
+        nf_ms.append(tn); ff_ms.append(tf); eg_ms.append(t_eg); sv_ms.append(ts)
+        print(f"# step: energy+grad {t_eg:.3f} ms, solve {ts:.1f} ms, {it_c[0]} iterations, rel {rel.max():.2e}",
+              file=sys.stderr, flush=True)
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    solve_iters = int(it_c[0])
+    print(f"# energy {en[0]:.15e}  step {ms_per_step:.3f} ms (device energy+grad {np.mean(eg_ms):.3f} ms: near {np.mean(nf_ms):.3f}, "
+          f"far {np.mean(ff_ms):.3f}; solve {np.mean(sv_ms):.1f} ms in {solve_iters} iterations)", file=sys.stderr, flush=True)
+    # operator timings (device events; host staging excluded)
     X = np.random.default_rng(11).standard_normal((M.V, 3))
     Y = np.zeros_like(X)
     A.MultiplyMetric(M, 1.0, X, 0.0, Y)
-    mm_ms, spmm_ms = [], []
+    mm_ms, spmm_ms, pcl_ms, h1l_ms = [], [], [], []
     for _ in range(max(3, args.steps)):
         A.MultiplyMetric(M, 1.0, X, 0.0, Y)
         mm_ms.append(M.last_timing(2)); spmm_ms.append(M.last_timing(6))
+        A.MultiplyPreconditioner(M, 1.0, X, 0.0, Y)
+        pcl_ms.append(M.last_timing(7))
+        M.h1_solve(X)
+        h1l_ms.append(M.last_timing(8))
+    pc_ms, h1_ms = float(np.mean(pcl_ms)), float(np.mean(h1l_ms))
+
+    # one solve with the reference preconditioner P0 alone, for comparison (capped to about a minute)
+    plain = None
+    if not args.no_solve and args.plain_pc_max_iter > 0:
+        A0 = R.TangentPointMetric0(wl.q, wl.p, precision=args.precision)
+        A0.SetPreconditionerDiagonalWeight(0.0)
+        per_it = float(np.mean(mm_ms)) + pc_ms
+        cap = int(max(10, min(args.plain_pc_max_iter, 60000.0 / max(per_it, 1e-3))))
+        st = H.rpm_metric_solve(A0._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, cap, args.tol, solve_flags, 1, it_c, _p(rel))
+        plain = {"solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]), "converged": st == 0,
+                 "rel_residuals": [float(x) for x in rel], "iteration_cap": cap}
+        print(f"# reference P0 alone: {plain}", file=sys.stderr, flush=True)
 
     pairs_nf = b["near_nnz"] // 2
     pairs_ff = b["far_nnz"] // 2
@@ -206,17 +266,31 @@ def main():
             "theta_far": wl.far_theta, "theta_near": wl.near_theta, "leaf_size": wl.leaf,
             "near_nnz": b["near_nnz"], "far_nnz": b["far_nnz"], "very_near_nnz": b["verynear_nnz"],
             "precision": args.precision,
-            "step": "energy + differential (device-resident); metric solve not yet in the step",
+            "step": ("energy + differential (device-resident) + metric solve to |B - A X|/|B| <= "
+                     f"{args.tol:g} per column with P0 = H^-1 L H^-1 ({args.solver})") if not args.no_solve
+                    else "energy + differential (device-resident)",
+            "preconditioner": ("P0 + w diag(A)^-1" if args.pc_weight != 0 else "reference P0"),
         },
         "energy": float(en[0]),
         "energy_grad_ms": round(float(np.mean(eg_ms)), 3),
         "near_kernel_ms": round(float(np.mean(nf_ms)), 3),
         "far_kernel_ms": round(float(np.mean(ff_ms)), 3),
-        "metric_assembly_ms": round(t_assemble * 1e3, 2),
+        "metric_assembly_ms": round(metric_assembly_dev_ms, 3),
+        "metric_assembly_wall_s": round(t_assemble, 3),
+        "pl_assembly_ms": round(pl_assembly_dev_ms, 3),
+        "h1_factor_upload_s": round(t_pc_prep, 2),
         "multiply_metric_ms": round(float(np.mean(mm_ms)), 3),
         "near_spmm_ms": round(float(np.mean(spmm_ms)), 3),
         "near_spmm_GBs_algorithmic": round(spmm_bytes / (float(np.mean(spmm_ms)) * 1e-3) / 1e9, 1),
-        "solve_ms": None,
+        "multiply_preconditioner_ms": round(pc_ms, 3),
+        "h1_solve_ms": round(h1_ms, 3),
+        "solve_ms": None if args.no_solve else round(float(np.mean(sv_ms)), 2),
+        "solve_iterations": None if args.no_solve else solve_iters,
+        "solve_rel_residuals": None if args.no_solve else [float(x) for x in rel],
+        "solver": args.solver,
+        "pc_weight": pc_weight,
+        "pc_setup_ms": round(pc_setup_dev_ms, 3),
+        "solve_reference_p0": plain,
         "tree_bct_build_s": round(t_build, 2),
         "upload_s": round(t_upload, 2),
         "pairs_per_s": round((pairs_nf + pairs_ff) / (float(np.mean(eg_ms)) * 1e-3), 1),
@@ -237,6 +311,7 @@ def main():
         except Exception as e:  # reported, not fatal
             result["cpu_baseline"] = {"error": str(e)}
     hip.hipFree(Yd)
+    hip.hipFree(Xd)
     print(json.dumps(result), flush=True)
 
 

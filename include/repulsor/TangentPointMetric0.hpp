@@ -185,12 +185,40 @@ namespace Repulsor
         mutable int iter = 0;
         mutable std::vector<double> rel_residuals;
 
+        double pc_diagonal_weight = 0.0;
+
         bool PreparePreconditioner( const Mesh_T & M ) const
         {
             try { M.DeviceH1(); }
             catch( const std::exception & e ) { eprint( ClassName() + "::MultiplyPreconditioner: " + e.what() ); return false; }
-            return PseudoLaplacianValues( M );
+            if( !PseudoLaplacianValues( M ) ) { return false; }
+            if( pc_diagonal_weight != 0.0 && !MetricValues( M ) ) { return false; }
+            return Check( rp_set_preconditioner( M.Device(), pc_diagonal_weight ), ClassName() + "::MultiplyPreconditioner" );
         }
+
+    public:
+        // Preconditioner P = P0 + w diag(A)^-1 (w = 0: the reference's P0; w < 0: w estimated per state
+        // as lambda_max(P0 A) / 4). The diagonal term has no counterpart in the reference: P0 alone
+        // under-weights the roughest modes, so its condition number grows under mesh refinement.
+        void SetPreconditionerDiagonalWeight( const double w ) { pc_diagonal_weight = w; }
+        double PreconditionerDiagonalWeight() const { return pc_diagonal_weight; }
+
+        // Weight in use on the mesh's device context (after estimation).
+        double PreconditionerWeightInUse( const Mesh_T & M ) const
+        {
+            double w = 0;
+            Check( rp_preconditioner_weight( M.Device(), &w ), ClassName() + "::PreconditionerWeightInUse" );
+            return w;
+        }
+
+        // Exact diagonal of the assembled metric (V values).
+        bool MetricDiagonal( const Mesh_T & M, double * diag ) const
+        {
+            if( !MetricValues( M ) ) { return false; }
+            return Check( rp_download_metric_diagonal( M.Device(), diag ), ClassName() + "::MetricDiagonal" );
+        }
+
+    protected:
 
         double QuadraticNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs, const bool pc ) const
         {

@@ -146,6 +146,21 @@ int rp_h1_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, doub
 int rp_multiply_preconditioner( rp_ctx * ctx, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
                                 int nrhs, int ptr_flags );
 
+/* Preconditioner used by rp_multiply_preconditioner and rp_solve: P = P0 + w diag(A)^-1.
+ * w = 0 (default): the reference preconditioner P0 alone. w > 0: fixed weight. w < 0: w is estimated
+ * once per metric/factor state as lambda_max(P0 A) / 4 (power iteration, timed as slot 10).
+ * The diagonal term is an addition of this library: P0 under-weights the roughest modes, so the
+ * condition number of P0 A grows under refinement; the additive diagonal (Jacobi) term bounds it.
+ * No reference counterpart. */
+int rp_set_preconditioner( rp_ctx * ctx, double diagonal_weight );
+
+/* Weight w in use (after estimation). */
+int rp_preconditioner_weight( rp_ctx * ctx, double * weight );
+
+/* Exact diagonal of the assembled metric A (V values; A acts identically on the three columns).
+ * No reference counterpart (used by the diagonal term of the preconditioner). */
+int rp_download_metric_diagonal( rp_ctx * ctx, double * diag );
+
 /* rp_solve flags */
 #define RP_SOLVE_NO_PRECONDITIONER 1
 #define RP_SOLVE_PC_NORM           2
@@ -168,7 +183,7 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
 /* Device timing of the last call of each entry point, in milliseconds (HIP events on the
  * context's stream). which: 0 = tp0_compute, 1 = assemble_vertex_gradient, 2 = multiply_metric,
  * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM, 7 = multiply_preconditioner,
- * 8 = h1_solve, 9 = pl_compute. */
+ * 8 = h1_solve, 9 = pl_compute, 10 = preconditioner setup (diagonal, weight estimate). */
 int rp_last_timing( rp_ctx * ctx, int which, double * ms );
 
 #ifdef __cplusplus

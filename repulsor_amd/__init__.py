@@ -97,6 +97,9 @@ _host_sigs = {
     "rpm_pl_values": (C.c_int, [_vp, _vp, _d, _d, _d, _d]),
     "rpm_pl_multiply": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_h1_solve_device": (C.c_int, [_vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
+    "rpm_op_set_pc_weight": (None, [_vp, C.c_double]),
+    "rpm_pc_weight": (C.c_int, [_vp, _vp, _d]),
+    "rpm_metric_diagonal": (C.c_int, [_vp, _vp, _d]),
 }
 H = type("H", (), {})()
 for _n, (_r, _a) in _host_sigs.items():
@@ -424,6 +427,24 @@ class TangentPointMetric0(_Op):
         assert Y.dtype == np.float64 and Y.flags.c_contiguous
         _check(H.rpm_metric_precondition(self._h, M._h, alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], nrhs, 0))
         return Y
+
+    def SetPreconditionerDiagonalWeight(self, w):
+        """P = P0 + w diag(A)^-1: w = 0 the reference P0 (default), w > 0 fixed, w < 0 estimated per state
+        as lambda_max(P0 A) / 4. The diagonal term has no counterpart in the reference."""
+        self.pc_weight = float(w)
+        H.rpm_op_set_pc_weight(self._h, self.pc_weight)
+
+    def PreconditionerWeightInUse(self, M: SimplicialMesh):
+        w = np.zeros(1)
+        _check(H.rpm_pc_weight(self._h, M._h, _p(w)))
+        return float(w[0])
+
+    def MetricDiagonal(self, M: SimplicialMesh):
+        """Exact diagonal of the assembled metric (V values, the same for the three columns)."""
+        M.build()
+        d = np.zeros(M.V)
+        _check(H.rpm_metric_diagonal(self._h, M._h, _p(d)))
+        return d
 
     def PreconditionerNorm(self, M: SimplicialMesh, X):
         Y = np.zeros_like(X)

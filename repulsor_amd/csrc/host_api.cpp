@@ -23,6 +23,7 @@ namespace
     {
         double q = 6, p = 12;
         int precision = 0;
+        double pc_weight = 0.0;
     };
 
     template<typename F>
@@ -48,6 +49,7 @@ namespace
     {
         TangentPointMetric0<Mesh> A( o->q, o->p );
         A.SetPrecision( (Precision)o->precision );
+        A.SetPreconditionerDiagonalWeight( o->pc_weight );
         return A;
     }
 
@@ -228,6 +230,30 @@ void * rpm_op_new( double q, double p, int precision )
 }
 
 void rpm_op_free( void * o ) { delete (OpHandle *)o; }
+
+// Diagonal weight of the metric's preconditioner P = P0 + w diag(A)^-1 (0: reference P0, < 0: estimated).
+void rpm_op_set_pc_weight( void * o, double w ) { ((OpHandle *)o)->pc_weight = w; }
+
+// Weight in use on the mesh's device (after MultiplyPreconditioner / Solve).
+int rpm_pc_weight( void * op, void * h, double * w )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        *w = make_metric<Mesh>( (OpHandle *)op ).PreconditionerWeightInUse( M );
+        return RP_OK;
+    });
+}
+
+// Exact diagonal of the assembled metric (V values).
+int rpm_metric_diagonal( void * op, void * h, double * diag )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        return make_metric<Mesh>( (OpHandle *)op ).MetricDiagonal( M, diag ) ? RP_OK : RP_EINVAL;
+    });
+}
 
 int rpm_energy_value( void * op, void * h, double * out )
 {

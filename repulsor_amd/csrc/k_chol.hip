@@ -67,6 +67,7 @@ namespace rp
         for( int a = tid; a < w; a += BS )
         {
             double acc[3] = { 0.0, 0.0, 0.0 };
+            #pragma unroll 8
             for( int c = 0; c <= a; ++c )
             {
                 const double l = Ls[(int64_t)c * r + a];
@@ -78,6 +79,7 @@ namespace rp
         for( int q = w + tid; q < r; q += BS )
         {
             double acc[3] = { z[3 * q], z[3 * q + 1], z[3 * q + 2] };
+            #pragma unroll 8
             for( int c = 0; c < w; ++c )
             {
                 const double l = Ls[(int64_t)c * r + q];
@@ -134,6 +136,7 @@ namespace rp
                     if( c < w )
                     {
                         const double * col = Ls + (int64_t)c * r + w + q0;
+                        #pragma unroll 8
                         for( int q = gl; q < qn; q += G1 )
                         {
                             const double l = col[q];
@@ -158,6 +161,7 @@ namespace rp
                 if( c < w )
                 {
                     const double * col = Ls + (int64_t)c * r;
+                    #pragma unroll 8
                     for( int a = c + gl; a < w; a += G2 )
                     {
                         const double l = col[a];
@@ -168,6 +172,149 @@ namespace rp
                 if( c < w && gl == 0 ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)(f + c) + k] = acc[k]; } }
             }
         }
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // Split form for levels with large supernodes: the work of one supernode is spread over many
+    // workgroups through task lists (int2 = supernode, first row or column).
+    //   forward  1: one workgroup per supernode: init, merge the children, y = inv(L_ss) z[0:w]
+    //   forward  2: one workgroup per 256 update rows: z[q] -= L(q, :) y
+    //   backward 1: one workgroup per group of columns: t[c] = y[c] - L(:, c)^T x(R[w:]) (in place)
+    //   backward 2: one workgroup per group of columns: x[f + c] = inv(L_ss)(:, c)^T t
+    constexpr int CHOL_SPLIT_BS = 256;
+
+    template<int BS>
+    __global__ __launch_bounds__(BS) void k_chol_fwd1( CholDev F, const int32_t * nodes, double alpha, const double * B, int64_t ldB )
+    {
+        extern __shared__ double sh[];
+        const int s = nodes[blockIdx.x];
+        const int f = F.sn_begin[s];
+        const int w = F.sn_begin[s + 1] - f;
+        const int64_t rb = F.row_ptr[s];
+        const int r = (int)( F.row_ptr[s + 1] - rb );
+        double * z = F.z + 3 * rb;
+        const double * Ls = F.vals + F.val_ptr[s];
+        double * zs = sh;            // 3 w
+        const int tid = threadIdx.x;
+        for( int a = tid; a < w; a += BS )
+        {
+            const int64_t o = F.perm[f + a];
+            for( int k = 0; k < 3; ++k ) { zs[3 * a + k] = alpha * B[ldB * o + k]; }
+        }
+        for( int q = w + tid; q < r; q += BS ) { for( int k = 0; k < 3; ++k ) { z[3 * q + k] = 0.0; } }
+        __syncthreads();
+        for( int kk = F.kid_ptr[s]; kk < F.kid_ptr[s + 1]; ++kk )
+        {
+            const int c = F.kids[kk];
+            const int wc = F.sn_begin[c + 1] - F.sn_begin[c];
+            const int64_t rbc = F.row_ptr[c];
+            const int rc = (int)( F.row_ptr[c + 1] - rbc );
+            const double * zc = F.z + 3 * rbc;
+            for( int q = wc + tid; q < rc; q += BS )
+            {
+                const int pos = F.cmap[rbc + q];
+                if( pos < w ) { for( int k = 0; k < 3; ++k ) { zs[3 * pos + k] += zc[3 * q + k]; } }
+                else          { for( int k = 0; k < 3; ++k ) { z[3 * pos + k]  += zc[3 * q + k]; } }
+            }
+            __syncthreads();
+        }
+        for( int a = tid; a < w; a += BS )
+        {
+            double acc[3] = { 0.0, 0.0, 0.0 };
+            #pragma unroll 8
+            for( int c = 0; c <= a; ++c )
+            {
+                const double l = Ls[(int64_t)c * r + a];
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * zs[3 * c + k]; }
+            }
+            for( int k = 0; k < 3; ++k ) { z[3 * a + k] = acc[k]; }
+        }
+    }
+
+    __global__ __launch_bounds__(CHOL_SPLIT_BS) void k_chol_fwd2( CholDev F, const int2 * tasks )
+    {
+        extern __shared__ double sh[];
+        const int2 tk = tasks[blockIdx.x];
+        const int s = tk.x;
+        const int w = F.sn_begin[s + 1] - F.sn_begin[s];
+        const int64_t rb = F.row_ptr[s];
+        const int r = (int)( F.row_ptr[s + 1] - rb );
+        double * z = F.z + 3 * rb;
+        const double * Ls = F.vals + F.val_ptr[s];
+        double * ys = sh;   // 3 w
+        for( int a = threadIdx.x; a < 3 * w; a += CHOL_SPLIT_BS ) { ys[a] = z[a]; }
+        __syncthreads();
+        const int q = tk.y + threadIdx.x;
+        if( q >= r ) { return; }
+        double acc[3] = { z[3 * q], z[3 * q + 1], z[3 * q + 2] };
+        #pragma unroll 8
+        for( int c = 0; c < w; ++c )
+        {
+            const double l = Ls[(int64_t)c * r + q];
+            for( int k = 0; k < 3; ++k ) { acc[k] -= l * ys[3 * c + k]; }
+        }
+        for( int k = 0; k < 3; ++k ) { z[3 * q + k] = acc[k]; }
+    }
+
+    // group size for reductions of length n: a power of two, at most 64
+    __device__ __forceinline__ int chol_group( int n ) { return pow2_at_least( n < 64 ? n : 64 ); }
+
+    __global__ __launch_bounds__(CHOL_SPLIT_BS) void k_chol_bwd1( CholDev F, const int2 * tasks )
+    {
+        const int2 tk = tasks[blockIdx.x];
+        const int s = tk.x;
+        const int w = F.sn_begin[s + 1] - F.sn_begin[s];
+        const int64_t rb = F.row_ptr[s];
+        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const int u = r - w;
+        double * z = F.z + 3 * rb;
+        const int32_t * R = F.rows + rb;
+        const double * Ls = F.vals + F.val_ptr[s];
+        const int G = chol_group( u );
+        const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+        const int c = tk.y + grp;
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        if( c < w )
+        {
+            const double * col = Ls + (int64_t)c * r;
+            #pragma unroll 8
+            for( int q = w + gl; q < r; q += G )
+            {
+                const double l = col[q];
+                const int64_t j = R[q];
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * F.xp[3 * j + k]; }
+            }
+        }
+        for( int k = 0; k < 3; ++k ) { acc[k] = group_sum( acc[k], G ); }
+        if( c < w && gl == 0 ) { for( int k = 0; k < 3; ++k ) { z[3 * c + k] -= acc[k]; } }
+    }
+
+    __global__ __launch_bounds__(CHOL_SPLIT_BS) void k_chol_bwd2( CholDev F, const int2 * tasks )
+    {
+        const int2 tk = tasks[blockIdx.x];
+        const int s = tk.x;
+        const int f = F.sn_begin[s];
+        const int w = F.sn_begin[s + 1] - f;
+        const int64_t rb = F.row_ptr[s];
+        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const double * z = F.z + 3 * rb;
+        const double * Ls = F.vals + F.val_ptr[s];
+        const int G = chol_group( w );
+        const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+        const int c = tk.y + grp;
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        if( c < w )
+        {
+            const double * col = Ls + (int64_t)c * r;
+            #pragma unroll 8
+            for( int a = c + gl; a < w; a += G )
+            {
+                const double l = col[a];
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * z[3 * a + k]; }
+            }
+        }
+        for( int k = 0; k < 3; ++k ) { acc[k] = group_sum( acc[k], G ); }
+        if( c < w && gl == 0 ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)(f + c) + k] = acc[k]; } }
     }
 
     __global__ void k_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY )
@@ -197,6 +344,23 @@ namespace rp
         const size_t shm = sizeof(double) * ( 3 * (size_t)wmax + 3 * (size_t)CHOL_QC );
         if( big ) { hipLaunchKernelGGL( (k_chol_bwd<256>), dim3((unsigned)count), dim3(256), shm, s, F, nodes ); }
         else      { hipLaunchKernelGGL( (k_chol_bwd<64>),  dim3((unsigned)count), dim3(64),  shm, s, F, nodes ); }
+    }
+
+    void launch_chol_fwd1_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax,
+                                 double alpha, const double * B, int64_t ldB, hipStream_t s )
+    {
+        if( count <= 0 ) { return; }
+        const size_t shm = sizeof(double) * 3 * (size_t)wmax;
+        hipLaunchKernelGGL( (k_chol_fwd1<CHOL_SPLIT_BS>), dim3((unsigned)count), dim3(CHOL_SPLIT_BS), shm, s, F, nodes, alpha, B, ldB );
+    }
+
+    void launch_chol_tasks( int kind, const CholDev & F, const int2 * tasks, int64_t count, int wmax, hipStream_t s )
+    {
+        if( count <= 0 ) { return; }
+        const dim3 grid( (unsigned)count ), block( CHOL_SPLIT_BS );
+        if( kind == 0 )      { hipLaunchKernelGGL( k_chol_fwd2, grid, block, sizeof(double) * 3 * (size_t)wmax, s, F, tasks ); }
+        else if( kind == 1 ) { hipLaunchKernelGGL( k_chol_bwd1, grid, block, 0, s, F, tasks ); }
+        else                 { hipLaunchKernelGGL( k_chol_bwd2, grid, block, 0, s, F, tasks ); }
     }
 
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s )

@@ -403,7 +403,7 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         finish_timing( ctx, 0 ); finish_timing( ctx, 4 ); finish_timing( ctx, 5 );
         if( energy_out ) { *energy_out = E ? en : 0.0; }
         if( D ) { ctx->have_diff = true; }
-        if( M ) { ctx->have_metric = true; ctx->metric_precision = precision; }
+        if( M ) { ctx->have_metric = true; ctx->metric_precision = precision; ++ctx->metric_version; }
     });
 }
 

@@ -203,12 +203,23 @@ struct rp_ctx
     std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
     std::vector<int>     h1_level_wmax;
     std::vector<int>     h1_level_rmax;
+    std::vector<char>    h1_level_split;           // levels run in the split (task) form
+    rp::DBuf<int2>       h1_tasks;                 // forward-2, backward-1, backward-2 tasks of split levels
+    std::vector<int64_t> h1_task_ptr[3];           // per kind: level offsets into h1_tasks
     int64_t h1_n = 0, h1_ns = 0;
     bool have_h1 = false;
 
     // Krylov solver state
     rp::DBuf<double> cg_x, cg_r, cg_p, cg_Ap, cg_z, cg_b, pc_tmp, cg_scal, cg_part;
     rp::DBuf<double> gm_V, gm_w, gm_t, gm_small, gm_part;   // GMRES basis (m+1) x 3V and work
+
+    // two-level preconditioner P0 + w diag(A)^-1: exact metric diagonal and the weight
+    rp::DBuf<double> dA, dA_path;          // V, 2C
+    int64_t metric_version = 0, pl_version = 0, h1_version = 0;
+    int64_t diag_version = -1;
+    double pc_weight_setting = 0.0;        // 0: P0 only, > 0: fixed weight, < 0: estimated per state
+    double pc_weight = 0.0;                // weight in use
+    int64_t pc_weight_key[3] = { -1, -1, -1 };
     int solve_iters = 0;
     std::vector<double> solve_relres;
 

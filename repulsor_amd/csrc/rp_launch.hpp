@@ -97,6 +97,11 @@ namespace rp
                                 double alpha, const double * B, int64_t ldB, hipStream_t s );
     void launch_chol_bwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big, hipStream_t s );
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
+    // split form for levels with large supernodes; tasks are (supernode, first row/column) pairs
+    constexpr int CHOL_TASK_ROWS = 256;   // rows per forward-2 task (= threads per workgroup)
+    void launch_chol_fwd1_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax,
+                                 double alpha, const double * B, int64_t ldB, hipStream_t s );
+    void launch_chol_tasks( int kind, const CholDev & F, const int2 * tasks, int64_t count, int wmax, hipStream_t s );
 
     // Krylov vector kernels (k_solve.hip)
     void launch_dot3( int64_t n, const double * u, const double * v, double * part, double * out, hipStream_t s );   // part: 3 x 1024
@@ -106,6 +111,33 @@ namespace rp
     void launch_cg_check( const double * rr, const double * bb, double tol2, double * active, hipStream_t s );
     void launch_copy3( int64_t n, const double * src, int64_t lds, double * dst, int64_t ldd, hipStream_t s );
     void launch_axpby3( int64_t n, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY, hipStream_t s );
+    // exact diagonal of the metric (k_diag.hip)
+    struct DiagArgs
+    {
+        int64_t V, C;
+        int size;
+        int have_near, use_far;
+        const int64_t  * vp_ptr;      // vertex -> (primitive, slot) entries
+        const int32_t  * vp_entry;
+        const double   * pre_vals;    // T x 4 x size
+        const int32_t  * leaf_of_prim;
+        const int32_t  * leaf_begin;
+        const int32_t  * leaf_end;
+        const int64_t  * col_ptr;
+        const int32_t  * col_prim;
+        const uint32_t * col_mask;
+        const int64_t  * val_ptr;
+        const double   * nf_diag;     // 2T
+        const int32_t  * leaf_cluster;
+        const int32_t  * C_parent;
+        const int32_t  * C_begin;
+        const int32_t  * C_end;
+        const double   * ff_diag;     // 2C
+        const double   * S;           // 2C root-path sums of ff_diag (set by launch_metric_diag)
+    };
+    void launch_metric_diag( bool mixed, const DiagArgs & A, const void * nf_vals, double * S, double * dA, hipStream_t s );
+    void launch_jacobi_add( int64_t V, double c, const double * X, int64_t ldX, const double * dA, double * Y, int64_t ldY, hipStream_t s );
+
     constexpr int MDOT_PART = 256 * 3;   // partial-sum doubles per basis vector of launch_mdot
     void launch_mdot( int64_t n, const double * Vb, int64_t stride, int nv, const double * w, double * part, double * out, int accumulate, hipStream_t s );
     void launch_maxpy( int64_t n, const double * Vb, int64_t stride, int nv, const double * coef, double sign, double * w, hipStream_t s );

@@ -63,23 +63,41 @@ namespace rp
     {
         const CholDev F = chol_dev( ctx );
         const size_t nl = ctx->h1_level_wmax.size();
+        const int2 * tasks = ctx->h1_tasks.get();
+        auto task_range = [&]( int kind, size_t l, int64_t & b, int64_t & e ) { b = ctx->h1_task_ptr[kind][l]; e = ctx->h1_task_ptr[kind][l + 1]; };
         for( size_t l = 0; l < nl; ++l )
         {
             const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-            launch_chol_fwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], ctx->h1_level_rmax[l] > 96,
-                                   alpha, Bd, ldb, ctx->stream );
+            if( !ctx->h1_level_split[l] )
+            {
+                launch_chol_fwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], false, alpha, Bd, ldb, ctx->stream );
+                continue;
+            }
+            launch_chol_fwd1_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], alpha, Bd, ldb, ctx->stream );
+            int64_t tb, te;
+            task_range( 0, l, tb, te );
+            launch_chol_tasks( 0, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
         }
         for( size_t l = nl; l-- > 0; )
         {
             const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-            launch_chol_bwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], ctx->h1_level_rmax[l] > 96, ctx->stream );
+            if( !ctx->h1_level_split[l] )
+            {
+                launch_chol_bwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], false, ctx->stream );
+                continue;
+            }
+            int64_t tb, te;
+            task_range( 1, l, tb, te );
+            launch_chol_tasks( 1, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
+            task_range( 2, l, tb, te );
+            launch_chol_tasks( 2, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
         }
         launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream );
     }
 
     // Y = alpha P0 X + beta Y with P0 = H^-1 L_{2-s} H^-1.
     // Reference: TangentPointMetric0::multiply_preconditioner (src/Metrics/TangentPointMetric0.hpp:88-126).
-    static void pc_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    static void pc0_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
     {
         const int64_t V = ctx->V;
         ctx->pc_tmp.require( (size_t)(3 * V) );
@@ -87,6 +105,77 @@ namespace rp
         h1_solve( ctx, alpha, Xd, ldx, 0.0, Z, 3 );
         if( alpha != 0.0 ) { pl_apply( ctx, 1.0, Z, 3, 0.0, Z, 3, true, true ); }
         h1_solve( ctx, 1.0, Z, 3, beta, Yd, ldy );
+    }
+
+    // Exact diagonal of the assembled metric (cached per assembly).
+    static void ensure_metric_diag( rp_ctx * ctx )
+    {
+        if( ctx->diag_version == ctx->metric_version ) { return; }
+        ctx->dA.require( (size_t)ctx->V );
+        ctx->dA_path.require( (size_t)(2 * ctx->C) );
+        DiagArgs A{};
+        A.V = ctx->V; A.C = ctx->C; A.size = ctx->size;
+        A.have_near = ctx->nf_cols > 0; A.use_far = ctx->far_nnz > 0;
+        A.vp_ptr = ctx->vp_ptr.get(); A.vp_entry = ctx->vp_entry.get(); A.pre_vals = ctx->pre_vals.get();
+        A.leaf_of_prim = ctx->leaf_of_prim.get(); A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
+        A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
+        A.nf_diag = ctx->nf_diag.get(); A.leaf_cluster = ctx->leaf_cluster.get(); A.C_parent = ctx->C_parent.get();
+        A.C_begin = ctx->C_begin.get(); A.C_end = ctx->C_end.get(); A.ff_diag = ctx->ff_diag.get();
+        launch_metric_diag( ctx->metric_precision == RP_MIXED, A, ctx->nf_vals.get(), ctx->dA_path.get(), ctx->dA.get(), ctx->stream );
+        RP_HIP( hipGetLastError() );
+        ctx->diag_version = ctx->metric_version;
+    }
+
+    // Weight w of the diagonal term. A negative setting estimates it per state as lambda_max(P0 A) / 4
+    // by power iteration from the (smooth) vertex coordinates: the top of the spectrum of P0 A belongs to
+    // smooth modes, while the rough modes that P0 under-weights get w diag(A)^-1 A ~ w.
+    static constexpr int PC_POWER_STEPS = 12;
+    static void prepare_pc( rp_ctx * ctx )
+    {
+        if( ctx->pc_weight_setting >= 0.0 ) { ctx->pc_weight = ctx->pc_weight_setting; }
+        else
+        {
+            const int64_t key[3] = { ctx->metric_version, ctx->pl_version, ctx->h1_version };
+            if( !( key[0] == ctx->pc_weight_key[0] && key[1] == ctx->pc_weight_key[1] && key[2] == ctx->pc_weight_key[2] ) )
+            {
+                const int64_t V = ctx->V;
+                const size_t n3 = (size_t)(3 * V);
+                ctx->cg_p.require( n3 ); ctx->cg_Ap.require( n3 ); ctx->cg_z.require( n3 );
+                ctx->cg_scal.require( 32 ); ctx->cg_part.require( 3 * 1024 );
+                double * v = ctx->cg_p.get(); double * t = ctx->cg_Ap.get(); double * u = ctx->cg_z.get();
+                double * sc = ctx->cg_scal.get() + 24; double * part = ctx->cg_part.get();
+                hipStream_t s = ctx->stream;
+                launch_copy3( V, ctx->coords.get(), 3, v, 3, s );
+                double lam = 0.0;
+                for( int k = 0; k < PC_POWER_STEPS; ++k )
+                {
+                    double nv[3], nu[3];
+                    launch_dot3( V, v, v, part, sc, s );
+                    RP_HIP( hipMemcpyAsync( nv, sc, sizeof(nv), hipMemcpyDeviceToHost, s ) );
+                    metric_apply( ctx, 1.0, v, 3, 0.0, t, 3, true, true, false );
+                    pc0_apply( ctx, 1.0, t, 3, 0.0, u, 3 );
+                    launch_dot3( V, u, u, part, sc, s );
+                    RP_HIP( hipMemcpyAsync( nu, sc, sizeof(nu), hipMemcpyDeviceToHost, s ) );
+                    RP_HIP( hipStreamSynchronize( s ) );
+                    lam = 0.0;
+                    for( int c = 0; c < 3; ++c ) { if( nv[c] > 0 ) { lam = std::max( lam, std::sqrt( nu[c] / nv[c] ) ); } }
+                    std::swap( v, u );
+                    launch_dot3( V, v, v, part, sc, s );
+                    launch_normalize3( V, v, sc, v, s );
+                }
+                ctx->pc_weight = 0.25 * lam;
+                for( int k = 0; k < 3; ++k ) { ctx->pc_weight_key[k] = key[k]; }
+            }
+        }
+        if( ctx->pc_weight > 0.0 ) { ensure_metric_diag( ctx ); }
+    }
+
+    // Y = alpha (P0 + w diag(A)^-1) X + beta Y (w = 0: the reference preconditioner). X and Y must not
+    // overlap. Call prepare_pc first.
+    static void pc_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    {
+        pc0_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy );
+        if( ctx->pc_weight > 0.0 && alpha != 0.0 ) { launch_jacobi_add( ctx->V, alpha * ctx->pc_weight, Xd, ldx, ctx->dA.get(), Yd, ldy, ctx->stream ); }
     }
 
     // Host staging of a V x 3 input/output pair for calls with host pointers.
@@ -373,6 +462,7 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision )
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         finish_timing( ctx, 9 );
         ctx->have_pl = true;
+        ++ctx->pl_version;
         ctx->pl_precision = precision;
         ctx->pl_s = s;
     });
@@ -495,6 +585,44 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 rmax[(size_t)h] = std::max( rmax[(size_t)h], (int)( row_ptr[s + 1] - row_ptr[s] ) );
             }
         }
+        // Levels holding a supernode with more than 64 rows run in the split form; their task lists:
+        // forward 2 by 256 rows, backward 1 and 2 by as many columns as groups fit in 256 threads.
+        std::vector<char> split( (size_t)nl, 0 );
+        std::vector<int2> tasks;
+        std::vector<int64_t> tptr[3];
+        for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
+        auto group_of = []( int64_t len ) { int g = 1; while( g < len && g < 64 ) { g <<= 1; } return g; };
+        {
+            std::vector<int2> tk[3];
+            for( int32_t l = 0; l < nl; ++l )
+            {
+                split[(size_t)l] = rmax[(size_t)l] > 64;
+                for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k )
+                {
+                    if( !split[(size_t)l] ) { break; }
+                    const int32_t s = lnodes[(size_t)k];
+                    const int32_t w = sn_begin[s + 1] - sn_begin[s];
+                    const int32_t r = (int32_t)( row_ptr[s + 1] - row_ptr[s] );
+                    for( int32_t q0 = w; q0 < r; q0 += CHOL_TASK_ROWS ) { tk[0].push_back( make_int2( s, q0 ) ); }
+                    if( r > w )
+                    {
+                        const int per = CHOL_TASK_ROWS / group_of( r - w );
+                        for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( make_int2( s, c0 ) ); }
+                    }
+                    const int per2 = CHOL_TASK_ROWS / group_of( w );
+                    for( int32_t c0 = 0; c0 < w; c0 += per2 ) { tk[2].push_back( make_int2( s, c0 ) ); }
+                }
+                for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
+            }
+            int64_t off = 0;
+            for( int kind = 0; kind < 3; ++kind )
+            {
+                for( auto & x : tptr[kind] ) { x += off; }
+                tasks.insert( tasks.end(), tk[kind].begin(), tk[kind].end() );
+                off = (int64_t)tasks.size();
+            }
+        }
+        if( !tasks.empty() ) { ctx->h1_tasks.upload( tasks.data(), tasks.size(), ctx->stream ); }
         ctx->h1_sn_begin.upload( sn_begin, (size_t)ns + 1, ctx->stream );
         ctx->h1_row_ptr.upload( row_ptr, (size_t)ns + 1, ctx->stream );
         ctx->h1_rows.upload( rows, (size_t)nrows, ctx->stream );
@@ -511,9 +639,12 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_level_ptr = lptr;
         ctx->h1_level_wmax = wmax;
         ctx->h1_level_rmax = rmax;
+        ctx->h1_level_split = split;
+        for( int kind = 0; kind < 3; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
         ctx->h1_n = n;
         ctx->h1_ns = ns;
         ctx->have_h1 = true;
+        ++ctx->h1_version;
     });
 }
 
@@ -547,7 +678,13 @@ int rp_multiply_preconditioner( rp_ctx * ctx, double alpha, const double * X, in
         require( ctx->have_h1 && ctx->have_pl, "the preconditioner needs the H1 factor and the pseudo-Laplacian", RP_ENOTREADY );
         require( X != nullptr && Y != nullptr && ldX >= 3 && ldY >= 3, "X and Y must be V x 3 with ld >= 3" );
         RP_HIP( hipSetDevice( ctx->device ) );
+        require( ctx->pc_weight_setting == 0.0 || ctx->have_metric, "the diagonal term needs the assembled metric", RP_ENOTREADY );
         Staged st( ctx, ptr_flags & RP_DEVICE_PTR, X, ldX, beta, Y, ldY );
+        {
+            EventScope ev( ctx, 10 );
+            prepare_pc( ctx );
+            ev.stop();
+        }
         EventScope ev( ctx, 7 );
         pc_apply( ctx, alpha, st.X, st.ldx, beta, st.Y, st.ldy );
         ev.stop();
@@ -575,6 +712,14 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
         Staged st( ctx, ptr_flags & RP_DEVICE_PTR, B, ldB, beta, X, ldX );
         ctx->cg_x.require( n3 ); ctx->cg_b.require( n3 );
         double * x = ctx->cg_x.get(); double * b = ctx->cg_b.get();
+        if( use_pc )
+        {
+            EventScope ev( ctx, 10 );
+            prepare_pc( ctx );
+            ev.stop();
+            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+            finish_timing( ctx, 10 );
+        }
         EventScope ev( ctx, 3 );
         launch_copy3( V, st.X, st.ldx, b, 3, ctx->stream );
         std::vector<double> rel( 3, 0.0 );
@@ -592,6 +737,41 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
         if( iterations ) { *iterations = it; }
         if( rel_residuals ) { for( int k = 0; k < 3; ++k ) { rel_residuals[k] = rel[(size_t)k]; } }
         if( !done ) { throw InvalidArg{ "no convergence to the requested tolerance within max_iter iterations", RP_ESTAGNATION }; }
+    });
+}
+
+int rp_set_preconditioner( rp_ctx * ctx, double diagonal_weight )
+{
+    return guarded( "rp_set_preconditioner", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require( std::isfinite( diagonal_weight ), "the weight must be finite" );
+        if( diagonal_weight == ctx->pc_weight_setting ) { return; }   // keeps an estimated weight
+        ctx->pc_weight_setting = diagonal_weight;
+        ctx->pc_weight = diagonal_weight > 0 ? diagonal_weight : 0.0;
+        for( int k = 0; k < 3; ++k ) { ctx->pc_weight_key[k] = -1; }
+    });
+}
+
+int rp_preconditioner_weight( rp_ctx * ctx, double * weight )
+{
+    return guarded( "rp_preconditioner_weight", [&]()
+    {
+        require( ctx != nullptr && weight != nullptr, "bad arguments" );
+        *weight = ctx->pc_weight;
+    });
+}
+
+int rp_download_metric_diagonal( rp_ctx * ctx, double * diag )
+{
+    return guarded( "rp_download_metric_diagonal", [&]()
+    {
+        require( ctx != nullptr && diag != nullptr, "bad arguments" );
+        require( ctx->have_metric, "metric values not assembled (call rp_tp0_compute with RP_METRIC)", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        ensure_metric_diag( ctx );
+        ctx->dA.download( diag, (size_t)ctx->V, ctx->stream );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
     });
 }
 

@@ -78,6 +78,21 @@ class OracleSolve:
         O.pl_multiply(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.npat, self.pl, self.fpat, 1.0, Z, 0.0, Y)
         return np.ascontiguousarray(self.lu.solve(Y))
 
+    def diagonal(self):
+        """diag(A) by probing: A applied to the unit vector of every vertex."""
+        V = self.M.V
+        d = np.zeros(V)
+        for v in range(V):
+            e = np.zeros((V, 3)); e[v, 0] = 1.0
+            d[v] = self.A(e)[v, 0]
+        return d
+
+    def with_diagonal_term(self, w, diag):
+        """Switches P to P0 + w diag^-1."""
+        P0 = self.P
+        self.P = lambda X: P0(X) + w * X / diag[:, None]
+        return self
+
     def pcg(self, B, tol=1e-6, max_iter=500, measure="l2"):
         """Textbook PCG per column (all columns iterate together); measure "l2" stops on
         |r|_2 <= tol |b|_2, "pc" on <r, P r> <= tol^2 <b, P b>."""
