@@ -41,6 +41,21 @@ PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (equal to the FP64 matrix peak
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per k_nf_spmm launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by
+# the counter's reading for a copy kernel of known size (profiles/k_nf_spmm_sphere_1310720_traffic.md).
+MEASURED_SPMM_TRAFFIC = {("sphere_1310720", "fp64"): 14.605e9}
+
+
+def spmm_algorithmic_bytes(b, precision):
+    """Compulsory bytes of one near-field SpMM launch: the metric values of every (row, column) pair
+    of the full near pattern (2 per pair: 16 B in FP64, 8 B in FP32), the tile column list (column
+    index + row mask, 8 B per tile column entry), and per primitive the input block (12 doubles), the
+    diagonal (2 doubles), the post-operator weights (4 x 3 doubles) and the output (3 x 3 doubles),
+    plus the far-field total of every leaf (12 doubles)."""
+    val_bytes = 16 if precision == "fp64" else 8
+    return (val_bytes * b["near_nnz"] + 8 * b["tile_cols"]
+            + b["T"] * (96 + 16 + 96 + 72) + 96 * b["L"])
+
 
 def _hip():
     lib = C.CDLL("libamdhip64.so")
@@ -50,11 +65,13 @@ def _hip():
     return lib
 
 
-def cpu_baseline(M, wl, target_pairs):
+def cpu_baseline(M, wl, target_pairs, solve_iters=0):
     """Oracle restatement (-O3 -ffast-math build, 1 thread) on a bounded sample of the same workload:
     the near-field rows [0, R) holding about `target_pairs` upper pairs, the complete far field.
-    Returns the extrapolated energy+grad ms for the whole workload."""
+    Returns the extrapolated energy+grad ms (plus, with solve_iters, the metric solve) for the whole
+    workload."""
     import oracle as O
+    t_start = time.perf_counter()
     f = O.fast_tp0()
     b = M.build()
     td = M.tree_data()
@@ -90,15 +107,71 @@ def cpu_baseline(M, wl, target_pairs):
     t0 = time.perf_counter()
     O.differential(M.dom_dim, M.VertexCoordinates(), M.simplices, tr, PDn, CDf)
     t_chain = time.perf_counter() - t0
-    total_s = t_near * (near_upper_total / max(1, sample_pairs)) + t_far + t_chain
+    eg_s = t_near * (near_upper_total / max(1, sample_pairs)) + t_far + t_chain
     sampled_s = t_near + t_far + t_chain
-    return {
-        "value": round(total_s * 1e3, 1), "unit": "ms (energy+grad, extrapolated)", "cores": 1, "kind": "port",
+    out = {
+        "value": round(eg_s * 1e3, 1), "unit": "ms (energy+grad, extrapolated)", "cores": 1, "kind": "port",
         "sample": (f"near-field rows [0,{R}) = {sample_pairs} of {near_upper_total} upper pairs timed and scaled; "
                    f"far field ({b['far_nnz'] // 2} pairs), derivative collection, chain rule and assembly complete; "
-                   f"{sampled_s:.1f} s of CPU work; oracle -O3 -ffast-math build"),
+                   f"oracle -O3 -ffast-math build"),
         "pairs_per_s": round(sample_pairs / t_near, 1),
+        "energy_grad_ms": round(eg_s * 1e3, 1),
     }
+    if solve_iters:
+        # One PCG iteration on the CPU: metric product A p and preconditioner (two H^1 solves around
+        # the pseudo-Laplacian product), same near rows sample, scaled; far field, tree passes and
+        # the pre/post operators complete. Iteration count taken from the GPU solve (same method).
+        s_pl = 2.0 - (wl.p - M.dom_dim) / wl.q
+        near_s = (nrp, nci, ndg)
+        empty_near = (nrp0, empty_ci, ndg0)
+        far_full = (frp, fci, fdg)
+        empty_far = (empty_rp, empty_ci, empty_dg)
+        mv_near = O.tp0_compute(M.dom_dim, P_near, C_far, near_s, empty_far, wl.q, wl.p, 4, impl=O.fast("orc_tp0_compute"))
+        mv_far = O.tp0_compute(M.dom_dim, P_near, C_far, empty_near, far_full, wl.q, wl.p, 4, impl=O.fast("orc_tp0_compute"))
+        pl_near = O.pl_compute(M.dom_dim, P_near, C_far, near_s, empty_far, s_pl, impl=O.fast("orc_pl_compute"))
+        pl_far = O.pl_compute(M.dom_dim, P_near, C_far, empty_near, far_full, s_pl, impl=O.fast("orc_pl_compute"))
+        Xv = np.random.default_rng(11).standard_normal((M.V, 3))
+        Yv = np.zeros_like(Xv)
+        mm, plm = O.fast("orc_multiply_metric"), O.fast("orc_pl_multiply")
+
+        def timed(fn):
+            t0 = time.perf_counter(); fn(); return time.perf_counter() - t0
+
+        t_a_near = timed(lambda: O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], near_s, mv_near["NF"],
+                                                   mv_near["NF_diag"], empty_far, np.zeros((1, 2)), mv_near["FF_diag"],
+                                                   1.0, Xv, 0.0, Yv, True, False, impl=mm))
+        t_a_far = timed(lambda: O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, np.zeros((1, 2)),
+                                                  mv_far["NF_diag"], far_full, mv_far["FF"], mv_far["FF_diag"],
+                                                  1.0, Xv, 0.0, Yv, False, True, impl=mm))
+        t_a_base = timed(lambda: O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, np.zeros((1, 2)),
+                                                   mv_far["NF_diag"], empty_far, np.zeros((1, 2)), mv_far["FF_diag"],
+                                                   1.0, Xv, 0.0, Yv, False, False, impl=mm))
+        pl_ne = {"NF": pl_near["NF"], "NF_diag": pl_near["NF_diag"], "FF": np.zeros(1), "FF_diag": pl_far["FF_diag"]}
+        pl_fa = {"NF": np.zeros(1), "NF_diag": pl_near["NF_diag"], "FF": pl_far["FF"], "FF_diag": pl_far["FF_diag"]}
+        t_l_near = timed(lambda: O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], near_s, pl_ne, empty_far,
+                                               1.0, Xv, 0.0, Yv, True, False, impl=plm))
+        t_l_far = timed(lambda: O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, pl_fa, far_full,
+                                              1.0, Xv, 0.0, Yv, False, True, impl=plm))
+        t_l_base = timed(lambda: O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, pl_fa, empty_far,
+                                               1.0, Xv, 0.0, Yv, False, False, impl=plm))
+        M.h1_solve_host(Xv)                      # factorisation (per state, not timed)
+        t_h = timed(lambda: M.h1_solve_host(Xv))
+        nnz_scale = b["near_nnz"] / max(1, int(nrp[-1]))
+        t_a = t_a_base + (t_a_near - t_a_base) * nnz_scale + (t_a_far - t_a_base)
+        t_l = t_l_base + (t_l_near - t_l_base) * nnz_scale + (t_l_far - t_l_base)
+        t_it = t_a + t_l + 2.0 * t_h
+        total_s = eg_s + solve_iters * t_it
+        out.update({
+            "value": round(total_s * 1e3, 1), "unit": "ms (energy+grad + metric solve, extrapolated)",
+            "solve_ms": round(solve_iters * t_it * 1e3, 1), "iteration_ms": round(t_it * 1e3, 1),
+            "multiply_metric_ms": round(t_a * 1e3, 1), "pseudo_laplacian_ms": round(t_l * 1e3, 1),
+            "h1_solve_ms": round(t_h * 1e3, 1), "iterations": solve_iters,
+            "sample": out["sample"] + (f"; solve: one metric product and one pseudo-Laplacian product on the same near rows "
+                                       f"({int(nrp[-1])} of {b['near_nnz']} near entries, scaled) with far field and tree passes "
+                                       f"complete, one host supernodal H^1 solve (1 thread), times {solve_iters} iterations"),
+        })
+    out["sampled_cpu_s"] = round(time.perf_counter() - t_start, 1)
+    return out
 
 
 def calibrate_copy(mb):
@@ -247,7 +320,10 @@ def main():
     nf_avg_s = float(np.mean(nf_ms)) * 1e-3
     achieved = nf_flops / nf_avg_s / 1e12
     peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
-    spmm_bytes = 20 * b["near_nnz"] + 2 * 96 * b["T"] + 16 * b["T"]
+    spmm_bytes = spmm_algorithmic_bytes(b, args.precision)
+    spmm_s = float(np.mean(spmm_ms)) * 1e-3
+    spmm_gbs = spmm_bytes / spmm_s / 1e9
+    traffic = MEASURED_SPMM_TRAFFIC.get((wl.name, args.precision))
     result = {
         "metric": "energy+grad ms & metric-solve ms to 1e-6 residual, learned vs plain PC, 1 GPU",
         "value": round(ms_per_step, 3),
@@ -281,7 +357,7 @@ def main():
         "h1_factor_upload_s": round(t_pc_prep, 2),
         "multiply_metric_ms": round(float(np.mean(mm_ms)), 3),
         "near_spmm_ms": round(float(np.mean(spmm_ms)), 3),
-        "near_spmm_GBs_algorithmic": round(spmm_bytes / (float(np.mean(spmm_ms)) * 1e-3) / 1e9, 1),
+        "near_spmm_GBs_algorithmic": round(spmm_gbs, 1),
         "multiply_preconditioner_ms": round(pc_ms, 3),
         "h1_solve_ms": round(h1_ms, 3),
         "solve_ms": None if args.no_solve else round(float(np.mean(sv_ms)), 2),
@@ -295,19 +371,26 @@ def main():
         "upload_s": round(t_upload, 2),
         "pairs_per_s": round((pairs_nf + pairs_ff) / (float(np.mean(eg_ms)) * 1e-3), 1),
         "roofline": {
-            "kernel": "near-field energy+differential (k_nf_f64<Q6P12,1,1,0>)" if args.precision == "fp64"
-                      else "near-field energy+differential (k_nf_mixed<Q6P12,1,1,0>)",
+            "kernel": "k_nf_spmm (near-field metric block SpMM with fused post-operator epilogue, one per MultiplyMetric)",
+            "bound": "hbm",
+            "achieved": round(spmm_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(spmm_gbs / PEAK_HBM_GBS, 4),
+            "algorithmic_bytes": int(spmm_bytes), "kernel_ms": round(spmm_s * 1e3, 4),
+            "traffic": traffic,
+            "traffic_source": "profiles/k_nf_spmm_sphere_1310720_traffic.md" if traffic else None,
+        },
+        "roofline_near_energy": {
+            "kernel": "k_nf<Q6P12,E,D> (near-field energy + differential)",
             "bound": "mfma", "compute_unit": "valu",
             "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
             "algorithmic_flops": nf_flops, "flops_per_pair": NF_FLOPS_VALUE_DIFF, "pairs": pairs_nf,
-            "traffic": None,
         },
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         try:
-            result["cpu_baseline"] = cpu_baseline(M, wl, args.cpu_sample_pairs)
+            result["cpu_baseline"] = cpu_baseline(M, wl, args.cpu_sample_pairs, 0 if args.no_solve else solve_iters)
         except Exception as e:  # reported, not fatal
             result["cpu_baseline"] = {"error": str(e)}
     hip.hipFree(Yd)

@@ -24,19 +24,27 @@ def _load(name):
 
 _lib = _load("liboracle.so")
 # Release-flag build of the same restatement (-O3 -ffast-math), timed as the CPU baseline.
-_fast = None
 
 
 def fast_tp0():
     """orc_tp0_compute from the -O3 -ffast-math build (CPU baseline timing only)."""
-    global _fast
-    if _fast is None:
-        lib = _load("liboracle_fast.so")
-        f = lib.orc_tp0_compute
-        f.restype = C.c_double
-        f.argtypes = _lib.orc_tp0_compute.argtypes
-        _fast = f
-    return _fast
+    return fast("orc_tp0_compute")
+
+
+_fast_lib = None
+
+
+def fast(name):
+    """A function of the -O3 -ffast-math build with the signature of the checker build's (CPU
+    baseline timing only)."""
+    global _fast_lib
+    if _fast_lib is None:
+        _fast_lib = _load("liboracle_fast.so")
+    f = getattr(_fast_lib, name)
+    ref = getattr(_lib, name)
+    f.restype = ref.restype
+    f.argtypes = ref.argtypes
+    return f
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)
@@ -89,7 +97,7 @@ def simplex_data(V, S, charges=None):
     return near, far, dop, srt
 
 
-def tp0_compute(n, P_near, C_far, near_pat, far_pat, q, p, flags):
+def tp0_compute(n, P_near, C_far, near_pat, far_pat, q, p, flags, impl=None):
     """TP0_Traversor::Compute restatement. *_pat = (rp, ci, diag) full symmetric CSR. flags 1|2|4."""
     P_near, C_far = c64(P_near), c64(C_far)
     T, Cn = P_near.shape[0], C_far.shape[0]
@@ -103,7 +111,7 @@ def tp0_compute(n, P_near, C_far, near_pat, far_pat, q, p, flags):
     ffv = np.zeros((max(1, len(fci)), 2)) if flags & 4 else None
     nfd = np.zeros((T, 2)) if flags & 4 else None
     ffd = np.zeros((Cn, 2)) if flags & 4 else None
-    en = _tp0(n, T, Cn, _p(P_near), _p(C_far), _p(nrp, _i64), _p(nci if len(nci) else np.zeros(1, np.int32), _i32), _p(ndg, _i64),
+    en = (impl or _tp0)(n, T, Cn, _p(P_near), _p(C_far), _p(nrp, _i64), _p(nci if len(nci) else np.zeros(1, np.int32), _i32), _p(ndg, _i64),
               _p(frp, _i64), _p(fci if len(fci) else np.zeros(1, np.int32), _i32), _p(fdg, _i64), q, p, flags,
               _p(PDn), _p(CDf), _p(nfv), _p(ffv), _p(nfd), _p(ffd))
     out["energy"] = en
@@ -149,14 +157,14 @@ def differential(n, V, S, tree, P_D_near_tree, C_D_far, charges=None):
 
 
 def multiply_metric(n, nV, tree, pre_cols, pre_vals, near_pat, nf_vals, nf_diag, far_pat, ff_vals, ff_diag,
-                    alpha, X, beta, Y, nf=True, ff=True):
+                    alpha, X, beta, Y, nf=True, ff=True, impl=None):
     T = len(tree["P_ordering"])
     Cn = len(tree["C_left"])
     X = c64(X)
     assert Y.dtype == np.float64 and Y.flags.c_contiguous
     z = np.zeros(1, np.int32)
     nci = c32(near_pat[1]); fci = c32(far_pat[1])
-    _mm(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
+    (impl or _mm)(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
         _p(c32(tree["C_end"]), _i32), _p(c32(pre_cols), _i32), _p(c64(pre_vals)),
         _p(c64i(near_pat[0]), _i64), _p(nci if len(nci) else z, _i32), _p(c64(nf_vals) if len(nci) else np.zeros(2)), _p(c64(nf_diag)),
         _p(c64i(far_pat[0]), _i64), _p(fci if len(fci) else z, _i32), _p(c64(ff_vals) if len(fci) else np.zeros(2)), _p(c64(ff_diag)),
@@ -184,7 +192,7 @@ _plm = _f("orc_pl_multiply", None, [C.c_int, C.c_int64, C.c_int64, C.c_int64, _i
                                     C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int])
 
 
-def pl_compute(n, P_near, C_far, near_pat, far_pat, s):
+def pl_compute(n, P_near, C_far, near_pat, far_pat, s, impl=None):
     """PseudoLaplacian (LowOrder) values restatement: NF (N_nf), FF (N_ff), NF_diag (T), FF_diag (C)."""
     P_near, C_far = c64(P_near), c64(C_far)
     T, Cn = P_near.shape[0], C_far.shape[0]
@@ -193,19 +201,19 @@ def pl_compute(n, P_near, C_far, near_pat, far_pat, s):
     z = np.zeros(1, np.int32)
     nfv = np.zeros(max(1, len(nci))); ffv = np.zeros(max(1, len(fci)))
     nfd = np.zeros(T); ffd = np.zeros(Cn)
-    _pl(n, T, Cn, _p(P_near), _p(C_far), _p(nrp, _i64), _p(nci if len(nci) else z, _i32), _p(ndg, _i64),
+    (impl or _pl)(n, T, Cn, _p(P_near), _p(C_far), _p(nrp, _i64), _p(nci if len(nci) else z, _i32), _p(ndg, _i64),
         _p(frp, _i64), _p(fci if len(fci) else z, _i32), _p(fdg, _i64), s, _p(nfv), _p(ffv), _p(nfd), _p(ffd))
     return {"NF": nfv[: len(nci)], "FF": ffv[: len(fci)], "NF_diag": nfd, "FF_diag": ffd}
 
 
-def pl_multiply(n, nV, tree, pre_cols, pre_vals, near_pat, pl, far_pat, alpha, X, beta, Y, nf=True, ff=True):
+def pl_multiply(n, nV, tree, pre_cols, pre_vals, near_pat, pl, far_pat, alpha, X, beta, Y, nf=True, ff=True, impl=None):
     T = len(tree["P_ordering"])
     Cn = len(tree["C_left"])
     X = c64(X)
     assert Y.dtype == np.float64 and Y.flags.c_contiguous
     z = np.zeros(1, np.int32)
     nci = c32(near_pat[1]); fci = c32(far_pat[1])
-    _plm(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
+    (impl or _plm)(n, nV, T, Cn, _p(c32(tree["C_left"]), _i32), _p(c32(tree["C_right"]), _i32), _p(c32(tree["C_begin"]), _i32),
          _p(c32(tree["C_end"]), _i32), _p(c32(pre_cols), _i32), _p(c64(pre_vals)),
          _p(c64i(near_pat[0]), _i64), _p(nci if len(nci) else z, _i32), _p(c64(pl["NF"]) if len(nci) else np.zeros(1)),
          _p(c64(pl["NF_diag"])),

@@ -49,7 +49,8 @@ namespace rp
         Chunk<R> & sh = smem.ch;
         double (&red)[64][NACC] = smem.red;
         const int lane = threadIdx.x;
-        const int64_t I = blockIdx.x;
+        const int64_t I = xcd_leaf( A.L );
+        if( I < 0 ) { return; }
         const int ib = A.leaf_begin[I];
         const int nI = A.leaf_end[I] - ib;
         const int nS = 64 / nI;
@@ -240,7 +241,7 @@ namespace rp
     // Epilogue (post-operator, ClusterTree::Post): ytot_i = y_i + far-field leaf total of leaf(i), then
     // S_out[i][s][c] = sum_r pre_vals[i][r][s] ytot_i[r][c] for the (n+1) vertex slots s.
     template<typename VT, int SIZE>
-    __global__ __launch_bounds__(64) void k_nf_spmm( int64_t T, const int32_t * leaf_begin, const int32_t * leaf_end,
+    __global__ __launch_bounds__(64) void k_nf_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
         const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
         const VT * vals, const double * diag, const double * xin,
         const double * leaf_out, int use_far, const double * pre_vals, double * S_out )
@@ -252,7 +253,8 @@ namespace rp
         __shared__ int32_t sj[2][CH];
         __shared__ uint32_t sm[2][CH];
         const int lane = threadIdx.x;
-        const int64_t I = blockIdx.x;
+        const int64_t I = xcd_leaf( L );
+        if( I < 0 ) { return; }
         const int ib = leaf_begin[I];
         const int nI = leaf_end[I] - ib;
         const int nS = 64 / nI;
@@ -363,7 +365,7 @@ namespace rp
     static void launch_nf_flags( int kind, bool mixed, const NFArgs & A, hipStream_t s )
     {
         if( A.L == 0 ) { return; }
-        const dim3 grid( (unsigned)A.L ), block( 64 );
+        const dim3 grid = leaf_grid( A.L ), block( 64 );
         if( !mixed )
         {
             if( kind == KIND_Q6P12 )      { hipLaunchKernelGGL( (k_nf<KIND_Q6P12,E,D,M,false>),  grid, block, 0, s, A ); }
@@ -396,16 +398,16 @@ namespace rp
                               const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s )
     {
         if( !L ) { return; }
-        const dim3 grid( (unsigned)L ), block( 64 );
+        const dim3 grid = leaf_grid( L ), block( 64 );
         if( size == 3 )
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,3>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
-            else        { hipLaunchKernelGGL( (k_nf_spmm<double2,3>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_spmm<double2,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
         else
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,2>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
-            else        { hipLaunchKernelGGL( (k_nf_spmm<double2,2>), grid, block, 0, s, T, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_spmm<double2,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
     }
 

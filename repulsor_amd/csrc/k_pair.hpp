@@ -16,6 +16,20 @@
 
 namespace rp
 {
+    // Leaf of a leaf-tile workgroup. Workgroups are dealt to the 8 XCDs round robin (workgroup b runs on
+    // XCD b mod 8), each with its own L2; mapping b -> (b mod 8) * per + b / 8 gives every XCD one
+    // contiguous run of leaves, whose column lists overlap, so the gathered column vectors are reused
+    // in that XCD's L2. The grid is padded to a multiple of 8 (launch with leaf_grid(L)); returns -1 for
+    // the padding workgroups.
+    constexpr int NUM_XCD = 8;
+    __device__ __forceinline__ int64_t xcd_leaf( int64_t L )
+    {
+        const int64_t per = gridDim.x / NUM_XCD;
+        const int64_t I = (int64_t)( blockIdx.x % NUM_XCD ) * per + blockIdx.x / NUM_XCD;
+        return ( I < L ) ? I : -1;
+    }
+    inline dim3 leaf_grid( int64_t L ) { return dim3( (unsigned)( ( (L + NUM_XCD - 1) / NUM_XCD ) * NUM_XCD ) ); }
+
     // Exponent kinds compiled as fast paths; KIND_GENERIC reads the exponents at run time.
     enum { KIND_GENERIC = 0, KIND_Q6P12 = 1, KIND_Q2P4 = 2 };
 

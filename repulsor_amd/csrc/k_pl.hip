@@ -27,7 +27,8 @@ namespace rp
         __shared__ uint32_t sm[2][PL_CH];
         __shared__ double red[64];
         const int lane = threadIdx.x;
-        const int64_t I = blockIdx.x;
+        const int64_t I = xcd_leaf( A.L );
+        if( I < 0 ) { return; }
         const int ib = A.leaf_begin[I];
         const int nI = A.leaf_end[I] - ib;
         const int nS = 64 / nI;
@@ -147,7 +148,7 @@ namespace rp
 
     // ------------------------------------------------------------------ near-field product + post epilogue
     template<typename VT, int SIZE>
-    __global__ __launch_bounds__(64) void k_nf_pl_spmm( const int32_t * leaf_begin, const int32_t * leaf_end,
+    __global__ __launch_bounds__(64) void k_nf_pl_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
         const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
         const VT * vals, const double * diag, const double * xin, const double * leaf_out, int use_far,
         const double * pre_vals, double * S_out )
@@ -156,7 +157,8 @@ namespace rp
         __shared__ uint32_t sm[2][PL_CH];
         __shared__ double red[64][3];
         const int lane = threadIdx.x;
-        const int64_t I = blockIdx.x;
+        const int64_t I = xcd_leaf( L );
+        if( I < 0 ) { return; }
         const int ib = leaf_begin[I];
         const int nI = leaf_end[I] - ib;
         const int nS = 64 / nI;
@@ -270,8 +272,8 @@ namespace rp
     {
         if( do_nf && A.L )
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_assemble<true>), dim3((unsigned)A.L), dim3(64), 0, s, A ); }
-            else        { hipLaunchKernelGGL( (k_nf_pl_assemble<false>), dim3((unsigned)A.L), dim3(64), 0, s, A ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_assemble<true>), leaf_grid( A.L ), dim3(64), 0, s, A ); }
+            else        { hipLaunchKernelGGL( (k_nf_pl_assemble<false>), leaf_grid( A.L ), dim3(64), 0, s, A ); }
         }
         if( do_ff && B.C )
         {
@@ -294,16 +296,16 @@ namespace rp
                             const double * xin, const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s )
     {
         if( !L ) { return; }
-        const dim3 grid( (unsigned)L ), block( 64 );
+        const dim3 grid = leaf_grid( L ), block( 64 );
         if( size == 3 )
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,3>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
-            else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,3>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
         else
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,2>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
-            else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,2>), grid, block, 0, s, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
     }
 
