@@ -1,0 +1,75 @@
+"""Mixed precision (FP32 pair arithmetic after an FP64 shift to a per-leaf origin, FP64 accumulation)
+against the FP64 oracle: energy, differential, metric product, pseudo-Laplacian product, and a
+solve whose residual is measured with the FP64 oracle metric.
+
+Stated errors (relative, 2-norm; measured on MI355X in brackets): energy <= 1e-7 [0.9-4.8e-9],
+differential <= 1e-5 for surfaces [0.9-1.8e-6] and <= 2e-4 for curves [7.2e-5], metric product
+<= 1e-6 [4.5-8.3e-8], pseudo-Laplacian product <= 1e-6 [1.4-2.5e-8]. The mixed solve stops at
+1e-6 on its FP32-valued operator; its residual against the FP64 operator is <= 3e-6 [1.2-1.7e-6].
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+import repulsor_amd as R
+from repulsor_amd.workloads import WORKLOADS, random_field
+from helpers import OracleSolve, oracle_energy_diff, oracle_multiply, oracle_state, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["torus_2400", "sphere_1280", "trefoil_3000"]
+
+
+def _mesh(name):
+    wl = WORKLOADS[name]
+    M = wl.mesh()
+    M.build()
+    return wl, M
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_energy_differential(name):
+    wl, M = _mesh(name)
+    E = R.TangentPointEnergy0(wl.q, wl.p, precision="mixed")
+    Y, en = E.Differential(M, return_energy=True)
+    en_ref, Y_ref, _, _ = oracle_energy_diff(M, wl.q, wl.p)
+    e_en, e_d = abs(en - en_ref) / abs(en_ref), rel(Y, Y_ref)
+    print(f"case {name}: mixed energy rel err {e_en:.2e}, differential {e_d:.2e}", flush=True)
+    assert e_en <= 1e-7
+    assert e_d <= (2e-4 if M.dom_dim == 1 else 1e-5)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_mixed_metric_products(name):
+    wl, M = _mesh(name)
+    A = R.TangentPointMetric0(wl.q, wl.p, precision="mixed")
+    X = random_field(M.V, seed=11)
+    Y = np.zeros_like(X)
+    A.MultiplyMetric(M, 1.0, X, 0.0, Y)
+    e_a = rel(Y, oracle_multiply(M, wl.q, wl.p, X))
+    Z = np.zeros_like(X)
+    A.MultiplyPseudoLaplacian(M, 1.0, X, 0.0, Z)
+    tr, td, npat, fpat = oracle_state(M)
+    pl = O.pl_compute(M.dom_dim, td["P_near"], td["C_far"], npat, fpat, 2.0 - (wl.p - M.dom_dim) / wl.q)
+    Zr = np.zeros_like(X)
+    O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], npat, pl, fpat, 1.0, X, 0.0, Zr)
+    e_l = rel(Z, Zr)
+    print(f"case {name}: mixed metric product rel err {e_a:.2e}, pseudo-Laplacian {e_l:.2e}", flush=True)
+    assert e_a <= 1e-6
+    assert e_l <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["torus_2400", "sphere_1280"])
+def test_mixed_solve(name):
+    wl, M = _mesh(name)
+    E = R.TangentPointEnergy0(wl.q, wl.p)
+    B = E.Differential(M)
+    A = R.TangentPointMetric0(wl.q, wl.p, precision="mixed")
+    A.SetPreconditionerDiagonalWeight(-1.0)
+    X = np.zeros_like(B)
+    A.Solve(M, 1.0, B, 0.0, X, max_iter=500, tolerance=1e-6)
+    S = OracleSolve(M, wl.q, wl.p)
+    true = np.linalg.norm(B - S.A(X), axis=0) / np.linalg.norm(B, axis=0)
+    print(f"case {name}: mixed solve {A.Solver_IterationCount()} iterations, FP64 residual {true.max():.2e}", flush=True)
+    assert np.all(true <= 3e-6), true
