@@ -5,6 +5,9 @@
 // vector overloads), MetricNorm. The metric blocks are assembled and applied on the GPU.
 
 #include "SimplicialMesh.hpp"
+#include "LearnedPreconditioner.hpp"
+
+#include <cstring>
 
 namespace Repulsor
 {
@@ -164,12 +167,31 @@ namespace Repulsor
             if( !MetricValues( M ) ) { return false; }
             if( use_preconditioner && !PreparePreconditioner( M ) ) { return false; }
             ScopedTimer timer( "TangentPointMetric0::Solve" );
+            const bool learned = use_preconditioner && !lp_params.empty();
+            const bool can_fall_back = learned && ( beta == 0.0 || !device_ptrs );
+            std::vector<double> X0;
+            if( can_fall_back && beta != 0.0 ) { X0.assign( X, X + ldX * ( M.VertexCount() - 1 ) + 3 ); }
             int it = 0;
             double rel[3] = { 0, 0, 0 };
             const int st = rp_solve( M.Device(), alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance,
                                      solver_flags, device_ptrs ? RP_DEVICE_PTR : 0, &it, rel );
             iter = it;
             rel_residuals.assign( rel, rel + 3 );
+            fell_back = false;
+            if( st == RP_ESTAGNATION && can_fall_back )
+            {
+                // the reference preconditioner (with the estimated diagonal term) is the fallback
+                wprint( ClassName() + "::Solve: learned preconditioner stagnated; falling back to P0 + w diag(A)^-1" );
+                if( !X0.empty() ) { std::copy( X0.begin(), X0.end(), X ); }
+                TangentPointMetric0 plain( q, p );
+                plain.SetPrecision( precision );
+                plain.SetPreconditionerDiagonalWeight( -1.0 );
+                const bool ok = plain.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance, device_ptrs, solver_flags );
+                iter = it + plain.Solver_IterationCount();
+                rel_residuals = plain.Solver_RelativeResiduals();
+                fell_back = true;
+                return ok;
+            }
             if( st == RP_ESTAGNATION )
             {
                 wprint( ClassName() + "::Solve: " + rp_last_error() );
@@ -178,23 +200,104 @@ namespace Repulsor
             return Check( st, ClassName() + "::Solve" );
         }
 
+        // Whether the last Solve had to fall back from the learned preconditioner.
+        bool Solver_FellBack() const { return fell_back; }
+
         int Solver_IterationCount() const { return iter; }
         std::vector<double> Solver_RelativeResiduals() const { return rel_residuals; }
 
     protected:
         mutable int iter = 0;
         mutable std::vector<double> rel_residuals;
+        mutable bool fell_back = false;
 
         double pc_diagonal_weight = 0.0;
+        std::vector<float> lp_params;      // learned preconditioner model (empty: none)
+        int lp_nfeat = 0;
+        bool lp_bf16 = true;
+
+        std::string LearnedKey() const
+        {
+            uint64_t h = 1469598103934665603ull;
+            for( float x : lp_params ) { uint32_t b; std::memcpy( &b, &x, 4 ); h = ( h ^ b ) * 1099511628211ull; }
+            return "lp:" + std::to_string( h ) + ( lp_bf16 ? ":bf16" : ":fp32" );
+        }
 
         bool PreparePreconditioner( const Mesh_T & M ) const
         {
             try { M.DeviceH1(); }
             catch( const std::exception & e ) { eprint( ClassName() + "::MultiplyPreconditioner: " + e.what() ); return false; }
             if( !PseudoLaplacianValues( M ) ) { return false; }
-            if( pc_diagonal_weight != 0.0 && !MetricValues( M ) ) { return false; }
-            return Check( rp_set_preconditioner( M.Device(), pc_diagonal_weight ), ClassName() + "::MultiplyPreconditioner" );
+            const bool learned = !lp_params.empty();
+            if( ( learned || pc_diagonal_weight != 0.0 ) && !MetricValues( M ) ) { return false; }
+            rp_ctx * c = M.Device();
+            // the device holds at most one learned model: switch it on or off as this object needs
+            std::string have;
+            for( auto & kv : M.device_state ) { if( kv.first.rfind( "lp:", 0 ) == 0 ) { have = kv.first; } }
+            const std::string want = learned ? LearnedKey() : std::string();
+            if( have != want )
+            {
+                if( !have.empty() ) { M.device_state.erase( have ); }
+                if( learned )
+                {
+                    const std::vector<float> feats = LearnedPreconditionerFeatures<DOM_DIM>( M.GetClusterTree() );
+                    if( !Check( rp_pc_set_model( c, lp_nfeat, feats.data(), lp_params.data(), lp_bf16 ? 1 : 0 ), ClassName() + "::MultiplyPreconditioner" ) ) { return false; }
+                    M.device_state[want] = 1;
+                }
+                else if( !Check( rp_pc_set_model( c, 0, nullptr, nullptr, 0 ), ClassName() + "::MultiplyPreconditioner" ) ) { return false; }
+            }
+            return Check( rp_set_preconditioner( c, pc_diagonal_weight ), ClassName() + "::MultiplyPreconditioner" );
         }
+
+    public:
+        // Learned preconditioner D^{1/2} (P0 + w diag(A)^-1) D^{1/2} (no counterpart in the reference):
+        // params = W1 (64 x nfeat), b1, w2, b2, log w as written by repulsor_amd/train_learned_pc.py.
+        void SetLearnedPreconditioner( const std::vector<float> & params, const int nfeat = LEARNED_PC_FEATURES, const bool bf16 = true )
+        {
+            if( (int)params.size() != LearnedPreconditionerParameterCount( nfeat ) )
+            {
+                eprint( ClassName() + "::SetLearnedPreconditioner: wrong parameter count" );
+                return;
+            }
+            lp_params = params; lp_nfeat = nfeat; lp_bf16 = bf16;
+        }
+
+        bool LoadLearnedPreconditioner( const std::string & path, const bool bf16 = true )
+        {
+            std::vector<float> p;
+            int nf = 0;
+            if( !ReadLearnedPreconditioner( path, p, nf ) ) { eprint( ClassName() + "::LoadLearnedPreconditioner: cannot read " + path ); return false; }
+            SetLearnedPreconditioner( p, nf, bf16 );
+            return true;
+        }
+
+        void ClearLearnedPreconditioner() { lp_params.clear(); lp_nfeat = 0; }
+        bool HasLearnedPreconditioner() const { return !lp_params.empty(); }
+
+        // One training step on the pairs recorded by the last Solve with RP_SOLVE_RECORD (see
+        // rp_pc_learn_step); params / m / v are updated in place.
+        bool PreconditionerLearnStep( const Mesh_T & M, const float lr, const int step, float * params, float * m, float * v,
+                                      double * loss, double * grad ) const
+        {
+            if( lp_params.empty() ) { eprint( ClassName() + "::PreconditionerLearnStep: no model set" ); return false; }
+            if( !PreparePreconditioner( M ) ) { return false; }
+            const bool ok = Check( rp_pc_learn_step( M.Device(), lr, step, params, m, v, loss, grad ), ClassName() + "::PreconditionerLearnStep" );
+            // the device model now differs from this object's parameters: force a reload next time
+            for( auto k = M.device_state.begin(); k != M.device_state.end(); )
+            {
+                if( k->first.rfind( "lp:", 0 ) == 0 ) { k = M.device_state.erase(k); } else { ++k; }
+            }
+            return ok;
+        }
+
+        // Vertex scalings sqrt(d_v) of the learned model on M.
+        bool LearnedScaling( const Mesh_T & M, double * sqrt_d ) const
+        {
+            if( lp_params.empty() || !PreparePreconditioner( M ) ) { return false; }
+            return Check( rp_pc_download_scaling( M.Device(), sqrt_d ), ClassName() + "::LearnedScaling" );
+        }
+
+    protected:
 
     public:
         // Preconditioner P = P0 + w diag(A)^-1 (w = 0: the reference's P0; w < 0: w estimated per state

@@ -161,10 +161,33 @@ int rp_preconditioner_weight( rp_ctx * ctx, double * weight );
  * No reference counterpart (used by the diagonal term of the preconditioner). */
 int rp_download_metric_diagonal( rp_ctx * ctx, double * diag );
 
+/* Learned preconditioner P = D^{1/2} (P0 + w diag(A)^-1) D^{1/2} (no reference counterpart; the
+ * reference preconditioner P0 stays the fallback). A 2-layer MLP of width 64 per cluster-tree node maps
+ * the node's features (C x nfeat floats, tree order) to s_c = softplus(o_c) > 0; the vertex scaling is
+ * d_v = exp(mean over the primitives containing v of the sum of log s_c along their leaf-to-root path).
+ * params: W1 (64 x nfeat, row-major), b1 (64), w2 (64), b2 (1), log w (1): 64 nfeat + 130 floats.
+ * The diagonal weight in use is exp(log w) times the base weight: the rp_set_preconditioner weight
+ * if positive, else the estimate lambda_max(P0 A) / 4.
+ * bf16 != 0 runs inference with BF16 weights and features. nfeat = 0 switches the model off. */
+int rp_pc_set_model( rp_ctx * ctx, int nfeat, const float * features, const float * params, int bf16 );
+
+/* Vertex scalings sqrt(d_v) of the current model (V values). */
+int rp_pc_download_scaling( rp_ctx * ctx, double * sqrt_d );
+
+/* One training step of the model on the pairs (r_k, e_k = x_last - x_k) recorded by the last
+ * rp_solve with RP_SOLVE_RECORD (residuals down to 1e-6 |r_0|): with y_k = r_k - r_last = A e_k,
+ * loss = min over alpha of mean_k |alpha P r_k - e_k|_A^2 / (e_k . y_k), in [0, 1] and invariant under
+ * scaling P (as PCG is); gradient by HIP
+ * kernels (FP32 MLP forward, FP64 reductions), then one Adam step (lr > 0; lr = 0 only computes the
+ * gradient) on params / m / v (64 nfeat + 130 floats each, updated in place; step counts from 1).
+ * loss and grad (64 nfeat + 130 doubles) may be NULL. */
+int rp_pc_learn_step( rp_ctx * ctx, float lr, int step, float * params, float * m, float * v, double * loss, double * grad );
+
 /* rp_solve flags */
 #define RP_SOLVE_NO_PRECONDITIONER 1
 #define RP_SOLVE_PC_NORM           2
 #define RP_SOLVE_GMRES             4
+#define RP_SOLVE_RECORD            8   /* PCG: keep r_k and x_k of every iteration for rp_pc_learn_step */
 
 /* X = alpha * A^+ B + beta * X on the three columns of B (zero initial guess).
  * Default: preconditioned conjugate gradients, stopping when every column has

@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 
 RP_OK, RP_EINVAL, RP_EHIP, RP_ENOMEM, RP_ESTAGNATION, RP_EUNSUPPORTED, RP_ENOTREADY = range(7)
-RP_SOLVE_NO_PRECONDITIONER, RP_SOLVE_PC_NORM, RP_SOLVE_GMRES = 1, 2, 4
+RP_SOLVE_NO_PRECONDITIONER, RP_SOLVE_PC_NORM, RP_SOLVE_GMRES, RP_SOLVE_RECORD = 1, 2, 4, 8
 PRECISIONS = {"fp64": 0, "mixed": 1}
 
 
@@ -98,6 +98,12 @@ _host_sigs = {
     "rpm_pl_multiply": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_h1_solve_device": (C.c_int, [_vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_op_set_pc_weight": (None, [_vp, C.c_double]),
+    "rpm_op_set_learned": (None, [_vp, C.c_int, C.POINTER(C.c_float), C.c_int]),
+    "rpm_learned_feature_count": (C.c_int, []),
+    "rpm_learned_features": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "rpm_pc_learn_step": (C.c_int, [_vp, _vp, C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float), _d, _d]),
+    "rpm_pc_scaling": (C.c_int, [_vp, _vp, _d]),
     "rpm_pc_weight": (C.c_int, [_vp, _vp, _d]),
     "rpm_metric_diagonal": (C.c_int, [_vp, _vp, _d]),
 }
@@ -452,23 +458,24 @@ class TangentPointMetric0(_Op):
         return float(np.sqrt(np.sum(X * Y)))
 
     def Solve(self, M: SimplicialMesh, alpha, B, beta, X, max_iter=1000, tolerance=1e-6, nrhs=3,
-              preconditioner=True, measure="l2", method="pcg", raise_on_stagnation=True):
+              preconditioner=True, measure="l2", method="pcg", raise_on_stagnation=True, record=False):
         """X = alpha A^+ B + beta X on the GPU; the record is kept for Solver_IterationCount and
         Solver_RelativeResiduals (reference MetricDimRestricted.hpp:232-289, MetricBase.hpp:126-134).
         method "pcg" with measure "l2" (|B - A X|_2 / |B|_2 <= tolerance per column) or "pc" (the same
         ratio in the preconditioner norm sqrt(<r, P r>)); method "gmres": the reference's
-        left-preconditioned GMRES, measure |P (B - A X)|_2 / |P B|_2."""
+        left-preconditioned GMRES, measure |P (B - A X)|_2 / |P B|_2. record keeps the PCG iterates on
+        the device for PreconditionerLearnStep."""
         M.build()
         B = np.ascontiguousarray(B, dtype=np.float64)
         assert X.dtype == np.float64 and X.flags.c_contiguous
         assert measure in ("l2", "pc") and method in ("pcg", "gmres")
-        it = (C.c_int * 1)()
+        it = (C.c_int * 2)()
         rel = np.zeros(3)
         flags = (0 if preconditioner else RP_SOLVE_NO_PRECONDITIONER) | (RP_SOLVE_PC_NORM if measure == "pc" else 0)
-        flags |= RP_SOLVE_GMRES if method == "gmres" else 0
+        flags |= (RP_SOLVE_GMRES if method == "gmres" else 0) | (RP_SOLVE_RECORD if record else 0)
         st = H.rpm_metric_solve(self._h, M._h, alpha, _p(B), B.shape[1], beta, _p(X), X.shape[1], nrhs,
                                 int(max_iter), float(tolerance), flags, 0, it, _p(rel))
-        self._iter, self._relres = int(it[0]), rel
+        self._iter, self._relres, self._fell_back = int(it[0]), rel, bool(it[1])
         if st == RP_ESTAGNATION and not raise_on_stagnation:
             return X
         _check(st)
@@ -479,3 +486,38 @@ class TangentPointMetric0(_Op):
 
     def Solver_RelativeResiduals(self):
         return getattr(self, "_relres", np.zeros(3))
+
+    def Solver_FellBack(self):
+        """Whether the last Solve fell back from the learned preconditioner to P0 + w diag(A)^-1."""
+        return getattr(self, "_fell_back", False)
+
+    # ------------------------------------------------------------------ learned preconditioner
+    def SetLearnedPreconditioner(self, params=None, path=None, bf16=True):
+        """P = D^{1/2} (P0 + w diag(A)^-1) D^{1/2} with the node MLP; params (float32, 64 F + 130) or a
+        model file written by repulsor_amd.learned_pc.save_model. None removes the model."""
+        from . import learned_pc
+        if path is not None:
+            params = learned_pc.load_model(path)
+        if params is None:
+            H.rpm_op_set_learned(self._h, 0, None, 0)
+            self._lp = None
+            return
+        self._lp = np.ascontiguousarray(params, dtype=np.float32)
+        F = learned_pc.feature_count()
+        assert self._lp.size == learned_pc.param_count(F)
+        H.rpm_op_set_learned(self._h, F, self._lp.ctypes.data_as(C.POINTER(C.c_float)), int(bool(bf16)))
+
+    def PreconditionerLearnStep(self, M: SimplicialMesh, params, m, v, step, lr):
+        """One Adam step of the model on the pairs recorded by the last Solve(record=True); params, m, v
+        (float32) are updated in place. Returns (loss, gradient)."""
+        fp = C.POINTER(C.c_float)
+        loss = np.zeros(1)
+        grad = np.zeros(params.size)
+        _check(H.rpm_pc_learn_step(self._h, M._h, float(lr), int(step), params.ctypes.data_as(fp), m.ctypes.data_as(fp),
+                                   v.ctypes.data_as(fp), _p(loss), _p(grad)))
+        return float(loss[0]), grad
+
+    def LearnedScaling(self, M: SimplicialMesh):
+        d = np.zeros(M.V)
+        _check(H.rpm_pc_scaling(self._h, M._h, _p(d)))
+        return d

@@ -24,6 +24,9 @@ namespace
         double q = 6, p = 12;
         int precision = 0;
         double pc_weight = 0.0;
+        std::vector<float> lp_params;
+        int lp_nfeat = 0;
+        bool lp_bf16 = true;
     };
 
     template<typename F>
@@ -50,6 +53,7 @@ namespace
         TangentPointMetric0<Mesh> A( o->q, o->p );
         A.SetPrecision( (Precision)o->precision );
         A.SetPreconditionerDiagonalWeight( o->pc_weight );
+        if( !o->lp_params.empty() ) { A.SetLearnedPreconditioner( o->lp_params, o->lp_nfeat, o->lp_bf16 ); }
         return A;
     }
 
@@ -234,6 +238,49 @@ void rpm_op_free( void * o ) { delete (OpHandle *)o; }
 // Diagonal weight of the metric's preconditioner P = P0 + w diag(A)^-1 (0: reference P0, < 0: estimated).
 void rpm_op_set_pc_weight( void * o, double w ) { ((OpHandle *)o)->pc_weight = w; }
 
+// Learned preconditioner model of the operator (nfeat = 0 removes it).
+void rpm_op_set_learned( void * o, int nfeat, const float * params, int bf16 )
+{
+    auto * h = (OpHandle *)o;
+    if( nfeat <= 0 || !params ) { h->lp_params.clear(); h->lp_nfeat = 0; return; }
+    h->lp_params.assign( params, params + LearnedPreconditionerParameterCount( nfeat ) );
+    h->lp_nfeat = nfeat;
+    h->lp_bf16 = bf16 != 0;
+}
+
+int rpm_learned_feature_count( void ) { return LEARNED_PC_FEATURES; }
+
+// Node features of the mesh's cluster tree (C x nfeat floats).
+int rpm_learned_features( void * h, float * out )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        constexpr int n = std::decay_t<decltype(M)>::DOM_DIM;
+        const auto f = LearnedPreconditionerFeatures<n>( M.GetClusterTree() );
+        std::copy( f.begin(), f.end(), out );
+        return RP_OK;
+    });
+}
+
+int rpm_pc_learn_step( void * op, void * h, float lr, int step, float * params, float * m, float * v, double * loss, double * grad )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        return A.PreconditionerLearnStep( M, lr, step, params, m, v, loss, grad ) ? RP_OK : RP_EINVAL;
+    });
+}
+
+int rpm_pc_scaling( void * op, void * h, double * sqrt_d )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        return make_metric<Mesh>( (OpHandle *)op ).LearnedScaling( M, sqrt_d ) ? RP_OK : RP_EINVAL;
+    });
+}
+
 // Weight in use on the mesh's device (after MultiplyPreconditioner / Solve).
 int rpm_pc_weight( void * op, void * h, double * w )
 {
@@ -333,7 +380,8 @@ int rpm_metric_precondition( void * op, void * h, double alpha, const double * X
     });
 }
 
-// X = alpha A^+ B + beta X; iters and relres (3) receive the solver's record. Returns RP_ESTAGNATION
+// X = alpha A^+ B + beta X; iters (2: iteration count, 1 if the learned preconditioner fell back) and
+// relres (3) receive the solver's record. Returns RP_ESTAGNATION
 // when max_iter was reached before the tolerance.
 int rpm_metric_solve( void * op, void * h, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
                       int nrhs, int max_iter, double tol, int solver_flags, int device_ptr, int * iters, double * relres )
@@ -345,7 +393,7 @@ int rpm_metric_solve( void * op, void * h, double alpha, const double * B, int64
         const bool ok = A.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tol, device_ptr, solver_flags );
         if( !ok ) { if( nrhs != 3 ) { g_err = "nrhs must be 3"; } return RP_EINVAL; }
         const auto rr = A.Solver_RelativeResiduals();
-        if( iters ) { *iters = A.Solver_IterationCount(); }
+        if( iters ) { iters[0] = A.Solver_IterationCount(); iters[1] = A.Solver_FellBack() ? 1 : 0; }
         bool conv = true;
         for( size_t k = 0; k < rr.size(); ++k ) { if( relres ) { relres[k] = rr[k]; } if( !( rr[k] <= tol ) ) { conv = false; } }
         return conv ? RP_OK : RP_ESTAGNATION;

@@ -159,6 +159,29 @@ namespace rp
 
     static dim3 grid256( int64_t n ) { return dim3( (unsigned)std::max<int64_t>( 1, (n + 255) / 256 ) ); }
 
+    // y += alpha x and x *= alpha on plain length-n vectors
+    __global__ __launch_bounds__(256) void k_axpy1( int64_t n, double alpha, const double * x, double * y )
+    {
+        const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        if( i < n ) { y[i] += alpha * x[i]; }
+    }
+
+    __global__ __launch_bounds__(256) void k_scale1( int64_t n, double alpha, double * x )
+    {
+        const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        if( i < n ) { x[i] *= alpha; }
+    }
+
+    void launch_axpy1( int64_t n, double alpha, const double * x, double * y, hipStream_t s )
+    {
+        if( n > 0 ) { hipLaunchKernelGGL( k_axpy1, grid256( n ), dim3(256), 0, s, n, alpha, x, y ); }
+    }
+
+    void launch_scale1( int64_t n, double alpha, double * x, hipStream_t s )
+    {
+        if( n > 0 ) { hipLaunchKernelGGL( k_scale1, grid256( n ), dim3(256), 0, s, n, alpha, x ); }
+    }
+
     void launch_mdot( int64_t n, const double * Vb, int64_t stride, int nv, const double * w, double * part, double * out, int accumulate, hipStream_t s )
     {
         if( nv <= 0 ) { return; }

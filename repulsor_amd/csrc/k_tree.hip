@@ -379,6 +379,9 @@ namespace rp
     void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s )
     { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<3>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
 
+    void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * src, double * leaf_out, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<1>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, src, (int64_t)0, leaf_out, 0 ); } }
+
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {
         if( !C ) { return; }

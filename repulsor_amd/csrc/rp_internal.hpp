@@ -219,7 +219,26 @@ struct rp_ctx
     int64_t diag_version = -1;
     double pc_weight_setting = 0.0;        // 0: P0 only, > 0: fixed weight, < 0: estimated per state
     double pc_weight = 0.0;                // weight in use
+    double pc_weight_est = 0.0;            // cached estimate lambda_max(P0 A) / 4
     int64_t pc_weight_key[3] = { -1, -1, -1 };
+
+    // learned preconditioner D^{1/2} (P0 + w diag(A)^-1) D^{1/2}
+    bool lp_enabled = false;
+    bool lp_bf16 = true;
+    int lp_F = 0;
+    rp::DBuf<float>  lp_feats;             // C x F node features
+    rp::DBuf<float>  lp_params;            // MLP parameters (FP32)
+    rp::DBuf<float>  lp_params16;          // BF16 copy (as raw storage)
+    rp::DBuf<float>  lp_m, lp_v;           // Adam moments
+    double lp_log_w = 0.0;                 // learned log factor on the base diagonal weight
+    double lp_alpha = 1.0;                 // optimal global scale of P from the last training step
+    bool lp_scale_ready = false;
+    rp::DBuf<double> lp_lam, lp_leaf, lp_sqrtd;     // C, L, V
+    rp::DBuf<double> lp_t1, lp_t2;                   // 3V work
+    rp::DBuf<double> lp_gl, lp_gp, lp_gc, lp_part, lp_grad, lp_acc;   // training
+    // PCG history for training pairs: r_k and x_k, k = 0..hist_n-1 (3V each)
+    rp::DBuf<double> hist_r, hist_x;
+    int hist_n = 0, hist_cap = 0;
     int solve_iters = 0;
     std::vector<double> solve_relres;
 

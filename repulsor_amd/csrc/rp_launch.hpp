@@ -138,6 +138,25 @@ namespace rp
     void launch_metric_diag( bool mixed, const DiagArgs & A, const void * nf_vals, double * S, double * dA, hipStream_t s );
     void launch_jacobi_add( int64_t V, double c, const double * X, int64_t ldX, const double * dA, double * Y, int64_t ldY, hipStream_t s );
 
+    // learned preconditioner (k_learn.hip)
+    constexpr int MLP_H = 64;      // hidden width
+    constexpr int MLP_FMAX = 16;   // maximum feature count
+    inline int mlp_param_count( int F ) { return MLP_H * F + 2 * MLP_H + 1; }
+    void launch_mlp_infer( bool bf16, int64_t C, int F, const float * feats, const float * p32, void * p16, double * lam, hipStream_t s );
+    void launch_vertex_scale( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_of_prim,
+                              const double * leaf_lam, double * sqrt_d, hipStream_t s );
+    void launch_row_scale( int64_t V, const double * sd, const double * X, int64_t ldX, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
+    void launch_grad_logd( int64_t V, const double * sd, const double * g, const double * q, const double * t, const double * r, double * out, hipStream_t s );
+    void launch_grad_prim( int64_t T, int size, const int32_t * pre_cols, const int64_t * vp_ptr, const double * g_logd, double * g_prim, hipStream_t s );
+    int64_t mlp_backward_blocks( int64_t C );
+    void launch_mlp_backward( int64_t C, int F, const float * feats, const float * p, const double * g_lam, double * part, double * grad, hipStream_t s );
+    void launch_adam( int NP, float * p, float * m, float * v, const double * grad, float lr, float b1, float b2, float eps, int t, hipStream_t s );
+    // root-path sums of one value per cluster, per leaf (k_tree.hip)
+    void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * src, double * leaf_out, hipStream_t s );
+
+    void launch_axpy1( int64_t n, double alpha, const double * x, double * y, hipStream_t s );
+    void launch_scale1( int64_t n, double alpha, double * x, hipStream_t s );
+
     constexpr int MDOT_PART = 256 * 3;   // partial-sum doubles per basis vector of launch_mdot
     void launch_mdot( int64_t n, const double * Vb, int64_t stride, int nv, const double * w, double * part, double * out, int accumulate, hipStream_t s );
     void launch_maxpy( int64_t n, const double * Vb, int64_t stride, int nv, const double * coef, double sign, double * w, hipStream_t s );

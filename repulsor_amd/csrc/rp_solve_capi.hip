@@ -130,52 +130,89 @@ namespace rp
     // by power iteration from the (smooth) vertex coordinates: the top of the spectrum of P0 A belongs to
     // smooth modes, while the rough modes that P0 under-weights get w diag(A)^-1 A ~ w.
     static constexpr int PC_POWER_STEPS = 12;
+
+    // Vertex scalings sqrt(d_v) of the learned preconditioner from the node MLP (FP32 or BF16 weights).
+    static void learned_scales( rp_ctx * ctx, bool bf16 )
+    {
+        const int F = ctx->lp_F;
+        ctx->lp_lam.require( (size_t)ctx->C ); ctx->lp_leaf.require( (size_t)ctx->L ); ctx->lp_sqrtd.require( (size_t)ctx->V );
+        ctx->lp_params16.require( (size_t)mlp_param_count( F ) );
+        launch_mlp_infer( bf16, ctx->C, F, ctx->lp_feats.get(), ctx->lp_params.get(), ctx->lp_params16.get(), ctx->lp_lam.get(), ctx->stream );
+        launch_leaf_path_sum1( ctx->L, ctx->C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->lp_lam.get(), ctx->lp_leaf.get(), ctx->stream );
+        launch_vertex_scale( ctx->V, ctx->size, ctx->vp_ptr.get(), ctx->vp_entry.get(), ctx->leaf_of_prim.get(), ctx->lp_leaf.get(),
+                             ctx->lp_sqrtd.get(), ctx->stream );
+        RP_HIP( hipGetLastError() );
+    }
+
+    // lambda_max(P0 A) / 4, cached per state of metric, pseudo-Laplacian and H1 factor.
+    static double estimated_weight( rp_ctx * ctx )
+    {
+        const int64_t key[3] = { ctx->metric_version, ctx->pl_version, ctx->h1_version };
+        if( key[0] == ctx->pc_weight_key[0] && key[1] == ctx->pc_weight_key[1] && key[2] == ctx->pc_weight_key[2] ) { return ctx->pc_weight_est; }
+        const int64_t V = ctx->V;
+        const size_t n3 = (size_t)(3 * V);
+        ctx->cg_p.require( n3 ); ctx->cg_Ap.require( n3 ); ctx->cg_z.require( n3 );
+        ctx->cg_scal.require( 32 ); ctx->cg_part.require( 3 * 1024 );
+        double * v = ctx->cg_p.get(); double * t = ctx->cg_Ap.get(); double * u = ctx->cg_z.get();
+        double * sc = ctx->cg_scal.get() + 24; double * part = ctx->cg_part.get();
+        hipStream_t s = ctx->stream;
+        launch_copy3( V, ctx->coords.get(), 3, v, 3, s );
+        double lam = 0.0;
+        for( int k = 0; k < PC_POWER_STEPS; ++k )
+        {
+            double nv[3], nu[3];
+            launch_dot3( V, v, v, part, sc, s );
+            RP_HIP( hipMemcpyAsync( nv, sc, sizeof(nv), hipMemcpyDeviceToHost, s ) );
+            metric_apply( ctx, 1.0, v, 3, 0.0, t, 3, true, true, false );
+            pc0_apply( ctx, 1.0, t, 3, 0.0, u, 3 );
+            launch_dot3( V, u, u, part, sc, s );
+            RP_HIP( hipMemcpyAsync( nu, sc, sizeof(nu), hipMemcpyDeviceToHost, s ) );
+            RP_HIP( hipStreamSynchronize( s ) );
+            lam = 0.0;
+            for( int c = 0; c < 3; ++c ) { if( nv[c] > 0 ) { lam = std::max( lam, std::sqrt( nu[c] / nv[c] ) ); } }
+            std::swap( v, u );
+            launch_dot3( V, v, v, part, sc, s );
+            launch_normalize3( V, v, sc, v, s );
+        }
+        ctx->pc_weight_est = 0.25 * lam;
+        for( int k = 0; k < 3; ++k ) { ctx->pc_weight_key[k] = key[k]; }
+        return ctx->pc_weight_est;
+    }
+
+    // Weight in use: the fixed setting or the estimate; a learned model multiplies its base weight (the
+    // setting if positive, else the estimate) by exp(learned log w).
+    static void update_weight( rp_ctx * ctx )
+    {
+        double w;
+        if( ctx->lp_enabled ) { w = ( ctx->pc_weight_setting > 0.0 ? ctx->pc_weight_setting : estimated_weight( ctx ) ) * std::exp( ctx->lp_log_w ); }
+        else { w = ctx->pc_weight_setting >= 0.0 ? ctx->pc_weight_setting : estimated_weight( ctx ); }
+        ctx->pc_weight = w;
+        if( w > 0.0 ) { ensure_metric_diag( ctx ); }
+    }
+
     static void prepare_pc( rp_ctx * ctx )
     {
-        if( ctx->pc_weight_setting >= 0.0 ) { ctx->pc_weight = ctx->pc_weight_setting; }
-        else
-        {
-            const int64_t key[3] = { ctx->metric_version, ctx->pl_version, ctx->h1_version };
-            if( !( key[0] == ctx->pc_weight_key[0] && key[1] == ctx->pc_weight_key[1] && key[2] == ctx->pc_weight_key[2] ) )
-            {
-                const int64_t V = ctx->V;
-                const size_t n3 = (size_t)(3 * V);
-                ctx->cg_p.require( n3 ); ctx->cg_Ap.require( n3 ); ctx->cg_z.require( n3 );
-                ctx->cg_scal.require( 32 ); ctx->cg_part.require( 3 * 1024 );
-                double * v = ctx->cg_p.get(); double * t = ctx->cg_Ap.get(); double * u = ctx->cg_z.get();
-                double * sc = ctx->cg_scal.get() + 24; double * part = ctx->cg_part.get();
-                hipStream_t s = ctx->stream;
-                launch_copy3( V, ctx->coords.get(), 3, v, 3, s );
-                double lam = 0.0;
-                for( int k = 0; k < PC_POWER_STEPS; ++k )
-                {
-                    double nv[3], nu[3];
-                    launch_dot3( V, v, v, part, sc, s );
-                    RP_HIP( hipMemcpyAsync( nv, sc, sizeof(nv), hipMemcpyDeviceToHost, s ) );
-                    metric_apply( ctx, 1.0, v, 3, 0.0, t, 3, true, true, false );
-                    pc0_apply( ctx, 1.0, t, 3, 0.0, u, 3 );
-                    launch_dot3( V, u, u, part, sc, s );
-                    RP_HIP( hipMemcpyAsync( nu, sc, sizeof(nu), hipMemcpyDeviceToHost, s ) );
-                    RP_HIP( hipStreamSynchronize( s ) );
-                    lam = 0.0;
-                    for( int c = 0; c < 3; ++c ) { if( nv[c] > 0 ) { lam = std::max( lam, std::sqrt( nu[c] / nv[c] ) ); } }
-                    std::swap( v, u );
-                    launch_dot3( V, v, v, part, sc, s );
-                    launch_normalize3( V, v, sc, v, s );
-                }
-                ctx->pc_weight = 0.25 * lam;
-                for( int k = 0; k < 3; ++k ) { ctx->pc_weight_key[k] = key[k]; }
-            }
-        }
-        if( ctx->pc_weight > 0.0 ) { ensure_metric_diag( ctx ); }
+        if( ctx->lp_enabled && !ctx->lp_scale_ready ) { learned_scales( ctx, ctx->lp_bf16 ); ctx->lp_scale_ready = true; }
+        update_weight( ctx );
     }
 
     // Y = alpha (P0 + w diag(A)^-1) X + beta Y (w = 0: the reference preconditioner). X and Y must not
     // overlap. Call prepare_pc first.
-    static void pc_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    static void pc2_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
     {
         pc0_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy );
         if( ctx->pc_weight > 0.0 && alpha != 0.0 ) { launch_jacobi_add( ctx->V, alpha * ctx->pc_weight, Xd, ldx, ctx->dA.get(), Yd, ldy, ctx->stream ); }
+    }
+
+    // Learned form: Y = alpha D^{1/2} (P0 + w diag(A)^-1) D^{1/2} X + beta Y.
+    static void pc_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    {
+        if( !ctx->lp_enabled ) { pc2_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy ); return; }
+        const int64_t V = ctx->V;
+        ctx->lp_t1.require( (size_t)(3 * V) ); ctx->lp_t2.require( (size_t)(3 * V) );
+        launch_row_scale( V, ctx->lp_sqrtd.get(), Xd, ldx, 1.0, 0.0, ctx->lp_t1.get(), 3, ctx->stream );
+        pc2_apply( ctx, 1.0, ctx->lp_t1.get(), 3, 0.0, ctx->lp_t2.get(), 3 );
+        launch_row_scale( V, ctx->lp_sqrtd.get(), ctx->lp_t2.get(), 3, alpha, beta, Yd, ldy, ctx->stream );
     }
 
     // Host staging of a V x 3 input/output pair for calls with host pointers.
@@ -221,9 +258,16 @@ namespace rp
     // start). Stopping measure: |r|_2 / |b|_2, or with pc_norm the preconditioner norm
     // sqrt(<r, P r>) / sqrt(<b, P b>). b, x: device V x 3 (ld 3). Returns the iteration count.
     static int pcg( rp_ctx * ctx, const double * b, double * x, int max_iter, double tol, bool use_pc, bool pc_norm,
-                    std::vector<double> & rel, bool & done )
+                    std::vector<double> & rel, bool & done, bool record )
     {
         const int64_t V = ctx->V;
+        auto keep = [&]( const double * rv, const double * xv )
+        {
+            if( !record || ctx->hist_n >= ctx->hist_cap ) { return; }
+            launch_copy3( V, rv, 3, ctx->hist_r.get() + (size_t)(3 * V) * ctx->hist_n, 3, ctx->stream );
+            launch_copy3( V, xv, 3, ctx->hist_x.get() + (size_t)(3 * V) * ctx->hist_n, 3, ctx->stream );
+            ++ctx->hist_n;
+        };
         const size_t n3 = (size_t)(3 * V);
         hipStream_t s = ctx->stream;
         ctx->cg_r.require( n3 ); ctx->cg_p.require( n3 ); ctx->cg_Ap.require( n3 ); ctx->cg_z.require( n3 );
@@ -243,6 +287,7 @@ namespace rp
         // x = 0, r = b, z = P r, p = z, rho = <r, z>
         launch_fill( x, (int64_t)n3, 0.0, s );
         launch_copy3( V, b, 3, r, 3, s );
+        keep( r, x );
         launch_dot3( V, r, r, part, bb, s );
         precondition( r, z );
         launch_copy3( V, z, 3, p, 3, s );
@@ -273,6 +318,7 @@ namespace rp
             metric_apply( ctx, 1.0, p, 3, 0.0, Ap, 3, true, true, false );
             launch_dot3( V, p, Ap, part, pAp, s );
             launch_cg_update( V, x, r, p, Ap, rho[cur], pAp, active, part, rr, s );
+            keep( r, x );
             if( !pc_norm )
             {
                 RP_HIP( hipMemcpyAsync( h_m, rr, sizeof(h_m), hipMemcpyDeviceToHost, s ) );
@@ -720,13 +766,21 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
             RP_HIP( hipStreamSynchronize( ctx->stream ) );
             finish_timing( ctx, 10 );
         }
+        const bool record = ( solver_flags & RP_SOLVE_RECORD ) && !( solver_flags & RP_SOLVE_GMRES );
+        ctx->hist_n = 0;
+        if( record )
+        {
+            ctx->hist_cap = std::min( max_iter, 1000 ) + 1;
+            ctx->hist_r.require( (size_t)( 3 * V ) * ctx->hist_cap );
+            ctx->hist_x.require( (size_t)( 3 * V ) * ctx->hist_cap );
+        }
         EventScope ev( ctx, 3 );
         launch_copy3( V, st.X, st.ldx, b, 3, ctx->stream );
         std::vector<double> rel( 3, 0.0 );
         bool done = false;
         const int it = ( solver_flags & RP_SOLVE_GMRES )
                      ? gmres( ctx, b, x, max_iter, tolerance, use_pc, rel, done )
-                     : pcg( ctx, b, x, max_iter, tolerance, use_pc, use_pc && ( solver_flags & RP_SOLVE_PC_NORM ), rel, done );
+                     : pcg( ctx, b, x, max_iter, tolerance, use_pc, use_pc && ( solver_flags & RP_SOLVE_PC_NORM ), rel, done, record );
         launch_axpby3( V, alpha, x, 3, beta, st.Y, st.ldy, ctx->stream );
         ev.stop();
         RP_HIP( hipGetLastError() );
@@ -746,10 +800,8 @@ int rp_set_preconditioner( rp_ctx * ctx, double diagonal_weight )
     {
         require( ctx != nullptr, "ctx is NULL" );
         require( std::isfinite( diagonal_weight ), "the weight must be finite" );
-        if( diagonal_weight == ctx->pc_weight_setting ) { return; }   // keeps an estimated weight
         ctx->pc_weight_setting = diagonal_weight;
         ctx->pc_weight = diagonal_weight > 0 ? diagonal_weight : 0.0;
-        for( int k = 0; k < 3; ++k ) { ctx->pc_weight_key[k] = -1; }
     });
 }
 
@@ -772,6 +824,169 @@ int rp_download_metric_diagonal( rp_ctx * ctx, double * diag )
         ensure_metric_diag( ctx );
         ctx->dA.download( diag, (size_t)ctx->V, ctx->stream );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
+    });
+}
+
+int rp_pc_set_model( rp_ctx * ctx, int nfeat, const float * features, const float * params, int bf16 )
+{
+    return guarded( "rp_pc_set_model", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree, "upload the tree first", RP_ENOTREADY );
+        if( nfeat == 0 ) { ctx->lp_enabled = false; ctx->lp_scale_ready = false; return; }
+        require( nfeat > 0 && nfeat <= MLP_FMAX && features && params, "bad model arguments" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int NP = mlp_param_count( nfeat );
+        for( int q = 0; q <= NP; ++q ) { require( std::isfinite( params[q] ), "model parameters must be finite" ); }
+        ctx->lp_F = nfeat;
+        ctx->lp_feats.upload( features, (size_t)( ctx->C * nfeat ), ctx->stream );
+        ctx->lp_params.upload( params, (size_t)NP, ctx->stream );
+        ctx->lp_log_w = params[NP];
+        ctx->lp_bf16 = bf16 != 0;
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->lp_enabled = true;
+        ctx->lp_scale_ready = false;
+    });
+}
+
+int rp_pc_download_scaling( rp_ctx * ctx, double * sqrt_d )
+{
+    return guarded( "rp_pc_download_scaling", [&]()
+    {
+        require( ctx != nullptr && sqrt_d != nullptr, "bad arguments" );
+        require( ctx->lp_enabled, "no learned preconditioner set", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        if( !ctx->lp_scale_ready ) { learned_scales( ctx, ctx->lp_bf16 ); ctx->lp_scale_ready = true; }
+        ctx->lp_sqrtd.download( sqrt_d, (size_t)ctx->V, ctx->stream );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+    });
+}
+
+int rp_pc_learn_step( rp_ctx * ctx, float lr, int step, float * params, float * m, float * v, double * loss, double * grad )
+{
+    return guarded( "rp_pc_learn_step", [&]()
+    {
+        require( ctx != nullptr && ctx->lp_F > 0, "set a learned preconditioner model first (rp_pc_set_model)", RP_ENOTREADY );
+        require( ctx->have_metric && ctx->have_pl && ctx->have_h1, "metric, pseudo-Laplacian and H1 factor are needed", RP_ENOTREADY );
+        require( ctx->hist_n >= 2, "record a PCG solve first (rp_solve with RP_SOLVE_RECORD)", RP_ENOTREADY );
+        require( params && m && v && step >= 1, "bad arguments" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int F = ctx->lp_F;
+        const int NP = mlp_param_count( F );
+        const int64_t V = ctx->V, n3 = 3 * V;
+        hipStream_t s = ctx->stream;
+        // model -> FP32 vertex scalings (training uses the FP32 forward)
+        ctx->lp_params.upload( params, (size_t)NP, s );
+        ctx->lp_log_w = params[NP];
+        learned_scales( ctx, false );
+        ctx->lp_scale_ready = false;
+        update_weight( ctx );
+        const double * sd = ctx->lp_sqrtd.get();
+        ctx->lp_acc.require( (size_t)( 7 * n3 + 3 * V ) );
+        double * e = ctx->lp_acc.get(); double * y = e + n3; double * u = y + n3; double * Au = u + n3;
+        double * sg = Au + n3; double * tv = sg + n3; double * Ja = tv + n3;
+        double * gl = Ja + n3; double * gacc1 = gl + V; double * gacc2 = gacc1 + V;
+        ctx->lp_t1.require( (size_t)n3 ); ctx->lp_t2.require( (size_t)std::max<int64_t>( n3, NP + 1 ) );
+        double * a = ctx->lp_t1.get(); double * q = ctx->lp_t2.get();
+        ctx->cg_scal.require( 32 ); ctx->cg_part.require( 3 * 1024 );
+        double * sc = ctx->cg_scal.get(); double * part = ctx->cg_part.get();
+        launch_fill( gacc1, 2 * V, 0.0, s );
+        const int ns = ctx->hist_n - 1;
+        const double * xfin = ctx->hist_x.get() + (size_t)n3 * ns;
+        const double * rfin = ctx->hist_r.get() + (size_t)n3 * ns;
+        auto dot = [&]( const double * X, const double * Y, int slot ) { launch_dot3( V, X, Y, part, sc + 3 * slot, s ); };
+        double h_rr0[3];
+        dot( ctx->hist_r.get(), ctx->hist_r.get(), 0 );
+        RP_HIP( hipMemcpyAsync( h_rr0, sc, sizeof(h_rr0), hipMemcpyDeviceToHost, s ) );
+        RP_HIP( hipStreamSynchronize( s ) );
+        const double rr0 = h_rr0[0] + h_rr0[1] + h_rr0[2];
+        // Per sample (r_k, e_k = x_last - x_k, y_k = r_k - r_last = A e_k, E_k = e_k . y_k) with u_k = P r_k:
+        //   N = sum_k u_k.A u_k / E_k,  Cs = sum_k u_k.y_k / E_k,  loss = 1 - Cs^2 / (N K)
+        // = min over alpha of mean_k |alpha u_k - e_k|_A^2 / |e_k|_A^2, invariant under scaling P. With the
+        // optimal alpha = Cs/N held fixed, dloss/du_k = (alpha^2 g1_k - alpha g2_k) / K, g1_k = 2 A u_k / E_k,
+        // g2_k = 2 y_k / E_k: the parameter gradients of g1 and g2 are accumulated apart and combined at the end.
+        double Nsum = 0.0, Csum = 0.0, gw1 = 0.0, gw2 = 0.0;
+        int used = 0;
+        for( int k = 0; k < ns; ++k )
+        {
+            const double * r = ctx->hist_r.get() + (size_t)n3 * k;
+            const double * xk = ctx->hist_x.get() + (size_t)n3 * k;
+            launch_copy3( V, xfin, 3, e, 3, s );
+            launch_axpby3( V, -1.0, xk, 3, 1.0, e, 3, s );              // e = x_last - x_k
+            launch_copy3( V, r, 3, y, 3, s );
+            launch_axpby3( V, -1.0, rfin, 3, 1.0, y, 3, s );             // y = r_k - r_last
+            launch_row_scale( V, sd, r, 3, 1.0, 0.0, a, 3, s );          // a = D^{1/2} r
+            pc2_apply( ctx, 1.0, a, 3, 0.0, q, 3 );                      // q = (P0 + w J) a
+            launch_row_scale( V, sd, q, 3, 1.0, 0.0, u, 3, s );          // u = D^{1/2} q
+            metric_apply( ctx, 1.0, u, 3, 0.0, Au, 3, true, true, false );
+            double h[12];
+            dot( e, y, 0 ); dot( u, Au, 1 ); dot( u, y, 2 ); dot( r, r, 3 );
+            RP_HIP( hipMemcpyAsync( h, sc, sizeof(h), hipMemcpyDeviceToHost, s ) );
+            RP_HIP( hipStreamSynchronize( s ) );
+            const double Ek = h[0] + h[1] + h[2], Nk = h[3] + h[4] + h[5], Ck = h[6] + h[7] + h[8], rr = h[9] + h[10] + h[11];
+            // e_k of the last iterates is dominated by the final error: use residuals down to 1e-6 |r_0|
+            if( !( Ek > 0.0 ) || !( Nk > 0.0 ) || rr < 1e-12 * rr0 ) { continue; }
+            ++used;
+            Nsum += Nk / Ek; Csum += Ck / Ek;
+            launch_fill( Ja, n3, 0.0, s );
+            launch_jacobi_add( V, 1.0, a, 3, ctx->dA.get(), Ja, 3, s );  // J a
+            for( int part_k = 0; part_k < 2; ++part_k )
+            {
+                double * g = part_k == 0 ? Au : y;                        // g1 = (2/E) A u, g2 = (2/E) y
+                launch_scale1( n3, 2.0 / Ek, g, s );
+                launch_row_scale( V, sd, g, 3, 1.0, 0.0, sg, 3, s );     // sg = D^{1/2} g
+                pc2_apply( ctx, 1.0, sg, 3, 0.0, tv, 3 );                // t = (P0 + w J) D^{1/2} g
+                launch_grad_logd( V, sd, g, q, tv, r, gl, s );
+                launch_axpy1( V, 1.0, gl, part_k == 0 ? gacc1 : gacc2, s );
+                double h_gw[3];
+                dot( sg, Ja, 0 );
+                RP_HIP( hipMemcpyAsync( h_gw, sc, sizeof(h_gw), hipMemcpyDeviceToHost, s ) );
+                RP_HIP( hipStreamSynchronize( s ) );
+                ( part_k == 0 ? gw1 : gw2 ) += ctx->pc_weight * ( h_gw[0] + h_gw[1] + h_gw[2] );
+            }
+        }
+        require( used > 0 && Nsum > 0.0, "the recorded solve holds no usable residual/error pairs", RP_ENOTREADY );
+        const double alpha = Csum / Nsum;
+        const double loss_val = 1.0 - Csum * alpha / used;
+        ctx->lp_alpha = alpha;
+        launch_scale1( V, alpha * alpha, gacc1, s );
+        launch_axpy1( V, -alpha, gacc2, gacc1, s );                      // gacc = alpha^2 gacc1 - alpha gacc2
+        const double gw_sum = alpha * alpha * gw1 - alpha * gw2;
+        const double inv = 1.0 / used;
+        // dL/dlam per cluster: per primitive, then subtree sums (leaf clusters, then levels)
+        ctx->lp_gp.require( (size_t)ctx->T ); ctx->lp_gc.require( (size_t)ctx->C );
+        launch_grad_prim( ctx->T, ctx->size, ctx->pre_cols.get(), ctx->vp_ptr.get(), gacc1, ctx->lp_gp.get(), s );
+        launch_up_leaves( ctx->L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->lp_gp.get(), ctx->lp_gc.get(), s, 1 );
+        for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
+        {
+            const int64_t b = ctx->h_level_ptr[d], en = ctx->h_level_ptr[d + 1];
+            launch_up_level( en - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->lp_gc.get(), s, 1 );
+        }
+        launch_scale1( ctx->C, inv, ctx->lp_gc.get(), s );                                        // mean over samples
+        ctx->lp_part.require( (size_t)( mlp_backward_blocks( ctx->C ) * NP ) );
+        ctx->lp_grad.require( (size_t)( NP + 1 ) );
+        launch_mlp_backward( ctx->C, F, ctx->lp_feats.get(), ctx->lp_params.get(), ctx->lp_gc.get(), ctx->lp_part.get(), ctx->lp_grad.get(), s );
+        const double gw = gw_sum * inv;
+        RP_HIP( hipMemcpyAsync( ctx->lp_grad.get() + NP, &gw, sizeof(double), hipMemcpyHostToDevice, s ) );
+        if( lr > 0.0f )
+        {
+            ctx->lp_m.upload( m, (size_t)( NP + 1 ), s ); ctx->lp_v.upload( v, (size_t)( NP + 1 ), s );
+            ctx->lp_t2.require( (size_t)( NP + 1 ) );
+            float * p32 = reinterpret_cast<float *>( ctx->lp_t2.get() );     // all parameters incl. log w
+            RP_HIP( hipMemcpyAsync( p32, params, sizeof(float) * ( NP + 1 ), hipMemcpyHostToDevice, s ) );
+            launch_adam( NP + 1, p32, ctx->lp_m.get(), ctx->lp_v.get(), ctx->lp_grad.get(), lr, 0.9f, 0.999f, 1e-8f, step, s );
+            RP_HIP( hipMemcpyAsync( params, p32, sizeof(float) * ( NP + 1 ), hipMemcpyDeviceToHost, s ) );
+            ctx->lp_m.download( m, (size_t)( NP + 1 ), s ); ctx->lp_v.download( v, (size_t)( NP + 1 ), s );
+        }
+        if( grad ) { ctx->lp_grad.download( grad, (size_t)( NP + 1 ), s ); }
+        RP_HIP( hipGetLastError() );
+        RP_HIP( hipStreamSynchronize( s ) );
+        if( lr > 0.0f )
+        {
+            ctx->lp_params.upload( params, (size_t)NP, s );
+            ctx->lp_log_w = params[NP];
+            RP_HIP( hipStreamSynchronize( s ) );
+        }
+        if( loss ) { *loss = loss_val; }
     });
 }
 
