@@ -238,11 +238,6 @@ namespace Repulsor
             {
                 const Tree_T & S = GetClusterTree();
                 const BCT_T & B = GetBlockClusterTree();
-                if( B.VeryNearFieldInteractionCount() > 0 )
-                {
-                    throw std::runtime_error( "very-near field is non-empty (" + std::to_string(B.VeryNearFieldInteractionCount())
-                        + " pairs); the adaptive very-near-field kernel is not implemented on the GPU path" );
-                }
                 auto chk = [&]( int st, const char * w ) { if( st != RP_OK ) { throw std::runtime_error( std::string(w) + ": " + rp_last_error() ); } };
                 chk( rp_upload_mesh( c, DOM_DIM, VertexCount(), SimplexCount(), simplices.data(), VertexCharges() ), "rp_upload_mesh" );
                 chk( rp_upload_tree( c, S.ClusterCount(), S.C_left.data(), S.C_right.data(), S.C_begin.data(), S.C_end.data(),
@@ -251,6 +246,9 @@ namespace Repulsor
                 chk( rp_upload_near_tiles( c, Tl.leaf_count, Tl.leaf_begin.data(), Tl.leaf_end.data(), Tl.col_ptr.data(),
                                            Tl.col_prim.data(), Tl.col_mask.data() ), "rp_upload_near_tiles" );
                 chk( rp_upload_far_pattern( c, B.far.rows, B.far.rp.data(), B.far.ci.data() ), "rp_upload_far_pattern" );
+                const SymmetricCSR vf = B.VeryNearCSR();
+                chk( rp_upload_verynear_pattern( c, vf.rows, vf.rp.data(), vf.ci.empty() ? nullptr : vf.ci.data(),
+                                                 B.NearFieldSeparationParameter(), B.Settings().max_refinement ), "rp_upload_verynear_pattern" );
                 dev_topology = true;
                 dev_data = false;
             }

@@ -70,6 +70,21 @@ namespace Repulsor
             return true;
         }
 
+        // Very-near-field metric values: vf_vals (a0, a1) per entry of the VeryNear pattern (CSR order),
+        // vf_diag T x 2 in tree order (TP0_Traversor::VF_Compute, TP0_Traversor.hpp:280-308).
+        bool DownloadVeryNearMetricValues( const Mesh_T & M, double * vf_vals, double * vf_diag ) const
+        {
+            if( !MetricValues( M ) ) { return false; }
+            return Check( rp_download_vf_metric( M.Device(), vf_vals, vf_diag ), ClassName() );
+        }
+
+        // Pseudo-Laplacian very-near-field values: vf_vals per VeryNear entry, vf_diag T (tree order).
+        bool DownloadVeryNearPseudoLaplacianValues( const Mesh_T & M, double * vf_vals, double * vf_diag ) const
+        {
+            if( !PseudoLaplacianValues( M ) ) { return false; }
+            return Check( rp_download_pl_vf( M.Device(), vf_vals, vf_diag ), ClassName() );
+        }
+
         // Y = alpha * A X + beta * Y (MetricDimRestricted::MultiplyMetric, MetricDimRestricted.hpp:109-139).
         // nrhs must equal 3; otherwise an error is printed and Y is left untouched.
         bool MultiplyMetric( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,

@@ -81,13 +81,26 @@ int rp_upload_near_tiles( rp_ctx * ctx, int32_t leaf_count,
  * Replaces: BlockClusterTree::Far() (reference src/BlockClusterTree.hpp:321-324). */
 int rp_upload_far_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp, const int32_t * ci );
 
+/* Very-near field: full symmetric CSR over primitives (rows = simplex count, tree order, no diagonal)
+ * of the pairs that are neither adjacent nor separated by the near-field MAC, with the near separation
+ * parameter theta and the deepest subdivision level of the tangent-point kernel (clamped as the
+ * reference clamps it to the hierarchy's maximal level). rp[rows] = 0 clears it.
+ * Replaces: BlockClusterTree::VeryNear() (reference src/BlockClusterTree.hpp:166, built at :313) and
+ * AdaptivitySettings{max_refinement, theta} (src/Settings.hpp:49-54). */
+int rp_upload_verynear_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp, const int32_t * ci,
+                                double theta, int max_refinement );
+
 /* Per-state data, after construction and after every SemiStaticUpdate:
  * V_coords V x 3, P_near T x (1+3(n+1)+6) and C_far C x 10 in tree order.
  * Replaces: ClusterTree::SemiStaticUpdate data path (reference src/ClusterTree.hpp:983-993). */
 int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near, const double * C_far );
 
-/* TangentPointEnergy0 kernels over the near and far fields (flags: RP_ENERGY | RP_DIFF | RP_METRIC).
- * energy_out may be NULL. RP_METRIC stores the metric blocks on the device for rp_multiply_metric.
+/* TangentPointEnergy0 kernels over the very-near, near and far fields (flags: RP_ENERGY | RP_DIFF |
+ * RP_METRIC). energy_out may be NULL. RP_METRIC stores the metric blocks on the device for
+ * rp_multiply_metric. The very-near field is computed in FP64 at either precision; its energy terms
+ * are a E b per adaptive evaluation without the quadrature weight, as the reference sums them
+ * (TP0_Kernel_VF.hpp:273), while its derivative and metric terms carry the weight. A very-near
+ * traversal that exceeds 2^24 evaluations on one level-1 sub-pair stops with RP_EUNSUPPORTED.
  * Replaces: TP0_Traversor<...>::Compute (reference src/Energies/TangentPointEnergy0/TP0_Traversor.hpp:129-253). */
 int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precision, double * energy_out );
 
@@ -104,6 +117,18 @@ int rp_assemble_vertex_gradient( rp_ctx * ctx, double alpha, double beta, double
  * Any pointer may be NULL. Replaces: MetricValueContainer read-out (reference src/FMM/FMM_Configurator.hpp:5-20). */
 int rp_download_metric( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * ff_vals, double * ff_diag );
 
+/* Very-near-field metric values of the last RP_METRIC compute: vf_vals nnz x 2 (a0, a1) in the CSR
+ * order of the VF pattern, vf_diag T x 2 (tree order). The values are the reference kernel's per-pair
+ * sums (TP0_Kernel_VF::compute, src/Energies/TangentPointEnergy0/TP0_Kernel_VF.hpp:300-331); the
+ * reference stores them with a stride of 7 (off-diagonal) and 16 (diagonal) but reads them with a
+ * stride of 2 (TP0_Traversor.hpp:453-478), which this library does not reproduce. Either pointer may
+ * be NULL. */
+int rp_download_vf_metric( rp_ctx * ctx, double * vf_vals, double * vf_diag );
+
+/* Statistics of the very-near field of the last rp_tp0_compute: adaptive evaluations and deepest
+ * subdivision level (FMM_Kernel_VF::PrintReport, src/FMM/FMM_Kernel_VF_Common.hpp:119-133). */
+int rp_vf_stats( rp_ctx * ctx, int64_t * evaluations, int * max_level );
+
 /* Y = alpha * A X + beta * Y, X and Y V x nrhs (nrhs must be 3), ld >= 3.
  * Replaces: MetricDimRestricted::MultiplyMetric (reference src/Metrics/MetricDimRestricted.hpp:109-139). */
 int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ldX,
@@ -119,6 +144,11 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision );
 /* Read-out of the pseudo-Laplacian values (near values in tile order, far values in CSR order).
  * Any pointer may be NULL. Replaces: the PseudoLaplacian MetricValues container read-out. */
 int rp_download_pl( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * ff_vals, double * ff_diag );
+
+/* Very-near-field pseudo-Laplacian values of the last rp_pl_compute: vf_vals nnz (CSR order of the VF
+ * pattern), vf_diag T. Replaces: PseudoLaplacian_Traversor::VF_Compute
+ * (reference src/Metrics/PseudoLaplacian/PseudoLaplacian_Traversor.hpp:176-202). */
+int rp_download_pl_vf( rp_ctx * ctx, double * vf_vals, double * vf_diag );
 
 /* Y = alpha * L X + beta * Y with the LowOrder pre/post operator (nrhs must be 3).
  * Replaces: PseudoLaplacian's MultiplyMetric (reference src/Metrics/MetricDimRestricted.hpp:109-139 with

@@ -221,3 +221,62 @@ def pl_multiply(n, nV, tree, pre_cols, pre_vals, near_pat, pl, far_pat, alpha, X
          _p(c64(pl["FF_diag"])),
          alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], int(nf), int(ff))
     return Y
+
+
+_vf = _f("orc_vf_compute", C.c_double, [C.c_int, C.c_int64, _d, _i64, _i32, _i64, C.c_double, C.c_int, C.c_double, C.c_double,
+                                        C.c_int, _d, _d, _d, _d])
+_plvf = _f("orc_pl_vf", None, [C.c_int, C.c_int64, _d, _i64, _i32, _i64, C.c_double, C.c_int, C.c_double, _d, _d, _d])
+
+
+def vf_compute(n, P_near, vf_pat, theta, max_refinement, q, p, flags):
+    """TP0 very-near field restatement. vf_pat = (rp, ci, diag) full symmetric CSR. flags 1|2|4, +8 for
+    the w-weighted energy (a pinning aid, not the reference's sum). Returns energy, P_D_near_tree
+    (T x ND, VF part only), VF (nnz x 2), VF_diag (T x 2), evaluations, max_level."""
+    P_near = c64(P_near)
+    T, nd = P_near.shape
+    rp, ci, dg = c64i(vf_pat[0]), c32(vf_pat[1]), c64i(vf_pat[2])
+    PDn = np.zeros((T, nd)); vals = np.zeros((max(1, len(ci)), 2)); diag = np.zeros((T, 2)); st = np.zeros(2)
+    en = _vf(n, T, _p(P_near), _p(rp, _i64), _p(ci if len(ci) else np.zeros(1, np.int32), _i32), _p(dg, _i64),
+             float(theta), int(max_refinement), float(q), float(p), int(flags), _p(PDn), _p(vals), _p(diag), _p(st))
+    return {"energy": en, "P_D_near_tree": PDn, "VF": vals[: len(ci)], "VF_diag": diag,
+            "evaluations": int(st[0]), "max_level": int(st[1])}
+
+
+def pl_vf(n, P_near, vf_pat, theta, max_refinement, s):
+    """Pseudo-Laplacian very-near field restatement: VF (nnz), VF_diag (T)."""
+    P_near = c64(P_near)
+    T = P_near.shape[0]
+    rp, ci, dg = c64i(vf_pat[0]), c32(vf_pat[1]), c64i(vf_pat[2])
+    vals = np.zeros(max(1, len(ci))); diag = np.zeros(T); st = np.zeros(2)
+    _plvf(n, T, _p(P_near), _p(rp, _i64), _p(ci if len(ci) else np.zeros(1, np.int32), _i32), _p(dg, _i64),
+          float(theta), int(max_refinement), float(s), _p(vals), _p(diag), _p(st))
+    return {"VF": vals[: len(ci)], "VF_diag": diag, "evaluations": int(st[0])}
+
+
+def merge_patterns(pat_a, vals_a, pat_b, vals_b):
+    """Row-wise union of two disjoint CSR patterns with their values (for applying NF and VF blocks in
+    one near-field product). Returns ((rp, ci, diag), vals); columns sorted per row."""
+    rpa, cia = np.asarray(pat_a[0]), np.asarray(pat_a[1])
+    rpb, cib = np.asarray(pat_b[0]), np.asarray(pat_b[1])
+    vals_a, vals_b = np.asarray(vals_a), np.asarray(vals_b)
+    T = len(rpa) - 1
+    rp = [0]; ci = []; vv = []; dg = []
+    for i in range(T):
+        cols = np.concatenate([cia[rpa[i]:rpa[i + 1]], cib[rpb[i]:rpb[i + 1]]])
+        vals = np.concatenate([vals_a[rpa[i]:rpa[i + 1]], vals_b[rpb[i]:rpb[i + 1]]])
+        o = np.argsort(cols, kind="stable")
+        cols, vals = cols[o], vals[o]
+        dg.append(rp[-1] + int(np.searchsorted(cols, i, side="right")))
+        ci.append(cols); vv.append(vals); rp.append(rp[-1] + len(cols))
+    ci = np.concatenate(ci).astype(np.int32) if ci else np.zeros(0, np.int32)
+    vv = np.concatenate(vv) if vv else np.zeros((0,) + vals_a.shape[1:])
+    return (np.array(rp, np.int64), ci, np.array(dg, np.int64)), vv
+
+
+_sd2 = _f("orc_simplex_dist2", C.c_double, [C.c_int, _d, _d])
+
+
+def simplex_dist2(P, Q):
+    """Squared distance of two disjoint simplices (2 or 3 points each, rows = points)."""
+    P, Q = c64(P), c64(Q)
+    return _sd2(P.shape[0], _p(P), _p(Q))

@@ -815,3 +815,417 @@ void orc_pl_multiply( int n, LInt V, LInt T, LInt C,
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Very-near field (VF): adaptive subdivision quadrature for primitive pairs that are neither
+// adjacent nor separated by the near-field MAC.
+//   FMM_Kernel_VF_Common::Compute        src/FMM/FMM_Kernel_VF_Common.hpp:3-116
+//   FMM_Kernel_VF::loadS/loadT/writeS/writeT src/FMM/FMM_Kernel_VF.hpp:243-305
+//   SimplexHierarchy RequireSimplex/RequireSubsimplex src/SimplexHierarchy/SimplexHierarchy_Details.hpp,
+//                    ToChild/ToParent src/SimplexHierarchy/SimplexHierarchy_1D.hpp, _2D.hpp
+//   TP0_Kernel_VF::compute               src/Energies/TangentPointEnergy0/TP0_Kernel_VF.hpp:158-338
+//   PseudoLaplacian_Kernel_VF::compute   src/Metrics/PseudoLaplacian/PseudoLaplacian_Kernel_VF.hpp:135-176
+// The GJK multipole acceptance criterion (src/GJK.hpp:1162-1172) is restated with the exact
+// simplex distance: separated <=> max(r_S^2, r_T^2) < theta^2 dist(S,T)^2, with r^2 of a sub-simplex
+// = scale^2 * r^2 of its primitive (max squared vertex distance from the barycenter).
+namespace
+{
+    struct V3o { double x, y, z; };
+    inline V3o sub3( V3o a, V3o b ) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+    inline V3o add3( V3o a, V3o b ) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+    inline V3o mul3( double s, V3o a ) { return { s * a.x, s * a.y, s * a.z }; }
+    inline double dot3( V3o a, V3o b ) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+    inline double clamp01o( double t ) { return t < 0 ? 0 : (t > 1 ? 1 : t); }
+
+    // Closest point on triangle abc to p by Voronoi regions of the vertices, edges and face.
+    V3o ClosestOnTriangle( V3o p, V3o a, V3o b, V3o c )
+    {
+        const V3o ab = sub3(b, a), ac = sub3(c, a), ap = sub3(p, a);
+        const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+        if( d1 <= 0 && d2 <= 0 ) { return a; }
+        const V3o bp = sub3(p, b);
+        const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+        if( d3 >= 0 && d4 <= d3 ) { return b; }
+        const double vc = d1 * d4 - d3 * d2;
+        if( vc <= 0 && d1 >= 0 && d3 <= 0 ) { return add3( a, mul3( d1 / (d1 - d3), ab ) ); }
+        const V3o cp = sub3(p, c);
+        const double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+        if( d6 >= 0 && d5 <= d6 ) { return c; }
+        const double vb = d5 * d2 - d1 * d6;
+        if( vb <= 0 && d2 >= 0 && d6 <= 0 ) { return add3( a, mul3( d2 / (d2 - d6), ac ) ); }
+        const double va = d3 * d6 - d5 * d4;
+        if( va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0 ) { return add3( b, mul3( (d4 - d3) / ((d4 - d3) + (d5 - d6)), sub3(c, b) ) ); }
+        const double den = 1.0 / (va + vb + vc);
+        return add3( a, add3( mul3( vb * den, ab ), mul3( vc * den, ac ) ) );
+    }
+
+    // Squared distance of the segments [p1,q1], [p2,q2] (clamped parametric closest points).
+    double SegmentsDist2( V3o p1, V3o q1, V3o p2, V3o q2 )
+    {
+        const V3o d1 = sub3(q1, p1), d2 = sub3(q2, p2), r = sub3(p1, p2);
+        const double a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r);
+        double s = 0, t = 0;
+        if( a <= 1e-300 && e <= 1e-300 ) { return dot3(r, r); }
+        if( a <= 1e-300 ) { t = clamp01o( f / e ); }
+        else
+        {
+            const double c = dot3(d1, r);
+            if( e <= 1e-300 ) { s = clamp01o( -c / a ); }
+            else
+            {
+                const double b = dot3(d1, d2), den = a * e - b * b;
+                s = (den != 0) ? clamp01o( (b * f - c * e) / den ) : 0.0;
+                t = (b * s + f) / e;
+                if( t < 0 ) { t = 0; s = clamp01o( -c / a ); }
+                else if( t > 1 ) { t = 1; s = clamp01o( (b - c) / a ); }
+            }
+        }
+        const V3o w = sub3( add3( p1, mul3( s, d1 ) ), add3( p2, mul3( t, d2 ) ) );
+        return dot3(w, w);
+    }
+
+    // Squared distance of two disjoint simplices with m = 2 (segments) or 3 (triangles) points each.
+    double SimplexDist2( const V3o * P, const V3o * Q, int m )
+    {
+        double d2 = 1e300;
+        if( m == 2 ) { return SegmentsDist2( P[0], P[1], Q[0], Q[1] ); }
+        for( int i = 0; i < 3; ++i )
+        {
+            V3o c = ClosestOnTriangle( P[i], Q[0], Q[1], Q[2] ); d2 = std::min( d2, dot3( sub3(P[i], c), sub3(P[i], c) ) );
+            c = ClosestOnTriangle( Q[i], P[0], P[1], P[2] );     d2 = std::min( d2, dot3( sub3(Q[i], c), sub3(Q[i], c) ) );
+        }
+        for( int i = 0; i < 3; ++i ) { for( int j = 0; j < 3; ++j ) { d2 = std::min( d2, SegmentsDist2( P[i], P[(i+1)%3], Q[j], Q[(j+1)%3] ) ); } }
+        return d2;
+    }
+
+    // Sub-simplex of one primitive: barycentric corners and center, level, child ids, path code.
+    struct Hier
+    {
+        int n = 2;
+        double corners[3][3], center[3];
+        int level = 0, child = 0, former = -1;
+        uint64_t column = 0;
+        double scale = 1, weight = 1;
+
+        int Children() const { return n == 2 ? 4 : 2; }
+        void Reset()
+        {
+            const int S = n + 1;
+            for( int i = 0; i < S; ++i ) { center[i] = 1.0 / S; for( int j = 0; j < S; ++j ) { corners[i][j] = (i == j) ? 1.0 : 0.0; } }
+            level = 0; child = 0; former = -1; column = 0; scale = 1; weight = 1;
+        }
+        // Child k < n+1 is the half-size copy at corner k; child 3 of a triangle the inverted middle.
+        void ToChild( int k )
+        {
+            const int S = n + 1;
+            former = child; child = k; column = (uint64_t)Children() * column + (uint64_t)k; ++level;
+            scale *= 0.5; weight *= (n == 2 ? 0.25 : 0.5);
+            if( k < S )
+            {
+                for( int i = 0; i < S; ++i )
+                {
+                    if( i != k ) { for( int j = 0; j < S; ++j ) { corners[i][j] = corners[k][j] + 0.5 * (corners[i][j] - corners[k][j]); } }
+                    else         { for( int j = 0; j < S; ++j ) { center[j] = corners[k][j] + 0.5 * (center[j] - corners[k][j]); } }
+                }
+            }
+            else
+            {
+                for( int i = 0; i < S; ++i ) { for( int j = 0; j < S; ++j ) { corners[i][j] = center[j] - 0.5 * (corners[i][j] - center[j]); } }
+            }
+        }
+        void ToParent()
+        {
+            const int S = n + 1;
+            const int k = child;
+            column = (column - (uint64_t)child) / (uint64_t)Children();
+            former = child; child = (int)(column % (uint64_t)Children()); --level;
+            scale *= 2.0; weight *= (n == 2 ? 4.0 : 2.0);
+            if( k < S )
+            {
+                for( int i = 0; i < S; ++i )
+                {
+                    if( i != k ) { for( int j = 0; j < S; ++j ) { corners[i][j] = corners[k][j] + 2.0 * (corners[i][j] - corners[k][j]); } }
+                    else         { for( int j = 0; j < S; ++j ) { center[j] = corners[k][j] + 2.0 * (center[j] - corners[k][j]); } }
+                }
+            }
+            else
+            {
+                for( int i = 0; i < S; ++i ) { for( int j = 0; j < S; ++j ) { corners[i][j] = center[j] - 2.0 * (corners[i][j] - center[j]); } }
+            }
+        }
+        // Vertices of the sub-simplex in space from the primitive's hull (SIZE x 3).
+        void Vertices( const double * hull, V3o * out ) const
+        {
+            const int S = n + 1;
+            for( int i = 0; i < S; ++i )
+            {
+                double c[3];
+                for( int k = 0; k < 3; ++k ) { c[k] = corners[i][0] * hull[k]; for( int j = 1; j < S; ++j ) { c[k] += corners[i][j] * hull[3 * j + k]; } }
+                out[i] = { c[0], c[1], c[2] };
+            }
+        }
+    };
+
+    // Squared radius of a primitive as the reference serializes it (Polytope::FromCoordinates,
+    // src/Primitives/Polytope.hpp:69-111): max squared vertex distance from the vertex average.
+    double PrimitiveSquaredRadius( const double * hull, int S )
+    {
+        double c[3];
+        for( int k = 0; k < 3; ++k ) { c[k] = hull[k]; for( int j = 1; j < S; ++j ) { c[k] += hull[3 * j + k]; } c[k] *= 1.0 / S; }
+        double r2 = 0;
+        for( int j = 0; j < S; ++j )
+        {
+            double d = hull[3 * j] - c[0];
+            double sq = d * d;
+            for( int k = 1; k < 3; ++k ) { d = hull[3 * j + k] - c[k]; sq += d * d; }
+            r2 = std::max( r2, sq );
+        }
+        return r2;
+    }
+
+    // Depth-first traversal of FMM_Kernel_VF_Common::Compute for one primitive pair: calls
+    // eval(S, T) for every admissible sub-pair or every sub-pair at level max_refinement.
+    template<typename Eval>
+    void VFTraverse( int n, const double * hullS, const double * hullT, double r2S, double r2T, double theta2, int max_refinement,
+                     Hier & S, Hier & T, double * stats, Eval && eval )
+    {
+        S.n = n; T.n = n; S.Reset(); T.Reset();
+        const int m = n + 1;
+        bool from_above = true, go_on = true;
+        S.ToChild(0); T.ToChild(0);
+        while( go_on )
+        {
+            if( from_above )
+            {
+                bool leaf = S.level >= max_refinement;
+                if( !leaf )
+                {
+                    V3o PS[3], PT[3];
+                    S.Vertices( hullS, PS ); T.Vertices( hullT, PT );
+                    const double rr = std::max( S.scale * S.scale * r2S, T.scale * T.scale * r2T );
+                    leaf = rr < theta2 * SimplexDist2( PS, PT, m );
+                }
+                if( leaf )
+                {
+                    stats[0] += 1.0; stats[1] = std::max( stats[1], (double)std::min( S.level, max_refinement ) );
+                    eval( S, T );
+                    S.ToParent(); T.ToParent(); from_above = false;
+                }
+                else { S.ToChild(0); T.ToChild(0); from_above = true; }
+            }
+            else
+            {
+                const int sk = S.former, tk = T.former;
+                if( tk + 1 < T.Children() ) { S.ToChild( sk ); T.ToChild( tk + 1 ); from_above = true; }
+                else if( sk + 1 < S.Children() ) { S.ToChild( sk + 1 ); T.ToChild( 0 ); from_above = true; }
+                else if( S.level == 0 ) { go_on = false; }
+                else { S.ToParent(); T.ToParent(); from_above = false; }
+            }
+        }
+    }
+}
+
+extern "C"
+{
+
+// Squared distance of two disjoint simplices with m = 2 (segments) or 3 (triangles) points, row-major.
+double orc_simplex_dist2( int m, const double * P, const double * Q )
+{
+    V3o p[3], q[3];
+    for( int i = 0; i < m; ++i ) { p[i] = { P[3*i], P[3*i+1], P[3*i+2] }; q[i] = { Q[3*i], Q[3*i+1], Q[3*i+2] }; }
+    return SimplexDist2( p, q, m );
+}
+
+// TP0_Traversor::VF_Compute for a symmetric BCT: upper part of the full symmetric VeryNear pattern
+// (rp, ci, diag as for the near field). P_near: T x ND (tree order). flags: 1 energy, 2 differential,
+// 4 metric, 8 multiply each energy term by the quadrature weight w (not what the reference does:
+// TP0_Kernel_VF.hpp:273 sums a E b without w; the weighted form is the quadrature whose derivative
+// the differential is, used to pin the differential). Outputs added into P_D_near (T x ND) and
+// vf_diag (T x 2, the first two entries of the reference's 16-value diagonal blocks); vf_vals
+// (full pattern x 2) receives (a0, a1) per entry, the lower part copied from the upper part.
+// stats[0] += evaluations, stats[1] = max(stats[1], deepest level). Returns the energy.
+double orc_vf_compute( int n, LInt T, const double * P_near, const LInt * rp, const Int * ci, const LInt * diag,
+    double theta, int max_refinement, double q, double p, int flags,
+    double * P_D_near, double * vf_vals, double * vf_diag, double * stats )
+{
+    const int SIZE = n + 1;
+    const int ND = 1 + 3 * SIZE + 6;
+    const bool E = flags & 1, D = flags & 2, M = flags & 4, W = flags & 8;
+    const Exponents ex( q, p, n );
+    const double theta2 = theta * theta;
+    Neumaier en;
+    std::vector<double> r2( (size_t)T );
+    for( LInt i = 0; i < T; ++i ) { r2[i] = PrimitiveSquaredRadius( P_near + ND * i + 1, SIZE ); }
+    Hier S, Tt;
+    for( LInt i = 0; i < T; ++i )
+    {
+        if( rp[i + 1] <= diag[i] ) { continue; }
+        const double * X = P_near + ND * i;
+        const double a = X[0];
+        const double * xb = X + 1, * P = X + 1 + 3 * SIZE;
+        double DX[16] = {0}, ii0 = 0, ii1 = 0;
+        for( LInt k = diag[i]; k < rp[i + 1]; ++k )
+        {
+            const Int j = ci[k];
+            const double * Y = P_near + ND * j;
+            const double b = Y[0];
+            const double * yb = Y + 1, * Q = Y + 1 + 3 * SIZE;
+            double DY[16] = {0}, ij0 = 0, ij1 = 0, jj0 = 0, jj1 = 0;
+            double sum = 0;
+            VFTraverse( n, xb, yb, r2[i], r2[j], theta2, max_refinement, S, Tt, stats, [&]( const Hier & hs, const Hier & ht )
+            {
+                const double w = hs.weight * ht.weight;
+                double x[3], y[3], v[3], Pv[3] = {0,0,0}, Qv[3] = {0,0,0}, Vp[6] = {0,0,0,0,0,0};
+                double rr2 = 0, phi = 0, psi = 0;
+                for( int c = 0; c < 3; ++c )
+                {
+                    x[c] = hs.center[0] * xb[c]; y[c] = ht.center[0] * yb[c];
+                    for( int h = 1; h < SIZE; ++h ) { x[c] += hs.center[h] * xb[3 * h + c]; }
+                    for( int h = 1; h < SIZE; ++h ) { y[c] += ht.center[h] * yb[3 * h + c]; }
+                    v[c] = y[c] - x[c]; rr2 += v[c] * v[c];
+                }
+                for( int r = 0; r < 3; ++r )
+                {
+                    for( int c = 0; c < 3; ++c )
+                    {
+                        const int kk = lin_k3(r, c);
+                        Pv[r] += P[kk] * v[c]; Qv[r] += Q[kk] * v[c];
+                        if( c >= r ) { Vp[kk] = (1.0 + (double)(r != c)) * v[r] * v[c]; }
+                    }
+                    phi += v[r] * Pv[r]; psi += v[r] * Qv[r];
+                }
+                if( E || D )
+                {
+                    const double rmp2 = ex.PowP( rr2, -ex.p_half - 1, -ex.p_half_i - 1 );
+                    const double rmp = rmp2 * rr2;
+                    double phq2, psq2, phq, psq;
+                    if( ex.q_flag >= 2 )
+                    {
+                        phq2 = ex.PowQ( std::abs(phi), ex.q_half - 1, ex.q_half_i - 1 );
+                        psq2 = ex.PowQ( std::abs(psi), ex.q_half - 1, ex.q_half_i - 1 );
+                        phq = phq2 * phi; psq = psq2 * psi;
+                    }
+                    else
+                    {
+                        phq2 = 1; psq2 = 1;
+                        phq = ex.PowQ( std::abs(phi), ex.q_half, ex.q_half_i );
+                        psq = ex.PowQ( std::abs(psi), ex.q_half, ex.q_half_i );
+                    }
+                    const double Num = phq + psq;
+                    const double Ev = Num * rmp;
+                    if( E ) { sum += W ? w * a * Ev * b : a * Ev * b; }
+                    if( D )
+                    {
+                        const double factor = (ex.q_flag == 0) ? 0.0 : ex.q * rmp;
+                        const double Kxy = (ex.q_flag == 0) ? 0.0 : factor * phq2;
+                        const double Kyx = (ex.q_flag == 0) ? 0.0 : factor * psq2;
+                        const double H = ex.p * rmp2 * Num;
+                        const double wa = w * a, wb = w * b;
+                        double dx = 0, dy = 0;
+                        for( int c = 0; c < 3; ++c )
+                        {
+                            const double g = ((ex.q_flag == 0) ? 0.0 : Kxy * Pv[c] + Kyx * Qv[c]) - H * v[c];
+                            dx += g * x[c]; dy += g * y[c];
+                            for( int h = 0; h < SIZE; ++h ) { DX[1 + 3 * h + c] -= hs.center[h] * wb * g; }
+                            for( int h = 0; h < SIZE; ++h ) { DY[1 + 3 * h + c] += ht.center[h] * wa * g; }
+                        }
+                        DX[0] += wb * ( Ev - 0.5 * factor * phq + dx );
+                        DY[0] += wa * ( Ev - 0.5 * factor * psq - dy );
+                        for( int kk = 0; kk < 6; ++kk )
+                        {
+                            DX[1 + 3 * SIZE + kk] += 0.5 * wb * Kxy * Vp[kk];
+                            DY[1 + 3 * SIZE + kk] += 0.5 * wa * Kyx * Vp[kk];
+                        }
+                    }
+                }
+                if( M )
+                {
+                    const double r4 = rr2 * rr2;
+                    const double a1 = w * -2.0 * std::pow( rr2, ex.s_exp );
+                    const double a0 = w * 0.5 * (phi + psi) / r4 * a1;     // w enters twice, as in TP0_Kernel_VF.hpp:320
+                    ij0 += a0; ij1 += a1;
+                    ii0 -= (b / a) * a0; ii1 -= (b / a) * a1;
+                    jj0 -= (a / b) * a0; jj1 -= (a / b) * a1;
+                }
+            });
+            en.add( sum );
+            if( D ) { for( int h = 0; h < ND; ++h ) { P_D_near[ND * j + h] += DY[h]; } }
+            if( M )
+            {
+                vf_vals[2 * k] = ij0; vf_vals[2 * k + 1] = ij1;
+                vf_diag[2 * j] += jj0; vf_diag[2 * j + 1] += jj1;
+            }
+        }
+        if( D ) { for( int h = 0; h < ND; ++h ) { P_D_near[ND * i + h] += DX[h]; } }
+        if( M ) { vf_diag[2 * i] += ii0; vf_diag[2 * i + 1] += ii1; }
+    }
+    if( M )
+    {
+        // lower entries from their transposed upper entries
+        for( LInt i = 0; i < T; ++i )
+        {
+            for( LInt k = rp[i]; k < diag[i]; ++k )
+            {
+                const Int j = ci[k];
+                const Int * f = std::lower_bound( ci + diag[j], ci + rp[j + 1], (Int)i );
+                vf_vals[2 * k] = vf_vals[2 * (f - ci)]; vf_vals[2 * k + 1] = vf_vals[2 * (f - ci) + 1];
+            }
+        }
+    }
+    return E ? en.value() : 0.0;
+}
+
+// PseudoLaplacian_Traversor::VF_Compute (LowOrder): val = -w Power(r2, s_exp), s_exp = -n/2 - s,
+// summed over the sub-pairs of the adaptive subdivision (max_refinement 20 in the reference,
+// PseudoLaplacian_Traversor.hpp:187); row diagonal -= (b/a) val, column diagonal -= (a/b) val.
+// vf_vals: full pattern (lower copied from upper), vf_diag T (added).
+void orc_pl_vf( int n, LInt T, const double * P_near, const LInt * rp, const Int * ci, const LInt * diag,
+                double theta, int max_refinement, double s, double * vf_vals, double * vf_diag, double * stats )
+{
+    const int SIZE = n + 1;
+    const int ND = 1 + 3 * SIZE + 6;
+    const double s_exp = -0.5 * n - s;
+    const double theta2 = theta * theta;
+    std::vector<double> r2( (size_t)T );
+    for( LInt i = 0; i < T; ++i ) { r2[i] = PrimitiveSquaredRadius( P_near + ND * i + 1, SIZE ); }
+    Hier S, Tt;
+    for( LInt i = 0; i < T; ++i )
+    {
+        const double a = P_near[ND * i];
+        double ii = 0;
+        for( LInt k = diag[i]; k < rp[i + 1]; ++k )
+        {
+            const Int j = ci[k];
+            const double b = P_near[ND * j];
+            double ij = 0, jj = 0;
+            const double * xb = P_near + ND * i + 1, * yb = P_near + ND * j + 1;
+            VFTraverse( n, xb, yb, r2[i], r2[j], theta2, max_refinement, S, Tt, stats, [&]( const Hier & hs, const Hier & ht )
+            {
+                const double w = hs.weight * ht.weight;
+                double rr2 = 0;
+                for( int c = 0; c < 3; ++c )
+                {
+                    double x = hs.center[0] * xb[c], y = ht.center[0] * yb[c];
+                    for( int h = 1; h < SIZE; ++h ) { x += hs.center[h] * xb[3 * h + c]; y += ht.center[h] * yb[3 * h + c]; }
+                    rr2 += (y - x) * (y - x);
+                }
+                const double val = -w * std::pow( rr2, s_exp );
+                ij += val; ii -= (b / a) * val; jj -= (a / b) * val;
+            });
+            vf_vals[k] = ij;
+            vf_diag[j] += jj;
+        }
+        vf_diag[i] += ii;
+    }
+    for( LInt i = 0; i < T; ++i )
+    {
+        for( LInt k = rp[i]; k < diag[i]; ++k )
+        {
+            const Int j = ci[k];
+            const Int * f = std::lower_bound( ci + diag[j], ci + rp[j + 1], (Int)i );
+            vf_vals[k] = vf_vals[f - ci];
+        }
+    }
+}
+
+} // extern "C"

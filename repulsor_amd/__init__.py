@@ -95,6 +95,9 @@ _host_sigs = {
     "rpm_metric_solve": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int,
                                    C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _d]),
     "rpm_pl_values": (C.c_int, [_vp, _vp, _d, _d, _d, _d]),
+    "rpm_vf_metric_values": (C.c_int, [_vp, _vp, _d, _d]),
+    "rpm_pl_vf_values": (C.c_int, [_vp, _vp, _d, _d]),
+    "rpm_vf_stats": (C.c_int, [_vp, _i64, C.POINTER(C.c_int)]),
     "rpm_pl_multiply": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_h1_solve_device": (C.c_int, [_vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_op_set_pc_weight": (None, [_vp, C.c_double]),
@@ -276,6 +279,13 @@ class SimplicialMesh:
         _check(H.rpm_mesh_leaf_blocks(self._h, _p(blocks, _i32), _p(vf, _i32), _p(it, _i32)))
         return blocks, vf, it
 
+    def VeryNearStats(self):
+        """(evaluations, deepest level) of the very-near field of the last tangent-point compute."""
+        ev = np.zeros(1, dtype=np.int64)
+        lv = C.c_int(0)
+        _check(H.rpm_vf_stats(self._h, _p(ev, _i64), C.byref(lv)))
+        return int(ev[0]), int(lv.value)
+
     def SemiStaticUpdate(self, coords):
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         assert coords.shape == (self.V, 3)
@@ -388,6 +398,23 @@ class TangentPointMetric0(_Op):
         out = {"NF": np.zeros((b["near_nnz"], 2)), "NF_diag": np.zeros((b["T"], 2)),
                "FF": np.zeros((b["far_nnz"], 2)), "FF_diag": np.zeros((b["C"], 2))}
         _check(H.rpm_metric_values(self._h, M._h, _p(out["NF"]), _p(out["NF_diag"]), _p(out["FF"]), _p(out["FF_diag"])))
+        return out
+
+    def VeryNearMetricValues(self, M: SimplicialMesh):
+        """Very-near-field metric values: VF (nnz x 2, CSR order of M.pattern('verynear')) and VF_diag
+        (T x 2, tree order)."""
+        b = M.build()
+        out = {"VF": np.zeros((max(1, b["verynear_nnz"]), 2)), "VF_diag": np.zeros((b["T"], 2))}
+        _check(H.rpm_vf_metric_values(self._h, M._h, _p(out["VF"]), _p(out["VF_diag"])))
+        out["VF"] = out["VF"][: b["verynear_nnz"]]
+        return out
+
+    def VeryNearPseudoLaplacianValues(self, M: SimplicialMesh):
+        """Very-near-field values of L_{2-s}: VF (nnz) and VF_diag (T, tree order)."""
+        b = M.build()
+        out = {"VF": np.zeros(max(1, b["verynear_nnz"])), "VF_diag": np.zeros(b["T"])}
+        _check(H.rpm_pl_vf_values(self._h, M._h, _p(out["VF"]), _p(out["VF_diag"])))
+        out["VF"] = out["VF"][: b["verynear_nnz"]]
         return out
 
     def assemble(self, M: SimplicialMesh):

@@ -340,6 +340,36 @@ int rpm_metric_values( void * op, void * h, double * nf_csr, double * nf_diag, d
     });
 }
 
+int rpm_vf_metric_values( void * op, void * h, double * vf_vals, double * vf_diag )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        return A.DownloadVeryNearMetricValues( M, vf_vals, vf_diag ) ? RP_OK : RP_EINVAL;
+    });
+}
+
+int rpm_pl_vf_values( void * op, void * h, double * vf_vals, double * vf_diag )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        using Mesh = std::decay_t<decltype(M)>;
+        auto A = make_metric<Mesh>( (OpHandle *)op );
+        return A.DownloadVeryNearPseudoLaplacianValues( M, vf_vals, vf_diag ) ? RP_OK : RP_EINVAL;
+    });
+}
+
+// Evaluations and deepest level of the very-near field of the mesh's last tangent-point compute.
+int rpm_vf_stats( void * h, int64_t * evaluations, int * max_level )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        if( !M.DeviceReady() ) { *evaluations = 0; *max_level = 0; return RP_OK; }
+        return rp_vf_stats( M.Device(), evaluations, max_level );
+    });
+}
+
 int rpm_metric_multiply( void * op, void * h, double alpha, const double * X, int64_t ldX, double beta, double * Y, int64_t ldY,
                          int nrhs, int vf, int nf, int ff, int device_ptr )
 {

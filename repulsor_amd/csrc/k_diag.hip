@@ -49,6 +49,7 @@ namespace rp
                 if( i == j )
                 {
                     k0 = A.nf_diag[2 * (int64_t)i]; k1 = A.nf_diag[2 * (int64_t)i + 1];
+                    if( A.vf_diag ) { k0 += A.vf_diag[2 * (int64_t)i]; k1 += A.vf_diag[2 * (int64_t)i + 1]; }
                 }
                 else if( A.have_near )
                 {

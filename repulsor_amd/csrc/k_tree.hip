@@ -90,7 +90,7 @@ namespace rp
     template<int N>
     __global__ void k_hulls( int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim,
                              const int32_t * S, const double * V, const double * charges,
-                             const double * nf_dx, const double * leaf_far, double * B )
+                             const double * nf_dx, const double * vf_dh, const double * leaf_far, double * B )
     {
         constexpr int SIZE = N + 1;
         const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,8 +111,11 @@ namespace rp
         // Nd: [0] d/da, [1..3] d/d(a hull_l) (same for every l), [4..9] d/d(a P)
         double mean[3] = { 0, 0, 0 };
         for( int j = 0; j < SIZE; ++j ) { for( int c = 0; c < 3; ++c ) { mean[c] += X[j][c] * (1.0 / SIZE); } }
+        // very-near-field hull-vertex terms (they differ per vertex), zero without a very-near field
+        double Hv[SIZE][3];
+        for( int j = 0; j < SIZE; ++j ) { for( int c = 0; c < 3; ++c ) { Hv[j][c] = vf_dh ? vf_dh[3 * SIZE * i + 3 * j + c] : 0.0; } }
         double K = Nd[0] + Fd[0];
-        for( int j = 0; j < SIZE; ++j ) { for( int c = 0; c < 3; ++c ) { K += Nd[1 + c] * X[j][c]; } }
+        for( int j = 0; j < SIZE; ++j ) { for( int c = 0; c < 3; ++c ) { K += ( Nd[1 + c] + Hv[j][c] ) * X[j][c]; } }
         for( int c = 0; c < 3; ++c ) { K += Fd[1 + c] * mean[c]; }
         double W[6];
         for( int k = 0; k < 6; ++k ) { W[k] = Nd[4 + k] + Fd[4 + k]; }
@@ -171,7 +174,7 @@ namespace rp
             }
             for( int c = 0; c < 3; ++c )
             {
-                B[ (SIZE * t + j) * 3 + c ] = charge * ( d[c] + vol * ( Nd[1 + c] + Fd[1 + c] * (1.0 / SIZE) ) );
+                B[ (SIZE * t + j) * 3 + c ] = charge * ( d[c] + vol * ( Nd[1 + c] + Hv[j][c] + Fd[1 + c] * (1.0 / SIZE) ) );
             }
         }
     }
@@ -353,11 +356,11 @@ namespace rp
     { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<12>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
 
     void launch_hulls( int n, int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim, const int32_t * S,
-                       const double * V, const double * charges, const double * nf_dx, const double * leaf_far, double * B, hipStream_t s )
+                       const double * V, const double * charges, const double * nf_dx, const double * vf_dh, const double * leaf_far, double * B, hipStream_t s )
     {
         if( !T ) { return; }
-        if( n == 2 ) { hipLaunchKernelGGL( (k_hulls<2>), grid1(T), dim3(256), 0, s, T, L, P_ordering, leaf_of_prim, S, V, charges, nf_dx, leaf_far, B ); }
-        else         { hipLaunchKernelGGL( (k_hulls<1>), grid1(T), dim3(256), 0, s, T, L, P_ordering, leaf_of_prim, S, V, charges, nf_dx, leaf_far, B ); }
+        if( n == 2 ) { hipLaunchKernelGGL( (k_hulls<2>), grid1(T), dim3(256), 0, s, T, L, P_ordering, leaf_of_prim, S, V, charges, nf_dx, vf_dh, leaf_far, B ); }
+        else         { hipLaunchKernelGGL( (k_hulls<1>), grid1(T), dim3(256), 0, s, T, L, P_ordering, leaf_of_prim, S, V, charges, nf_dx, vf_dh, leaf_far, B ); }
     }
 
     void launch_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )

@@ -22,7 +22,7 @@ namespace rp
     // cluster sums, far-field SpMM and down-sweep, near-field SpMM with fused post epilogue, vertex gather.
     // Reference: MetricDimRestricted::MultiplyMetric (src/Metrics/MetricDimRestricted.hpp:109-139).
     void metric_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy,
-                       bool nf_flag, bool ff_flag, bool time_nf )
+                       bool nf_flag, bool ff_flag, bool time_nf, bool vf_flag )
     {
         const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
         const bool mixed = ctx->metric_precision == RP_MIXED;
@@ -56,6 +56,11 @@ namespace rp
         else
         {
             launch_post_prim( ctx->size, T, ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->leaf_out.get(), do_ff ? 1 : 0, S_out, ctx->stream );
+        }
+        if( vf_flag && ctx->have_vf_metric )
+        {
+            launch_vf_spmm_post( ctx->size, ctx->vf_nrows, ctx->vf_rows.get(), ctx->vf_rp.get(), ctx->vf_ci.get(), ctx->vf_vals.get(),
+                                 ctx->vf_diag.get(), ctx->P_in.get(), ctx->pre_vals.get(), S_out, ctx->stream );
         }
         if( time_nf ) { RP_HIP( hipEventRecord( ctx->ev[6][1], ctx->stream ) ); }
         launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
@@ -231,6 +236,8 @@ int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_tree = true;
         ctx->have_near = ctx->have_far = ctx->have_data = ctx->have_diff = ctx->have_metric = ctx->have_pl = false;
+        ctx->vf_nnz = 0; ctx->vf_nrows = 0;
+        ctx->have_vf_dh = ctx->have_vf_metric = ctx->have_pl_vf = false;
     });
 }
 
@@ -294,6 +301,68 @@ int rp_upload_far_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp_, cons
     });
 }
 
+int rp_upload_verynear_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp_, const int32_t * ci, double theta, int max_refinement )
+{
+    return guarded( "rp_upload_verynear_pattern", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree, "upload the tree first", RP_ENOTREADY );
+        require( rows == ctx->T && rp_ != nullptr, "very-near pattern must have one row per simplex" );
+        require( std::isfinite( theta ) && theta > 0.0, "theta must be positive" );
+        require( max_refinement >= 1, "max_refinement must be at least 1" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = rows, nnz = rp_[rows];
+        require( rp_[0] == 0 && nnz >= 0, "bad row pointers" );
+        std::vector<int64_t> dg( (size_t)T ), tr( (size_t)std::max<int64_t>( 1, nnz ) );
+        std::vector<int32_t> row( (size_t)std::max<int64_t>( 1, nnz ) ), rws;
+        for( int64_t i = 0; i < T; ++i )
+        {
+            require( rp_[i + 1] >= rp_[i], "row pointers must be non-decreasing" );
+            dg[(size_t)i] = rp_[i + 1];
+            for( int64_t k = rp_[i]; k < rp_[i + 1]; ++k )
+            {
+                require( ci[k] >= 0 && ci[k] < T && ci[k] != i, "very-near column out of range or on the diagonal" );
+                require( k == rp_[i] || ci[k] > ci[k - 1], "very-near columns must be strictly increasing per row" );
+                if( ci[k] > i && dg[(size_t)i] == rp_[i + 1] ) { dg[(size_t)i] = k; }
+                row[(size_t)k] = (int32_t)i;
+            }
+            if( rp_[i + 1] > rp_[i] ) { rws.push_back( (int32_t)i ); }
+        }
+        // metric values of a lower entry (i, j), j < i, come from the upper entry (j, i)
+        for( int64_t i = 0; i < T; ++i )
+        {
+            for( int64_t k = rp_[i]; k < rp_[i + 1]; ++k )
+            {
+                const int32_t j = ci[k];
+                if( j > i ) { tr[(size_t)k] = k; continue; }
+                const int32_t * b = ci + dg[(size_t)j], * e = ci + rp_[j + 1];
+                const int32_t * f = std::lower_bound( b, e, (int32_t)i );
+                require( f != e && *f == (int32_t)i, "very-near pattern is not symmetric" );
+                tr[(size_t)k] = f - ci;
+            }
+        }
+        // deepest level: the reference clamps max_refinement to the largest level whose path code
+        // fits a 64-bit integer (FMM_Kernel_VF.hpp:178-183, SimplexHierarchy_{1D,2D}.hpp MaxLevel)
+        const double children = ( ctx->dom_dim == 2 ) ? 4.0 : 2.0;
+        const int max_level = (int)std::floor( std::log( (double)std::numeric_limits<uint64_t>::max() ) / std::log( children ) );
+        ctx->vf_max_ref = std::min( max_refinement, max_level );
+        ctx->vf_theta = theta;
+        ctx->vf_rp.upload( rp_, (size_t)T + 1, ctx->stream );
+        ctx->vf_dg.upload( dg.data(), (size_t)T, ctx->stream );
+        if( nnz > 0 )
+        {
+            ctx->vf_ci.upload( ci, (size_t)nnz, ctx->stream );
+            ctx->vf_row.upload( row.data(), (size_t)nnz, ctx->stream );
+            ctx->vf_tr.upload( tr.data(), (size_t)nnz, ctx->stream );
+            ctx->vf_rows.upload( rws.data(), rws.size(), ctx->stream );
+        }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        ctx->vf_nnz = nnz;
+        ctx->vf_nrows = (int64_t)rws.size();
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = false;
+        ctx->have_vf_dh = ctx->have_vf_metric = ctx->have_pl_vf = false;
+    });
+}
+
 int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near, const double * C_far )
 {
     return guarded( "rp_upload_data", [&]()
@@ -339,8 +408,10 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         const bool mixed = ( precision == RP_MIXED );
         const bool E = flags & RP_ENERGY, D = flags & RP_DIFF, M = flags & RP_METRIC;
 
-        ctx->partials.require( (size_t)(L + C) );
-        ctx->scalar.require( (size_t)(1 + REDUCE_BLOCKS) );
+        const bool have_vf = ctx->vf_nnz > 0;
+        const int64_t R = have_vf ? ctx->vf_nrows : 0;
+        ctx->partials.require( (size_t)(L + C + R) );
+        ctx->scalar.require( (size_t)(1 + REDUCE_BLOCKS + 3) );
         if( D ) { ctx->nf_dx.require( (size_t)(10 * T) ); ctx->ff_dx.require( (size_t)(10 * C) ); }
         if( M )
         {
@@ -394,16 +465,56 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
             RP_HIP( hipGetLastError() );
         }
         RP_HIP( hipEventRecord( ctx->ev[5][1], ctx->stream ) );
-        if( E ) { launch_reduce_sum( ctx->partials.get(), L + C, ctx->scalar.get(), ctx->stream ); }
+        RP_HIP( hipEventRecord( ctx->ev[11][0], ctx->stream ) );
+        double * vf_stat = ctx->scalar.get() + 1 + REDUCE_BLOCKS;
+        if( D ) { ctx->have_vf_dh = false; }
+        if( M ) { ctx->have_vf_metric = false; }
+        if( have_vf )
+        {
+            // very-near field in FP64: per-entry adaptive traversals, then per-row sums
+            const int64_t nnz = ctx->vf_nnz;
+            ctx->vf_rec.require( (size_t)(VF_REC * nnz) );
+            if( D ) { ctx->vf_dh.require( (size_t)(3 * ctx->size * T) ); launch_fill( ctx->vf_dh.get(), 3 * ctx->size * T, 0.0, ctx->stream ); }
+            if( M )
+            {
+                ctx->vf_vals.require( (size_t)(2 * nnz) ); ctx->vf_diag.require( (size_t)(2 * T) );
+                launch_fill( ctx->vf_diag.get(), 2 * T, 0.0, ctx->stream );
+            }
+            VFArgs A{};
+            A.nnz = nnz; A.T = T; A.row = ctx->vf_row.get(); A.col = ctx->vf_ci.get();
+            A.P_ordering = ctx->P_ordering.get(); A.simplices = ctx->simplices.get(); A.coords = ctx->coords.get();
+            A.a = ctx->p_a.get(); A.P = ctx->p_P.get();
+            A.theta2 = ctx->vf_theta * ctx->vf_theta; A.max_ref = ctx->vf_max_ref; A.ex = ex;
+            A.energy = E; A.diff = D; A.metric = M; A.pl_sexp = 0.0;
+            A.cap = VF_LANE_EVALUATION_LIMIT; A.rec = ctx->vf_rec.get();
+            launch_vf_pairs( n, 0, A, ctx->stream );
+            VFRowArgs W{};
+            W.nrows = R; W.T = T; W.rows = ctx->vf_rows.get(); W.rp = ctx->vf_rp.get(); W.dg = ctx->vf_dg.get(); W.tr = ctx->vf_tr.get();
+            W.rec = ctx->vf_rec.get();
+            W.dx = D ? ctx->nf_dx.get() : nullptr; W.dh = D ? ctx->vf_dh.get() : nullptr;
+            W.vals = M ? ctx->vf_vals.get() : nullptr; W.diag = M ? ctx->vf_diag.get() : nullptr;
+            W.epart = E ? ctx->partials.get() + L + C : nullptr;
+            launch_vf_rows( n, 0, W, ctx->stream );
+            launch_vf_stats( nnz, ctx->vf_rec.get(), vf_stat, ctx->stream );
+            RP_HIP( hipGetLastError() );
+        }
+        RP_HIP( hipEventRecord( ctx->ev[11][1], ctx->stream ) );
+        if( E ) { launch_reduce_sum( ctx->partials.get(), L + C + R, ctx->scalar.get(), ctx->stream ); }
         ev.stop();
         RP_HIP( hipGetLastError() );
         double en = 0;
         if( E ) { RP_HIP( hipMemcpyAsync( &en, ctx->scalar.get(), sizeof(double), hipMemcpyDeviceToHost, ctx->stream ) ); }
+        double st[3] = { 0, 0, 0 };
+        if( have_vf ) { RP_HIP( hipMemcpyAsync( st, vf_stat, sizeof(st), hipMemcpyDeviceToHost, ctx->stream ) ); }
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
-        finish_timing( ctx, 0 ); finish_timing( ctx, 4 ); finish_timing( ctx, 5 );
+        finish_timing( ctx, 0 ); finish_timing( ctx, 4 ); finish_timing( ctx, 5 ); finish_timing( ctx, 11 );
+        for( int k = 0; k < 3; ++k ) { ctx->vf_stats[k] = st[k]; }
+        if( st[2] != 0.0 ) { ctx->have_diff = ctx->have_metric = false; }
+        require( st[2] == 0.0, "very-near-field subdivision exceeded 2^24 evaluations on one level-1 sub-pair "
+                 "(primitives almost touching); refine the mesh or lower adaptivity_settings.max_refinement", RP_EUNSUPPORTED );
         if( energy_out ) { *energy_out = E ? en : 0.0; }
-        if( D ) { ctx->have_diff = true; }
-        if( M ) { ctx->have_metric = true; ctx->metric_precision = precision; ++ctx->metric_version; }
+        if( D ) { ctx->have_diff = true; ctx->have_vf_dh = have_vf; }
+        if( M ) { ctx->have_metric = true; ctx->have_vf_metric = have_vf; ctx->metric_precision = precision; ++ctx->metric_version; }
     });
 }
 
@@ -421,6 +532,8 @@ int rp_download_derivatives( rp_ctx * ctx, double * P_D_near, double * P_D_far )
         ctx->leaf_far.download( lf.data(), lf.size(), ctx->stream );
         ctx->P_ordering.download( ord.data(), ord.size(), ctx->stream );
         ctx->leaf_of_prim.download( lop.data(), lop.size(), ctx->stream );
+        std::vector<double> dh;
+        if( ctx->have_vf_dh ) { dh.resize( (size_t)(3 * ctx->size * T) ); ctx->vf_dh.download( dh.data(), dh.size(), ctx->stream ); }
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         const int nd = ctx->nd, size = ctx->size;
         for( int64_t i = 0; i < T; ++i )
@@ -431,6 +544,7 @@ int rp_download_derivatives( rp_ctx * ctx, double * P_D_near, double * P_D_far )
                 double * o = P_D_near + nd * t;
                 o[0] = dx[(size_t)i];
                 for( int h = 0; h < size; ++h ) { for( int c = 0; c < 3; ++c ) { o[1 + 3 * h + c] = dx[(size_t)((1 + c) * T + i)]; } }
+                if( !dh.empty() ) { for( int h = 0; h < 3 * size; ++h ) { o[1 + h] += dh[(size_t)(3 * size * i + h)]; } }
                 for( int k = 0; k < 6; ++k ) { o[1 + 3 * size + k] = dx[(size_t)((4 + k) * T + i)]; }
             }
             if( P_D_far )
@@ -464,7 +578,8 @@ int rp_assemble_vertex_gradient( rp_ctx * ctx, double alpha, double beta, double
         EventScope ev( ctx, 1 );
         collect_far( ctx );
         launch_hulls( ctx->dom_dim, T, L, ctx->P_ordering.get(), ctx->leaf_of_prim.get(), ctx->simplices.get(), ctx->coords.get(),
-                      ctx->have_charges ? ctx->charges.get() : nullptr, ctx->nf_dx.get(), ctx->leaf_far.get(), ctx->hull_grad.get(), ctx->stream );
+                      ctx->have_charges ? ctx->charges.get() : nullptr, ctx->nf_dx.get(), ctx->have_vf_dh ? ctx->vf_dh.get() : nullptr,
+                      ctx->leaf_far.get(), ctx->hull_grad.get(), ctx->stream );
         launch_assemble( V, ctx->vc_ptr.get(), ctx->vc_corner.get(), ctx->hull_grad.get(), alpha, beta, Yd, ld, ctx->stream );
         ev.stop();
         RP_HIP( hipGetLastError() );
@@ -476,6 +591,34 @@ int rp_assemble_vertex_gradient( rp_ctx * ctx, double alpha, double beta, double
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         finish_timing( ctx, 1 );
         if( !dev ) { for( int64_t v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = hy[(size_t)(3 * v + k)]; } } }
+    });
+}
+
+int rp_download_vf_metric( rp_ctx * ctx, double * vf_vals, double * vf_diag )
+{
+    return guarded( "rp_download_vf_metric", [&]()
+    {
+        require( ctx != nullptr && ctx->have_metric, "no metric computed", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T;
+        if( !ctx->have_vf_metric )
+        {
+            if( vf_diag ) { std::fill( vf_diag, vf_diag + 2 * T, 0.0 ); }
+            return;
+        }
+        if( vf_vals ) { ctx->vf_vals.download( vf_vals, (size_t)(2 * ctx->vf_nnz), ctx->stream ); }
+        if( vf_diag ) { ctx->vf_diag.download( vf_diag, (size_t)(2 * T), ctx->stream ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+    });
+}
+
+int rp_vf_stats( rp_ctx * ctx, int64_t * evaluations, int * max_level )
+{
+    return guarded( "rp_vf_stats", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        if( evaluations ) { *evaluations = (int64_t)ctx->vf_stats[0]; }
+        if( max_level ) { *max_level = (int)ctx->vf_stats[1]; }
     });
 }
 
@@ -532,7 +675,7 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
             Yd = ctx->Yd.get(); ldy = 3;
         }
         EventScope ev( ctx, 2 );
-        metric_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy, nf_flag != 0, ff_flag != 0, true );
+        metric_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy, nf_flag != 0, ff_flag != 0, true, vf_flag != 0 );
         ev.stop();
         RP_HIP( hipGetLastError() );
         if( !dev )

@@ -162,6 +162,19 @@ struct rp_ctx
     int64_t far_nnz = 0;
     bool have_far = false;
 
+    // very-near field: full symmetric CSR over primitives (tree order)
+    rp::DBuf<int64_t> vf_rp, vf_dg, vf_tr;   // T+1, T, nnz
+    rp::DBuf<int32_t> vf_ci, vf_row, vf_rows; // nnz, nnz, rows with entries
+    int64_t vf_nnz = 0, vf_nrows = 0;
+    double vf_theta = 10.0;              // near separation parameter
+    int vf_max_ref = 30;                 // deepest subdivision level of the tangent-point kernel
+    rp::DBuf<double> vf_rec;             // nnz x VF_REC per-entry records
+    rp::DBuf<double> vf_dh;              // T x 3(n+1) hull-vertex moment derivatives of the last differential
+    rp::DBuf<double> vf_vals, vf_diag;   // 2 x nnz, 2T metric values
+    rp::DBuf<double> pl_vf_vals, pl_vf_diag;   // nnz, T pseudo-Laplacian values
+    bool have_vf_dh = false, have_vf_metric = false, have_pl_vf = false;
+    double vf_stats[3] = { 0, 0, 0 };    // evaluations, deepest level, limit hit (last tangent-point compute)
+
     // data (tree order, SoA)
     rp::DBuf<double> p_a, p_x, p_P;      // T, 3T, 6T
     rp::DBuf<double> c_a, c_x, c_P;      // C, 3C, 6C

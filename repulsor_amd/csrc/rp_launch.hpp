@@ -54,7 +54,7 @@ namespace rp
     void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * ff_dx, double * leaf_far, hipStream_t s );
     void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s );
     void launch_hulls( int n, int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim, const int32_t * S,
-                       const double * V, const double * charges, const double * nf_dx, const double * leaf_far, double * B, hipStream_t s );
+                       const double * V, const double * charges, const double * nf_dx, const double * vf_dh, const double * leaf_far, double * B, hipStream_t s );
     void launch_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
     void launch_pre( int size, int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
     void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s, int W = 12 );
@@ -134,9 +134,53 @@ namespace rp
         const int32_t  * C_end;
         const double   * ff_diag;     // 2C
         const double   * S;           // 2C root-path sums of ff_diag (set by launch_metric_diag)
+        const double   * vf_diag;     // 2T very-near-field diagonal terms (NULL if none)
     };
     void launch_metric_diag( bool mixed, const DiagArgs & A, const void * nf_vals, double * S, double * dA, hipStream_t s );
     void launch_jacobi_add( int64_t V, double c, const double * X, int64_t ldX, const double * dA, double * Y, int64_t ldY, hipStream_t s );
+
+    // very-near field (k_vf.hip)
+    constexpr int VF_REC = 24;                          // doubles per pattern-entry record
+    constexpr int64_t VF_LANE_EVALUATION_LIMIT = (int64_t)1 << 24;
+    struct VFArgs
+    {
+        int64_t nnz, T;
+        const int32_t * row;          // nnz: row primitive of each full-pattern entry (tree order)
+        const int32_t * col;          // nnz: column primitive
+        const int32_t * P_ordering;   // tree position -> simplex
+        const int32_t * simplices;    // simplex vertices (caller's order)
+        const double  * coords;       // V x 3
+        const double  * a;            // T areas (tree order)
+        const double  * P;            // 6 x T projectors
+        double theta2;                // near separation parameter squared
+        int max_ref;                  // deepest subdivision level
+        Expo ex;
+        int energy, diff, metric;     // tangent-point outputs requested
+        double pl_sexp;               // pseudo-Laplacian exponent -n/2 - s
+        int64_t cap;                  // evaluations per lane before the traversal stops with an error
+        double * rec;                 // nnz x VF_REC
+    };
+    struct VFRowArgs
+    {
+        int64_t nrows, T;
+        const int32_t * rows;         // rows with VF entries
+        const int64_t * rp;           // T+1
+        const int64_t * dg;           // T: first strictly upper entry
+        const int64_t * tr;           // nnz: entry holding the metric values (itself or the transposed upper entry)
+        const double  * rec;
+        double * dx;                  // 10 x T (added), or NULL
+        double * dh;                  // T x 3(n+1) (written on VF rows), with dx
+        double * vals;                // tangent-point: 2 x nnz; pseudo-Laplacian: nnz
+        double * diag;                // tangent-point: 2 x T; pseudo-Laplacian: T (written on VF rows)
+        double * epart;               // nrows energy partials, or NULL
+    };
+    void launch_vf_pairs( int n, int mode, const VFArgs & A, hipStream_t s );
+    void launch_vf_rows( int n, int mode, const VFRowArgs & A, hipStream_t s );
+    void launch_vf_stats( int64_t nnz, const double * rec, double * out, hipStream_t s );
+    void launch_vf_spmm_post( int size, int64_t nrows, const int32_t * rows, const int64_t * rp, const int32_t * ci, const double * vals,
+                              const double * diag, const double * P_in, const double * pre_vals, double * S_out, hipStream_t s );
+    void launch_vf_pl_spmm_post( int size, int64_t nrows, const int32_t * rows, const int64_t * rp, const int32_t * ci, const double * vals,
+                                 const double * diag, const double * P_in, const double * pre_vals, double * S_out, hipStream_t s );
 
     // learned preconditioner (k_learn.hip)
     constexpr int MLP_H = 64;      // hidden width

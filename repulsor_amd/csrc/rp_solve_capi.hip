@@ -45,6 +45,11 @@ namespace rp
         {
             launch_pl_post_prim( ctx->size, T, ctx->leaf_of_prim.get(), ctx->pre_vals.get(), ctx->leaf_out.get(), do_ff ? 1 : 0, S_out, ctx->stream );
         }
+        if( ctx->have_pl_vf )
+        {
+            launch_vf_pl_spmm_post( ctx->size, ctx->vf_nrows, ctx->vf_rows.get(), ctx->vf_rp.get(), ctx->vf_ci.get(), ctx->pl_vf_vals.get(),
+                                    ctx->pl_vf_diag.get(), ctx->P_in.get(), ctx->pre_vals.get(), S_out, ctx->stream );
+        }
         launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
     }
 
@@ -116,6 +121,7 @@ namespace rp
         DiagArgs A{};
         A.V = ctx->V; A.C = ctx->C; A.size = ctx->size;
         A.have_near = ctx->nf_cols > 0; A.use_far = ctx->far_nnz > 0;
+        A.vf_diag = ctx->have_vf_metric ? ctx->vf_diag.get() : nullptr;
         A.vp_ptr = ctx->vp_ptr.get(); A.vp_entry = ctx->vp_entry.get(); A.pre_vals = ctx->pre_vals.get();
         A.leaf_of_prim = ctx->leaf_of_prim.get(); A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
         A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
@@ -503,14 +509,61 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision )
         B.a = ctx->c_a.get(); B.x = ctx->c_x.get(); B.ex = ex;
         B.vals = ctx->pl_ff_vals.get(); B.diag = ctx->pl_ff_diag.get();
         launch_pl_assemble( mixed, A, B, have_nf, have_ff, ctx->stream );
+        ctx->have_pl_vf = false;
+        double st[3] = { 0, 0, 0 };
+        if( ctx->vf_nnz > 0 )
+        {
+            // very-near field in FP64 (PseudoLaplacian_Traversor::VF_Compute, max_refinement 20 as in
+            // PseudoLaplacian_Traversor.hpp:187)
+            const int64_t nnz = ctx->vf_nnz;
+            ctx->vf_rec.require( (size_t)(VF_REC * nnz) );
+            ctx->pl_vf_vals.require( (size_t)nnz ); ctx->pl_vf_diag.require( (size_t)T );
+            ctx->scalar.require( (size_t)(1 + REDUCE_BLOCKS + 3) );
+            launch_fill( ctx->pl_vf_diag.get(), T, 0.0, ctx->stream );
+            VFArgs V{};
+            V.nnz = nnz; V.T = T; V.row = ctx->vf_row.get(); V.col = ctx->vf_ci.get();
+            V.P_ordering = ctx->P_ordering.get(); V.simplices = ctx->simplices.get(); V.coords = ctx->coords.get();
+            V.a = ctx->p_a.get(); V.P = ctx->p_P.get();
+            V.theta2 = ctx->vf_theta * ctx->vf_theta; V.max_ref = 20;
+            V.ex = ex; V.pl_sexp = s_exp; V.cap = VF_LANE_EVALUATION_LIMIT; V.rec = ctx->vf_rec.get();
+            launch_vf_pairs( n, 1, V, ctx->stream );
+            VFRowArgs W{};
+            W.nrows = ctx->vf_nrows; W.T = T; W.rows = ctx->vf_rows.get(); W.rp = ctx->vf_rp.get(); W.dg = ctx->vf_dg.get();
+            W.tr = ctx->vf_tr.get(); W.rec = ctx->vf_rec.get(); W.vals = ctx->pl_vf_vals.get(); W.diag = ctx->pl_vf_diag.get();
+            launch_vf_rows( n, 1, W, ctx->stream );
+            double * vf_stat = ctx->scalar.get() + 1 + REDUCE_BLOCKS;
+            launch_vf_stats( nnz, ctx->vf_rec.get(), vf_stat, ctx->stream );
+            RP_HIP( hipMemcpyAsync( st, vf_stat, sizeof(st), hipMemcpyDeviceToHost, ctx->stream ) );
+        }
         ev.stop();
         RP_HIP( hipGetLastError() );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         finish_timing( ctx, 9 );
+        if( st[2] != 0.0 ) { ctx->have_pl = false; }
+        require( st[2] == 0.0, "very-near-field subdivision exceeded 2^24 evaluations on one level-1 sub-pair "
+                 "(primitives almost touching); refine the mesh", RP_EUNSUPPORTED );
+        ctx->have_pl_vf = ctx->vf_nnz > 0;
         ctx->have_pl = true;
         ++ctx->pl_version;
         ctx->pl_precision = precision;
         ctx->pl_s = s;
+    });
+}
+
+int rp_download_pl_vf( rp_ctx * ctx, double * vf_vals, double * vf_diag )
+{
+    return guarded( "rp_download_pl_vf", [&]()
+    {
+        require( ctx != nullptr && ctx->have_pl, "no pseudo-Laplacian computed", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        if( !ctx->have_pl_vf )
+        {
+            if( vf_diag ) { std::fill( vf_diag, vf_diag + ctx->T, 0.0 ); }
+            return;
+        }
+        if( vf_vals ) { ctx->pl_vf_vals.download( vf_vals, (size_t)ctx->vf_nnz, ctx->stream ); }
+        if( vf_diag ) { ctx->pl_vf_diag.download( vf_diag, (size_t)ctx->T, ctx->stream ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
     });
 }
 

@@ -17,11 +17,25 @@ def oracle_state(M):
     return tr, td, npat, fpat
 
 
+def oracle_vf(M, q, p, flags):
+    """Oracle very-near field of M (None if its VeryNear pattern is empty) and the pattern."""
+    if M.build()["verynear_nnz"] == 0:
+        return None, None
+    pat = M.pattern("verynear")
+    td = M.tree_data()
+    return O.vf_compute(M.dom_dim, td["P_near"], pat, M.adaptivity_settings.theta,
+                        M.adaptivity_settings.max_refinement, q, p, flags), pat
+
+
 def oracle_energy_diff(M, q, p, X=None, charges=None):
-    """Oracle energy and vertex differential (V x 3) on M's hierarchy."""
+    """Oracle energy and vertex differential (V x 3) on M's hierarchy (very-near field included)."""
     tr, td, npat, fpat = oracle_state(M)
     n = M.dom_dim
     o = O.tp0_compute(n, td["P_near"], td["C_far"], npat, fpat, q, p, 3)
+    vf, _ = oracle_vf(M, q, p, 3)
+    if vf is not None:
+        o["energy"] += vf["energy"]
+        o["P_D_near_tree"] = o["P_D_near_tree"] + vf["P_D_near_tree"]
     V = M.VertexCoordinates() if X is None else X
     Y, N, F = O.differential(n, V, M.simplices, tr, o["P_D_near_tree"], o["C_D_far"], charges)
     return o["energy"], Y, N, F
@@ -33,10 +47,28 @@ def oracle_metric(M, q, p):
     return o, tr, td, npat, fpat
 
 
-def oracle_multiply(M, q, p, X, alpha=1.0, beta=0.0, Y=None, nf=True, ff=True):
+def with_verynear_metric(M, q, p, o, npat):
+    """Near pattern and values with the very-near blocks merged in (same block form), and the near
+    diagonal plus the very-near diagonal, for the oracle's metric product."""
+    vf, vpat = oracle_vf(M, q, p, 4)
+    if vf is None:
+        return npat, o["NF"], o["NF_diag"]
+    pat, vals = O.merge_patterns(npat, o["NF"], vpat, vf["VF"])
+    return pat, vals, o["NF_diag"] + vf["VF_diag"]
+
+
+def oracle_multiply(M, q, p, X, alpha=1.0, beta=0.0, Y=None, nf=True, ff=True, vf=True):
     o, tr, td, npat, fpat = oracle_metric(M, q, p)
     Y = np.zeros_like(X) if Y is None else Y.copy()
-    O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], npat, o["NF"], o["NF_diag"],
+    if vf and nf:
+        npat, nvals, ndiag = with_verynear_metric(M, q, p, o, npat)
+    elif vf:
+        v, vpat = oracle_vf(M, q, p, 4)
+        npat, nvals, ndiag = (vpat, v["VF"], v["VF_diag"]) if v is not None else (npat, o["NF"], np.zeros_like(o["NF_diag"]))
+        nf = v is not None
+    else:
+        nvals, ndiag = o["NF"], o["NF_diag"]
+    O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], npat, nvals, ndiag,
                       fpat, o["FF"], o["FF_diag"], alpha, X, beta, Y, nf, ff)
     return Y
 
@@ -63,19 +95,27 @@ class OracleSolve:
         s = (p - n) / q
         self.o, self.tr, self.td, self.npat, self.fpat = oracle_metric(M, q, p)
         self.pl = O.pl_compute(n, self.td["P_near"], self.td["C_far"], self.npat, self.fpat, 2.0 - s)
+        # very-near blocks merged into the near pattern of both operators
+        self.mpat, self.mvals, self.mdiag = with_verynear_metric(M, q, p, self.o, self.npat)
+        self.plpat = self.npat
+        if M.build()["verynear_nnz"] > 0:
+            vpat = M.pattern("verynear")
+            v = O.pl_vf(n, self.td["P_near"], vpat, M.adaptivity_settings.theta, 20, 2.0 - s)
+            self.plpat, nfv = O.merge_patterns(self.npat, self.pl["NF"], vpat, v["VF"])
+            self.pl = dict(self.pl, NF=nfv, NF_diag=self.pl["NF_diag"] + v["VF_diag"])
         self.lu = spla.splu(h1_matrix(M).tocsc())
 
     def A(self, X):
         M, td = self.M, self.td
         Y = np.zeros_like(X)
-        return O.multiply_metric(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.npat, self.o["NF"], self.o["NF_diag"],
+        return O.multiply_metric(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.mpat, self.mvals, self.mdiag,
                                  self.fpat, self.o["FF"], self.o["FF_diag"], 1.0, X, 0.0, Y)
 
     def P(self, X):
         M, td = self.M, self.td
         Z = np.ascontiguousarray(self.lu.solve(np.ascontiguousarray(X)))
         Y = np.zeros(Z.shape)
-        O.pl_multiply(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.npat, self.pl, self.fpat, 1.0, Z, 0.0, Y)
+        O.pl_multiply(M.dom_dim, M.V, self.tr, td["pre_cols"], td["pre_vals"], self.plpat, self.pl, self.fpat, 1.0, Z, 0.0, Y)
         return np.ascontiguousarray(self.lu.solve(Y))
 
     def diagonal(self):
@@ -153,3 +193,31 @@ class OracleSolve:
             X[:, k] = x
             its = max(its, it)
         return X, its
+
+
+# ---------------------------------------------------------------------------------------------
+# Meshes with a non-empty very-near field: primitives of different components closer than a tenth
+# of their radius.
+
+def vf_shells(level=3, gap=0.004, seed=7):
+    """Two nested icospheres (unit sphere and a radially jittered copy at distance ~gap, reversed
+    orientation): every triangle has very-near partners on the other shell."""
+    import repulsor_amd as R
+    X, S = R.make_icosphere(level, 0, 0.0, seed)
+    X = X / np.linalg.norm(X, axis=1)[:, None]
+    rng = np.random.default_rng(seed)
+    Y = X * (1.0 + gap * (1.0 + 0.5 * rng.uniform(-1, 1, len(X))))[:, None]
+    S = np.asarray(S, dtype=np.int32)
+    return R.SimplicialMesh(np.vstack([X, Y]), np.vstack([S, S[:, ::-1] + len(X)]).astype(np.int32))
+
+
+def vf_circles(N=400, gap=3e-4, seed=8):
+    """Two unit circles with N edges each, the second lifted by ~gap along z (jittered)."""
+    import repulsor_amd as R
+    t = 2 * np.pi * np.arange(N) / N
+    rng = np.random.default_rng(seed)
+    A = np.stack([np.cos(t), np.sin(t), np.zeros(N)], 1)
+    B = A.copy()
+    B[:, 2] = gap * (1 + 0.5 * rng.uniform(-1, 1, N))
+    E = np.stack([np.arange(N), (np.arange(N) + 1) % N], 1)
+    return R.SimplicialMesh(np.vstack([A, B]), np.vstack([E, E + N]).astype(np.int32))
