@@ -9,6 +9,10 @@ of P r_k and e_k after the best common scale; PCG does not see the scale of P), 
 library's HIP kernels (rp_pc_learn_step). The model is written to repulsor_amd/models/learned_pc_tori.rplp.
 
 Usage: python -m repulsor_amd.train_learned_pc [--meshes 256] [--grid 320 240] [--lr 3e-3]
+       [--start K] [--count N] [--checkpoint state.npz]
+With --checkpoint, the parameters, Adam moments and step count are read from the file if it exists
+and written back after every mesh, so that a long run can be split into several processes (meshes
+start .. start+count-1 per process).
 """
 
 from __future__ import annotations
@@ -48,15 +52,23 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--record-tol", type=float, default=1e-9)
     ap.add_argument("--out", default=LP.DEFAULT_MODEL)
+    ap.add_argument("--start", type=int, default=0)
+    ap.add_argument("--count", type=int, default=-1)
+    ap.add_argument("--checkpoint", default="")
     args = ap.parse_args()
 
     F = LP.feature_count()
     params = LP.init_params(F)
     m = np.zeros_like(params)
     v = np.zeros_like(params)
+    if args.checkpoint and os.path.exists(args.checkpoint):
+        st = np.load(args.checkpoint)
+        params, m, v = st["params"].astype(np.float32), st["m"].astype(np.float32), st["v"].astype(np.float32)
+        print(f"resumed from {args.checkpoint} after {int(st['step'])} steps", flush=True)
     q, p = 6.0, 12.0
     t0 = time.time()
-    for k in range(args.meshes):
+    end = args.meshes if args.count < 0 else min(args.meshes, args.start + args.count)
+    for k in range(args.start, end):
         M, sigma = training_mesh(k, args.grid[0], args.grid[1], args.seed)
         E = R.TangentPointEnergy0(q, p)
         B = E.Differential(M)
@@ -66,10 +78,13 @@ def main():
         A.Solve(M, 1.0, B, 0.0, X, max_iter=300, tolerance=args.record_tol, record=True, raise_on_stagnation=False)
         it = A.Solver_IterationCount()
         loss, grad = A.PreconditionerLearnStep(M, params, m, v, k + 1, args.lr)
+        del A, E, M
         print(f"mesh {k:3d} sigma {sigma:.4f}: {it} PCG iterations to {args.record_tol:g}, loss {loss:.4e}, "
               f"|grad| {np.linalg.norm(grad):.3e}, w factor {np.exp(params[-1]):.3f}, {time.time() - t0:.1f} s", flush=True)
         if not np.all(np.isfinite(params)):
             raise RuntimeError("training diverged")
+        if args.checkpoint:
+            np.savez(args.checkpoint, params=params, m=m, v=v, step=k + 1)
     LP.save_model(args.out, params, F)
     print(f"wrote {args.out}", flush=True)
 
