@@ -190,7 +190,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="sphere_1310720")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                    help="energy + differential: FP64, or FP32 pair arithmetic with FP64 accumulation")
+    ap.add_argument("--metric-precision", default="fp64", choices=["fp64", "mixed"],
+                    help="metric blocks of the solve (FP64 keeps the final residual the FP64 one)")
+    ap.add_argument("--learned", default=None,
+                    help="learned preconditioner model (default: the packaged model if present; '' for the two-level form)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the comparison solves (two-level FP64, reference P0 FP64) and the FP64 energy timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-pairs", type=float, default=1.2e8)
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run only the copy calibration kernel (MiB)")
@@ -226,9 +233,13 @@ def main():
     M.upload()
     t_upload = time.perf_counter() - t0
 
+    from repulsor_amd import learned_pc as LP
+    learned = args.learned if args.learned is not None else (LP.DEFAULT_MODEL if os.path.exists(LP.DEFAULT_MODEL) else "")
     E = R.TangentPointEnergy0(wl.q, wl.p, precision=args.precision)
-    A = R.TangentPointMetric0(wl.q, wl.p, precision=args.precision)
+    A = R.TangentPointMetric0(wl.q, wl.p, precision=args.metric_precision)
     A.SetPreconditionerDiagonalWeight(args.pc_weight)
+    if learned:
+        A.SetLearnedPreconditioner(path=learned, bf16=True)
 
     hip = _hip()
     Yd = C.c_void_p()
@@ -255,11 +266,13 @@ def main():
           f"preconditioner setup {pc_setup_dev_ms:.2f} ms (device, weight {pc_weight:.4g}), "
           f"H1 assembly+factor+upload+setup {t_pc_prep:.2f} s (wall)", file=sys.stderr, flush=True)
 
-    it_c = (C.c_int * 1)()
+    it_c = (C.c_int * 2)()     # iteration count, 1 if the learned preconditioner fell back
     rel = np.zeros(3)
     solve_flags = 0 if args.solver == "pcg" else R.RP_SOLVE_GMRES
     dY = C.cast(Yd, C.POINTER(C.c_double))
     dX = C.cast(Xd, C.POINTER(C.c_double))
+
+    fell_back = []
 
     def step():
         _check(H.rpm_energy_differential(E._h, M._h, 1.0, 0.0, dY, 3, _p(en), 1))
@@ -270,6 +283,7 @@ def main():
         st = H.rpm_metric_solve(A._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, args.max_iter, args.tol, solve_flags, 1, it_c, _p(rel))
         if st not in (0, RP_ESTAGNATION):
             _check(st)
+        fell_back.append(int(it_c[1]))
         return t_eg, tn, tf, M.last_timing(3)
 
     for _ in range(args.warmup):
@@ -302,17 +316,36 @@ def main():
         h1l_ms.append(M.last_timing(8))
     pc_ms, h1_ms = float(np.mean(pcl_ms)), float(np.mean(h1l_ms))
 
-    # one solve with the reference preconditioner P0 alone, for comparison (capped to about a minute)
-    plain = None
-    if not args.no_solve and args.plain_pc_max_iter > 0:
-        A0 = R.TangentPointMetric0(wl.q, wl.p, precision=args.precision)
-        A0.SetPreconditionerDiagonalWeight(0.0)
-        per_it = float(np.mean(mm_ms)) + pc_ms
-        cap = int(max(10, min(args.plain_pc_max_iter, 60000.0 / max(per_it, 1e-3))))
-        st = H.rpm_metric_solve(A0._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, cap, args.tol, solve_flags, 1, it_c, _p(rel))
-        plain = {"solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]), "converged": st == 0,
-                 "rel_residuals": [float(x) for x in rel], "iteration_cap": cap}
-        print(f"# reference P0 alone: {plain}", file=sys.stderr, flush=True)
+    # comparison forms on the same right-hand side: the two-level preconditioner without the learned
+    # scalings, and the plain FP64 form (FP64 energy + differential, reference P0 alone; the P0 solve is
+    # capped to about a minute of iterations)
+    plain = two_level = eg64 = None
+    headline_learned = bool(learned) and not any(fell_back)
+    if not args.no_solve and not args.no_compare:
+        E64 = R.TangentPointEnergy0(wl.q, wl.p, precision="fp64")
+        t64 = []
+        for _ in range(3):
+            _check(H.rpm_energy_differential(E64._h, M._h, 1.0, 0.0, dY, 3, _p(en), 1))
+            t64.append(M.last_timing(0) + M.last_timing(1))
+        eg64 = float(np.mean(t64))
+        A2 = R.TangentPointMetric0(wl.q, wl.p, precision="fp64")
+        A2.SetPreconditionerDiagonalWeight(-1.0)
+        st = H.rpm_metric_solve(A2._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, args.max_iter, args.tol, solve_flags, 1, it_c, _p(rel))
+        two_level = {"solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]), "converged": st == 0,
+                     "rel_residuals": [float(x) for x in rel]}
+        print(f"# two-level P0 + w diag(A)^-1, FP64: {two_level}", file=sys.stderr, flush=True)
+        if args.plain_pc_max_iter > 0:
+            A0 = R.TangentPointMetric0(wl.q, wl.p, precision="fp64")
+            A0.SetPreconditionerDiagonalWeight(0.0)
+            per_it = float(np.mean(mm_ms)) + pc_ms
+            cap = int(max(10, min(args.plain_pc_max_iter, 60000.0 / max(per_it, 1e-3))))
+            st = H.rpm_metric_solve(A0._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, cap, args.tol, solve_flags, 1, it_c, _p(rel))
+            plain = {"solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]), "converged": st == 0,
+                     "rel_residuals": [float(x) for x in rel], "iteration_cap": cap,
+                     "energy_grad_fp64_ms": round(eg64, 3),
+                     "step_ms": round(eg64 + M.last_timing(3), 1)}
+            plain["step_ms_is_lower_bound"] = not plain["converged"]
+            print(f"# plain FP64 form (reference P0 alone): {plain}", file=sys.stderr, flush=True)
 
     pairs_nf = b["near_nnz"] // 2
     pairs_ff = b["far_nnz"] // 2
@@ -320,10 +353,10 @@ def main():
     nf_avg_s = float(np.mean(nf_ms)) * 1e-3
     achieved = nf_flops / nf_avg_s / 1e12
     peak = PEAK_FP32_TFLOPS if args.precision == "mixed" else PEAK_FP64_TFLOPS
-    spmm_bytes = spmm_algorithmic_bytes(b, args.precision)
+    spmm_bytes = spmm_algorithmic_bytes(b, args.metric_precision)
     spmm_s = float(np.mean(spmm_ms)) * 1e-3
     spmm_gbs = spmm_bytes / spmm_s / 1e9
-    traffic = MEASURED_SPMM_TRAFFIC.get((wl.name, args.precision))
+    traffic = MEASURED_SPMM_TRAFFIC.get((wl.name, args.metric_precision))
     result = {
         "metric": "energy+grad ms & metric-solve ms to 1e-6 residual, learned vs plain PC, 1 GPU",
         "value": round(ms_per_step, 3),
@@ -335,17 +368,20 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-pairs/f64-accumulate",
+        "dtype": ("f64" if args.precision == "fp64" else "f32-pairs/f64-accumulate")
+                 + ("" if args.metric_precision == args.precision else f", metric {args.metric_precision}"),
         "data": "synthetic",
         "config": {
             "workload": wl.name, "triangles": b["T"], "vertices": b["V"], "q": wl.q, "p": wl.p,
             "theta_far": wl.far_theta, "theta_near": wl.near_theta, "leaf_size": wl.leaf,
             "near_nnz": b["near_nnz"], "far_nnz": b["far_nnz"], "very_near_nnz": b["verynear_nnz"],
-            "precision": args.precision,
+            "precision": args.precision, "metric_precision": args.metric_precision,
             "step": ("energy + differential (device-resident) + metric solve to |B - A X|/|B| <= "
-                     f"{args.tol:g} per column with P0 = H^-1 L H^-1 ({args.solver})") if not args.no_solve
+                     f"{args.tol:g} per column ({args.solver})") if not args.no_solve
                     else "energy + differential (device-resident)",
-            "preconditioner": ("P0 + w diag(A)^-1" if args.pc_weight != 0 else "reference P0"),
+            "preconditioner": ("learned D^1/2 (P0 + w diag(A)^-1) D^1/2 (BF16 inference), model "
+                               + os.path.relpath(learned, ROOT)) if learned else
+                              ("P0 + w diag(A)^-1" if args.pc_weight != 0 else "reference P0 = H^-1 L H^-1"),
         },
         "energy": float(en[0]),
         "energy_grad_ms": round(float(np.mean(eg_ms)), 3),
@@ -366,7 +402,10 @@ def main():
         "solver": args.solver,
         "pc_weight": pc_weight,
         "pc_setup_ms": round(pc_setup_dev_ms, 3),
-        "solve_reference_p0": plain,
+        "solve_learned_fell_back": (not headline_learned) if learned else None,
+        "solve_two_level_fp64": two_level,
+        "plain_fp64_form": plain,
+        "speedup_vs_plain_fp64_form": (round(plain["step_ms"] / ms_per_step, 2) if plain else None),
         "tree_bct_build_s": round(t_build, 2),
         "upload_s": round(t_upload, 2),
         "pairs_per_s": round((pairs_nf + pairs_ff) / (float(np.mean(eg_ms)) * 1e-3), 1),

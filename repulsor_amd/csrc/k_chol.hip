@@ -218,22 +218,62 @@ namespace rp
             }
             __syncthreads();
         }
-        for( int a = tid; a < w; a += BS )
+        (void)Ls;
+        double * aux = F.aux + 3 * (int64_t)f;
+        for( int a = tid; a < 3 * w; a += BS ) { aux[a] = zs[a]; }
+    }
+
+    // Forward, split form, diagonal block: y = Inv * aux (Inv lower triangular, the top w x w of the
+    // supernode block) into the front's first w rows. 64 rows per task, one per lane; the 4 waves
+    // take a quarter of the columns each and their partial sums meet in LDS.
+    __global__ __launch_bounds__(256) void k_chol_fwd_diag( CholDev F, const int2 * tasks )
+    {
+        extern __shared__ double sh[];
+        double * xs = sh;                       // 3 w
+        double * part = sh + 3 * CHOL_MAX_WIDTH; // 4 x 64 x 3
+        const int2 tk = tasks[blockIdx.x];
+        const int s = tk.x;
+        const int f = F.sn_begin[s];
+        const int w = F.sn_begin[s + 1] - f;
+        const int64_t rb = F.row_ptr[s];
+        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const double * Ls = F.vals + F.val_ptr[s];
+        const double * aux = F.aux + 3 * (int64_t)f;
+        for( int a = threadIdx.x; a < 3 * w; a += 256 ) { xs[a] = aux[a]; }
+        __syncthreads();
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int a = tk.y + lane;
+        const int cw = ( w + 3 ) / 4;
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        if( a < w )
         {
-            double acc[3] = { 0.0, 0.0, 0.0 };
+            const int cb = wv * cw, ce = min( a + 1, min( w, cb + cw ) );
             #pragma unroll 8
-            for( int c = 0; c <= a; ++c )
+            for( int c = cb; c < ce; ++c )
             {
                 const double l = Ls[(int64_t)c * r + a];
-                for( int k = 0; k < 3; ++k ) { acc[k] += l * zs[3 * c + k]; }
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * xs[3 * c + k]; }
             }
-            for( int k = 0; k < 3; ++k ) { z[3 * a + k] = acc[k]; }
+        }
+        for( int k = 0; k < 3; ++k ) { part[(wv * 64 + lane) * 3 + k] = acc[k]; }
+        __syncthreads();
+        if( wv == 0 && a < w )
+        {
+            double * z = F.z + 3 * rb;
+            for( int k = 0; k < 3; ++k )
+            {
+                z[3 * a + k] = ( part[lane * 3 + k] + part[(64 + lane) * 3 + k] ) + ( part[(128 + lane) * 3 + k] + part[(192 + lane) * 3 + k] );
+            }
         }
     }
 
-    __global__ __launch_bounds__(CHOL_SPLIT_BS) void k_chol_fwd2( CholDev F, const int2 * tasks )
+    // Forward, split form, update rows: z[q] -= L(q, 0:w) y for 64 rows q per task (one per lane),
+    // the 4 waves taking a quarter of the columns each, partial sums added in LDS.
+    __global__ __launch_bounds__(256) void k_chol_fwd2( CholDev F, const int2 * tasks )
     {
         extern __shared__ double sh[];
+        double * ys = sh;                        // 3 w
+        double * part = sh + 3 * CHOL_MAX_WIDTH; // 4 x 64 x 3
         const int2 tk = tasks[blockIdx.x];
         const int s = tk.x;
         const int w = F.sn_begin[s + 1] - F.sn_begin[s];
@@ -241,19 +281,31 @@ namespace rp
         const int r = (int)( F.row_ptr[s + 1] - rb );
         double * z = F.z + 3 * rb;
         const double * Ls = F.vals + F.val_ptr[s];
-        double * ys = sh;   // 3 w
-        for( int a = threadIdx.x; a < 3 * w; a += CHOL_SPLIT_BS ) { ys[a] = z[a]; }
+        for( int a = threadIdx.x; a < 3 * w; a += 256 ) { ys[a] = z[a]; }
         __syncthreads();
-        const int q = tk.y + threadIdx.x;
-        if( q >= r ) { return; }
-        double acc[3] = { z[3 * q], z[3 * q + 1], z[3 * q + 2] };
-        #pragma unroll 8
-        for( int c = 0; c < w; ++c )
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int q = tk.y + lane;
+        const int cw = ( w + 3 ) / 4;
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        if( q < r )
         {
-            const double l = Ls[(int64_t)c * r + q];
-            for( int k = 0; k < 3; ++k ) { acc[k] -= l * ys[3 * c + k]; }
+            const int cb = wv * cw, ce = min( w, cb + cw );
+            #pragma unroll 8
+            for( int c = cb; c < ce; ++c )
+            {
+                const double l = Ls[(int64_t)c * r + q];
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * ys[3 * c + k]; }
+            }
         }
-        for( int k = 0; k < 3; ++k ) { z[3 * q + k] = acc[k]; }
+        for( int k = 0; k < 3; ++k ) { part[(wv * 64 + lane) * 3 + k] = acc[k]; }
+        __syncthreads();
+        if( wv == 0 && q < r )
+        {
+            for( int k = 0; k < 3; ++k )
+            {
+                z[3 * q + k] -= ( part[lane * 3 + k] + part[(64 + lane) * 3 + k] ) + ( part[(128 + lane) * 3 + k] + part[(192 + lane) * 3 + k] );
+            }
+        }
     }
 
     // group size for reductions of length n: a power of two, at most 64
@@ -358,9 +410,12 @@ namespace rp
     {
         if( count <= 0 ) { return; }
         const dim3 grid( (unsigned)count ), block( CHOL_SPLIT_BS );
-        if( kind == 0 )      { hipLaunchKernelGGL( k_chol_fwd2, grid, block, sizeof(double) * 3 * (size_t)wmax, s, F, tasks ); }
+        const size_t fwd_shm = sizeof(double) * ( 3 * (size_t)CHOL_MAX_WIDTH + 4 * 64 * 3 );
+        (void)wmax;
+        if( kind == 0 )      { hipLaunchKernelGGL( k_chol_fwd2, grid, block, fwd_shm, s, F, tasks ); }
         else if( kind == 1 ) { hipLaunchKernelGGL( k_chol_bwd1, grid, block, 0, s, F, tasks ); }
-        else                 { hipLaunchKernelGGL( k_chol_bwd2, grid, block, 0, s, F, tasks ); }
+        else if( kind == 2 ) { hipLaunchKernelGGL( k_chol_bwd2, grid, block, 0, s, F, tasks ); }
+        else                 { hipLaunchKernelGGL( k_chol_fwd_diag, grid, block, fwd_shm, s, F, tasks ); }
     }
 
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s )

@@ -92,13 +92,16 @@ namespace rp
         const int32_t * perm;       // new -> old
         double        * z;          // 3 x rows: front vectors
         double        * xp;         // 3 x n: solution in the new numbering
+        double        * aux;        // 3 x n: merged right-hand sides of the supernode columns (split levels)
     };
     void launch_chol_fwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big,
                                 double alpha, const double * B, int64_t ldB, hipStream_t s );
     void launch_chol_bwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big, hipStream_t s );
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
     // split form for levels with large supernodes; tasks are (supernode, first row/column) pairs
-    constexpr int CHOL_TASK_ROWS = 256;   // rows per forward-2 task (= threads per workgroup)
+    constexpr int CHOL_TASK_ROWS = 256;   // threads per workgroup of the split backward tasks
+    constexpr int CHOL_FWD_ROWS = 64;     // rows per split forward task (one per lane)
+    constexpr int CHOL_MAX_WIDTH = 512;   // columns per supernode
     void launch_chol_fwd1_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax,
                                  double alpha, const double * B, int64_t ldB, hipStream_t s );
     void launch_chol_tasks( int kind, const CholDev & F, const int2 * tasks, int64_t count, int wmax, hipStream_t s );

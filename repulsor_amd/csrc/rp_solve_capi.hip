@@ -59,7 +59,7 @@ namespace rp
         F.sn_begin = ctx->h1_sn_begin.get(); F.row_ptr = ctx->h1_row_ptr.get(); F.rows = ctx->h1_rows.get();
         F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get();
         F.kid_ptr = ctx->h1_kid_ptr.get(); F.kids = ctx->h1_kids.get(); F.cmap = ctx->h1_cmap.get();
-        F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get();
+        F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get(); F.aux = ctx->h1_aux.get();
         return F;
     }
 
@@ -80,6 +80,8 @@ namespace rp
             }
             launch_chol_fwd1_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], alpha, Bd, ldb, ctx->stream );
             int64_t tb, te;
+            task_range( 3, l, tb, te );
+            launch_chol_tasks( 3, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
             task_range( 0, l, tb, te );
             launch_chol_tasks( 0, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
         }
@@ -685,14 +687,15 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
             }
         }
         // Levels holding a supernode with more than 64 rows run in the split form; their task lists:
-        // forward 2 by 256 rows, backward 1 and 2 by as many columns as groups fit in 256 threads.
+        // forward updates and forward diagonal blocks by 64 rows, backward 1 and 2 by as many columns
+        // as groups fit in 256 threads.
         std::vector<char> split( (size_t)nl, 0 );
         std::vector<int2> tasks;
-        std::vector<int64_t> tptr[3];
-        for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
+        std::vector<int64_t> tptr[4];
+        for( int kind = 0; kind < 4; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
         auto group_of = []( int64_t len ) { int g = 1; while( g < len && g < 64 ) { g <<= 1; } return g; };
         {
-            std::vector<int2> tk[3];
+            std::vector<int2> tk[4];
             for( int32_t l = 0; l < nl; ++l )
             {
                 split[(size_t)l] = rmax[(size_t)l] > 64;
@@ -702,7 +705,8 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                     const int32_t s = lnodes[(size_t)k];
                     const int32_t w = sn_begin[s + 1] - sn_begin[s];
                     const int32_t r = (int32_t)( row_ptr[s + 1] - row_ptr[s] );
-                    for( int32_t q0 = w; q0 < r; q0 += CHOL_TASK_ROWS ) { tk[0].push_back( make_int2( s, q0 ) ); }
+                    for( int32_t q0 = w; q0 < r; q0 += CHOL_FWD_ROWS ) { tk[0].push_back( make_int2( s, q0 ) ); }
+                    for( int32_t a0 = 0; a0 < w; a0 += CHOL_FWD_ROWS ) { tk[3].push_back( make_int2( s, a0 ) ); }
                     if( r > w )
                     {
                         const int per = CHOL_TASK_ROWS / group_of( r - w );
@@ -711,10 +715,10 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                     const int per2 = CHOL_TASK_ROWS / group_of( w );
                     for( int32_t c0 = 0; c0 < w; c0 += per2 ) { tk[2].push_back( make_int2( s, c0 ) ); }
                 }
-                for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
+                for( int kind = 0; kind < 4; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
             }
             int64_t off = 0;
-            for( int kind = 0; kind < 3; ++kind )
+            for( int kind = 0; kind < 4; ++kind )
             {
                 for( auto & x : tptr[kind] ) { x += off; }
                 tasks.insert( tasks.end(), tk[kind].begin(), tk[kind].end() );
@@ -734,12 +738,13 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_level_nodes.upload( lnodes.data(), lnodes.size(), ctx->stream );
         ctx->h1_z.require( (size_t)(3 * nrows) );
         ctx->h1_xp.require( (size_t)(3 * (int64_t)n) );
+        ctx->h1_aux.require( (size_t)(3 * (int64_t)n) );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->h1_level_ptr = lptr;
         ctx->h1_level_wmax = wmax;
         ctx->h1_level_rmax = rmax;
         ctx->h1_level_split = split;
-        for( int kind = 0; kind < 3; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
+        for( int kind = 0; kind < 4; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
         ctx->h1_n = n;
         ctx->h1_ns = ns;
         ctx->have_h1 = true;
