@@ -67,7 +67,8 @@ namespace Repulsor
     protected:
         int threads = 1;
         int device  = 0;
-        std::vector<double> V_coords, V_coords_frozen, V_charges;
+        mutable std::vector<double> V_coords;        // host copy (refreshed from the device when stale)
+        std::vector<double> V_coords_frozen, V_charges;
         std::vector<Int>    simplices;
         bool have_charges = false;
 
@@ -78,6 +79,11 @@ namespace Repulsor
         mutable std::unique_ptr<DeviceContext> dev;
         mutable bool dev_topology = false;
         mutable bool dev_data     = false;
+        // after a device-resident SemiStaticUpdate the device holds newer coordinates (coords_stale);
+        // after any SemiStaticUpdate the host tree rows are recomputed only when a host computation
+        // needs them (rows_stale)
+        mutable bool coords_stale = false;
+        mutable bool rows_stale   = false;
 
     public:
         // Row-major V x 3 coordinates and T x (n+1) simplices, as the reference's pointer constructor
@@ -108,13 +114,34 @@ namespace Repulsor
         int  ThreadCount()  const { return threads; }
         void SetDevice( int d ) { device = d; }
 
-        const std::vector<double> & VertexCoordinates() const { return V_coords; }
+        const std::vector<double> & VertexCoordinates() const { RequireHostCoordinates(); return V_coords; }
+
+        // Host copy of the coordinates of the current state.
+        void RequireHostCoordinates() const
+        {
+            if( !coords_stale ) { return; }
+            if( rp_download_state_rows( dev->get(), nullptr, nullptr, V_coords.data() ) != RP_OK )
+            {
+                throw std::runtime_error( std::string("rp_download_state_rows: ") + rp_last_error() );
+            }
+            coords_stale = false;
+        }
         const std::vector<Int>    & Simplices()         const { return simplices; }
         const double * VertexCharges() const { return have_charges ? V_charges.data() : nullptr; }
 
         // ---------------------------------------------------------------- persistent cache
         const Tree_T & GetClusterTree() const
         {
+            RequireHostCoordinates();
+            if( tree && rows_stale )
+            {
+                ScopedTimer timer("ClusterTree::SemiStaticUpdate");
+                const LInt T = SimplexCount();
+                std::vector<double> P_near( (size_t)(NEAR_DIM * T) ), P_far( (size_t)(FAR_DIM * T) );
+                ComputeNearFarData<DOM_DIM>( V_coords.data(), simplices.data(), VertexCharges(), T, P_near.data(), P_far.data(), threads );
+                tree->SemiStaticUpdate( P_near.data(), P_far.data() );
+            }
+            rows_stale = false;
             if( !tree )
             {
                 ScopedTimer timer("SimplicialMesh::GetClusterTree");
@@ -156,21 +183,45 @@ namespace Repulsor
         }
 
         // Reads new coordinates without rebuilding the tree (src/SimplicialMesh.hpp:354-376).
-        void SemiStaticUpdate( const double * V_new, const bool transp = false )
+        // With device_ptr, V_new is a V x 3 row-major array in HBM and the state is computed on the GPU
+        // (rp_update_coordinates); the host reads it back only when a host computation needs it
+        // (H^1 metric, learned-preconditioner features, read-outs). Without it, the coordinates are
+        // copied and, once the device is set up, sent to the device, which computes the per-state data
+        // itself (15.7 MB of coordinates instead of 168 MB of rows at 1.31M triangles).
+        void SemiStaticUpdate( const double * V_new, const bool transp = false, const bool device_ptr = false )
         {
             ScopedTimer timer("SimplicialMesh::SemiStaticUpdate");
             const LInt V = VertexCount();
+            ClearCache();
+            if( device_ptr )
+            {
+                if( transp ) { throw std::runtime_error( "SemiStaticUpdate: device coordinates must be row-major (transp = false)" ); }
+                rp_ctx * c = Device();
+                const int singletons = GetClusterTree().SplitThreshold() <= 1 ? 1 : 0;
+                if( rp_update_coordinates( c, V_new, 3, singletons, RP_DEVICE_PTR ) != RP_OK )
+                {
+                    throw std::runtime_error( std::string("rp_update_coordinates: ") + rp_last_error() );
+                }
+                coords_stale = true;
+                rows_stale = (bool)tree;
+                dev_data = true;
+                return;
+            }
             if( transp ) { for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { V_coords[(size_t)(3 * v + k)] = V_new[k * V + v]; } } }
             else { std::copy( V_new, V_new + 3 * V, V_coords.begin() ); }
-            ClearCache();
-            if( tree )
+            coords_stale = false;
+            rows_stale = (bool)tree;
+            if( dev && dev_topology )
             {
-                const LInt T = SimplexCount();
-                std::vector<double> P_near( (size_t)(NEAR_DIM * T) ), P_far( (size_t)(FAR_DIM * T) );
-                ComputeNearFarData<DOM_DIM>( V_coords.data(), simplices.data(), VertexCharges(), T, P_near.data(), P_far.data(), threads );
-                tree->SemiStaticUpdate( P_near.data(), P_far.data() );
+                const int singletons = GetClusterTree().SplitThreshold() <= 1 ? 1 : 0;
+                if( rp_update_coordinates( dev->get(), V_coords.data(), 3, singletons, 0 ) != RP_OK )
+                {
+                    throw std::runtime_error( std::string("rp_update_coordinates: ") + rp_last_error() );
+                }
+                dev_data = true;
+                device_state.clear();
             }
-            dev_data = false;
+            else { dev_data = false; }
         }
 
         // ---------------------------------------------------------------- value cache
@@ -204,6 +255,7 @@ namespace Repulsor
 
         const CSRMatrix & H1Metric() const
         {
+            RequireHostCoordinates();
             if( !h1 )
             {
                 h1 = std::make_unique<CSRMatrix>( AssembleH1Metric<DOM_DIM>( V_coords.data(), VertexCount(), simplices.data(), SimplexCount(),

@@ -95,6 +95,23 @@ int rp_upload_verynear_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp, 
  * Replaces: ClusterTree::SemiStaticUpdate data path (reference src/ClusterTree.hpp:983-993). */
 int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near, const double * C_far );
 
+/* Per-state data computed on the device from new vertex coordinates (V x 3, leading dimension ldV,
+ * host pointer or RP_DEVICE_PTR): simplex rows (area, hull, barycenter, projector) in tree order and
+ * the cluster moments (area-weighted averages, leaves then levels up to the root). The results equal,
+ * bit for bit, rp_upload_data of the host's rows for the same coordinates (the same operations in the
+ * same order, no contraction). singletons != 0: leaves copy their primitive's row (split threshold
+ * <= 1). Tree, patterns and pre/post operators stay as uploaded. Device-pointer coordinates are not
+ * checked for NaN.
+ * Replaces: SimplicialMesh::SemiStaticUpdate (reference src/SimplicialMesh.hpp:354-376) with
+ * ComputeNearFarData (src/SimplicialMesh/ComputeNearFarData.hpp:3-27) and ClusterTree::ComputeClusterData
+ * (src/ClusterTree/ComputeClusterData_Thread.hpp:50-129). */
+int rp_update_coordinates( rp_ctx * ctx, const double * V_coords, int64_t ldV, int singletons, int ptr_flags );
+
+/* After rp_update_coordinates: the device's near rows (T x ND, tree order), cluster rows (C x 10) and
+ * vertex coordinates (V x 3). Any pointer may be NULL. No reference counterpart (the host mesh reads
+ * its state back through it when a host computation needs the coordinates). */
+int rp_download_state_rows( rp_ctx * ctx, double * P_near, double * C_far, double * V_coords );
+
 /* TangentPointEnergy0 kernels over the very-near, near and far fields (flags: RP_ENERGY | RP_DIFF |
  * RP_METRIC). energy_out may be NULL. RP_METRIC stores the metric blocks on the device for
  * rp_multiply_metric. The very-near field is computed in FP64 at either precision; its energy terms

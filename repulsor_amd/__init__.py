@@ -72,7 +72,7 @@ _host_sigs = {
     "rpm_mesh_tree_data": (C.c_int, [_vp, _d, _d, _d, _d, _d, _i32, _d]),
     "rpm_mesh_pattern": (C.c_int, [_vp, C.c_int, _i64, _i32, _i64, _i64, C.c_int32]),
     "rpm_mesh_leaf_blocks": (C.c_int, [_vp, _i32, _i32, _i32]),
-    "rpm_mesh_semistatic_update": (C.c_int, [_vp, _d]),
+    "rpm_mesh_semistatic_update": (C.c_int, [_vp, _d, C.c_int]),
     "rpm_mesh_coords": (C.c_int, [_vp, _d]),
     "rpm_mesh_device": (C.c_int, [_vp]),
     "rpm_mesh_timing": (C.c_int, [_vp, C.c_int, _d]),
@@ -95,6 +95,7 @@ _host_sigs = {
     "rpm_metric_solve": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int, C.c_int,
                                    C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _d]),
     "rpm_pl_values": (C.c_int, [_vp, _vp, _d, _d, _d, _d]),
+    "rpm_mesh_device_state_rows": (C.c_int, [_vp, _d, _d]),
     "rpm_vf_metric_values": (C.c_int, [_vp, _vp, _d, _d]),
     "rpm_pl_vf_values": (C.c_int, [_vp, _vp, _d, _d]),
     "rpm_vf_stats": (C.c_int, [_vp, _i64, C.POINTER(C.c_int)]),
@@ -286,10 +287,23 @@ class SimplicialMesh:
         _check(H.rpm_vf_stats(self._h, _p(ev, _i64), C.byref(lv)))
         return int(ev[0]), int(lv.value)
 
-    def SemiStaticUpdate(self, coords):
+    def SemiStaticUpdate(self, coords=None, device_ptr=None):
+        """New vertex coordinates (V x 3) from host memory, or from a device pointer (V x 3 row-major
+        doubles in HBM, an int or ctypes pointer): the per-state data is then computed on the GPU."""
+        if device_ptr is not None:
+            _check(H.rpm_mesh_semistatic_update(self._h, C.cast(C.c_void_p(int(device_ptr) if not isinstance(device_ptr, C.c_void_p) else device_ptr.value), _d), 1))
+            return
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         assert coords.shape == (self.V, 3)
-        _check(H.rpm_mesh_semistatic_update(self._h, _p(coords)))
+        _check(H.rpm_mesh_semistatic_update(self._h, _p(coords), 0))
+
+    def device_state_rows(self):
+        """Near rows (T x ND) and cluster rows (C x 10) computed on the device by the last update."""
+        b = self.build()
+        nd = 1 + 3 * (self.dom_dim + 1) + 6
+        N = np.zeros((b["T"], nd)); Cf = np.zeros((b["C"], 10))
+        _check(H.rpm_mesh_device_state_rows(self._h, _p(N), _p(Cf)))
+        return N, Cf
 
     def VertexCoordinates(self):
         X = np.zeros((self.V, 3))

@@ -197,9 +197,15 @@ int rpm_mesh_leaf_blocks( void * h, int32_t * blocks, int32_t * vf_pairs, int32_
     });
 }
 
-int rpm_mesh_semistatic_update( void * h, const double * coords )
+int rpm_mesh_semistatic_update( void * h, const double * coords, int device_ptr )
 {
-    return call( (MeshHandle *)h, [&]( auto & M ) { M.SemiStaticUpdate( coords ); return RP_OK; } );
+    return call( (MeshHandle *)h, [&]( auto & M ) { M.SemiStaticUpdate( coords, false, device_ptr != 0 ); return RP_OK; } );
+}
+
+// Device-computed rows of the last device-resident update (tree order), for comparison with the host's.
+int rpm_mesh_device_state_rows( void * h, double * P_near, double * C_far )
+{
+    return call( (MeshHandle *)h, [&]( auto & M ) { return rp_download_state_rows( M.Device(), P_near, C_far, nullptr ); } );
 }
 
 int rpm_mesh_coords( void * h, double * coords )

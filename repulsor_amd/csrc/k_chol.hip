@@ -177,8 +177,11 @@ namespace rp
     // ---------------------------------------------------------------------------------------------
     // Split form for levels with large supernodes: the work of one supernode is spread over many
     // workgroups through task lists (int2 = supernode, first row or column).
-    //   forward  1: one workgroup per supernode: init, merge the children, y = inv(L_ss) z[0:w]
-    //   forward  2: one workgroup per 256 update rows: z[q] -= L(q, :) y
+    //   forward  1: one workgroup per supernode: init and merge the children; the supernode's own
+    //               rows go to the column buffer aux, the update rows stay in z
+    //   forward  d: one workgroup per 64 rows of the diagonal block: z[0:w] = inv(L_ss) aux
+    //   forward  2: one workgroup per 64 update rows: z[q] -= L(q, :) y
+    //               (forward d and 2: lane = row, the 4 waves split the columns, sums meet in LDS)
     //   backward 1: one workgroup per group of columns: t[c] = y[c] - L(:, c)^T x(R[w:]) (in place)
     //   backward 2: one workgroup per group of columns: x[f + c] = inv(L_ss)(:, c)^T t
     constexpr int CHOL_SPLIT_BS = 256;

@@ -385,7 +385,70 @@ int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near
         RP_HIP( hipGetLastError() );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_data = true;
+        ctx->have_state_rows = false;
         ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = false;
+        ctx->have_vf_dh = ctx->have_vf_metric = ctx->have_pl_vf = false;
+    });
+}
+
+int rp_update_coordinates( rp_ctx * ctx, const double * V_coords, int64_t ldV, int singletons, int ptr_flags )
+{
+    return guarded( "rp_update_coordinates", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree, "upload the mesh and the tree first", RP_ENOTREADY );
+        require( V_coords != nullptr && ldV >= 3, "V_coords must be V x 3 with ldV >= 3" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T, C = ctx->C, L = ctx->L, V = ctx->V;
+        hipStream_t s = ctx->stream;
+        if( ptr_flags & RP_DEVICE_PTR )
+        {
+            launch_copy3( V, V_coords, ldV, ctx->coords.get(), 3, s );
+        }
+        else
+        {
+            std::vector<double> h;
+            pack_rows( V_coords, ldV, V, h );
+            for( double x : h ) { require( std::isfinite( x ), "vertex coordinates contain NaN or Inf" ); }
+            ctx->coords.upload( h.data(), h.size(), s );
+        }
+        ctx->stage.require( (size_t)( ctx->nd * T ) );
+        ctx->upd_far.require( (size_t)( 10 * ( T + C ) ) );
+        double * near = ctx->stage.get();
+        double * pfar = ctx->upd_far.get();
+        double * cfar = pfar + 10 * T;
+        launch_near_far_rows( ctx->dom_dim, T, ctx->P_ordering.get(), ctx->simplices.get(), ctx->coords.get(), 3,
+                              ctx->have_charges ? ctx->charges.get() : nullptr, near, pfar, s );
+        launch_cluster_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), pfar, singletons, cfar, s );
+        for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
+        {
+            const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
+            launch_cluster_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), cfar, s );
+        }
+        ctx->p_a.require( (size_t)T ); ctx->p_x.require( (size_t)(3 * T) ); ctx->p_P.require( (size_t)(6 * T) );
+        launch_prep_prims( T, ctx->nd, ctx->size, near, ctx->p_a.get(), ctx->p_x.get(), ctx->p_P.get(), s );
+        ctx->c_a.require( (size_t)C ); ctx->c_x.require( (size_t)(3 * C) ); ctx->c_P.require( (size_t)(6 * C) );
+        launch_prep_clusters( C, cfar, ctx->c_a.get(), ctx->c_x.get(), ctx->c_P.get(), s );
+        ctx->leaf_origin.require( (size_t)(3 * L) );
+        launch_leaf_origin( L, C, ctx->leaf_cluster.get(), ctx->c_x.get(), ctx->leaf_origin.get(), s );
+        RP_HIP( hipGetLastError() );
+        RP_HIP( hipStreamSynchronize( s ) );
+        ctx->have_data = true;
+        ctx->have_state_rows = true;
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = false;
+        ctx->have_vf_dh = ctx->have_vf_metric = ctx->have_pl_vf = false;
+    });
+}
+
+int rp_download_state_rows( rp_ctx * ctx, double * P_near, double * C_far, double * V_coords )
+{
+    return guarded( "rp_download_state_rows", [&]()
+    {
+        require( ctx != nullptr && ctx->have_state_rows, "no rows from rp_update_coordinates", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        if( P_near ) { ctx->stage.download( P_near, (size_t)( ctx->nd * ctx->T ), ctx->stream ); }
+        if( C_far ) { RP_HIP( hipMemcpyAsync( C_far, ctx->upd_far.get() + 10 * ctx->T, sizeof(double) * 10 * ctx->C, hipMemcpyDeviceToHost, ctx->stream ) ); }
+        if( V_coords ) { ctx->coords.download( V_coords, (size_t)( 3 * ctx->V ), ctx->stream ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
     });
 }
 

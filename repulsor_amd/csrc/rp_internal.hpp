@@ -143,7 +143,7 @@ struct rp_ctx
     rp::DBuf<double>  pre_vals;          // T x 4 x (n+1)
     rp::DBuf<int64_t> vp_ptr;            // vertex -> (tree primitive, slot), sorted by primitive
     rp::DBuf<int32_t> vp_entry;          // primitive * (n+1) + slot
-    std::vector<int32_t> h_level_ptr;    // clusters grouped by depth (deepest last)
+    std::vector<int32_t> h_level_ptr;    // internal clusters grouped by depth, deepest first
     rp::DBuf<int32_t> level_nodes;
     bool have_tree = false;
 
@@ -179,7 +179,9 @@ struct rp_ctx
     rp::DBuf<double> p_a, p_x, p_P;      // T, 3T, 6T
     rp::DBuf<double> c_a, c_x, c_P;      // C, 3C, 6C
     rp::DBuf<double> leaf_origin;        // 3L
-    rp::DBuf<double> stage;              // upload staging
+    rp::DBuf<double> stage;              // upload staging; after rp_update_coordinates: the near rows T x ND
+    rp::DBuf<double> upd_far;            // after rp_update_coordinates: primitive far rows T x 10, cluster rows C x 10
+    bool have_state_rows = false;
     bool have_data = false;
 
     // energy/derivative outputs

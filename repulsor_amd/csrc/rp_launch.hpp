@@ -142,6 +142,13 @@ namespace rp
     void launch_metric_diag( bool mixed, const DiagArgs & A, const void * nf_vals, double * S, double * dA, hipStream_t s );
     void launch_jacobi_add( int64_t V, double c, const double * X, int64_t ldX, const double * dA, double * Y, int64_t ldY, hipStream_t s );
 
+    // per-state data from vertex coordinates (k_update.hip)
+    void launch_near_far_rows( int n, int64_t T, const int32_t * P_ordering, const int32_t * S, const double * V, int64_t ldV,
+                               const double * charges, double * near, double * far, hipStream_t s );
+    void launch_cluster_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end,
+                                const double * pfar, int singletons, double * C_far, hipStream_t s );
+    void launch_cluster_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_far, hipStream_t s );
+
     // very-near field (k_vf.hip)
     constexpr int VF_REC = 24;                          // doubles per pattern-entry record
     constexpr int64_t VF_LANE_EVALUATION_LIMIT = (int64_t)1 << 24;
