@@ -224,6 +224,39 @@ namespace Repulsor
             else { dev_data = false; }
         }
 
+        // Largest t in [0, max_time] such that moving every vertex by t vecs causes no collision between
+        // simplices without a common vertex (SimplicialMesh::MaximumSafeStepSize, src/SimplicialMesh.hpp:441-468).
+        // vecs: V x 3 (3 x V with transp), host memory, or V x 3 row-major in HBM with device_ptr.
+        double MaximumSafeStepSize( const double * vecs, const double max_time, const double TOL = 0.0625,
+                                    const bool transp = false, const bool device_ptr = false ) const
+        {
+            ScopedTimer timer("SimplicialMesh::MaximumSafeStepSize");
+            rp_ctx * c = Device();
+            std::vector<double> buf;
+            const double * U = vecs;
+            if( transp )
+            {
+                if( device_ptr ) { throw std::runtime_error( "MaximumSafeStepSize: device update vectors must be row-major" ); }
+                const LInt V = VertexCount();
+                buf.resize( (size_t)(3 * V) );
+                for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { buf[(size_t)(3 * v + k)] = vecs[k * V + v]; } }
+                U = buf.data();
+            }
+            double t = 0.0;
+            if( rp_max_safe_step( c, U, 3, max_time, TOL, device_ptr ? RP_DEVICE_PTR : 0, &t, last_collision_stats ) != RP_OK )
+            {
+                throw std::runtime_error( std::string("rp_max_safe_step: ") + rp_last_error() );
+            }
+            return t;
+        }
+
+        // Leaf pairs, simplex pairs tested and hull tests of the last MaximumSafeStepSize.
+        const int64_t * CollisionStats() const { return last_collision_stats; }
+
+    protected:
+        mutable int64_t last_collision_stats[3] = { 0, 0, 0 };
+
+    public:
         // ---------------------------------------------------------------- value cache
         mutable std::map<std::string,double> scalar_cache;
         mutable std::map<std::string,std::vector<double>> vector_cache;

@@ -107,6 +107,21 @@ int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near
  * (src/ClusterTree/ComputeClusterData_Thread.hpp:50-129). */
 int rp_update_coordinates( rp_ctx * ctx, const double * V_coords, int64_t ldV, int singletons, int ptr_flags );
 
+/* Maximal step t in [0, max_time] such that moving the vertices by t U (U: V x 3 update vectors with
+ * leading dimension ldU, host memory or RP_DEVICE_PTR) causes no collision between simplices that share
+ * no vertex, as the reference computes it: per simplex pair, bisection of [0, max_time] until the
+ * interval is shorter than TOL times its end, testing whether the space-time convex hulls of the two
+ * moving simplices over the interval meet (GJK in R^4 with the reference's tolerance); pairs whose
+ * boxes over [0, max_time] do not meet are skipped after a traversal of the cluster tree. Every pair
+ * starts at max_time (the reference starts later pairs at its running minimum), so the result is
+ * independent of the order of evaluation. stats (may be NULL): leaf pairs, simplex pairs tested, hull
+ * tests. Uses the coordinates of the current state (also after rp_update_coordinates).
+ * Replaces: SimplicialMesh::MaximumSafeStepSize (reference src/SimplicialMesh.hpp:441-468),
+ * CollisionTree::MaximumSafeStepSize (src/CollisionTree.hpp), CollisionFinder::FindMaximumSafeStepSize
+ * (src/CollisionFinder.hpp). */
+int rp_max_safe_step( rp_ctx * ctx, const double * U, int64_t ldU, double max_time, double TOL, int ptr_flags,
+                      double * t_out, int64_t * stats );
+
 /* After rp_update_coordinates: the device's near rows (T x ND, tree order), cluster rows (C x 10) and
  * vertex coordinates (V x 3). Any pointer may be NULL. No reference counterpart (the host mesh reads
  * its state back through it when a host computation needs the coordinates). */
@@ -253,7 +268,8 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
 /* Device timing of the last call of each entry point, in milliseconds (HIP events on the
  * context's stream). which: 0 = tp0_compute, 1 = assemble_vertex_gradient, 2 = multiply_metric,
  * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM, 7 = multiply_preconditioner,
- * 8 = h1_solve, 9 = pl_compute, 10 = preconditioner setup (diagonal, weight estimate). */
+ * 8 = h1_solve, 9 = pl_compute, 10 = preconditioner setup (diagonal, weight estimate),
+ * 11 = very-near field of tp0_compute, 12 = pair stage of max_safe_step, 13 = update_coordinates. */
 int rp_last_timing( rp_ctx * ctx, int which, double * ms );
 
 #ifdef __cplusplus

@@ -280,3 +280,28 @@ def simplex_dist2(P, Q):
     """Squared distance of two disjoint simplices (2 or 3 points each, rows = points)."""
     P, Q = c64(P), c64(Q)
     return _sd2(P.shape[0], _p(P), _p(Q))
+
+
+_ssp = _f("orc_safe_step_pair", C.c_double, [C.c_int, _d, _d, _d, _d, C.c_double, C.c_double])
+_mhm = _f("orc_moving_hulls_meet", C.c_int, [C.c_int, _d, _d, _d, _d, C.c_double, C.c_double])
+_mss = _f("orc_max_safe_step", C.c_double, [C.c_int, C.c_int64, C.c_int64, _d, _d, _i32, C.c_double, C.c_double, _d])
+
+
+def safe_step_pair(p, u, q, v, tinit, eps=0.0625):
+    """The reference's bisection for one pair of moving simplices (rows = points)."""
+    p, u, q, v = c64(p), c64(u), c64(q), c64(v)
+    return _ssp(p.shape[0], _p(p), _p(u), _p(q), _p(v), float(tinit), float(eps))
+
+
+def moving_hulls_meet(p, u, q, v, a, b):
+    p, u, q, v = c64(p), c64(u), c64(q), c64(v)
+    return bool(_mhm(p.shape[0], _p(p), _p(u), _p(q), _p(v), float(a), float(b)))
+
+
+def max_safe_step(X, U, S, t_max, eps=0.0625):
+    """MaximumSafeStepSize over all non-adjacent pairs (brute force over pairs with meeting swept boxes).
+    Returns (t, candidate pairs, hull tests)."""
+    X, U, S = c64(X), c64(U), c32(S)
+    st = np.zeros(2)
+    t = _mss(S.shape[1] - 1, X.shape[0], S.shape[0], _p(X), _p(U), _p(S, _i32), float(t_max), float(eps), _p(st))
+    return t, int(st[0]), int(st[1])

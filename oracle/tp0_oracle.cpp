@@ -1229,3 +1229,190 @@ void orc_pl_vf( int n, LInt T, const double * P_near, const LInt * rp, const Int
 }
 
 } // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// MaximumSafeStepSize (the line-search bound after the metric solve):
+//   SimplicialMesh::MaximumSafeStepSize           src/SimplicialMesh.hpp:441-468
+//   CollisionTree::MaximumSafeStepSize           src/CollisionTree.hpp (minimum over the kernels)
+//   MaximumSafeStepSize_Kernel::ComputeLeaf      src/CollisionTree/MaximumSafeStepSize_Kernel.hpp
+//                                                (adjacent primitives are skipped)
+//   CollisionFinder::FindMaximumSafeStepSize     src/CollisionFinder.hpp (bisection of [0, t_init])
+//   MovingPolytope                               src/Primitives/MovingPolytope.hpp: the space-time convex
+//                                                hull of the simplex at times a and b
+//   GJK::IntersectingQ                           src/GJK.hpp:777-797, restated as an exact test: the two
+//                                                hulls meet iff sum l_k P_k = sum m_k Q_k, sum l = sum m = 1,
+//                                                l, m >= 0 is feasible (phase-one simplex, Bland's rule)
+// Every pair starts its bisection at t_init = t_max (the reference starts later pairs at its running
+// minimum, which changes results only within the tolerance).
+namespace
+{
+    // Feasibility of { x >= 0 : A x = b } (R rows, N columns, b >= 0) by phase one of the simplex method.
+    bool LPFeasible( int R, int N, const double * A, const double * b )
+    {
+        const int W = N + R + 1;
+        std::vector<double> t( (size_t)( (R + 1) * W ), 0.0 );
+        std::vector<int> basis( (size_t)R );
+        for( int r = 0; r < R; ++r )
+        {
+            for( int c = 0; c < N; ++c ) { t[(size_t)(r * W + c)] = A[r * N + c]; }
+            t[(size_t)(r * W + N + r)] = 1.0;
+            t[(size_t)(r * W + W - 1)] = b[r];
+            basis[(size_t)r] = N + r;
+        }
+        // objective row: minimize the sum of the artificials -> reduced costs
+        double * obj = &t[(size_t)(R * W)];
+        for( int r = 0; r < R; ++r ) { for( int c = 0; c < W; ++c ) { if( c < N || c == W - 1 ) { obj[c] -= t[(size_t)(r * W + c)]; } } }
+        double scale = 1.0;
+        for( int k = 0; k < R * N; ++k ) { scale = std::max( scale, std::abs( A[k] ) ); }
+        const double eps = 1e-12 * scale;
+        for( int it = 0; it < 500; ++it )
+        {
+            int enter = -1;
+            for( int c = 0; c < N + R; ++c ) { if( obj[c] < -eps ) { enter = c; break; } }   // Bland: first improving column
+            if( enter < 0 ) { break; }
+            int leave = -1; double best = 0;
+            for( int r = 0; r < R; ++r )
+            {
+                const double a = t[(size_t)(r * W + enter)];
+                if( a > eps )
+                {
+                    const double ratio = t[(size_t)(r * W + W - 1)] / a;
+                    if( leave < 0 || ratio < best - 1e-15 || ( ratio <= best + 1e-15 && basis[(size_t)r] < basis[(size_t)leave] ) ) { leave = r; best = ratio; }
+                }
+            }
+            if( leave < 0 ) { break; }
+            const double piv = t[(size_t)(leave * W + enter)];
+            for( int c = 0; c < W; ++c ) { t[(size_t)(leave * W + c)] /= piv; }
+            for( int r = 0; r <= R; ++r )
+            {
+                if( r == leave ) { continue; }
+                const double f = t[(size_t)(r * W + enter)];
+                if( f != 0.0 ) { for( int c = 0; c < W; ++c ) { t[(size_t)(r * W + c)] -= f * t[(size_t)(leave * W + c)]; } }
+            }
+            basis[(size_t)leave] = enter;
+        }
+        return -obj[W - 1] <= 1e-10;
+    }
+
+    // Do the space-time hulls of the two moving simplices (m points each) over [a, b] meet?
+    bool MovingHullsMeet( int m, const double * p, const double * u, const double * q, const double * v, double a, double b, double Tscale )
+    {
+        // columns: l (2m) then mu (2m); rows: sum l = 1, sum mu = 1, 4 coordinates
+        const int N = 4 * m, R = 6;
+        std::vector<double> A( (size_t)(R * N), 0.0 );
+        double rhs[6] = { 1, 1, 0, 0, 0, 0 };
+        for( int i = 0; i < m; ++i )
+        {
+            for( int s = 0; s < 2; ++s )
+            {
+                const double t = s == 0 ? a : b;
+                const int cl = 2 * i + s, cm = 2 * m + 2 * i + s;
+                A[(size_t)(0 * N + cl)] = 1.0;
+                A[(size_t)(1 * N + cm)] = 1.0;
+                for( int k = 0; k < 3; ++k )
+                {
+                    A[(size_t)((2 + k) * N + cl)] =   p[3 * i + k] + t * u[3 * i + k];
+                    A[(size_t)((2 + k) * N + cm)] = -(q[3 * i + k] + t * v[3 * i + k]);
+                }
+                A[(size_t)(5 * N + cl)] =   t * Tscale;
+                A[(size_t)(5 * N + cm)] = -(t * Tscale);
+            }
+        }
+        return LPFeasible( R, N, A.data(), rhs );
+    }
+
+    // CollisionFinder::FindMaximumSafeStepSize with the exact hull test.
+    double SafeStepPair( int m, const double * p, const double * u, const double * q, const double * v, double tinit, double eps, double * tests )
+    {
+        double a = 0.0, b = tinit;
+        // time scale of the space-time coordinates (it does not change whether the hulls meet)
+        double cp[3] = {0,0,0}, cq[3] = {0,0,0}, r2 = 0;
+        for( int i = 0; i < m; ++i ) { for( int k = 0; k < 3; ++k ) { cp[k] += p[3 * i + k] / m; cq[k] += q[3 * i + k] / m; } }
+        for( int k = 0; k < 3; ++k ) { r2 += (cp[k] - cq[k]) * (cp[k] - cq[k]); }
+        const double Tscale = std::sqrt( r2 ) / tinit + 1e-300;
+        const int max_iter = 126;
+        std::vector<double> stack( (size_t)max_iter + 2, 0.0 );
+        int iter = 0, sp = 0;
+        while( ( b - a > eps * b ) && ( iter < max_iter ) )
+        {
+            ++iter;
+            if( tests ) { *tests += 1; }
+            const bool meet = MovingHullsMeet( m, p, u, q, v, a, b, Tscale );
+            if( !meet )
+            {
+                if( sp > 0 ) { a = b; b = stack[(size_t)sp--]; }
+                else { return b; }
+            }
+            else { stack[(size_t)++sp] = b; b = 0.5 * ( a + b ); }
+        }
+        if( sp >= max_iter || iter >= max_iter ) { return a; }
+        return ( a <= 0.0 ) ? b : a;
+    }
+}
+
+extern "C"
+{
+
+// One pair (m = n+1 points each, rows p, u, q, v): the reference's bisection with the exact hull test.
+double orc_safe_step_pair( int m, const double * p, const double * u, const double * q, const double * v, double tinit, double eps )
+{
+    return SafeStepPair( m, p, u, q, v, tinit, eps, nullptr );
+}
+
+// Do the space-time hulls over [a, b] meet?
+int orc_moving_hulls_meet( int m, const double * p, const double * u, const double * q, const double * v, double a, double b )
+{
+    return MovingHullsMeet( m, p, u, q, v, a, b, 1.0 ) ? 1 : 0;
+}
+
+// Maximal safe step size over all pairs of non-adjacent simplices whose swept boxes over [0, t_max]
+// meet (other pairs return t_max). X, U: V x 3 coordinates and update vectors; S: T x (n+1).
+// stats: [0] candidate pairs, [1] hull tests.
+double orc_max_safe_step( int n, LInt V, LInt T, const double * X, const double * U, const Int * S, double t_max, double eps, double * stats )
+{
+    (void)V;
+    const int m = n + 1;
+    std::vector<double> box( (size_t)(6 * T) );
+    for( LInt i = 0; i < T; ++i )
+    {
+        double * bx = &box[(size_t)(6 * i)];
+        for( int k = 0; k < 3; ++k ) { bx[k] = 1e300; bx[3 + k] = -1e300; }
+        for( int l = 0; l < m; ++l )
+        {
+            const LInt w = S[m * i + l];
+            for( int k = 0; k < 3; ++k )
+            {
+                const double x0 = X[3 * w + k], x1 = X[3 * w + k] + t_max * U[3 * w + k];
+                bx[k] = std::min( bx[k], std::min( x0, x1 ) ); bx[3 + k] = std::max( bx[3 + k], std::max( x0, x1 ) );
+            }
+        }
+    }
+    double t = t_max;
+    for( LInt i = 0; i < T; ++i )
+    {
+        for( LInt j = i + 1; j < T; ++j )
+        {
+            bool adjacent = false;
+            for( int a = 0; a < m; ++a ) { for( int c = 0; c < m; ++c ) { adjacent = adjacent || ( S[m * i + a] == S[m * j + c] ); } }
+            if( adjacent ) { continue; }
+            const double * bi = &box[(size_t)(6 * i)], * bj = &box[(size_t)(6 * j)];
+            bool overlap = true;
+            for( int k = 0; k < 3; ++k ) { overlap = overlap && bi[k] <= bj[3 + k] && bj[k] <= bi[3 + k]; }
+            if( !overlap ) { continue; }
+            if( stats ) { stats[0] += 1; }
+            double p[9], u[9], q[9], v[9];
+            for( int l = 0; l < m; ++l )
+            {
+                for( int k = 0; k < 3; ++k )
+                {
+                    p[3 * l + k] = X[3 * (LInt)S[m * i + l] + k]; u[3 * l + k] = U[3 * (LInt)S[m * i + l] + k];
+                    q[3 * l + k] = X[3 * (LInt)S[m * j + l] + k]; v[3 * l + k] = U[3 * (LInt)S[m * j + l] + k];
+                }
+            }
+            t = std::min( t, SafeStepPair( m, p, u, q, v, t_max, eps, stats ? stats + 1 : nullptr ) );
+        }
+    }
+    return t;
+}
+
+} // extern "C"

@@ -96,6 +96,7 @@ _host_sigs = {
                                    C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _d]),
     "rpm_pl_values": (C.c_int, [_vp, _vp, _d, _d, _d, _d]),
     "rpm_mesh_device_state_rows": (C.c_int, [_vp, _d, _d]),
+    "rpm_mesh_max_safe_step": (C.c_int, [_vp, _d, C.c_double, C.c_double, C.c_int, _d, _i64]),
     "rpm_vf_metric_values": (C.c_int, [_vp, _vp, _d, _d]),
     "rpm_pl_vf_values": (C.c_int, [_vp, _vp, _d, _d]),
     "rpm_vf_stats": (C.c_int, [_vp, _i64, C.POINTER(C.c_int)]),
@@ -296,6 +297,16 @@ class SimplicialMesh:
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         assert coords.shape == (self.V, 3)
         _check(H.rpm_mesh_semistatic_update(self._h, _p(coords), 0))
+
+    def MaximumSafeStepSize(self, vecs, max_time, TOL=0.0625, return_stats=False):
+        """Largest t in [0, max_time] such that moving the vertices by t vecs (V x 3) causes no collision
+        between simplices without a common vertex (reference SimplicialMesh::MaximumSafeStepSize)."""
+        self.build()
+        U = np.ascontiguousarray(vecs, dtype=np.float64)
+        assert U.shape == (self.V, 3)
+        t = np.zeros(1); st = np.zeros(3, dtype=np.int64)
+        _check(H.rpm_mesh_max_safe_step(self._h, _p(U), float(max_time), float(TOL), 0, _p(t), _p(st, _i64)))
+        return (float(t[0]), {"leaf_pairs": int(st[0]), "pairs": int(st[1]), "hull_tests": int(st[2])}) if return_stats else float(t[0])
 
     def device_state_rows(self):
         """Near rows (T x ND) and cluster rows (C x 10) computed on the device by the last update."""

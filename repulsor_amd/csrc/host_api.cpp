@@ -202,6 +202,16 @@ int rpm_mesh_semistatic_update( void * h, const double * coords, int device_ptr 
     return call( (MeshHandle *)h, [&]( auto & M ) { M.SemiStaticUpdate( coords, false, device_ptr != 0 ); return RP_OK; } );
 }
 
+int rpm_mesh_max_safe_step( void * h, const double * U, double max_time, double TOL, int device_ptr, double * t, int64_t * stats )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        *t = M.MaximumSafeStepSize( U, max_time, TOL, false, device_ptr != 0 );
+        if( stats ) { for( int k = 0; k < 3; ++k ) { stats[k] = M.CollisionStats()[k]; } }
+        return RP_OK;
+    });
+}
+
 // Device-computed rows of the last device-resident update (tree order), for comparison with the host's.
 int rpm_mesh_device_state_rows( void * h, double * P_near, double * C_far )
 {

@@ -98,7 +98,7 @@ int rp_ctx_create( int device, rp_ctx ** out )
         try
         {
             RP_HIP( hipStreamCreateWithFlags( &c->stream, hipStreamNonBlocking ) );
-            for( int k = 0; k < 12; ++k ) { RP_HIP( hipEventCreate( &c->ev[k][0] ) ); RP_HIP( hipEventCreate( &c->ev[k][1] ) ); }
+            for( int k = 0; k < 16; ++k ) { RP_HIP( hipEventCreate( &c->ev[k][0] ) ); RP_HIP( hipEventCreate( &c->ev[k][1] ) ); }
         }
         catch( ... ) { delete c; throw; }
         *out = c;
@@ -110,7 +110,7 @@ int rp_ctx_destroy( rp_ctx * ctx )
     if( !ctx ) { return RP_OK; }
     (void)hipSetDevice( ctx->device );
     if( ctx->stream ) { (void)hipStreamSynchronize( ctx->stream ); }
-    for( int k = 0; k < 12; ++k ) { (void)hipEventDestroy( ctx->ev[k][0] ); (void)hipEventDestroy( ctx->ev[k][1] ); }
+    for( int k = 0; k < 16; ++k ) { (void)hipEventDestroy( ctx->ev[k][0] ); (void)hipEventDestroy( ctx->ev[k][1] ); }
     if( ctx->stream ) { (void)hipStreamDestroy( ctx->stream ); }
     delete ctx;
     return RP_OK;
@@ -400,6 +400,7 @@ int rp_update_coordinates( rp_ctx * ctx, const double * V_coords, int64_t ldV, i
         RP_HIP( hipSetDevice( ctx->device ) );
         const int64_t T = ctx->T, C = ctx->C, L = ctx->L, V = ctx->V;
         hipStream_t s = ctx->stream;
+        EventScope ev( ctx, 13 );
         if( ptr_flags & RP_DEVICE_PTR )
         {
             launch_copy3( V, V_coords, ldV, ctx->coords.get(), 3, s );
@@ -430,12 +431,96 @@ int rp_update_coordinates( rp_ctx * ctx, const double * V_coords, int64_t ldV, i
         launch_prep_clusters( C, cfar, ctx->c_a.get(), ctx->c_x.get(), ctx->c_P.get(), s );
         ctx->leaf_origin.require( (size_t)(3 * L) );
         launch_leaf_origin( L, C, ctx->leaf_cluster.get(), ctx->c_x.get(), ctx->leaf_origin.get(), s );
+        ev.stop();
         RP_HIP( hipGetLastError() );
         RP_HIP( hipStreamSynchronize( s ) );
+        finish_timing( ctx, 13 );
         ctx->have_data = true;
         ctx->have_state_rows = true;
         ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = false;
         ctx->have_vf_dh = ctx->have_vf_metric = ctx->have_pl_vf = false;
+    });
+}
+
+int rp_max_safe_step( rp_ctx * ctx, const double * U, int64_t ldU, double t_max, double TOL, int ptr_flags,
+                      double * t_out, int64_t * stats )
+{
+    return guarded( "rp_max_safe_step", [&]()
+    {
+        require( ctx != nullptr && ctx->have_tree && ctx->have_data, "mesh, tree and data must be uploaded", RP_ENOTREADY );
+        require( U != nullptr && ldU >= 3 && t_out != nullptr, "U must be V x 3 with ldU >= 3; t_out must not be NULL" );
+        require( std::isfinite( t_max ) && t_max > 0.0, "max_time must be positive" );
+        require( std::isfinite( TOL ) && TOL > 0.0 && TOL < 1.0, "TOL must lie in (0, 1)" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const int64_t T = ctx->T, C = ctx->C, L = ctx->L, V = ctx->V;
+        hipStream_t s = ctx->stream;
+        ctx->col_U.require( (size_t)(3 * V) );
+        if( ptr_flags & RP_DEVICE_PTR ) { launch_copy3( V, U, ldU, ctx->col_U.get(), 3, s ); }
+        else
+        {
+            std::vector<double> h;
+            pack_rows( U, ldU, V, h );
+            for( double x : h ) { require( std::isfinite( x ), "update vectors contain NaN or Inf" ); }
+            ctx->col_U.upload( h.data(), h.size(), s );
+        }
+        ctx->col_pbox.require( (size_t)(6 * T) ); ctx->col_cbox.require( (size_t)(6 * C) );
+        launch_prim_swept_boxes( T, ctx->size, ctx->P_ordering.get(), ctx->simplices.get(), ctx->coords.get(), ctx->col_U.get(), t_max,
+                                 ctx->col_pbox.get(), s );
+        launch_leaf_boxes( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->col_pbox.get(), ctx->col_cbox.get(), s );
+        for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
+        {
+            const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
+            launch_level_boxes( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->col_cbox.get(), s );
+        }
+        // dual-tree traversal, one level per launch; work lists grow (and the traversal restarts) on overflow
+        int64_t cap = std::max<int64_t>( (int64_t)1 << 20, 8 * C ), leaf_cap = std::max<int64_t>( (int64_t)1 << 20, 64 * L );
+        ctx->col_counters.require( 3 );
+        int64_t n_leaves = 0;
+        for( int attempt = 0; ; ++attempt )
+        {
+            require( attempt < 12, "collision traversal work lists exceed 2^31 entries", RP_ENOMEM );
+            ctx->col_list_a.require( (size_t)cap ); ctx->col_list_b.require( (size_t)cap ); ctx->col_leaves.require( (size_t)leaf_cap );
+            int2 * in = ctx->col_list_a.get(), * out = ctx->col_list_b.get();
+            const int2 root = make_int2( 0, 0 );
+            RP_HIP( hipMemcpyAsync( in, &root, sizeof(int2), hipMemcpyHostToDevice, s ) );
+            RP_HIP( hipMemsetAsync( ctx->col_counters.get(), 0, 3 * sizeof(unsigned long long), s ) );
+            int64_t n_in = 1;
+            bool overflow = false;
+            while( n_in > 0 )
+            {
+                RP_HIP( hipMemsetAsync( ctx->col_counters.get(), 0, sizeof(unsigned long long), s ) );
+                launch_traverse_step( n_in, in, out, cap, ctx->col_leaves.get(), leaf_cap, ctx->col_counters.get(),
+                                      ctx->C_left.get(), ctx->C_right.get(), ctx->col_cbox.get(), s );
+                unsigned long long cnt[3];
+                RP_HIP( hipMemcpyAsync( cnt, ctx->col_counters.get(), sizeof(cnt), hipMemcpyDeviceToHost, s ) );
+                RP_HIP( hipStreamSynchronize( s ) );
+                if( cnt[2] ) { overflow = true; break; }
+                n_in = (int64_t)cnt[0];
+                n_leaves = (int64_t)cnt[1];
+                std::swap( in, out );
+            }
+            if( !overflow ) { break; }
+            cap *= 2; leaf_cap *= 2;
+        }
+        // per leaf pair: the reference's bisection for every non-adjacent simplex pair whose boxes meet
+        unsigned long long res[3];
+        double tm = t_max;
+        std::memcpy( &res[0], &tm, sizeof(double) );
+        res[1] = res[2] = 0;
+        RP_HIP( hipMemcpyAsync( ctx->col_counters.get(), res, sizeof(res), hipMemcpyHostToDevice, s ) );
+        EventScope ev( ctx, 12 );
+        launch_leaf_pair_steps( ctx->dom_dim, n_leaves, ctx->col_leaves.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_ordering.get(),
+                                ctx->simplices.get(), ctx->coords.get(), ctx->col_U.get(), ctx->col_pbox.get(), t_max, TOL,
+                                ctx->col_counters.get(), s );
+        ev.stop();
+        RP_HIP( hipGetLastError() );
+        RP_HIP( hipMemcpyAsync( res, ctx->col_counters.get(), sizeof(res), hipMemcpyDeviceToHost, s ) );
+        RP_HIP( hipStreamSynchronize( s ) );
+        finish_timing( ctx, 12 );
+        double t;
+        std::memcpy( &t, &res[0], sizeof(double) );
+        *t_out = t;
+        if( stats ) { stats[0] = n_leaves; stats[1] = (int64_t)res[1]; stats[2] = (int64_t)res[2]; }
     });
 }
 
@@ -756,7 +841,7 @@ int rp_last_timing( rp_ctx * ctx, int which, double * ms )
 {
     return guarded( "rp_last_timing", [&]()
     {
-        require( ctx != nullptr && ms != nullptr && which >= 0 && which < 12, "bad arguments" );
+        require( ctx != nullptr && ms != nullptr && which >= 0 && which < 16, "bad arguments" );
         *ms = ctx->timings.ms[which];
     });
 }

@@ -109,7 +109,7 @@ namespace rp
 
     struct Timings
     {
-        double ms[12] = {0,0,0,0,0,0,0,0,0,0,0,0};
+        double ms[16] = {};
     };
 
 } // namespace rp
@@ -118,7 +118,7 @@ struct rp_ctx
 {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[12][2];
+    hipEvent_t ev[16][2];
 
     // sizes
     int dom_dim = 0, size = 0, nd = 0;
@@ -174,6 +174,11 @@ struct rp_ctx
     rp::DBuf<double> pl_vf_vals, pl_vf_diag;   // nnz, T pseudo-Laplacian values
     bool have_vf_dh = false, have_vf_metric = false, have_pl_vf = false;
     double vf_stats[3] = { 0, 0, 0 };    // evaluations, deepest level, limit hit (last tangent-point compute)
+
+    // maximal safe step size: swept boxes, traversal work lists, leaf pairs, counters
+    rp::DBuf<double> col_pbox, col_cbox, col_U;
+    rp::DBuf<int2> col_list_a, col_list_b, col_leaves;
+    rp::DBuf<unsigned long long> col_counters;
 
     // data (tree order, SoA)
     rp::DBuf<double> p_a, p_x, p_P;      // T, 3T, 6T
