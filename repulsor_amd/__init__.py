@@ -206,7 +206,7 @@ class SimplicialMesh:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and H is not None:          # the module's globals may already be cleared at interpreter exit
             H.rpm_mesh_free(h)
             self._h = None
 
@@ -298,13 +298,17 @@ class SimplicialMesh:
         assert coords.shape == (self.V, 3)
         _check(H.rpm_mesh_semistatic_update(self._h, _p(coords), 0))
 
-    def MaximumSafeStepSize(self, vecs, max_time, TOL=0.0625, return_stats=False):
+    def MaximumSafeStepSize(self, vecs=None, max_time=1.0, TOL=0.0625, return_stats=False, device_ptr=None):
         """Largest t in [0, max_time] such that moving the vertices by t vecs (V x 3) causes no collision
-        between simplices without a common vertex (reference SimplicialMesh::MaximumSafeStepSize)."""
+        between simplices without a common vertex (reference SimplicialMesh::MaximumSafeStepSize).
+        device_ptr: address of V x 3 row-major doubles on the mesh's GPU instead of vecs."""
         self.build()
+        t = np.zeros(1); st = np.zeros(3, dtype=np.int64)
+        if device_ptr is not None:
+            _check(H.rpm_mesh_max_safe_step(self._h, C.cast(C.c_void_p(int(device_ptr) if not isinstance(device_ptr, C.c_void_p) else device_ptr.value), _d), float(max_time), float(TOL), 1, _p(t), _p(st, _i64)))
+            return (float(t[0]), {"leaf_pairs": int(st[0]), "pairs": int(st[1]), "hull_tests": int(st[2])}) if return_stats else float(t[0])
         U = np.ascontiguousarray(vecs, dtype=np.float64)
         assert U.shape == (self.V, 3)
-        t = np.zeros(1); st = np.zeros(3, dtype=np.int64)
         _check(H.rpm_mesh_max_safe_step(self._h, _p(U), float(max_time), float(TOL), 0, _p(t), _p(st, _i64)))
         return (float(t[0]), {"leaf_pairs": int(st[0]), "pairs": int(st[1]), "hull_tests": int(st[2])}) if return_stats else float(t[0])
 
@@ -383,7 +387,7 @@ class _Op:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and H is not None:
             H.rpm_op_free(h)
             self._h = None
 

@@ -32,138 +32,144 @@ namespace rp
             return a[0] <= b[3] && b[0] <= a[3] && a[1] <= b[4] && b[1] <= a[4] && a[2] <= b[5] && b[2] <= a[5];
         }
 
-        // Two moving simplices of M points over [a, b], lifted to R^4 with time scale Ts.
-        template<int M>
-        struct MovingPair
-        {
-            double p[M][3], u[M][3], q[M][3], v[M][3];
-            double a, b, Ts;
-        };
+        __device__ __forceinline__ double dot3( const double * a, const double * b ) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-        // argmin over the Minkowski difference D = hull(P) - hull(Q) of d . x
+        // Moving simplex data, per primitive in cluster order: for each of its M vertices the position
+        // (3 doubles) and then the velocity (3 doubles).
+        //
+        // The space-time hulls of the two simplices over [a, b] (their points at a and at b, lifted by
+        // time) meet exactly when the origin lies in the convex hull of the 3D differences
+        //     D = { p_i(a) - q_j(a) } u { p_i(b) - q_j(b) }:
+        // a common point has equal time coordinates, so both sides put the same weight s on their points
+        // at b, and (1 - s) A_P + s B_P meets (1 - s) A_Q + s B_Q for hulls A, B of the points at a and b.
+        // GJK therefore runs in 3D on D (at most 2 M^2 points, never formed), with the reference's
+        // tolerance (eps^2 times the smaller squared radius of the two space-time hulls) and stopping rules.
+
+        // argmin over D of v . x: returns v . w and writes w
         template<int M>
-        __device__ __forceinline__ void support_min( const MovingPair<M> & S, const double d[4], double w[4] )
+        __device__ __forceinline__ double support3( const double * P, const double * Q, double a, double b, const double v[3], double w[3] )
         {
-            double bestP = 0, bestQ = 0, xP[4] = {0,0,0,0}, yQ[4] = {0,0,0,0};
-            bool first = true;
+            double vp[M], vu[M], vq[M], vv[M];
+            #pragma unroll
+            for( int i = 0; i < M; ++i )
+            {
+                vp[i] = dot3( v, P + 6 * i ); vu[i] = dot3( v, P + 6 * i + 3 );
+                vq[i] = dot3( v, Q + 6 * i ); vv[i] = dot3( v, Q + 6 * i + 3 );
+            }
+            double best = 0.0, bt = a;
+            int bi = 0, bj = 0;
+            #pragma unroll
+            for( int s = 0; s < 2; ++s )
+            {
+                const double t = s == 0 ? a : b;
+                double mp = vp[0] + t * vu[0], mq = vq[0] + t * vv[0];
+                int ip = 0, jq = 0;
+                #pragma unroll
+                for( int i = 1; i < M; ++i )
+                {
+                    const double x = vp[i] + t * vu[i];
+                    if( x < mp ) { mp = x; ip = i; }
+                    const double y = vq[i] + t * vv[i];
+                    if( y > mq ) { mq = y; jq = i; }
+                }
+                const double val = mp - mq;
+                if( s == 0 || val < best ) { best = val; bi = ip; bj = jq; bt = t; }
+            }
+            const double * pb = P + 6 * bi, * qb = Q + 6 * bj;
+            #pragma unroll
+            for( int c = 0; c < 3; ++c ) { w[c] = ( pb[c] + bt * pb[3 + c] ) - ( qb[c] + bt * qb[3 + c] ); }
+            return best;
+        }
+
+        // Affine minimum-norm point of segment {W0, B}, if it lies strictly inside.
+        __device__ __forceinline__ void try_segment( const double W0[3], const double B[3], int id, double & best, int & best_id, double v[3] )
+        {
+            const double e[3] = { B[0] - W0[0], B[1] - W0[1], B[2] - W0[2] };
+            const double ee = dot3( e, e );
+            if( !( ee > 0.0 ) ) { return; }
+            const double t = -dot3( W0, e ) / ee;
+            if( !( t > 0.0 && t < 1.0 ) ) { return; }
+            double x[3];
+            #pragma unroll
+            for( int c = 0; c < 3; ++c ) { x[c] = W0[c] + t * e[c]; }
+            const double n2 = dot3( x, x );
+            if( n2 < best ) { best = n2; best_id = id; v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; }
+        }
+
+        // Affine minimum-norm point of triangle {W0, B, C}, if it lies strictly inside.
+        __device__ __forceinline__ void try_triangle( const double W0[3], const double B[3], const double C[3], int id, double & best, int & best_id,
+                                                      double v[3] )
+        {
+            const double e1[3] = { B[0] - W0[0], B[1] - W0[1], B[2] - W0[2] };
+            const double e2[3] = { C[0] - W0[0], C[1] - W0[1], C[2] - W0[2] };
+            const double g11 = dot3( e1, e1 ), g12 = dot3( e1, e2 ), g22 = dot3( e2, e2 );
+            const double r1 = -dot3( W0, e1 ), r2 = -dot3( W0, e2 );
+            const double det = g11 * g22 - g12 * g12;
+            if( !( det > 1e-13 * g11 * g22 ) ) { return; }
+            const double m1 = ( r1 * g22 - r2 * g12 ) / det, m2 = ( g11 * r2 - g12 * r1 ) / det;
+            if( !( m1 > 0.0 && m2 > 0.0 && 1.0 - m1 - m2 > 0.0 ) ) { return; }
+            double x[3];
+            #pragma unroll
+            for( int c = 0; c < 3; ++c ) { x[c] = W0[c] + m1 * e1[c] + m2 * e2[c]; }
+            const double n2 = dot3( x, x );
+            if( n2 < best ) { best = n2; best_id = id; v[0] = x[0]; v[1] = x[1]; v[2] = x[2]; }
+        }
+
+        // Closest point of conv(W[0..n)) to the origin, over the faces that contain the newest point W[0]
+        // (it belongs to the closest face whenever it improved the support value). Subsets are numbered
+        // by the bit mask of the other slots they use. W is compacted to the chosen face; returns true
+        // when the origin lies inside the tetrahedron.
+        __device__ __forceinline__ bool closest_face( double W[4][3], int & n, double v[3] )
+        {
+            double best = dot3( W[0], W[0] );
+            int id = 0;
+            v[0] = W[0][0]; v[1] = W[0][1]; v[2] = W[0][2];
+            if( n >= 2 ) { try_segment( W[0], W[1], 1, best, id, v ); }
+            if( n >= 3 ) { try_segment( W[0], W[2], 2, best, id, v ); try_triangle( W[0], W[1], W[2], 3, best, id, v ); }
+            if( n >= 4 )
+            {
+                try_segment( W[0], W[3], 4, best, id, v );
+                try_triangle( W[0], W[1], W[3], 5, best, id, v );
+                try_triangle( W[0], W[2], W[3], 6, best, id, v );
+                double e[3][3];
+                #pragma unroll
+                for( int k = 0; k < 3; ++k ) { for( int c = 0; c < 3; ++c ) { e[k][c] = W[k + 1][c] - W[0][c]; } }
+                const double c23[3] = { e[1][1] * e[2][2] - e[1][2] * e[2][1], e[1][2] * e[2][0] - e[1][0] * e[2][2], e[1][0] * e[2][1] - e[1][1] * e[2][0] };
+                const double c31[3] = { e[2][1] * e[0][2] - e[2][2] * e[0][1], e[2][2] * e[0][0] - e[2][0] * e[0][2], e[2][0] * e[0][1] - e[2][1] * e[0][0] };
+                const double c12[3] = { e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2], e[0][0] * e[1][1] - e[0][1] * e[1][0] };
+                const double det = dot3( e[0], c23 );
+                const double scale = sqrt( dot3( e[0], e[0] ) * dot3( e[1], e[1] ) * dot3( e[2], e[2] ) );
+                if( fabs( det ) > 1e-13 * scale )
+                {
+                    const double m1 = -dot3( W[0], c23 ) / det, m2 = -dot3( W[0], c31 ) / det, m3 = -dot3( W[0], c12 ) / det;
+                    if( m1 > 0.0 && m2 > 0.0 && m3 > 0.0 && 1.0 - m1 - m2 - m3 > 0.0 ) { v[0] = v[1] = v[2] = 0.0; return true; }
+                }
+            }
+            switch( id )
+            {
+                case 0: n = 1; break;
+                case 1: n = 2; break;
+                case 2: for( int c = 0; c < 3; ++c ) { W[1][c] = W[2][c]; } n = 2; break;
+                case 3: n = 3; break;
+                case 4: for( int c = 0; c < 3; ++c ) { W[1][c] = W[3][c]; } n = 2; break;
+                case 5: for( int c = 0; c < 3; ++c ) { W[2][c] = W[3][c]; } n = 3; break;
+                default: for( int c = 0; c < 3; ++c ) { W[1][c] = W[2][c]; W[2][c] = W[3][c]; } n = 3; break;
+            }
+            return false;
+        }
+
+        // GJK::IntersectingQ of the two space-time hulls over [a, b] (time lifted by Ts for the radii).
+        template<int M>
+        __device__ bool hulls_meet( const double * P, const double * Q, double a, double b, double Ts )
+        {
+            const double tm = 0.5 * ( a + b );
+            double cP[3] = {0,0,0}, cQ[3] = {0,0,0};
             #pragma unroll
             for( int i = 0; i < M; ++i )
             {
                 #pragma unroll
-                for( int s = 0; s < 2; ++s )
-                {
-                    const double t = s == 0 ? S.a : S.b;
-                    const double x0 = S.p[i][0] + t * S.u[i][0], x1 = S.p[i][1] + t * S.u[i][1], x2 = S.p[i][2] + t * S.u[i][2], x3 = t * S.Ts;
-                    const double vp = d[0] * x0 + d[1] * x1 + d[2] * x2 + d[3] * x3;
-                    const double y0 = S.q[i][0] + t * S.v[i][0], y1 = S.q[i][1] + t * S.v[i][1], y2 = S.q[i][2] + t * S.v[i][2];
-                    const double vq = d[0] * y0 + d[1] * y1 + d[2] * y2 + d[3] * x3;
-                    if( first || vp < bestP ) { bestP = vp; xP[0] = x0; xP[1] = x1; xP[2] = x2; xP[3] = x3; }
-                    if( first || vq > bestQ ) { bestQ = vq; yQ[0] = y0; yQ[1] = y1; yQ[2] = y2; yQ[3] = x3; }
-                    first = false;
-                }
+                for( int c = 0; c < 3; ++c ) { cP[c] += ( P[6 * i + c] + tm * P[6 * i + 3 + c] ) / M; cQ[c] += ( Q[6 * i + c] + tm * Q[6 * i + 3 + c] ) / M; }
             }
-            #pragma unroll
-            for( int k = 0; k < 4; ++k ) { w[k] = xP[k] - yQ[k]; }
-        }
-
-        __device__ __forceinline__ double dot4( const double * a, const double * b ) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
-
-        // Minimum-norm point of conv(W), |W| = n <= 5 points of R^4: over all subsets, the affine
-        // minimum-norm point of the subset if it lies in the subset's relative interior; the smallest of
-        // these is the minimum over the hull. W is compacted to the chosen subset.
-        __device__ bool min_norm_point( double W[5][4], int & n, double v[4] )
-        {
-            double best = 1e300;
-            int best_mask = 0;
-            double bx[4] = {0,0,0,0};
-            const int full = ( 1 << n ) - 1;
-            for( int mask = 1; mask <= full; ++mask )
-            {
-                int idx[5], k = 0;
-                for( int i = 0; i < n; ++i ) { if( mask & (1 << i) ) { idx[k++] = i; } }
-                double lam[5];
-                double x[4];
-                const double * x0 = W[idx[0]];
-                if( k == 1 )
-                {
-                    lam[0] = 1.0;
-                    for( int c = 0; c < 4; ++c ) { x[c] = x0[c]; }
-                }
-                else
-                {
-                    double E[4][4], G[4][5];
-                    const int r = k - 1;
-                    for( int j = 0; j < r; ++j ) { for( int c = 0; c < 4; ++c ) { E[j][c] = W[idx[j + 1]][c] - x0[c]; } }
-                    double scale = 0.0;
-                    for( int i = 0; i < r; ++i )
-                    {
-                        for( int j = 0; j < r; ++j ) { G[i][j] = dot4( E[i], E[j] ); }
-                        G[i][r] = -dot4( E[i], x0 );
-                        scale = fmax( scale, G[i][i] );
-                    }
-                    bool ok = true;
-                    for( int c = 0; c < r && ok; ++c )
-                    {
-                        int piv = c;
-                        for( int i = c + 1; i < r; ++i ) { if( fabs( G[i][c] ) > fabs( G[piv][c] ) ) { piv = i; } }
-                        if( !( fabs( G[piv][c] ) > 1e-13 * scale ) ) { ok = false; break; }
-                        if( piv != c ) { for( int j = 0; j <= r; ++j ) { const double t = G[c][j]; G[c][j] = G[piv][j]; G[piv][j] = t; } }
-                        for( int i = c + 1; i < r; ++i )
-                        {
-                            const double f = G[i][c] / G[c][c];
-                            for( int j = c; j <= r; ++j ) { G[i][j] -= f * G[c][j]; }
-                        }
-                    }
-                    if( !ok ) { continue; }
-                    double mu[4];
-                    for( int i = r - 1; i >= 0; --i )
-                    {
-                        double s = G[i][r];
-                        for( int j = i + 1; j < r; ++j ) { s -= G[i][j] * mu[j]; }
-                        mu[i] = s / G[i][i];
-                    }
-                    double l0 = 1.0;
-                    for( int j = 0; j < r; ++j ) { lam[j + 1] = mu[j]; l0 -= mu[j]; }
-                    lam[0] = l0;
-                    bool inside = true;
-                    for( int j = 0; j < k; ++j ) { inside = inside && lam[j] > 0.0; }
-                    if( !inside ) { continue; }
-                    for( int c = 0; c < 4; ++c )
-                    {
-                        double s = x0[c];
-                        for( int j = 0; j < r; ++j ) { s += mu[j] * E[j][c]; }
-                        x[c] = s;
-                    }
-                }
-                const double n2 = dot4( x, x );
-                if( n2 < best ) { best = n2; best_mask = mask; for( int c = 0; c < 4; ++c ) { bx[c] = x[c]; } }
-            }
-            if( best_mask == 0 ) { return false; }
-            int k = 0;
-            for( int i = 0; i < n; ++i )
-            {
-                if( best_mask & (1 << i) ) { if( k != i ) { for( int c = 0; c < 4; ++c ) { W[k][c] = W[i][c]; } } ++k; }
-            }
-            n = k;
-            for( int c = 0; c < 4; ++c ) { v[c] = bx[c]; }
-            return true;
-        }
-
-        // Do the space-time hulls meet? GJK on D = hull(P) - hull(Q) with the reference's stopping rules:
-        // meeting once |v|^2 <= TOL2 or the simplex encloses the origin, separated once a support value
-        // exceeds the tolerance on the far side or the progress stalls above it.
-        template<int M>
-        __device__ bool hulls_meet( const MovingPair<M> & S )
-        {
-            // interior points and squared radii (tolerance scale)
-            const double tm = 0.5 * ( S.a + S.b );
-            double cP[4] = {0,0,0,0}, cQ[4] = {0,0,0,0};
-            #pragma unroll
-            for( int i = 0; i < M; ++i )
-            {
-                for( int c = 0; c < 3; ++c ) { cP[c] += ( S.p[i][c] + tm * S.u[i][c] ) / M; cQ[c] += ( S.q[i][c] + tm * S.v[i][c] ) / M; }
-            }
-            cP[3] = cQ[3] = tm * S.Ts;
             double rP = 0, rQ = 0;
             #pragma unroll
             for( int i = 0; i < M; ++i )
@@ -171,104 +177,129 @@ namespace rp
                 #pragma unroll
                 for( int s = 0; s < 2; ++s )
                 {
-                    const double t = s == 0 ? S.a : S.b;
-                    double dp = 0, dq = 0;
+                    const double t = s == 0 ? a : b;
+                    const double z = ( t - tm ) * Ts;
+                    double dp = z * z, dq = z * z;
+                    #pragma unroll
                     for( int c = 0; c < 3; ++c )
                     {
-                        const double x = S.p[i][c] + t * S.u[i][c] - cP[c]; dp += x * x;
-                        const double y = S.q[i][c] + t * S.v[i][c] - cQ[c]; dq += y * y;
+                        const double x = P[6 * i + c] + t * P[6 * i + 3 + c] - cP[c]; dp += x * x;
+                        const double y = Q[6 * i + c] + t * Q[6 * i + 3 + c] - cQ[c]; dq += y * y;
                     }
-                    const double z = t * S.Ts - cP[3];
-                    rP = fmax( rP, dp + z * z ); rQ = fmax( rQ, dq + z * z );
+                    rP = fmax( rP, dp ); rQ = fmax( rQ, dq );
                 }
             }
-            const double eps2 = 2.220446049250313e-16;
+            const double eps2 = 2.220446049250313e-16;   // squared tolerance factor: the double epsilon
             double TOL2 = eps2 * fmin( rP, rQ );
             if( !( TOL2 > 0.0 ) ) { TOL2 = eps2 * fmax( rP, rQ ); }
             if( !( TOL2 > 0.0 ) ) { TOL2 = 1e-300; }
-            double v[4];
-            for( int c = 0; c < 4; ++c ) { v[c] = cP[c] - cQ[c]; }
-            double W[5][4];
-            int n = 0;
-            {
-                double w[4];
-                support_min( S, v, w );
-                for( int c = 0; c < 4; ++c ) { W[0][c] = w[c]; v[c] = w[c]; }
-                n = 1;
-            }
-            const double eps = 1.4901161193847656e-08;   // sqrt of the double epsilon
+            const double eps = 1.4901161193847656e-08;    // sqrt of the double epsilon
+            double v[3] = { cP[0] - cQ[0], cP[1] - cQ[1], cP[2] - cQ[2] };
+            double W[4][3];
+            int n = 1;
+            support3<M>( P, Q, a, b, v, W[0] );
+            v[0] = W[0][0]; v[1] = W[0][1]; v[2] = W[0][2];
+            double vv = dot3( v, v );
             for( int iter = 0; iter < 64; ++iter )
             {
-                const double vv = dot4( v, v );
                 if( vv < TOL2 ) { return true; }
-                if( n >= 5 ) { return true; }
-                double w[4];
-                support_min( S, v, w );
-                const double vw = dot4( v, w );
+                double w[3];
+                const double vw = support3<M>( P, Q, a, b, v, w );
                 if( vw > 0.0 && vw * vw > TOL2 ) { return false; }
                 if( fabs( vv - vw ) <= eps * vv ) { return !( TOL2 < vv ); }
                 bool dup = false;
-                for( int i = 0; i < n; ++i )
+                const double ww = dot3( w, w );
+                #pragma unroll
+                for( int k = 0; k < 4; ++k )
                 {
-                    double d2 = 0;
-                    for( int c = 0; c < 4; ++c ) { const double t = W[i][c] - w[c]; d2 += t * t; }
-                    dup = dup || ( d2 <= 1e-28 * ( dot4( w, w ) + 1e-300 ) );
+                    if( k < n )
+                    {
+                        double d2 = 0;
+                        for( int c = 0; c < 3; ++c ) { const double t = W[k][c] - w[c]; d2 += t * t; }
+                        dup = dup || ( d2 <= 1e-28 * ( ww + 1e-300 ) );
+                    }
                 }
                 if( dup ) { return !( TOL2 < vv ); }
-                for( int c = 0; c < 4; ++c ) { W[n][c] = w[c]; }
+                #pragma unroll
+                for( int c = 0; c < 3; ++c ) { W[3][c] = W[2][c]; W[2][c] = W[1][c]; W[1][c] = W[0][c]; W[0][c] = w[c]; }
                 ++n;
-                double vn[4];
-                if( !min_norm_point( W, n, vn ) ) { return !( TOL2 < vv ); }
-                const double vvn = dot4( vn, vn );
-                for( int c = 0; c < 4; ++c ) { v[c] = vn[c]; }
+                double vn[3];
+                if( closest_face( W, n, vn ) ) { return true; }
+                const double vvn = dot3( vn, vn );
+                v[0] = vn[0]; v[1] = vn[1]; v[2] = vn[2];
                 if( fabs( vv - vvn ) <= eps * vvn ) { return !( TOL2 < vvn ); }
+                vv = vvn;
             }
-            return !( TOL2 < dot4( v, v ) );
+            return !( TOL2 < vv );
         }
 
-        // CollisionFinder::FindMaximumSafeStepSize
+        // CollisionFinder::FindMaximumSafeStepSize: depth-first bisection of [0, tinit] for the earliest
+        // small interval whose hulls meet. The reference keeps the right ends of the intervals still to
+        // visit on a stack; here the position is a path of left (0) and right (1) turns, and a pop
+        // replays a <- (a + b)/2 or b <- (a + b)/2 from the root, which reproduces the stacked doubles
+        // exactly (the same operations on the same operands).
         template<int M>
-        __device__ double safe_step_pair( MovingPair<M> & S, double tinit, double eps, int64_t & tests )
+        __device__ double safe_step_pair( const double * P, const double * Q, double tinit, double eps, unsigned & tests )
         {
-            double a = 0.0, b = tinit;
             double cp[3] = {0,0,0}, cq[3] = {0,0,0};
-            for( int i = 0; i < M; ++i ) { for( int k = 0; k < 3; ++k ) { cp[k] += S.p[i][k] / M; cq[k] += S.q[i][k] / M; } }
+            #pragma unroll
+            for( int i = 0; i < M; ++i ) { for( int k = 0; k < 3; ++k ) { cp[k] += P[6 * i + k] / M; cq[k] += Q[6 * i + k] / M; } }
             double r2 = 0;
             for( int k = 0; k < 3; ++k ) { r2 += ( cp[k] - cq[k] ) * ( cp[k] - cq[k] ); }
-            S.Ts = sqrt( r2 ) / tinit + 1e-300;
+            const double Ts = sqrt( r2 ) / tinit + 1e-300;
             constexpr int MAX_ITER = 126;
-            double stack[MAX_ITER + 2];
-            int iter = 0, sp = 0;
+            double a = 0.0, b = tinit;
+            unsigned long long path0 = 0, path1 = 0;     // turn at depth l: bit l of path0 (l < 64) or path1
+            int depth = 0, sp = 0, iter = 0;             // sp: left turns on the path (the reference's stack size)
             while( ( b - a > eps * b ) && ( iter < MAX_ITER ) )
             {
                 ++iter; ++tests;
-                S.a = a; S.b = b;
-                if( !hulls_meet<M>( S ) )
+                if( !hulls_meet<M>( P, Q, a, b, Ts ) )
                 {
-                    if( sp > 0 ) { a = b; b = stack[sp--]; }
-                    else { return b; }
+                    if( sp == 0 ) { return b; }
+                    int l = depth - 1;                   // deepest left turn
+                    while( ( ( l < 64 ? path0 >> l : path1 >> ( l - 64 ) ) & 1ull ) != 0 ) { --l; }
+                    if( l < 64 ) { path0 |= 1ull << l; path0 &= ( l == 63 ) ? ~0ull : ( ( 2ull << l ) - 1 ); path1 = 0; }
+                    else { path1 |= 1ull << ( l - 64 ); path1 &= ( l == 127 ) ? ~0ull : ( ( 2ull << ( l - 64 ) ) - 1 ); }
+                    depth = l + 1; --sp;
+                    double A = 0.0, B = tinit;
+                    for( int m = 0; m < l; ++m )
+                    {
+                        const double mid = 0.5 * ( A + B );
+                        if( ( ( m < 64 ? path0 >> m : path1 >> ( m - 64 ) ) & 1ull ) != 0 ) { A = mid; } else { B = mid; }
+                    }
+                    a = b; b = B;
                 }
-                else { stack[++sp] = b; b = 0.5 * ( a + b ); }
+                else
+                {
+                    ++depth; ++sp;                       // the new turn (bit depth - 1) is left: already 0
+                    b = 0.5 * ( a + b );
+                }
             }
             if( sp >= MAX_ITER || iter >= MAX_ITER ) { return a; }
             return ( a <= 0.0 ) ? b : a;
         }
     }
 
-    __global__ __launch_bounds__(256) void k_prim_swept_boxes( int64_t T, int size, const int32_t * P_ordering, const int32_t * S,
-                                                               const double * X, const double * U, double t_max, double * box )
+    // Per primitive in cluster order: swept box over [0, t_max], vertex ids, moving data (positions, velocities).
+    template<int M>
+    __global__ __launch_bounds__(256) void k_prim_stage( int64_t T, const int32_t * P_ordering, const int32_t * S, const double * X, const double * U,
+                                                         double t_max, double * box, int32_t * SV, double * PU )
     {
         const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
         if( i >= T ) { return; }
         const int64_t t = P_ordering[i];
         double lo[3] = { 1e300, 1e300, 1e300 }, hi[3] = { -1e300, -1e300, -1e300 };
-        for( int l = 0; l < size; ++l )
+        for( int l = 0; l < M; ++l )
         {
-            const int64_t w = S[size * t + l];
+            const int32_t w = S[M * t + l];
+            SV[M * i + l] = w;
             for( int k = 0; k < 3; ++k )
             {
-                const double x0 = X[3 * w + k], x1 = X[3 * w + k] + t_max * U[3 * w + k];
-                lo[k] = fmin( lo[k], fmin( x0, x1 ) ); hi[k] = fmax( hi[k], fmax( x0, x1 ) );
+                const double x = X[3 * (int64_t)w + k], u = U[3 * (int64_t)w + k];
+                const double x1 = x + t_max * u;
+                lo[k] = fmin( lo[k], fmin( x, x1 ) ); hi[k] = fmax( hi[k], fmax( x, x1 ) );
+                PU[( M * i + l ) * 6 + k] = x; PU[( M * i + l ) * 6 + 3 + k] = u;
             }
         }
         for( int k = 0; k < 3; ++k ) { box[6 * i + k] = lo[k]; box[6 * i + 3 + k] = hi[k]; }
@@ -302,115 +333,196 @@ namespace rp
         }
     }
 
-    // counters: [0] pairs written to `out`, [1] leaf pairs written, [2] overflow flag
+    // counters: [0] pairs written to `out`, [1] leaf pairs written, [2] overflow flag. Each wave reserves
+    // room for all of its lanes' children (and leaf pairs) with one atomic per list.
     __global__ __launch_bounds__(256) void k_traverse_step( int64_t n_in, const int2 * in, int2 * out, int64_t out_cap,
                                                             int2 * leaves, int64_t leaf_cap, unsigned long long * counters,
                                                             const int32_t * C_left, const int32_t * C_right, const double * cbox )
     {
         const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-        if( k >= n_in ) { return; }
-        const int2 pr = in[k];
-        const int32_t i = pr.x, j = pr.y;
-        if( !boxes_meet( cbox + 6 * (int64_t)i, cbox + 6 * (int64_t)j ) ) { return; }
-        const int32_t Li = C_left[i], Lj = C_left[j];
+        const int lane = threadIdx.x & 63;
         int2 kids[4];
         int nk = 0;
-        if( Li < 0 && Lj < 0 )
+        bool leaf = false;
+        int2 pr = make_int2( 0, 0 );
+        if( k < n_in )
         {
-            const unsigned long long pos = atomicAdd( counters + 1, 1ull );
-            if( pos < (unsigned long long)leaf_cap ) { leaves[pos] = make_int2( i, j ); } else { atomicMax( counters + 2, 1ull ); }
-            return;
+            pr = in[k];
+            const int32_t i = pr.x, j = pr.y;
+            if( boxes_meet( cbox + 6 * (int64_t)i, cbox + 6 * (int64_t)j ) )
+            {
+                const int32_t Li = C_left[i], Lj = C_left[j];
+                if( Li < 0 && Lj < 0 ) { leaf = true; }
+                else if( i == j )
+                {
+                    const int32_t Ri = C_right[i];
+                    kids[0] = make_int2( Li, Li ); kids[1] = make_int2( Li, Ri ); kids[2] = make_int2( Ri, Ri ); nk = 3;
+                }
+                else if( Li >= 0 && Lj >= 0 )
+                {
+                    const int32_t Ri = C_right[i], Rj = C_right[j];
+                    kids[0] = make_int2( Li, Lj ); kids[1] = make_int2( Li, Rj ); kids[2] = make_int2( Ri, Lj ); kids[3] = make_int2( Ri, Rj ); nk = 4;
+                }
+                else if( Li >= 0 ) { kids[0] = make_int2( Li, j ); kids[1] = make_int2( C_right[i], j ); nk = 2; }
+                else               { kids[0] = make_int2( i, Lj ); kids[1] = make_int2( i, C_right[j] ); nk = 2; }
+            }
         }
-        if( i == j )
+        // inclusive wave scan of the child counts
+        int x = nk;
+        #pragma unroll
+        for( int off = 1; off < 64; off <<= 1 )
         {
-            const int32_t Ri = C_right[i];
-            kids[0] = make_int2( Li, Li ); kids[1] = make_int2( Li, Ri ); kids[2] = make_int2( Ri, Ri ); nk = 3;
+            const int y = __shfl_up( x, off );
+            if( lane >= off ) { x += y; }
         }
-        else if( Li >= 0 && Lj >= 0 )
+        const int total = __shfl( x, 63 );
+        const unsigned long long lmask = __ballot( leaf );
+        unsigned long long kbase = 0, lbase = 0;
+        if( lane == 0 )
         {
-            const int32_t Ri = C_right[i], Rj = C_right[j];
-            kids[0] = make_int2( Li, Lj ); kids[1] = make_int2( Li, Rj ); kids[2] = make_int2( Ri, Lj ); kids[3] = make_int2( Ri, Rj ); nk = 4;
+            if( total ) { kbase = atomicAdd( counters, (unsigned long long)total ); }
+            if( lmask ) { lbase = atomicAdd( counters + 1, (unsigned long long)__popcll( lmask ) ); }
         }
-        else if( Li >= 0 )
+        kbase = __shfl( kbase, 0 ); lbase = __shfl( lbase, 0 );
+        if( leaf )
         {
-            kids[0] = make_int2( Li, j ); kids[1] = make_int2( C_right[i], j ); nk = 2;
+            const unsigned long long pos = lbase + __popcll( lmask & ( ( 1ull << lane ) - 1 ) );
+            if( pos < (unsigned long long)leaf_cap ) { leaves[pos] = pr; } else { atomicMax( counters + 2, 1ull ); }
         }
-        else
+        if( nk )
         {
-            kids[0] = make_int2( i, Lj ); kids[1] = make_int2( i, C_right[j] ); nk = 2;
+            const unsigned long long pos = kbase + (unsigned long long)( x - nk );
+            if( pos + nk <= (unsigned long long)out_cap ) { for( int c = 0; c < nk; ++c ) { out[pos + c] = kids[c]; } }
+            else { atomicMax( counters + 2, 1ull ); }
         }
-        const unsigned long long pos = atomicAdd( counters, (unsigned long long)nk );
-        if( pos + nk <= (unsigned long long)out_cap ) { for( int c = 0; c < nk; ++c ) { out[pos + c] = kids[c]; } }
-        else { atomicMax( counters + 2, 1ull ); }
     }
 
-    // result: [0] minimum step (bits of a non-negative double), [1] candidate pairs, [2] hull tests
-    template<int N>
-    __global__ __launch_bounds__(256) void k_leaf_pair_steps( int64_t n_pairs, const int2 * pairs, const int32_t * C_begin, const int32_t * C_end,
-                                                              const int32_t * P_ordering, const int32_t * S, const double * X, const double * U,
-                                                              const double * pbox, double t_max, double eps, unsigned long long * result )
+    // Candidate simplex pairs of the leaf pairs. A group of CAND_GROUP lanes takes one leaf pair, a lane one
+    // primitive i of the first leaf at a time: if i's swept box meets the second leaf's box (the union of
+    // its primitives' boxes) the lane runs over that leaf's primitives j (j > i within one leaf) and keeps
+    // the pairs whose boxes meet and that share no vertex. The first sweep counts, the workgroup reserves
+    // room with one atomic on `counter` (which counts every candidate, also beyond cap), the second sweep
+    // writes the pairs (cluster-ordered primitive indices) to `cand`. Many small groups per wave keep
+    // enough independent loads in flight.
+    constexpr int CAND_GROUP = 16;
+
+    template<int M, bool COUNT>
+    __device__ __forceinline__ void leaf_pair_rows( int2 pr, int32_t ib, int ni, int32_t jb, int nj, int r, const int32_t * SV, const double * pbox,
+                                                    const double * cbox, unsigned & count, unsigned long long & pos, int2 * cand, int64_t cap )
     {
-        constexpr int M = N + 1;
-        __shared__ double smin[256];
-        __shared__ unsigned long long scand[256], stest[256];
-        const int64_t b = blockIdx.x;
-        double tmin = t_max;
-        int64_t cand = 0, tests = 0;
-        if( b < n_pairs )
+        const bool same = pr.x == pr.y;
+        for( int a = r; a < ni; a += CAND_GROUP )
         {
-            const int2 pr = pairs[b];
-            const int32_t ib = C_begin[pr.x], ie = C_end[pr.x], jb = C_begin[pr.y], je = C_end[pr.y];
-            const int ni = ie - ib, nj = je - jb;
-            const bool same = pr.x == pr.y;
-            for( int k = threadIdx.x; k < ni * nj; k += 256 )
+            const int32_t i = ib + a;
+            double bi[6];
+            #pragma unroll
+            for( int c = 0; c < 6; ++c ) { bi[c] = pbox[6 * (int64_t)i + c]; }
+            if( !same && !boxes_meet( bi, cbox + 6 * (int64_t)pr.y ) ) { continue; }
+            int32_t vi[M];
+            #pragma unroll
+            for( int l = 0; l < M; ++l ) { vi[l] = SV[M * (int64_t)i + l]; }
+            for( int b = same ? a + 1 : 0; b < nj; ++b )
             {
-                const int32_t i = ib + k / nj, j = jb + k % nj;
-                if( same && j <= i ) { continue; }
-                if( !boxes_meet( pbox + 6 * (int64_t)i, pbox + 6 * (int64_t)j ) ) { continue; }
-                const int64_t ti = P_ordering[i], tj = P_ordering[j];
-                int32_t vi[M], vj[M];
-                bool adjacent = false;
-                for( int l = 0; l < M; ++l ) { vi[l] = S[M * ti + l]; vj[l] = S[M * tj + l]; }
-                for( int l = 0; l < M; ++l ) { for( int r = 0; r < M; ++r ) { adjacent = adjacent || vi[l] == vj[r]; } }
-                if( adjacent ) { continue; }
-                ++cand;
-                MovingPair<M> P;
-                for( int l = 0; l < M; ++l )
+                const int32_t j = jb + b;
+                if( !boxes_meet( bi, pbox + 6 * (int64_t)j ) ) { continue; }
+                bool ok = true;
+                #pragma unroll
+                for( int r2 = 0; r2 < M; ++r2 )
                 {
-                    for( int c = 0; c < 3; ++c )
-                    {
-                        P.p[l][c] = X[3 * (int64_t)vi[l] + c]; P.u[l][c] = U[3 * (int64_t)vi[l] + c];
-                        P.q[l][c] = X[3 * (int64_t)vj[l] + c]; P.v[l][c] = U[3 * (int64_t)vj[l] + c];
-                    }
+                    const int32_t y = SV[M * (int64_t)j + r2];
+                    #pragma unroll
+                    for( int l = 0; l < M; ++l ) { ok = ok && vi[l] != y; }
                 }
-                tmin = fmin( tmin, safe_step_pair<M>( P, t_max, eps, tests ) );
+                if( !ok ) { continue; }
+                if( COUNT ) { ++count; }
+                else
+                {
+                    if( pos < (unsigned long long)cap ) { cand[pos] = make_int2( i, j ); }
+                    ++pos;
+                }
             }
         }
-        smin[threadIdx.x] = tmin; scand[threadIdx.x] = (unsigned long long)cand; stest[threadIdx.x] = (unsigned long long)tests;
+    }
+
+    template<int M>
+    __global__ __launch_bounds__(256) void k_leaf_pair_candidates( int64_t n_pairs, const int2 * pairs, const int32_t * C_begin, const int32_t * C_end,
+                                                                   const int32_t * SV, const double * pbox, const double * cbox, int2 * cand,
+                                                                   int64_t cap, unsigned long long * counter )
+    {
+        __shared__ unsigned long long s_count[4];
+        __shared__ unsigned long long s_base;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int64_t g = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / CAND_GROUP;
+        const int r = threadIdx.x % CAND_GROUP;
+        int2 pr = make_int2( 0, 0 );
+        int32_t ib = 0, jb = 0;
+        int ni = 0, nj = 0;
+        if( g < n_pairs )
+        {
+            pr = pairs[g];
+            ib = C_begin[pr.x]; ni = C_end[pr.x] - ib;
+            jb = C_begin[pr.y]; nj = C_end[pr.y] - jb;
+        }
+        unsigned count = 0;
+        unsigned long long pos = 0;
+        leaf_pair_rows<M, true>( pr, ib, ni, jb, nj, r, SV, pbox, cbox, count, pos, cand, cap );
+        // inclusive wave scan of the counts
+        unsigned x = count;
+        #pragma unroll
+        for( int off = 1; off < 64; off <<= 1 )
+        {
+            const unsigned y = __shfl_up( x, off );
+            if( lane >= off ) { x += y; }
+        }
+        if( lane == 63 ) { s_count[wave] = x; }
         __syncthreads();
-        for( int w = 128; w > 0; w >>= 1 )
+        if( threadIdx.x == 0 )
         {
-            if( (int)threadIdx.x < w )
-            {
-                smin[threadIdx.x] = fmin( smin[threadIdx.x], smin[threadIdx.x + w] );
-                scand[threadIdx.x] += scand[threadIdx.x + w];
-                stest[threadIdx.x] += stest[threadIdx.x + w];
-            }
-            __syncthreads();
+            const unsigned long long total = s_count[0] + s_count[1] + s_count[2] + s_count[3];
+            s_base = total ? atomicAdd( counter, total ) : 0ull;
         }
-        if( threadIdx.x == 0 && b < n_pairs )
+        __syncthreads();
+        if( count == 0 ) { return; }
+        pos = s_base + ( x - count );
+        for( int w = 0; w < wave; ++w ) { pos += s_count[w]; }
+        leaf_pair_rows<M, false>( pr, ib, ni, jb, nj, r, SV, pbox, cbox, count, pos, cand, cap );
+    }
+
+    // One lane per candidate pair: the reference's bisection; the minimum by an atomic minimum on the
+    // double's bits (non-negative doubles order like their bit patterns), so the result does not depend
+    // on the order. result: [0] minimum step (bits), [2] hull tests.
+    template<int M>
+    __global__ __launch_bounds__(256) void k_pair_steps( int64_t n, const int2 * cand, const double * PU, double t_max, double eps,
+                                                         unsigned long long * result )
+    {
+        const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        double tmin = t_max;
+        unsigned tests = 0;
+        if( k < n )
         {
-            atomicMin( result, (unsigned long long)__double_as_longlong( smin[0] ) );
-            atomicAdd( result + 1, scand[0] );
-            atomicAdd( result + 2, stest[0] );
+            const int2 c = cand[k];
+            tmin = safe_step_pair<M>( PU + 6 * M * (int64_t)c.x, PU + 6 * M * (int64_t)c.y, t_max, eps, tests );
+        }
+        #pragma unroll
+        for( int off = 32; off > 0; off >>= 1 )
+        {
+            tmin = fmin( tmin, __shfl_xor( tmin, off ) );
+            tests += __shfl_xor( tests, off );
+        }
+        if( ( threadIdx.x & 63 ) == 0 && (int64_t)blockIdx.x * 256 + threadIdx.x < n )
+        {
+            atomicMin( result, (unsigned long long)__double_as_longlong( tmin ) );
+            atomicAdd( result + 2, (unsigned long long)tests );
         }
     }
 
     // ------------------------------------------------------------------ launchers
-    void launch_prim_swept_boxes( int64_t T, int size, const int32_t * P_ordering, const int32_t * S, const double * X, const double * U,
-                                  double t_max, double * box, hipStream_t s )
+    void launch_prim_stage( int n, int64_t T, const int32_t * P_ordering, const int32_t * S, const double * X, const double * U,
+                            double t_max, double * box, int32_t * SV, double * PU, hipStream_t s )
     {
-        if( T > 0 ) { hipLaunchKernelGGL( k_prim_swept_boxes, col_grid( T ), dim3(256), 0, s, T, size, P_ordering, S, X, U, t_max, box ); }
+        if( T <= 0 ) { return; }
+        if( n == 2 ) { hipLaunchKernelGGL( k_prim_stage<3>, col_grid( T ), dim3(256), 0, s, T, P_ordering, S, X, U, t_max, box, SV, PU ); }
+        else         { hipLaunchKernelGGL( k_prim_stage<2>, col_grid( T ), dim3(256), 0, s, T, P_ordering, S, X, U, t_max, box, SV, PU ); }
     }
 
     void launch_leaf_boxes( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * pbox,
@@ -430,13 +542,21 @@ namespace rp
         if( n_in > 0 ) { hipLaunchKernelGGL( k_traverse_step, col_grid( n_in ), dim3(256), 0, s, n_in, in, out, out_cap, leaves, leaf_cap, counters, C_left, C_right, cbox ); }
     }
 
-    void launch_leaf_pair_steps( int n, int64_t n_pairs, const int2 * pairs, const int32_t * C_begin, const int32_t * C_end,
-                                 const int32_t * P_ordering, const int32_t * S, const double * X, const double * U, const double * pbox,
-                                 double t_max, double eps, unsigned long long * result, hipStream_t s )
+    void launch_leaf_pair_candidates( int n, int64_t n_pairs, const int2 * pairs, const int32_t * C_begin, const int32_t * C_end,
+                                      const int32_t * SV, const double * pbox, const double * cbox, int2 * cand, int64_t cap,
+                                      unsigned long long * counter, hipStream_t s )
     {
         if( n_pairs <= 0 ) { return; }
-        const dim3 grid( (unsigned)n_pairs );
-        if( n == 2 ) { hipLaunchKernelGGL( k_leaf_pair_steps<2>, grid, dim3(256), 0, s, n_pairs, pairs, C_begin, C_end, P_ordering, S, X, U, pbox, t_max, eps, result ); }
-        else         { hipLaunchKernelGGL( k_leaf_pair_steps<1>, grid, dim3(256), 0, s, n_pairs, pairs, C_begin, C_end, P_ordering, S, X, U, pbox, t_max, eps, result ); }
+        const dim3 grid( (unsigned)( ( n_pairs * CAND_GROUP + 255 ) / 256 ) );
+        if( n == 2 ) { hipLaunchKernelGGL( k_leaf_pair_candidates<3>, grid, dim3(256), 0, s, n_pairs, pairs, C_begin, C_end, SV, pbox, cbox, cand, cap, counter ); }
+        else         { hipLaunchKernelGGL( k_leaf_pair_candidates<2>, grid, dim3(256), 0, s, n_pairs, pairs, C_begin, C_end, SV, pbox, cbox, cand, cap, counter ); }
+    }
+
+    void launch_pair_steps( int n, int64_t n_cand, const int2 * cand, const double * PU, double t_max, double eps, unsigned long long * result,
+                            hipStream_t s )
+    {
+        if( n_cand <= 0 ) { return; }
+        if( n == 2 ) { hipLaunchKernelGGL( k_pair_steps<3>, col_grid( n_cand ), dim3(256), 0, s, n_cand, cand, PU, t_max, eps, result ); }
+        else         { hipLaunchKernelGGL( k_pair_steps<2>, col_grid( n_cand ), dim3(256), 0, s, n_cand, cand, PU, t_max, eps, result ); }
     }
 }

@@ -464,8 +464,9 @@ int rp_max_safe_step( rp_ctx * ctx, const double * U, int64_t ldU, double t_max,
             ctx->col_U.upload( h.data(), h.size(), s );
         }
         ctx->col_pbox.require( (size_t)(6 * T) ); ctx->col_cbox.require( (size_t)(6 * C) );
-        launch_prim_swept_boxes( T, ctx->size, ctx->P_ordering.get(), ctx->simplices.get(), ctx->coords.get(), ctx->col_U.get(), t_max,
-                                 ctx->col_pbox.get(), s );
+        ctx->col_SV.require( (size_t)( ctx->size * T ) ); ctx->col_PU.require( (size_t)( 6 * ctx->size * T ) );
+        launch_prim_stage( ctx->dom_dim, T, ctx->P_ordering.get(), ctx->simplices.get(), ctx->coords.get(), ctx->col_U.get(), t_max,
+                           ctx->col_pbox.get(), ctx->col_SV.get(), ctx->col_PU.get(), s );
         launch_leaf_boxes( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->col_pbox.get(), ctx->col_cbox.get(), s );
         for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
         {
@@ -474,7 +475,7 @@ int rp_max_safe_step( rp_ctx * ctx, const double * U, int64_t ldU, double t_max,
         }
         // dual-tree traversal, one level per launch; work lists grow (and the traversal restarts) on overflow
         int64_t cap = std::max<int64_t>( (int64_t)1 << 20, 8 * C ), leaf_cap = std::max<int64_t>( (int64_t)1 << 20, 64 * L );
-        ctx->col_counters.require( 3 );
+        ctx->col_counters.require( 4 );
         int64_t n_leaves = 0;
         for( int attempt = 0; ; ++attempt )
         {
@@ -502,16 +503,31 @@ int rp_max_safe_step( rp_ctx * ctx, const double * U, int64_t ldU, double t_max,
             if( !overflow ) { break; }
             cap *= 2; leaf_cap *= 2;
         }
-        // per leaf pair: the reference's bisection for every non-adjacent simplex pair whose boxes meet
+        // candidate simplex pairs (non-adjacent, swept boxes meet), then the reference's bisection for each;
+        // the candidate list grows to the count and is rebuilt when it overflows
+        EventScope ev( ctx, 12 );
+        int64_t n_cand = 0;
+        for( int attempt = 0; attempt < 2; ++attempt )
+        {
+            ctx->col_cand.require( std::max<size_t>( ctx->col_cand.n, (size_t)std::max<int64_t>( (int64_t)1 << 22, 32 * n_leaves ) ) );
+            const int64_t cand_cap = (int64_t)ctx->col_cand.n;
+            RP_HIP( hipMemsetAsync( ctx->col_counters.get() + 3, 0, sizeof(unsigned long long), s ) );
+            launch_leaf_pair_candidates( ctx->dom_dim, n_leaves, ctx->col_leaves.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->col_SV.get(),
+                                         ctx->col_pbox.get(), ctx->col_cbox.get(), ctx->col_cand.get(), cand_cap, ctx->col_counters.get() + 3, s );
+            unsigned long long cnt = 0;
+            RP_HIP( hipMemcpyAsync( &cnt, ctx->col_counters.get() + 3, sizeof(cnt), hipMemcpyDeviceToHost, s ) );
+            RP_HIP( hipStreamSynchronize( s ) );
+            n_cand = (int64_t)cnt;
+            if( n_cand <= cand_cap ) { break; }
+            require( attempt == 0, "collision candidate list overflow", RP_ENOMEM );
+            ctx->col_cand.require( (size_t)n_cand );
+        }
         unsigned long long res[3];
         double tm = t_max;
         std::memcpy( &res[0], &tm, sizeof(double) );
-        res[1] = res[2] = 0;
+        res[1] = (unsigned long long)n_cand; res[2] = 0;
         RP_HIP( hipMemcpyAsync( ctx->col_counters.get(), res, sizeof(res), hipMemcpyHostToDevice, s ) );
-        EventScope ev( ctx, 12 );
-        launch_leaf_pair_steps( ctx->dom_dim, n_leaves, ctx->col_leaves.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_ordering.get(),
-                                ctx->simplices.get(), ctx->coords.get(), ctx->col_U.get(), ctx->col_pbox.get(), t_max, TOL,
-                                ctx->col_counters.get(), s );
+        launch_pair_steps( ctx->dom_dim, n_cand, ctx->col_cand.get(), ctx->col_PU.get(), t_max, TOL, ctx->col_counters.get(), s );
         ev.stop();
         RP_HIP( hipGetLastError() );
         RP_HIP( hipMemcpyAsync( res, ctx->col_counters.get(), sizeof(res), hipMemcpyDeviceToHost, s ) );

@@ -176,8 +176,9 @@ struct rp_ctx
     double vf_stats[3] = { 0, 0, 0 };    // evaluations, deepest level, limit hit (last tangent-point compute)
 
     // maximal safe step size: swept boxes, traversal work lists, leaf pairs, counters
-    rp::DBuf<double> col_pbox, col_cbox, col_U;
-    rp::DBuf<int2> col_list_a, col_list_b, col_leaves;
+    rp::DBuf<double> col_pbox, col_cbox, col_U, col_PU;
+    rp::DBuf<int32_t> col_SV;
+    rp::DBuf<int2> col_list_a, col_list_b, col_leaves, col_cand;
     rp::DBuf<unsigned long long> col_counters;
 
     // data (tree order, SoA)

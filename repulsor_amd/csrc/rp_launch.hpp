@@ -150,16 +150,18 @@ namespace rp
     void launch_cluster_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_far, hipStream_t s );
 
     // maximal safe step size (k_collide.hip)
-    void launch_prim_swept_boxes( int64_t T, int size, const int32_t * P_ordering, const int32_t * S, const double * X, const double * U,
-                                  double t_max, double * box, hipStream_t s );
+    void launch_prim_stage( int n, int64_t T, const int32_t * P_ordering, const int32_t * S, const double * X, const double * U,
+                            double t_max, double * box, int32_t * SV, double * PU, hipStream_t s );
     void launch_leaf_boxes( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * pbox,
                             double * cbox, hipStream_t s );
     void launch_level_boxes( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * cbox, hipStream_t s );
     void launch_traverse_step( int64_t n_in, const int2 * in, int2 * out, int64_t out_cap, int2 * leaves, int64_t leaf_cap,
                                unsigned long long * counters, const int32_t * C_left, const int32_t * C_right, const double * cbox, hipStream_t s );
-    void launch_leaf_pair_steps( int n, int64_t n_pairs, const int2 * pairs, const int32_t * C_begin, const int32_t * C_end,
-                                 const int32_t * P_ordering, const int32_t * S, const double * X, const double * U, const double * pbox,
-                                 double t_max, double eps, unsigned long long * result, hipStream_t s );
+    void launch_leaf_pair_candidates( int n, int64_t n_pairs, const int2 * pairs, const int32_t * C_begin, const int32_t * C_end,
+                                      const int32_t * SV, const double * pbox, const double * cbox, int2 * cand, int64_t cap,
+                                      unsigned long long * counter, hipStream_t s );
+    void launch_pair_steps( int n, int64_t n_cand, const int2 * cand, const double * PU, double t_max, double eps, unsigned long long * result,
+                            hipStream_t s );
 
     // very-near field (k_vf.hip)
     constexpr int VF_REC = 24;                          // doubles per pattern-entry record
