@@ -62,6 +62,7 @@ def _hip():
     lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     lib.hipFree.argtypes = [C.c_void_p]
     lib.hipDeviceSynchronize.argtypes = []
+    lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     return lib
 
 
@@ -317,6 +318,32 @@ def main():
         h1l_ms.append(M.last_timing(8))
     pc_ms, h1_ms = float(np.mean(pcl_ms)), float(np.mean(h1l_ms))
 
+    # the line-search bound that follows the solve in a gradient flow: MaximumSafeStepSize along the
+    # descent direction -X (staged in HBM, device-pointer entry), max_time such that the fastest vertex
+    # moves one mean edge length
+    safe = None
+    if not args.no_solve:
+        Xh = np.zeros((M.V, 3))
+        assert hip.hipMemcpy(Xh.ctypes.data, Xd, M.V * 3 * 8, 2) == 0
+        Dh = np.ascontiguousarray(-Xh)
+        Dd = C.c_void_p()
+        assert hip.hipMalloc(C.byref(Dd), M.V * 3 * 8) == 0
+        assert hip.hipMemcpy(Dd, Dh.ctypes.data, M.V * 3 * 8, 1) == 0
+        Vx = M.VertexCoordinates()
+        Sx = M.simplices
+        h_mean = float(np.mean(np.linalg.norm(Vx[Sx[:, 1]] - Vx[Sx[:, 0]], axis=1)))
+        t_cap = h_mean / float(np.max(np.linalg.norm(Xh, axis=1)))
+        ss_ms = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            t_safe, ss = M.MaximumSafeStepSize(None, t_cap, 0.0625, return_stats=True, device_ptr=Dd.value)
+            ss_ms.append((time.perf_counter() - t0) * 1e3)
+        hip.hipFree(Dd)
+        safe = {"direction": "-(metric solve)", "max_time": t_cap, "max_displacement": "one mean edge length",
+                "t": t_safe, "ms": round(float(np.median(ss_ms)), 3), "pair_stage_ms": round(M.last_timing(12), 3),
+                "leaf_pairs": ss["leaf_pairs"], "simplex_pairs": ss["pairs"], "hull_tests": ss["hull_tests"]}
+        print(f"# MaximumSafeStepSize: {safe}", file=sys.stderr, flush=True)
+
     # comparison forms on the same right-hand side: the two-level preconditioner without the learned
     # scalings, and the plain FP64 form (FP64 energy + differential, reference P0 alone; the P0 solve is
     # capped to about a minute of iterations)
@@ -416,6 +443,7 @@ def main():
         "solve_learned_fp64": learned_cmp,
         "plain_fp64_form": plain,
         "speedup_vs_plain_fp64_form": (round(plain["step_ms"] / ms_per_step, 2) if plain else None),
+        "max_safe_step": safe,
         "tree_bct_build_s": round(t_build, 2),
         "upload_s": round(t_upload, 2),
         "pairs_per_s": round((pairs_nf + pairs_ff) / (float(np.mean(eg_ms)) * 1e-3), 1),
