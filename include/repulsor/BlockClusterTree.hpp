@@ -68,6 +68,17 @@ namespace Repulsor
         LInt masked_pairs = 0;                        // Near pairs counted with multiplicity 2
     };
 
+    // Block cluster tree lists built on the device (rp_build_topology).
+    struct BuiltBlockClusterTree
+    {
+        std::vector<LInt> far_rp;
+        std::vector<Int>  far_ci;
+        std::vector<std::pair<Int,Int>> leaf_blocks, verynear_pairs, inter_pairs;   // sorted, upper
+        std::vector<LInt> col_ptr;
+        std::vector<Int>  col_prim;
+        std::vector<uint32_t> col_mask;
+    };
+
     template<int DOM_DIM>
     class BlockClusterTree
     {
@@ -98,6 +109,58 @@ namespace Repulsor
         ,   threads( std::max( 1, threads_ ) )
         {
             if( S.PrimitiveCount() > 0 ) { ComputeBlocks(); }
+        }
+
+        // Takes over the lists of a device build (same settings); derives the counts and the tile value
+        // offsets as ComputeBlocks and BuildTiles do.
+        BlockClusterTree( const Tree_T & S_, const BlockClusterTreeSettings & settings_, const int threads_, BuiltBlockClusterTree && B )
+        :   S( S_ ), settings( settings_ )
+        ,   far_theta2 ( settings_.far_field_separation_parameter  * settings_.far_field_separation_parameter  )
+        ,   near_theta2( settings_.near_field_separation_parameter * settings_.near_field_separation_parameter )
+        ,   intersection_theta2( near_theta2 )
+        ,   threads( std::max( 1, threads_ ) )
+        {
+            ScopedTimer timer("BlockClusterTree::Adopt");
+            const Int C = S.ClusterCount(), L = S.LeafClusterCount();
+            if( (Int)B.far_rp.size() != C + 1 || (Int)B.col_ptr.size() != L + 1 )
+            {
+                throw std::runtime_error( "BlockClusterTree: inconsistent device-built lists" );
+            }
+            far.rows = C;
+            far.rp = std::move( B.far_rp );
+            far.ci = std::move( B.far_ci );
+            far.diag.resize( (size_t)C );
+            for( Int i = 0; i < C; ++i )
+            {
+                far.diag[(size_t)i] = std::lower_bound( far.ci.begin() + far.rp[(size_t)i], far.ci.begin() + far.rp[(size_t)i + 1], i ) - far.ci.begin();
+            }
+            leaf_blocks = std::move( B.leaf_blocks );
+            verynear_pairs = std::move( B.verynear_pairs );
+            inter_pairs = std::move( B.inter_pairs );
+            LInt pairs = 0;
+            for( auto & b : leaf_blocks )
+            {
+                const LInt ni = S.leaf_cluster_ptr[(size_t)b.first + 1]  - S.leaf_cluster_ptr[(size_t)b.first];
+                const LInt nj = S.leaf_cluster_ptr[(size_t)b.second + 1] - S.leaf_cluster_ptr[(size_t)b.second];
+                pairs += (b.first == b.second) ? (ni * (ni - 1)) / 2 : ni * nj;
+            }
+            near_nnz = 2 * ( pairs - (LInt)verynear_pairs.size() - (LInt)inter_pairs.size() );
+            NearTiles & Tl = tiles;
+            Tl.leaf_count = L;
+            Tl.leaf_begin.resize( (size_t)L );
+            Tl.leaf_end.resize( (size_t)L );
+            for( Int l = 0; l < L; ++l ) { Tl.leaf_begin[(size_t)l] = S.leaf_cluster_ptr[(size_t)l]; Tl.leaf_end[(size_t)l] = S.leaf_cluster_ptr[(size_t)l+1]; }
+            Tl.col_ptr = std::move( B.col_ptr );
+            Tl.col_prim = std::move( B.col_prim );
+            Tl.col_mask = std::move( B.col_mask );
+            Tl.val_ptr.assign( (size_t)L + 1, 0 );
+            Tl.masked_pairs = 0;
+            for( Int l = 0; l < L; ++l )
+            {
+                const LInt m = Tl.col_ptr[(size_t)l + 1] - Tl.col_ptr[(size_t)l];
+                Tl.val_ptr[(size_t)l + 1] = Tl.val_ptr[(size_t)l] + m * (Tl.leaf_end[(size_t)l] - Tl.leaf_begin[(size_t)l]);
+                for( LInt k = Tl.col_ptr[(size_t)l]; k < Tl.col_ptr[(size_t)l + 1]; ++k ) { Tl.masked_pairs += __builtin_popcount( Tl.col_mask[(size_t)k] ); }
+            }
         }
 
         const Tree_T & GetS() const { return S; }

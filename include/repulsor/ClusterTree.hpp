@@ -20,6 +20,14 @@
 
 namespace Repulsor
 {
+    // Tree arrays built on the device (rp_build_topology): depth-first numbering, as ComputeClusters
+    // produces them.
+    struct BuiltClusterTree
+    {
+        std::vector<Int> P_ordering, C_left, C_right, C_begin, C_end, C_depth;
+        std::vector<double> C_serialized;   // C x 7
+    };
+
     template<int DOM_DIM>
     class ClusterTree
     {
@@ -73,7 +81,7 @@ namespace Repulsor
         ClusterTree(
             cptr<double> V_frozen, cptr<Int> S, const LInt T,
             cptr<double> P_near_orig, cptr<double> P_far_orig, cptr<double> DiffOp_vals, cptr<Int> sorted_simplices,
-            const ClusterTreeSettings & settings_, const int threads_ )
+            const ClusterTreeSettings & settings_, const int threads_, const BuiltClusterTree * built = nullptr )
         :   settings( settings_ ), threads( std::max(1, threads_) ), primitive_count( (Int)T )
         {
             ScopedTimer timer("ClusterTree::ClusterTree");
@@ -109,7 +117,7 @@ namespace Repulsor
             P_ordering.resize( (size_t)T );
             for( LInt i = 0; i < T; ++i ) { P_ordering[(size_t)i] = (Int)i; }
 
-            ComputeClusters();
+            if( built ) { AdoptClusters( *built ); } else { ComputeClusters(); }
 
             // Tree-ordered primitive data (ComputePrimitiveData, src/ClusterTree.hpp:319-338).
             P_near.resize( (size_t)T * NEAR_DIM );
@@ -216,7 +224,10 @@ namespace Repulsor
             }
             if( split_index == begin || split_index == end )
             {
-                // Median split fallback with std::nth_element.
+                // Median split fallback. The reference partitions with std::nth_element, which leaves the
+                // order inside the two halves to the standard library; this restatement sorts the range
+                // stably by the interior coordinate, one of the orders nth_element may produce, so that
+                // the tree is defined independently of the library (and the device build reproduces it).
                 split_index = begin + (end - begin) / 2;
                 for( Int i = begin; i < end; ++i )
                 {
@@ -224,7 +235,7 @@ namespace Repulsor
                     perm [(size_t)i] = i;
                 }
                 const double * sc = score.data();
-                std::nth_element( perm.data() + begin, perm.data() + split_index, perm.data() + end,
+                std::stable_sort( perm.data() + begin, perm.data() + end,
                     [sc]( const Int i, const Int j ) { return sc[i] < sc[j]; } );
                 for( Int i = begin; i < end; ++i ) { inv_perm[(size_t)perm[(size_t)i]] = i; }
                 for( Int i = begin; i < end; ++i )
@@ -334,6 +345,60 @@ namespace Repulsor
             }
             P_inverse_ordering.assign( (size_t)primitive_count, 0 );
             for( Int i = 0; i < primitive_count; ++i ) { P_inverse_ordering[(size_t)P_ordering[(size_t)i]] = i; }
+            leaf_cluster_ptr.assign( (size_t)L + 1, 0 );
+            for( Int i = 0; i < L; ++i ) { leaf_cluster_ptr[(size_t)i + 1] = C_end[(size_t)leaf_clusters[(size_t)i]]; }
+        }
+
+        // Takes over a tree built elsewhere (rp_build_topology) and derives the same members as
+        // ComputeClusters: tree-ordered serialized primitives, parents, subtree ends, leaf lists.
+        void AdoptClusters( const BuiltClusterTree & B )
+        {
+            ScopedTimer timer("ClusterTree::AdoptClusters");
+            const Int T = primitive_count;
+            const Int C = (Int)B.C_left.size();
+            if( (Int)B.P_ordering.size() != T || C < 1 || (Int)B.C_serialized.size() != C_SIZE * C )
+            {
+                throw std::runtime_error( "AdoptClusters: inconsistent tree arrays" );
+            }
+            P_ordering = B.P_ordering;
+            {
+                std::vector<double> ser( P_serialized.size() );
+                ParallelFor( T, threads, [&]( LInt i )
+                {
+                    std::memcpy( &ser[(size_t)P_SIZE * i], &P_serialized[(size_t)P_SIZE * P_ordering[(size_t)i]], sizeof(double) * P_SIZE );
+                });
+                P_serialized.swap( ser );
+            }
+            C_left = B.C_left; C_right = B.C_right; C_begin = B.C_begin; C_end = B.C_end; C_depth = B.C_depth;
+            C_serialized = B.C_serialized;
+            C_parent.assign( (size_t)C, -1 );
+            C_next.assign( (size_t)C, 0 );
+            std::vector<Int> desc( (size_t)C, 1 );
+            for( Int c = C; c-- > 0; )
+            {
+                if( C_left[(size_t)c] >= 0 )
+                {
+                    desc[(size_t)c] = 1 + desc[(size_t)C_left[(size_t)c]] + desc[(size_t)C_right[(size_t)c]];
+                    C_parent[(size_t)C_left[(size_t)c]] = c;
+                    C_parent[(size_t)C_right[(size_t)c]] = c;
+                }
+            }
+            depth = 0;
+            leaf_clusters.clear();
+            leaf_cluster_lookup.assign( (size_t)C, -1 );
+            for( Int c = 0; c < C; ++c )
+            {
+                C_next[(size_t)c] = c + desc[(size_t)c];
+                depth = std::max( depth, C_depth[(size_t)c] );
+                if( C_left[(size_t)c] < 0 )
+                {
+                    leaf_cluster_lookup[(size_t)c] = (Int)leaf_clusters.size();
+                    leaf_clusters.push_back( c );
+                }
+            }
+            const Int L = (Int)leaf_clusters.size();
+            P_inverse_ordering.assign( (size_t)T, 0 );
+            for( Int i = 0; i < T; ++i ) { P_inverse_ordering[(size_t)P_ordering[(size_t)i]] = i; }
             leaf_cluster_ptr.assign( (size_t)L + 1, 0 );
             for( Int i = 0; i < L; ++i ) { leaf_cluster_ptr[(size_t)i + 1] = C_end[(size_t)leaf_clusters[(size_t)i]]; }
         }

@@ -24,6 +24,9 @@ namespace Repulsor
         int split_threshold     = 1;
         int thread_count        = 1;
         int parallel_perc_depth = 5;
+        // Build the cluster tree and the block cluster tree on the GPU (rp_build_topology) instead of the
+        // host; same results. No reference counterpart.
+        bool device_build       = false;
     };
 
     struct BlockClusterTreeSettings

@@ -145,6 +145,8 @@ namespace Repulsor
             if( !tree )
             {
                 ScopedTimer timer("SimplicialMesh::GetClusterTree");
+                std::unique_ptr<BuiltClusterTree> built;
+                if( cluster_tree_settings.device_build ) { built = BuildTopologyOnDevice(); }
                 const LInt T = SimplexCount();
                 std::vector<double> P_near( (size_t)(NEAR_DIM * T) ), P_far( (size_t)(FAR_DIM * T) );
                 std::vector<double> diffop( (size_t)(3 * SIZE * T) );
@@ -162,7 +164,7 @@ namespace Repulsor
                 if( s.thread_count <= 0 ) { s.thread_count = threads; }
                 tree = std::make_unique<Tree_T>( V_coords_frozen.data(), simplices.data(), T,
                                                  P_near.data(), P_far.data(), diffop.data(), sorted.data(),
-                                                 s, threads );
+                                                 s, threads, built.get() );
             }
             return *tree;
         }
@@ -177,11 +179,57 @@ namespace Repulsor
                 s.near_field_separation_parameter   = adaptivity_settings.theta;
                 s.near_field_intersection_parameter = adaptivity_settings.intersection_theta;
                 s.max_refinement                    = adaptivity_settings.max_refinement;
-                bct = std::make_unique<BCT_T>( GetClusterTree(), s, threads );
+                const Tree_T & S = GetClusterTree();
+                if( built_bct ) { bct = std::make_unique<BCT_T>( S, s, threads, std::move( *built_bct ) ); built_bct.reset(); }
+                else            { bct = std::make_unique<BCT_T>( S, s, threads ); }
             }
             return *bct;
         }
 
+    protected:
+        mutable std::unique_ptr<BuiltBlockClusterTree> built_bct;   // lists of the device build until the BCT adopts them
+
+        // Cluster tree and block cluster tree on the GPU from the frozen coordinates (rp_build_topology),
+        // with the block cluster tree settings in force now (the near parameters from the adaptivity
+        // settings, as GetBlockClusterTree takes them).
+        std::unique_ptr<BuiltClusterTree> BuildTopologyOnDevice() const
+        {
+            rp_ctx * c = Context();
+            const LInt T = SimplexCount();
+            int64_t cnt[8] = {};
+            const double near_theta = adaptivity_settings.theta;
+            if( rp_build_topology( c, DOM_DIM, VertexCount(), T, V_coords_frozen.data(), simplices.data(), cluster_tree_settings.split_threshold,
+                                   block_cluster_tree_settings.far_field_separation_parameter, near_theta, near_theta, cnt ) != RP_OK )
+            {
+                throw std::runtime_error( std::string("rp_build_topology: ") + rp_last_error() );
+            }
+            const size_t C = (size_t)cnt[0], L = (size_t)cnt[1];
+            auto B = std::make_unique<BuiltClusterTree>();
+            B->P_ordering.resize( (size_t)T );
+            B->C_left.resize( C ); B->C_right.resize( C ); B->C_begin.resize( C ); B->C_end.resize( C ); B->C_depth.resize( C );
+            B->C_serialized.resize( 7 * C );
+            built_bct = std::make_unique<BuiltBlockClusterTree>();
+            BuiltBlockClusterTree & F = *built_bct;
+            F.far_rp.resize( C + 1 ); F.far_ci.resize( (size_t)cnt[3] );
+            std::vector<Int> blk( 2 * (size_t)cnt[4] ), vf( 2 * (size_t)cnt[5] ), inter( 2 * (size_t)cnt[6] );
+            F.col_ptr.resize( L + 1 ); F.col_prim.resize( (size_t)cnt[7] ); F.col_mask.resize( (size_t)cnt[7] );
+            if( rp_download_built_topology( c, B->P_ordering.data(), B->C_left.data(), B->C_right.data(), B->C_begin.data(), B->C_end.data(),
+                                            B->C_depth.data(), B->C_serialized.data(), F.far_rp.data(), F.far_ci.data(), blk.data(), vf.data(),
+                                            inter.data(), F.col_ptr.data(), F.col_prim.data(), F.col_mask.data(), 1 ) != RP_OK )
+            {
+                built_bct.reset();
+                throw std::runtime_error( std::string("rp_download_built_topology: ") + rp_last_error() );
+            }
+            auto to_pairs = []( const std::vector<Int> & a, std::vector<std::pair<Int,Int>> & out )
+            {
+                out.resize( a.size() / 2 );
+                for( size_t k = 0; k < out.size(); ++k ) { out[k] = { a[2 * k], a[2 * k + 1] }; }
+            };
+            to_pairs( blk, F.leaf_blocks ); to_pairs( vf, F.verynear_pairs ); to_pairs( inter, F.inter_pairs );
+            return B;
+        }
+
+    public:
         // Reads new coordinates without rebuilding the tree (src/SimplicialMesh.hpp:354-376).
         // With device_ptr, V_new is a V x 3 row-major array in HBM and the state is computed on the GPU
         // (rp_update_coordinates); the host reads it back only when a host computation needs it
@@ -315,10 +363,16 @@ namespace Repulsor
 
         // ---------------------------------------------------------------- device
         // Uploads topology once and the per-state data after every SemiStaticUpdate.
-        rp_ctx * Device() const
+        // The device context, created on first use (nothing uploaded yet).
+        rp_ctx * Context() const
         {
             if( !dev ) { dev = std::make_unique<DeviceContext>( device ); dev_topology = false; dev_data = false; }
-            rp_ctx * c = dev->get();
+            return dev->get();
+        }
+
+        rp_ctx * Device() const
+        {
+            rp_ctx * c = Context();
             if( !dev_topology )
             {
                 const Tree_T & S = GetClusterTree();

@@ -90,6 +90,37 @@ int rp_upload_far_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp, const
 int rp_upload_verynear_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp, const int32_t * ci,
                                 double theta, int max_refinement );
 
+/* Builds the cluster tree and the block cluster tree on the device from the clustering coordinates
+ * V_coords (V x 3 row-major) and the simplices (T x (n+1)): longest-axis splits down to split_threshold
+ * (1..32) primitives per leaf, depth-first numbering, the far field (AABB MAC with far_theta), the leaf
+ * blocks and, inside them, the very-near and intersecting primitive pairs (primitive MAC with near_theta,
+ * intersection test with intersection_theta). The results stay on the device until
+ * rp_download_built_topology and equal the host construction of include/repulsor/ClusterTree.hpp and
+ * BlockClusterTree.hpp element for element. The median-split fallback sorts the range stably by the
+ * interior coordinate (one of the orders std::nth_element may leave); a fallback on a range of more than
+ * 2048 primitives returns RP_EUNSUPPORTED (build on the host then). counts (8): clusters, leaves, depth,
+ * far nnz (full symmetric), leaf blocks, very-near pairs, intersecting pairs, near-tile columns.
+ * Replaces: ClusterTree construction (reference src/ClusterTree.hpp:115-180, ClusterTree/Split_Thread.hpp:3-166,
+ * ClusterTree/Serialize_Thread.hpp:3-156, BoundingVolumes/AABB_LongestAxisSplit.hpp:134-253) and
+ * BlockClusterTree::ComputeBlocks (src/BlockClusterTree.hpp:234-330, ClusterTreePairTraversor/Traverse_DepthFirst.hpp:3-267,
+ * BlockClusterTree/BlockSplit_Kernel.hpp:164-251). */
+int rp_build_topology( rp_ctx * ctx, int dom_dim, int64_t vertex_count, int64_t simplex_count, const double * V_coords,
+                       const int32_t * simplices, int split_threshold, double far_theta, double near_theta, double intersection_theta,
+                       int64_t * counts );
+
+/* Copies the device-built topology to host arrays (any pointer may be NULL) and, with release != 0,
+ * frees its device memory. Tree: P_ordering (T), C_left, C_right, C_begin, C_end, C_depth (C),
+ * C_serialized (C x 7: r^2, box center, half extents). Block cluster tree: far_rp (C + 1), far_ci
+ * (far nnz; full symmetric, sorted, no diagonal), leaf_blocks (2 x blocks: (I, J), I <= J, sorted),
+ * verynear_pairs and inter_pairs (2 x count: tree-ordered primitive pairs (i, j), i < j, sorted), near
+ * tiles col_ptr (leaves + 1), col_prim and col_mask (tile columns) in the rp_upload_near_tiles layout.
+ * Replaces: the reference's accessors of the built trees (ClusterTree::PrimitiveOrdering, ClusterLeft, ...,
+ * BlockClusterTree::Near/Far/VeryNear, src/ClusterTreeBase.hpp, src/BlockClusterTreeBase.hpp:25-28). */
+int rp_download_built_topology( rp_ctx * ctx, int32_t * P_ordering, int32_t * C_left, int32_t * C_right, int32_t * C_begin,
+                                int32_t * C_end, int32_t * C_depth, double * C_serialized, int64_t * far_rp, int32_t * far_ci,
+                                int32_t * leaf_blocks, int32_t * verynear_pairs, int32_t * inter_pairs, int64_t * col_ptr,
+                                int32_t * col_prim, uint32_t * col_mask, int release );
+
 /* Per-state data, after construction and after every SemiStaticUpdate:
  * V_coords V x 3, P_near T x (1+3(n+1)+6) and C_far C x 10 in tree order.
  * Replaces: ClusterTree::SemiStaticUpdate data path (reference src/ClusterTree.hpp:983-993). */
@@ -269,7 +300,8 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
  * context's stream). which: 0 = tp0_compute, 1 = assemble_vertex_gradient, 2 = multiply_metric,
  * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM, 7 = multiply_preconditioner,
  * 8 = h1_solve, 9 = pl_compute, 10 = preconditioner setup (diagonal, weight estimate),
- * 11 = very-near field of tp0_compute, 12 = pair stage of max_safe_step, 13 = update_coordinates. */
+ * 11 = very-near field of tp0_compute, 12 = pair stage of max_safe_step, 13 = update_coordinates,
+ * 14 = build_topology. */
 int rp_last_timing( rp_ctx * ctx, int which, double * ms );
 
 #ifdef __cplusplus

@@ -66,12 +66,14 @@ _host_sigs = {
     "rpm_gen_trefoil": (C.c_int, [C.c_int64, C.c_int, C.c_double, C.c_uint64, _d, _i32, _i64]),
     "rpm_mesh_new": (_vp, [C.c_int, _d, C.c_int64, _i32, C.c_int64, _d, C.c_int]),
     "rpm_mesh_free": (None, [_vp]),
-    "rpm_mesh_settings": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "rpm_mesh_settings": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
     "rpm_mesh_build": (C.c_int, [_vp, _i64]),
     "rpm_mesh_tree": (C.c_int, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "rpm_mesh_tree_data": (C.c_int, [_vp, _d, _d, _d, _d, _d, _i32, _d]),
     "rpm_mesh_pattern": (C.c_int, [_vp, C.c_int, _i64, _i32, _i64, _i64, C.c_int32]),
     "rpm_mesh_leaf_blocks": (C.c_int, [_vp, _i32, _i32, _i32]),
+    "rpm_mesh_tiles": (C.c_int, [_vp, _i64, _i32, C.POINTER(C.c_uint32), _i64]),
+    "rpm_mesh_build_device_ms": (C.c_int, [_vp, _d]),
     "rpm_mesh_semistatic_update": (C.c_int, [_vp, _d, C.c_int]),
     "rpm_mesh_coords": (C.c_int, [_vp, _d]),
     "rpm_mesh_device": (C.c_int, [_vp]),
@@ -199,7 +201,7 @@ class SimplicialMesh:
         self.simplices = simplices
         self.device = device
         # Library defaults (reference src/Settings.hpp:5-54).
-        self.cluster_tree_settings = _Settings(split_threshold=1)
+        self.cluster_tree_settings = _Settings(split_threshold=1, device_build=False)
         self.block_cluster_tree_settings = _Settings(far_field_separation_parameter=0.5)
         self.adaptivity_settings = _Settings(theta=10.0, max_refinement=30)
         self._built = None
@@ -220,7 +222,7 @@ class SimplicialMesh:
         _check(H.rpm_mesh_settings(self._h, int(self.cluster_tree_settings.split_threshold),
                                    float(self.block_cluster_tree_settings.far_field_separation_parameter),
                                    float(self.adaptivity_settings.theta), int(self.adaptivity_settings.max_refinement),
-                                   int(self.device)))
+                                   int(self.device), int(bool(self.cluster_tree_settings.device_build))))
 
     def build(self):
         """Builds (once) the cluster tree and the block cluster tree; returns the counts."""
@@ -280,6 +282,22 @@ class SimplicialMesh:
         it = np.zeros((b["inter_nnz"] // 2, 2), dtype=np.int32)
         _check(H.rpm_mesh_leaf_blocks(self._h, _p(blocks, _i32), _p(vf, _i32), _p(it, _i32)))
         return blocks, vf, it
+
+    def tiles(self):
+        """Near tiles: col_ptr (leaves + 1), col_prim, col_mask (tile columns), val_ptr (leaves + 1)."""
+        b = self.build()
+        cp = np.zeros(b["L"] + 1, dtype=np.int64)
+        prim = np.zeros(b["tile_cols"], dtype=np.int32)
+        mask = np.zeros(b["tile_cols"], dtype=np.uint32)
+        vp = np.zeros(b["L"] + 1, dtype=np.int64)
+        _check(H.rpm_mesh_tiles(self._h, _p(cp, _i64), _p(prim, _i32), mask.ctypes.data_as(C.POINTER(C.c_uint32)), _p(vp, _i64)))
+        return cp, prim, mask, vp
+
+    def build_device_ms(self):
+        """Device time (ms) of the GPU build of the cluster tree and block cluster tree, or -1."""
+        ms = np.zeros(1)
+        _check(H.rpm_mesh_build_device_ms(self._h, _p(ms)))
+        return float(ms[0])
 
     def VeryNearStats(self):
         """(evaluations, deepest level) of the very-near field of the last tangent-point compute."""

@@ -106,11 +106,12 @@ void * rpm_mesh_new( int dom_dim, const double * coords, int64_t V, const int32_
 
 void rpm_mesh_free( void * h ) { delete (MeshHandle *)h; }
 
-int rpm_mesh_settings( void * h, int split_threshold, double far_theta, double near_theta, int max_refinement, int device )
+int rpm_mesh_settings( void * h, int split_threshold, double far_theta, double near_theta, int max_refinement, int device, int device_build )
 {
     return call( (MeshHandle *)h, [&]( auto & M )
     {
         M.cluster_tree_settings.split_threshold = split_threshold;
+        M.cluster_tree_settings.device_build = ( device_build != 0 );
         M.block_cluster_tree_settings.far_field_separation_parameter = far_theta;
         M.adaptivity_settings.theta = near_theta;
         M.adaptivity_settings.max_refinement = max_refinement;
@@ -193,6 +194,29 @@ int rpm_mesh_leaf_blocks( void * h, int32_t * blocks, int32_t * vf_pairs, int32_
         if( blocks ) { for( size_t k = 0; k < B.leaf_blocks.size(); ++k ) { blocks[2*k] = B.leaf_blocks[k].first; blocks[2*k+1] = B.leaf_blocks[k].second; } }
         if( vf_pairs ) { for( size_t k = 0; k < B.verynear_pairs.size(); ++k ) { vf_pairs[2*k] = B.verynear_pairs[k].first; vf_pairs[2*k+1] = B.verynear_pairs[k].second; } }
         if( inter_pairs ) { for( size_t k = 0; k < B.inter_pairs.size(); ++k ) { inter_pairs[2*k] = B.inter_pairs[k].first; inter_pairs[2*k+1] = B.inter_pairs[k].second; } }
+        return RP_OK;
+    });
+}
+
+// Near tiles: col_ptr (leaves + 1), col_prim and col_mask (tile columns), val_ptr (leaves + 1).
+int rpm_mesh_tiles( void * h, int64_t * col_ptr, int32_t * col_prim, uint32_t * col_mask, int64_t * val_ptr )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        const auto & Tl = M.GetBlockClusterTree().tiles;
+        auto cp = []( const auto & v, auto * o ) { if( o ) { std::copy( v.begin(), v.end(), o ); } };
+        cp( Tl.col_ptr, col_ptr ); cp( Tl.col_prim, col_prim ); cp( Tl.col_mask, col_mask ); cp( Tl.val_ptr, val_ptr );
+        return RP_OK;
+    });
+}
+
+// Device time of the last device build of the tree and block cluster tree (ms), or -1.
+int rpm_mesh_build_device_ms( void * h, double * ms )
+{
+    return call( (MeshHandle *)h, [&]( auto & M )
+    {
+        *ms = -1.0;
+        if( M.DeviceReady() ) { rp_last_timing( M.Context(), 14, ms ); }
         return RP_OK;
     });
 }

@@ -363,6 +363,81 @@ int rp_upload_verynear_pattern( rp_ctx * ctx, int32_t rows, const int64_t * rp_,
     });
 }
 
+int rp_build_topology( rp_ctx * ctx, int dom_dim, int64_t vertex_count, int64_t simplex_count, const double * V_coords,
+                       const int32_t * simplices, int split_threshold, double far_theta, double near_theta, double intersection_theta,
+                       int64_t * counts )
+{
+    return guarded( "rp_build_topology", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require( dom_dim == 1 || dom_dim == 2, "only curves (dom_dim 1) and surfaces (dom_dim 2) in R^3 are supported", RP_EUNSUPPORTED );
+        require( vertex_count > 0 && simplex_count > 0 && V_coords && simplices, "empty mesh" );
+        require( vertex_count < (int64_t)INT32_MAX && 2 * simplex_count * ( dom_dim + 1 ) < (int64_t)INT32_MAX, "mesh too large for 32-bit indices" );
+        require( split_threshold >= 1 && split_threshold <= 32, "split_threshold must lie in 1..32", RP_EUNSUPPORTED );
+        require( std::isfinite( far_theta ) && far_theta >= 0.0 && std::isfinite( near_theta ) && near_theta >= 0.0
+                 && std::isfinite( intersection_theta ) && intersection_theta >= 0.0, "separation parameters must be finite and non-negative" );
+        const int size = dom_dim + 1;
+        for( int64_t r = 0; r < simplex_count * size; ++r )
+        {
+            require( simplices[r] >= 0 && simplices[r] < vertex_count, "simplex vertex index out of range" );
+        }
+        for( int64_t r = 0; r < 3 * vertex_count; ++r ) { require( std::isfinite( V_coords[r] ), "coordinates contain NaN or Inf" ); }
+        RP_HIP( hipSetDevice( ctx->device ) );
+        ctx->built.reset();
+        auto B = std::make_unique<BuiltTopology>();
+        EventScope ev( ctx, 14 );
+        build_topology( *B, dom_dim, vertex_count, simplex_count, V_coords, simplices, split_threshold,
+                        far_theta * far_theta, near_theta * near_theta, intersection_theta * intersection_theta, ctx->stream );
+        ev.stop();
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        finish_timing( ctx, 14 );
+        if( counts )
+        {
+            counts[0] = B->C; counts[1] = B->L; counts[2] = B->depth; counts[3] = B->far_nnz; counts[4] = B->n_blocks;
+            counts[5] = B->n_vf; counts[6] = B->n_inter; counts[7] = B->n_cols;
+        }
+        ctx->built = std::move( B );
+    });
+}
+
+int rp_download_built_topology( rp_ctx * ctx, int32_t * P_ordering, int32_t * C_left, int32_t * C_right, int32_t * C_begin, int32_t * C_end,
+                                int32_t * C_depth, double * C_serialized, int64_t * far_rp, int32_t * far_ci, int32_t * leaf_blocks,
+                                int32_t * verynear_pairs, int32_t * inter_pairs, int64_t * col_ptr, int32_t * col_prim, uint32_t * col_mask,
+                                int release )
+{
+    return guarded( "rp_download_built_topology", [&]()
+    {
+        require( ctx != nullptr && ctx->built, "no device-built topology (call rp_build_topology first)", RP_ENOTREADY );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        BuiltTopology & B = *ctx->built;
+        hipStream_t s = ctx->stream;
+        auto get = [&]( auto * dst, const void * src, int64_t n )
+        {
+            if( dst && n > 0 ) { RP_HIP( hipMemcpyAsync( dst, src, (size_t)n * sizeof(*dst), hipMemcpyDeviceToHost, s ) ); }
+        };
+        get( P_ordering, B.final_ord, B.T );
+        get( C_left, B.C_left.get(), B.C ); get( C_right, B.C_right.get(), B.C ); get( C_begin, B.C_begin.get(), B.C );
+        get( C_end, B.C_end.get(), B.C ); get( C_depth, B.C_depth.get(), B.C ); get( C_serialized, B.C_ser.get(), 7 * B.C );
+        get( far_rp, B.far_rp.get(), B.C + 1 ); get( far_ci, B.far_ci.get(), B.far_nnz );
+        get( col_ptr, B.col_ptr.get(), B.L + 1 ); get( col_prim, B.col_prim.get(), B.n_cols ); get( col_mask, B.col_mask.get(), B.n_cols );
+        // sorted (I << 32 | J) keys to (I, J) pairs
+        std::vector<uint64_t> k;
+        auto pairs = [&]( int32_t * dst, const uint64_t * src, int64_t n )
+        {
+            if( !dst || n <= 0 ) { return; }
+            k.resize( (size_t)n );
+            RP_HIP( hipMemcpyAsync( k.data(), src, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, s ) );
+            RP_HIP( hipStreamSynchronize( s ) );
+            for( int64_t i = 0; i < n; ++i ) { dst[2 * i] = (int32_t)( k[(size_t)i] >> 32 ); dst[2 * i + 1] = (int32_t)( k[(size_t)i] & 0xffffffffu ); }
+        };
+        pairs( leaf_blocks, B.blocks.get(), B.n_blocks );
+        pairs( verynear_pairs, B.vf.get(), B.n_vf );
+        pairs( inter_pairs, B.inter.get(), B.n_inter );
+        RP_HIP( hipStreamSynchronize( s ) );
+        if( release ) { ctx->built.reset(); }
+    });
+}
+
 int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near, const double * C_far )
 {
     return guarded( "rp_upload_data", [&]()

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <string>
 #include <vector>
+#include <memory>
 
 #include "../../include/repulsor_hip.h"
 
@@ -110,6 +111,39 @@ namespace rp
     struct Timings
     {
         double ms[16] = {};
+    };
+
+    // Cluster tree and block cluster tree built on the device (k_build.hip), kept until downloaded.
+    struct BuiltTopology
+    {
+        int dom_dim = 0;
+        int64_t T = 0, C = 0, L = 0;
+        int depth = 0;
+        // input and primitive data
+        DBuf<double>  coords, pser, tser;          // 3V; (4 + 3(n+1)) T in the caller's order; same in tree order
+        DBuf<int32_t> simp, ssimp, pre_cols;       // simplices, sorted simplices (caller's order), sorted simplices (tree order)
+        // tree levels
+        DBuf<int32_t> nb, ne, ndepth, nleft, nright, nnl, nj0, lvl_ind, lvl_incl;
+        DBuf<int8_t>  nflag, ndir;
+        DBuf<double>  nmid, nbox;
+        DBuf<int32_t> ord, ord2, pnode, pnode2, gcnt, fb_list;
+        DBuf<uint8_t> isl;
+        DBuf<unsigned int> fb_ctr;
+        std::vector<int64_t> level_ptr;
+        const int32_t * final_ord = nullptr;       // points into ord or ord2
+        DBuf<int32_t> desc, dleaf, g, lb;
+        // tree (depth-first numbering)
+        DBuf<int32_t> C_left, C_right, C_begin, C_end, C_depth, leaf_lookup, leaf_cluster, leaf_b, leaf_e;
+        DBuf<double>  C_ser;
+        // block cluster tree
+        DBuf<unsigned long long> ctr;
+        DBuf<int2>    front_a, front_b, far_pairs, blk_pairs;
+        DBuf<uint64_t> keys_a, keys_b, far_up, blocks, vf_raw, inter_raw, vf, inter, nbr;
+        int64_t n_far_up = 0, n_blocks = 0, n_vf = 0, n_inter = 0, n_nbr = 0, n_cols = 0, far_nnz = 0;
+        DBuf<int64_t> nbr_rp, col_cnt, col_ptr, far_rp, nsel;
+        DBuf<int32_t> col_prim, far_ci;
+        DBuf<uint32_t> col_mask;
+        DBuf<char>    tmp;                         // hipCUB temporary storage
     };
 
 } // namespace rp
@@ -264,4 +298,7 @@ struct rp_ctx
     std::vector<double> solve_relres;
 
     rp::Timings timings;
+
+    // device-built topology awaiting download (rp_build_topology)
+    std::unique_ptr<rp::BuiltTopology> built;
 };

@@ -222,6 +222,10 @@ namespace rp
     // root-path sums of one value per cluster, per leaf (k_tree.hip)
     void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * src, double * leaf_out, hipStream_t s );
 
+    // cluster tree and block cluster tree from coordinates (k_build.hip); the squared separation parameters
+    void build_topology( BuiltTopology & B, int dom_dim, int64_t V, int64_t T, const double * V_coords, const int32_t * simplices,
+                         int split_threshold, double far_theta2, double near_theta2, double inter_theta2, hipStream_t s );
+
     void launch_axpy1( int64_t n, double alpha, const double * x, double * y, hipStream_t s );
     void launch_scale1( int64_t n, double alpha, double * x, hipStream_t s );
 
