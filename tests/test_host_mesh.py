@@ -179,6 +179,76 @@ def test_cluster_tree_structure(torus_small):
             assert np.all(left < mid) and np.all(right >= mid)
 
 
+def _reference_split_order(hull, centers, leaf):
+    """The reference's tree construction order (AABB_LongestAxisSplit::split,
+    src/BoundingVolumes/AABB_LongestAxisSplit.hpp:134-253): per range, the box of the hull points, the
+    longest axis, then the in-place scan that swaps every primitive whose interior point lies below the
+    box center to the front; if all lie on one side, the range is ordered by that coordinate (stably,
+    the order this build fixes for the nth_element fallback) and split at the middle."""
+    order = np.arange(len(centers))
+    stack = [(0, len(centers))]
+    while stack:
+        b, e = stack.pop()
+        if e - b <= leaf:
+            continue
+        pts = hull[order[b:e]].reshape(-1, 3)
+        lo, hi = pts.min(axis=0), pts.max(axis=0)
+        diff = 0.5 * (hi - lo)
+        mid_all = lo + diff
+        k = 0
+        for a in (1, 2):
+            if diff[a] > diff[k]:
+                k = a
+        mid = mid_all[k]
+        s = b
+        for i in range(b, e):
+            if centers[order[i], k] < mid:
+                order[s], order[i] = order[i], order[s]
+                s += 1
+        if s == b or s == e:
+            s = b + (e - b) // 2
+            seg = order[b:e]
+            order[b:e] = seg[np.argsort(centers[seg, k], kind="stable")]
+        stack.append((s, e))
+        stack.append((b, s))
+    return order
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 16])
+def test_cluster_tree_order_is_the_reference_partition(leaf):
+    """P_ordering equals the order the reference's swap scan produces (restated loop by loop in Python)."""
+    X, S = R.make_torus(30, 12, 1.0, 0.4, 1e-3, 3)
+    M = R.SimplicialMesh(X, S)
+    M.cluster_tree_settings.split_threshold = leaf
+    tr, td = M.tree(), M.tree_data()
+    T = len(S)
+    inv = tr["P_inverse_ordering"]
+    hull = td["P_serialized"][:, 4:].reshape(T, 3, 3)[inv]       # original order
+    centers = td["P_serialized"][:, 1:4][inv]
+    assert np.array_equal(tr["P_ordering"], _reference_split_order(hull, centers, leaf))
+
+
+def test_cluster_tree_median_fallback_order():
+    """A fan of triangles around a far apex: every interior point lies below the box center along the
+    long axis, so every split is the median fallback, and the tree order is the stable order of the
+    interior points' coordinate along that axis (ties keep the input order)."""
+    n = 600
+    rng = np.random.default_rng(5)
+    t = 2 * np.pi * np.arange(n) / n
+    for jitter in (0.0, 1e-3):
+        ring = np.stack([jitter * rng.standard_normal(n), np.cos(t), np.sin(t)], 1)
+        X = np.vstack([ring, [[10.0, 0.0, 0.0]]])
+        S = np.stack([np.full(n, n), np.arange(n), (np.arange(n) + 1) % n], 1).astype(np.int32)
+        M = R.SimplicialMesh(X, S)
+        M.cluster_tree_settings.split_threshold = 16
+        tr, td = M.tree(), M.tree_data()
+        bx = td["P_serialized"][:, 1][tr["P_inverse_ordering"]]
+        assert np.array_equal(tr["P_ordering"], np.argsort(bx, kind="stable"))
+        L, B, E = tr["C_left"], tr["C_begin"], tr["C_end"]
+        for c in np.flatnonzero(L >= 0):
+            assert E[L[c]] == B[c] + (E[c] - B[c]) // 2
+
+
 def test_cluster_data(torus_small):
     X, S, M = torus_small
     tr, td = M.tree(), M.tree_data()
