@@ -254,13 +254,13 @@ struct rp_ctx
     // supernodal Cholesky factor of the H^1 metric
     rp::DBuf<int32_t> h1_sn_begin, h1_rows, h1_kid_ptr, h1_kids, h1_cmap, h1_perm, h1_level_nodes;
     rp::DBuf<int64_t> h1_row_ptr, h1_val_ptr;
-    rp::DBuf<double>  h1_vals, h1_z, h1_xp, h1_aux;
+    rp::DBuf<double>  h1_vals, h1_z, h1_xp;  // h1_vals: inv(L_ss) over M = L_off inv(L_ss) per supernode
     std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
     std::vector<int>     h1_level_wmax;
     std::vector<int>     h1_level_rmax;
-    std::vector<char>    h1_level_split;           // levels run in the split (task) form
-    rp::DBuf<int2>       h1_tasks;                 // forward-2, backward-1, backward-2 tasks of split levels
-    std::vector<int64_t> h1_task_ptr[4];           // per kind (forward updates, backward 1, backward 2, forward diagonal): level offsets into h1_tasks
+    std::vector<char>    h1_level_small;           // levels run a wavefront per supernode
+    rp::DBuf<int2>       h1_tasks;                 // forward, backward and M-forming tasks
+    std::vector<int64_t> h1_task_ptr[3];           // per kind (forward, backward, forming M): level offsets into h1_tasks
     int64_t h1_n = 0, h1_ns = 0;
     bool have_h1 = false;
 

@@ -85,26 +85,23 @@ namespace rp
         const int64_t * row_ptr;    // ns + 1
         const int32_t * rows;       // row sets, sorted, own columns first
         const int64_t * val_ptr;    // ns + 1
-        const double  * vals;       // column-major r x w blocks
+        const double  * vals;       // column-major r x w blocks: inv(L_ss) on top, M = L_off inv(L_ss) below
         const int32_t * kid_ptr;    // ns + 1
         const int32_t * kids;       // children in increasing order
         const int32_t * cmap;       // per row entry: position in the parent's row list
         const int32_t * perm;       // new -> old
         double        * z;          // 3 x rows: front vectors
         double        * xp;         // 3 x n: solution in the new numbering
-        double        * aux;        // 3 x n: merged right-hand sides of the supernode columns (split levels)
     };
-    void launch_chol_fwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big,
-                                double alpha, const double * B, int64_t ldB, hipStream_t s );
-    void launch_chol_bwd_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax, bool big, hipStream_t s );
-    void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
-    // split form for levels with large supernodes; tasks are (supernode, first row/column) pairs
-    constexpr int CHOL_TASK_ROWS = 256;   // threads per workgroup of the split backward tasks
-    constexpr int CHOL_FWD_ROWS = 64;     // rows per split forward task (one per lane)
     constexpr int CHOL_MAX_WIDTH = 512;   // columns per supernode
-    void launch_chol_fwd1_level( const CholDev & F, const int32_t * nodes, int64_t count, int wmax,
-                                 double alpha, const double * B, int64_t ldB, hipStream_t s );
-    void launch_chol_tasks( int kind, const CholDev & F, const int2 * tasks, int64_t count, int wmax, hipStream_t s );
+    constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a wavefront per supernode
+    constexpr int CHOL_FWD_ROWS = 64;     // front rows per forward task (one per lane)
+    void launch_chol_form_m( const CholDev & F, const int2 * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s );
+    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, double alpha, const double * B, int64_t ldB, hipStream_t s );
+    void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, double alpha, const double * B, int64_t ldB, hipStream_t s );
+    void launch_chol_bwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s );
+    void launch_chol_bwd_big( const CholDev & F, const int2 * tasks, int64_t count, hipStream_t s );
+    void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
 
     // Krylov vector kernels (k_solve.hip)
     void launch_dot3( int64_t n, const double * u, const double * v, double * part, double * out, hipStream_t s );   // part: 3 x 1024

@@ -59,45 +59,41 @@ namespace rp
         F.sn_begin = ctx->h1_sn_begin.get(); F.row_ptr = ctx->h1_row_ptr.get(); F.rows = ctx->h1_rows.get();
         F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get();
         F.kid_ptr = ctx->h1_kid_ptr.get(); F.kids = ctx->h1_kids.get(); F.cmap = ctx->h1_cmap.get();
-        F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get(); F.aux = ctx->h1_aux.get();
+        F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get();
         return F;
     }
 
-    // X = alpha H^-1 B + beta X (device vectors, V x 3).
+    // X = alpha H^-1 B + beta X (device vectors, V x 3): one launch per elimination level and sweep.
     static void h1_solve( rp_ctx * ctx, double alpha, const double * Bd, int64_t ldb, double beta, double * Xd, int64_t ldx )
     {
         const CholDev F = chol_dev( ctx );
-        const size_t nl = ctx->h1_level_wmax.size();
+        const size_t nl = ctx->h1_level_small.size();
         const int2 * tasks = ctx->h1_tasks.get();
-        auto task_range = [&]( int kind, size_t l, int64_t & b, int64_t & e ) { b = ctx->h1_task_ptr[kind][l]; e = ctx->h1_task_ptr[kind][l + 1]; };
         for( size_t l = 0; l < nl; ++l )
         {
-            const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-            if( !ctx->h1_level_split[l] )
+            if( ctx->h1_level_small[l] )
             {
-                launch_chol_fwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], false, alpha, Bd, ldb, ctx->stream );
-                continue;
+                const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
+                launch_chol_fwd_small( F, ctx->h1_level_nodes.get() + b, e - b, alpha, Bd, ldb, ctx->stream );
             }
-            launch_chol_fwd1_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], alpha, Bd, ldb, ctx->stream );
-            int64_t tb, te;
-            task_range( 3, l, tb, te );
-            launch_chol_tasks( 3, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
-            task_range( 0, l, tb, te );
-            launch_chol_tasks( 0, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
+            else
+            {
+                const int64_t tb = ctx->h1_task_ptr[0][l], te = ctx->h1_task_ptr[0][l + 1];
+                launch_chol_fwd_big( F, tasks + tb, te - tb, alpha, Bd, ldb, ctx->stream );
+            }
         }
         for( size_t l = nl; l-- > 0; )
         {
-            const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-            if( !ctx->h1_level_split[l] )
+            if( ctx->h1_level_small[l] )
             {
-                launch_chol_bwd_level( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_wmax[l], false, ctx->stream );
-                continue;
+                const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
+                launch_chol_bwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->stream );
             }
-            int64_t tb, te;
-            task_range( 1, l, tb, te );
-            launch_chol_tasks( 1, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
-            task_range( 2, l, tb, te );
-            launch_chol_tasks( 2, F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
+            else
+            {
+                const int64_t tb = ctx->h1_task_ptr[1][l], te = ctx->h1_task_ptr[1][l + 1];
+                launch_chol_bwd_big( F, tasks + tb, te - tb, ctx->stream );
+            }
         }
         launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream );
     }
@@ -686,39 +682,35 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 rmax[(size_t)h] = std::max( rmax[(size_t)h], (int)( row_ptr[s + 1] - row_ptr[s] ) );
             }
         }
-        // Levels holding a supernode with more than 64 rows run in the split form; their task lists:
-        // forward updates and forward diagonal blocks by 64 rows, backward 1 and 2 by as many columns
-        // as groups fit in 256 threads.
-        std::vector<char> split( (size_t)nl, 0 );
+        // Levels whose supernodes all have at most CHOL_SMALL_ROWS rows run a wavefront per supernode;
+        // the others run task lists: forward (supernode, 64 front rows), backward (supernode, as many
+        // columns as groups of G = min(64, r rounded up to a power of two) lanes fit in 256 threads).
+        // A third list, (supernode, 64 update rows) over all supernodes, forms M = L_off inv(L_ss).
+        std::vector<char> small( (size_t)nl, 0 );
         std::vector<int2> tasks;
-        std::vector<int64_t> tptr[4];
-        for( int kind = 0; kind < 4; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
+        std::vector<int64_t> tptr[3];
+        for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
         auto group_of = []( int64_t len ) { int g = 1; while( g < len && g < 64 ) { g <<= 1; } return g; };
         {
-            std::vector<int2> tk[4];
+            std::vector<int2> tk[3];
             for( int32_t l = 0; l < nl; ++l )
             {
-                split[(size_t)l] = rmax[(size_t)l] > 64;
+                small[(size_t)l] = rmax[(size_t)l] <= CHOL_SMALL_ROWS;
                 for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k )
                 {
-                    if( !split[(size_t)l] ) { break; }
                     const int32_t s = lnodes[(size_t)k];
                     const int32_t w = sn_begin[s + 1] - sn_begin[s];
                     const int32_t r = (int32_t)( row_ptr[s + 1] - row_ptr[s] );
-                    for( int32_t q0 = w; q0 < r; q0 += CHOL_FWD_ROWS ) { tk[0].push_back( make_int2( s, q0 ) ); }
-                    for( int32_t a0 = 0; a0 < w; a0 += CHOL_FWD_ROWS ) { tk[3].push_back( make_int2( s, a0 ) ); }
-                    if( r > w )
-                    {
-                        const int per = CHOL_TASK_ROWS / group_of( r - w );
-                        for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( make_int2( s, c0 ) ); }
-                    }
-                    const int per2 = CHOL_TASK_ROWS / group_of( w );
-                    for( int32_t c0 = 0; c0 < w; c0 += per2 ) { tk[2].push_back( make_int2( s, c0 ) ); }
+                    for( int32_t q0 = 0; q0 < std::max( r - w, 1 ); q0 += 64 ) { tk[2].push_back( make_int2( s, q0 ) ); }
+                    if( small[(size_t)l] ) { continue; }
+                    for( int32_t a0 = 0; a0 < r; a0 += CHOL_FWD_ROWS ) { tk[0].push_back( make_int2( s, a0 ) ); }
+                    const int per = 256 / group_of( r );
+                    for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( make_int2( s, c0 ) ); }
                 }
-                for( int kind = 0; kind < 4; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
+                for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
             }
             int64_t off = 0;
-            for( int kind = 0; kind < 4; ++kind )
+            for( int kind = 0; kind < 3; ++kind )
             {
                 for( auto & x : tptr[kind] ) { x += off; }
                 tasks.insert( tasks.end(), tk[kind].begin(), tk[kind].end() );
@@ -730,7 +722,18 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_row_ptr.upload( row_ptr, (size_t)ns + 1, ctx->stream );
         ctx->h1_rows.upload( rows, (size_t)nrows, ctx->stream );
         ctx->h1_val_ptr.upload( val_ptr, (size_t)ns + 1, ctx->stream );
-        ctx->h1_vals.upload( vals, (size_t)val_ptr[ns], ctx->stream );
+        {
+            // the uploaded blocks hold L_off below inv(L_ss); the device keeps M = L_off inv(L_ss) there
+            rp::DBuf<double> lin;
+            lin.upload( vals, (size_t)val_ptr[ns], ctx->stream );
+            ctx->h1_vals.require( (size_t)val_ptr[ns] );
+            CholDev F{};
+            F.sn_begin = ctx->h1_sn_begin.get(); F.row_ptr = ctx->h1_row_ptr.get(); F.val_ptr = ctx->h1_val_ptr.get();
+            const int64_t mb = tptr[2][0], me = tptr[2][(size_t)nl];
+            launch_chol_form_m( F, ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
+            RP_HIP( hipGetLastError() );
+            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        }
         ctx->h1_kid_ptr.upload( kid_ptr.data(), kid_ptr.size(), ctx->stream );
         ctx->h1_kids.upload( kids.data(), kids.size(), ctx->stream );
         ctx->h1_cmap.upload( cmap.data(), cmap.size(), ctx->stream );
@@ -738,13 +741,12 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_level_nodes.upload( lnodes.data(), lnodes.size(), ctx->stream );
         ctx->h1_z.require( (size_t)(3 * nrows) );
         ctx->h1_xp.require( (size_t)(3 * (int64_t)n) );
-        ctx->h1_aux.require( (size_t)(3 * (int64_t)n) );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->h1_level_ptr = lptr;
         ctx->h1_level_wmax = wmax;
         ctx->h1_level_rmax = rmax;
-        ctx->h1_level_split = split;
-        for( int kind = 0; kind < 4; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
+        ctx->h1_level_small = small;
+        for( int kind = 0; kind < 3; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
         ctx->h1_n = n;
         ctx->h1_ns = ns;
         ctx->have_h1 = true;
