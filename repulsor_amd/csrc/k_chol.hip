@@ -9,17 +9,17 @@
 // uploaded L_off. With it both sweeps need one pass per elimination level:
 //
 //   forward (leaves to root):  z_s starts as [b_s; 0] plus the children's update vectors (in child
-//       order, through the index map cmap of each child update row into the parent's row list), then
+//       order; each front row pulls the child update rows that map to it, from lists built at upload), then
 //           y_s = inv(L_ss) z_s[0:w],   z_s[w:r] -= M z_s[0:w]      (= L_off y_s)
 //       z_s[0:w] keeps y_s for the backward sweep;
 //   backward (root to leaves): x_s = inv(L_ss)^T y_s - M^T x(R_s[w:]), i.e. for column c the dot product
 //       of the contiguous block column c (rows c..r-1) with v = [y_s; -x(R_s[w:])].
 //
 // Levels whose supernodes have at most 64 rows run one wavefront per supernode (a lane per front row
-// forward, a lane per column backward); larger levels run workgroups over tasks (supernode, block of 64
-// front rows) forward, the 4 waves splitting the columns, and (supernode, group of columns) backward,
-// a group of lanes per column. Children are merged one after another in the same order by every task
-// that needs them, so all sums are deterministic and no atomics are used.
+// forward, a lane per row of v and a wavefront sum per column backward); larger levels run workgroups
+// over tasks (supernode, block of 64 front rows) forward, 4 or 16 waves splitting the columns, and
+// (supernode, group of columns) backward, a group of lanes per column. Every sum has a fixed order, so
+// the solve is deterministic and uses no atomics.
 
 #include "rp_launch.hpp"
 
@@ -78,8 +78,33 @@ namespace rp
     }
 
     // ---------------------------------------------------------------------------------------------
+    // The merged front value of row a of supernode s: the supernode's own right-hand side (rows a < w,
+    // already multiplied by alpha and permuted) or 0, plus the children's update rows that map to it, in
+    // child order (pull lists built at upload, the same additions in the same order as pushing the
+    // children one after another).
+    __device__ __forceinline__ void chol_front_row( const CholDev & F, int64_t rb, int f, int w, int a, double v[3] )
+    {
+        if( a < w ) { for( int k = 0; k < 3; ++k ) { v[k] = F.bp[3 * (int64_t)( f + a ) + k]; } }
+        else        { for( int k = 0; k < 3; ++k ) { v[k] = 0.0; } }
+        const int32_t e1 = F.pull_ptr[rb + a + 1];
+        for( int32_t e = F.pull_ptr[rb + a]; e < e1; ++e )
+        {
+            const int64_t q = F.pull_src[e];
+            for( int k = 0; k < 3; ++k ) { v[k] += F.z[3 * q + k]; }
+        }
+    }
+
+    // Bp = alpha B(perm) in the factor's numbering.
+    __global__ __launch_bounds__(256) void k_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp )
+    {
+        const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        if( j >= n ) { return; }
+        const int64_t o = perm[j];
+        for( int k = 0; k < 3; ++k ) { bp[3 * j + k] = alpha * B[ldB * o + k]; }
+    }
+
     // Forward, levels with r <= 64: one wavefront per supernode, a lane per front row.
-    __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const int32_t * nodes, int64_t count, double alpha, const double * B, int64_t ldB )
+    __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const int32_t * nodes, int64_t count )
     {
         __shared__ double fr_all[4][3 * 64];
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -91,56 +116,36 @@ namespace rp
         const int w = F.sn_begin[s + 1] - f;
         const int64_t rb = F.row_ptr[s];
         const int r = (int)( F.row_ptr[s + 1] - rb );
-        if( lane < r )
-        {
-            if( lane < w )
-            {
-                const int64_t o = F.perm[f + lane];
-                for( int k = 0; k < 3; ++k ) { fr[3 * lane + k] = alpha * B[ldB * o + k]; }
-            }
-            else { for( int k = 0; k < 3; ++k ) { fr[3 * lane + k] = 0.0; } }
-        }
-        // children in order; within one child the target rows are distinct; the wavefront's LDS
-        // accesses execute in program order
-        for( int kk = F.kid_ptr[s]; kk < F.kid_ptr[s + 1]; ++kk )
-        {
-            const int c = F.kids[kk];
-            const int wc = F.sn_begin[c + 1] - F.sn_begin[c];
-            const int64_t rbc = F.row_ptr[c];
-            const int rc = (int)( F.row_ptr[c + 1] - rbc );
-            const double * zc = F.z + 3 * rbc;
-            for( int q = wc + lane; q < rc; q += 64 )
-            {
-                const int pos = F.cmap[rbc + q];
-                for( int k = 0; k < 3; ++k ) { fr[3 * pos + k] += zc[3 * q + k]; }
-            }
-            __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-            __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-        }
+        double v[3] = { 0.0, 0.0, 0.0 };
+        if( lane < r ) { chol_front_row( F, rb, f, w, lane, v ); }
+        for( int k = 0; k < 3; ++k ) { fr[3 * lane + k] = v[k]; }
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
         if( lane >= r ) { return; }
         const double * Ls = F.vals + F.val_ptr[s];
         const int a = lane;
-        double acc[3] = { 0.0, 0.0, 0.0 };
         const int ce = ( a < w ) ? a + 1 : w;
-        for( int c = 0; c < ce; ++c )
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        #pragma unroll 8
+        for( int c = 0; c < w; ++c )
         {
-            const double l = Ls[(int64_t)c * r + a];
+            const double l = ( c < ce ) ? Ls[(int64_t)c * r + a] : 0.0;
             for( int k = 0; k < 3; ++k ) { acc[k] += l * fr[3 * c + k]; }
         }
         double * z = F.z + 3 * rb;
         if( a < w ) { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = acc[k]; } }
-        else        { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = fr[3 * a + k] - acc[k]; } }
+        else        { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = v[k] - acc[k]; } }
     }
 
-    // Forward, larger levels: one workgroup per task (supernode, 64 front rows); the 4 waves split the
-    // columns and add their partial sums in LDS.
-    __global__ __launch_bounds__(256) void k_chol_fwd_big( CholDev F, const int2 * tasks, double alpha, const double * B, int64_t ldB )
+    // Forward, larger levels: one workgroup of NW waves per task (supernode, 64 front rows); the waves
+    // split the columns and add their partial sums in LDS.
+    template<int NW>
+    __global__ __launch_bounds__(64 * NW) void k_chol_fwd_big( CholDev F, const int2 * tasks )
     {
+        constexpr int NT = 64 * NW;
         __shared__ double zt[3 * CHOL_MAX_WIDTH];
-        __shared__ double zb[3 * 64];
-        __shared__ double part[4 * 64 * 3];
+        __shared__ double part[NW * 64 * 3];
         const int2 tk = tasks[blockIdx.x];
         const int s = tk.x, row0 = tk.y;
         const int f = F.sn_begin[s];
@@ -148,38 +153,25 @@ namespace rp
         const int64_t rb = F.row_ptr[s];
         const int r = (int)( F.row_ptr[s + 1] - rb );
         const int tid = threadIdx.x;
-        for( int a = tid; a < w; a += 256 )
+        for( int a = tid; a < w; a += NT )
         {
-            const int64_t o = F.perm[f + a];
-            for( int k = 0; k < 3; ++k ) { zt[3 * a + k] = alpha * B[ldB * o + k]; }
+            double v[3];
+            chol_front_row( F, rb, f, w, a, v );
+            for( int k = 0; k < 3; ++k ) { zt[3 * a + k] = v[k]; }
         }
-        if( tid < 3 * 64 ) { zb[tid] = 0.0; }
-        __syncthreads();
-        for( int kk = F.kid_ptr[s]; kk < F.kid_ptr[s + 1]; ++kk )
-        {
-            const int c = F.kids[kk];
-            const int wc = F.sn_begin[c + 1] - F.sn_begin[c];
-            const int64_t rbc = F.row_ptr[c];
-            const int rc = (int)( F.row_ptr[c + 1] - rbc );
-            const double * zc = F.z + 3 * rbc;
-            for( int q = wc + tid; q < rc; q += 256 )
-            {
-                const int pos = F.cmap[rbc + q];
-                if( pos < w ) { for( int k = 0; k < 3; ++k ) { zt[3 * pos + k] += zc[3 * q + k]; } }
-                else if( pos >= row0 && pos < row0 + 64 ) { for( int k = 0; k < 3; ++k ) { zb[3 * (pos - row0) + k] += zc[3 * q + k]; } }
-            }
-            __syncthreads();
-        }
-        const double * Ls = F.vals + F.val_ptr[s];
         const int lane = tid & 63, wv = tid >> 6;
         const int a = row0 + lane;
-        const int cw = ( w + 3 ) / 4;
+        double vb[3] = { 0.0, 0.0, 0.0 };
+        if( wv == 0 && a >= w && a < r ) { chol_front_row( F, rb, f, w, a, vb ); }
+        __syncthreads();
+        const double * Ls = F.vals + F.val_ptr[s];
+        const int cw = ( w + NW - 1 ) / NW;
         double acc[3] = { 0.0, 0.0, 0.0 };
         if( a < r )
         {
             const int cb = wv * cw;
             const int ce = min( ( a < w ) ? a + 1 : w, cb + cw );
-            #pragma unroll 4
+            #pragma unroll 8
             for( int c = cb; c < ce; ++c )
             {
                 const double l = Ls[(int64_t)c * r + a];
@@ -193,47 +185,44 @@ namespace rp
             double * z = F.z + 3 * rb;
             for( int k = 0; k < 3; ++k )
             {
-                const double sum = ( part[lane * 3 + k] + part[( 64 + lane ) * 3 + k] ) + ( part[( 128 + lane ) * 3 + k] + part[( 192 + lane ) * 3 + k] );
-                z[3 * a + k] = ( a < w ) ? sum : zb[3 * lane + k] - sum;
+                double sum = part[lane * 3 + k];
+                #pragma unroll
+                for( int u = 1; u < NW; ++u ) { sum += part[( u * 64 + lane ) * 3 + k]; }
+                z[3 * a + k] = ( a < w ) ? sum : vb[k] - sum;
             }
         }
     }
 
     // ---------------------------------------------------------------------------------------------
-    // Backward, levels with r <= 64: one wavefront per supernode, a lane per column.
+    // Backward, levels with r <= 64: one wavefront per supernode, a lane per row of v; the column sums
+    // are wavefront reductions.
     __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const int32_t * nodes, int64_t count )
     {
-        __shared__ double v_all[4][3 * 64];
         const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
         const int64_t t = (int64_t)blockIdx.x * 4 + wv;
         if( t >= count ) { return; }
-        double * v = v_all[wv];
         const int s = nodes[t];
         const int f = F.sn_begin[s];
         const int w = F.sn_begin[s + 1] - f;
         const int64_t rb = F.row_ptr[s];
         const int r = (int)( F.row_ptr[s + 1] - rb );
-        const double * z = F.z + 3 * rb;
-        if( lane < r )
+        double v[3] = { 0.0, 0.0, 0.0 };
+        if( lane < w ) { for( int k = 0; k < 3; ++k ) { v[k] = F.z[3 * ( rb + lane ) + k]; } }
+        else if( lane < r )
         {
-            if( lane < w ) { for( int k = 0; k < 3; ++k ) { v[3 * lane + k] = z[3 * lane + k]; } }
-            else
-            {
-                const int64_t j = F.rows[rb + lane];
-                for( int k = 0; k < 3; ++k ) { v[3 * lane + k] = -F.xp[3 * j + k]; }
-            }
+            const int64_t j = F.rows[rb + lane];
+            for( int k = 0; k < 3; ++k ) { v[k] = -F.xp[3 * j + k]; }
         }
-        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
-        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-        if( lane >= w ) { return; }
-        const double * col = F.vals + F.val_ptr[s] + (int64_t)lane * r;
-        double acc[3] = { 0.0, 0.0, 0.0 };
-        for( int i = lane; i < r; ++i )
+        const double * Ls = F.vals + F.val_ptr[s];
+        double mine[3] = { 0.0, 0.0, 0.0 };
+        for( int c = 0; c < w; ++c )
         {
-            const double l = col[i];
-            for( int k = 0; k < 3; ++k ) { acc[k] += l * v[3 * i + k]; }
+            const double l = ( lane >= c && lane < r ) ? Ls[(int64_t)c * r + lane] : 0.0;
+            double p[3];
+            for( int k = 0; k < 3; ++k ) { p[k] = group_sum( l * v[k], 64 ); }
+            if( lane == c ) { for( int k = 0; k < 3; ++k ) { mine[k] = p[k]; } }
         }
-        for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = acc[k]; }
+        if( lane < w ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = mine[k]; } }
     }
 
     // Backward, larger levels: one workgroup per task (supernode, group of columns), a group of G lanes
@@ -272,7 +261,7 @@ namespace rp
             if( c < w )
             {
                 const int ib = max( c, i0 );
-                #pragma unroll 4
+                #pragma unroll 8
                 for( int i = ib + gl; i < i0 + in; i += G )
                 {
                     const double l = col[i];
@@ -302,16 +291,23 @@ namespace rp
         hipLaunchKernelGGL( k_chol_form_m, dim3( (unsigned)count ), dim3( 256 ), 0, s, F, tasks, Lin, Lout );
     }
 
-    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, double alpha, const double * B, int64_t ldB, hipStream_t s )
+    void launch_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp, hipStream_t s )
     {
-        if( count <= 0 ) { return; }
-        hipLaunchKernelGGL( k_chol_fwd_small, dim3( (unsigned)( ( count + 3 ) / 4 ) ), dim3( 256 ), 0, s, F, nodes, count, alpha, B, ldB );
+        if( n <= 0 ) { return; }
+        hipLaunchKernelGGL( k_chol_gather, dim3( (unsigned)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, n, perm, alpha, B, ldB, bp );
     }
 
-    void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, double alpha, const double * B, int64_t ldB, hipStream_t s )
+    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        hipLaunchKernelGGL( k_chol_fwd_big, dim3( (unsigned)count ), dim3( 256 ), 0, s, F, tasks, alpha, B, ldB );
+        hipLaunchKernelGGL( k_chol_fwd_small, dim3( (unsigned)( ( count + 3 ) / 4 ) ), dim3( 256 ), 0, s, F, nodes, count );
+    }
+
+    void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, bool wide, hipStream_t s )
+    {
+        if( count <= 0 ) { return; }
+        if( wide ) { hipLaunchKernelGGL( (k_chol_fwd_big<16>), dim3( (unsigned)count ), dim3( 1024 ), 0, s, F, tasks ); }
+        else       { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
     }
 
     void launch_chol_bwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s )

@@ -90,6 +90,9 @@ namespace rp
         const int32_t * kids;       // children in increasing order
         const int32_t * cmap;       // per row entry: position in the parent's row list
         const int32_t * perm;       // new -> old
+        const int32_t * pull_ptr;   // rows + 1: per front row, its sources among the children's update rows
+        const int32_t * pull_src;   // source rows (global row index), in child order
+        const double  * bp;         // 3 x n: alpha B(perm)
         double        * z;          // 3 x rows: front vectors
         double        * xp;         // 3 x n: solution in the new numbering
     };
@@ -97,8 +100,9 @@ namespace rp
     constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a wavefront per supernode
     constexpr int CHOL_FWD_ROWS = 64;     // front rows per forward task (one per lane)
     void launch_chol_form_m( const CholDev & F, const int2 * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s );
-    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, double alpha, const double * B, int64_t ldB, hipStream_t s );
-    void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, double alpha, const double * B, int64_t ldB, hipStream_t s );
+    void launch_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp, hipStream_t s );
+    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s );
+    void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, bool wide, hipStream_t s );
     void launch_chol_bwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s );
     void launch_chol_bwd_big( const CholDev & F, const int2 * tasks, int64_t count, hipStream_t s );
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );

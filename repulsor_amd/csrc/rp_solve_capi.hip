@@ -60,6 +60,7 @@ namespace rp
         F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get();
         F.kid_ptr = ctx->h1_kid_ptr.get(); F.kids = ctx->h1_kids.get(); F.cmap = ctx->h1_cmap.get();
         F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get();
+        F.pull_ptr = ctx->h1_pull_ptr.get(); F.pull_src = ctx->h1_pull_src.get(); F.bp = ctx->h1_bp.get();
         return F;
     }
 
@@ -69,17 +70,18 @@ namespace rp
         const CholDev F = chol_dev( ctx );
         const size_t nl = ctx->h1_level_small.size();
         const int2 * tasks = ctx->h1_tasks.get();
+        launch_chol_gather( ctx->h1_n, ctx->h1_perm.get(), alpha, Bd, ldb, ctx->h1_bp.get(), ctx->stream );
         for( size_t l = 0; l < nl; ++l )
         {
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_fwd_small( F, ctx->h1_level_nodes.get() + b, e - b, alpha, Bd, ldb, ctx->stream );
+                launch_chol_fwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->stream );
             }
             else
             {
                 const int64_t tb = ctx->h1_task_ptr[0][l], te = ctx->h1_task_ptr[0][l + 1];
-                launch_chol_fwd_big( F, tasks + tb, te - tb, alpha, Bd, ldb, ctx->stream );
+                launch_chol_fwd_big( F, tasks + tb, te - tb, ctx->h1_level_wmax[l] > 128, ctx->stream );
             }
         }
         for( size_t l = nl; l-- > 0; )
@@ -665,6 +667,34 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 cmap[(size_t)q] = (int32_t)( it - Rp );
             }
         }
+        // pull lists: per front row, the child update rows mapping to it, children in increasing order
+        require( nrows < (int64_t)INT32_MAX, "factor has too many row entries for 32-bit pull lists", RP_EUNSUPPORTED );
+        std::vector<int32_t> pull_ptr( (size_t)nrows + 1, 0 );
+        for( int32_t s = 0; s < ns; ++s )
+        {
+            const int32_t p = sn_parent[s];
+            if( p < 0 ) { continue; }
+            const int32_t w = sn_begin[s + 1] - sn_begin[s];
+            for( int64_t q = row_ptr[s] + w; q < row_ptr[s + 1]; ++q ) { ++pull_ptr[(size_t)( row_ptr[p] + cmap[(size_t)q] ) + 1]; }
+        }
+        for( int64_t q = 0; q < nrows; ++q ) { pull_ptr[(size_t)q + 1] += pull_ptr[(size_t)q]; }
+        std::vector<int32_t> pull_src( (size_t)std::max<int64_t>( 1, pull_ptr.back() ) );
+        {
+            std::vector<int32_t> pos( pull_ptr.begin(), pull_ptr.end() - 1 );
+            for( int32_t p = 0; p < ns; ++p )
+            {
+                for( int32_t kk = kid_ptr[(size_t)p]; kk < kid_ptr[(size_t)p + 1]; ++kk )
+                {
+                    const int32_t c = kids[(size_t)kk];
+                    const int32_t wc = sn_begin[c + 1] - sn_begin[c];
+                    for( int64_t q = row_ptr[c] + wc; q < row_ptr[c + 1]; ++q )
+                    {
+                        const int64_t tgt = row_ptr[p] + cmap[(size_t)q];
+                        pull_src[(size_t)pos[(size_t)tgt]++] = (int32_t)q;
+                    }
+                }
+            }
+        }
         int32_t nl = 0;
         for( auto h : height ) { nl = std::max( nl, h + 1 ); }
         std::vector<int64_t> lptr( (size_t)nl + 1, 0 );
@@ -737,6 +767,9 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_kid_ptr.upload( kid_ptr.data(), kid_ptr.size(), ctx->stream );
         ctx->h1_kids.upload( kids.data(), kids.size(), ctx->stream );
         ctx->h1_cmap.upload( cmap.data(), cmap.size(), ctx->stream );
+        ctx->h1_pull_ptr.upload( pull_ptr.data(), pull_ptr.size(), ctx->stream );
+        ctx->h1_pull_src.upload( pull_src.data(), pull_src.size(), ctx->stream );
+        ctx->h1_bp.require( (size_t)( 3 * (int64_t)n ) );
         ctx->h1_perm.upload( perm, (size_t)n, ctx->stream );
         ctx->h1_level_nodes.upload( lnodes.data(), lnodes.size(), ctx->stream );
         ctx->h1_z.require( (size_t)(3 * nrows) );
