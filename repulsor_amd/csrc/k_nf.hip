@@ -246,12 +246,12 @@ namespace rp
         const VT * vals, const double * diag, const double * xin,
         const double * leaf_out, int use_far, const double * pre_vals, double * S_out )
     {
-        union Smem { double sx[2][12][CH]; double red[64][12]; };
-        __shared__ Smem smem;
-        double (&sx)[2][12][CH] = smem.sx;
-        double (&red)[64][12] = smem.red;
-        __shared__ int32_t sj[2][CH];
-        __shared__ uint32_t sm[2][CH];
+        // 32 columns per chunk and the row sums by lane shuffles: 6.5 KB of LDS per wavefront, so that
+        // 24 leaves share a CU
+        constexpr int CS = 32;
+        __shared__ double sx[2][12][CS];
+        __shared__ int32_t sj[2][CS];
+        __shared__ uint32_t sm[2][CS];
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( L );
         if( I < 0 ) { return; }
@@ -267,38 +267,39 @@ namespace rp
         double acc[12];
         #pragma unroll
         for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
-        double rx[12];
+        // lane l loads half of the input block of column l mod 32 of the chunk (doubles 6 (l / 32) ..)
+        double rx[6];
         int32_t rj = 0;
         uint32_t rm = 0;
+        const int half = lane >> 5, lc = lane & ( CS - 1 );
         auto gload = [&]( int64_t base )
         {
-            const int64_t m = base + lane;
+            const int64_t m = base + lc;
             if( m < M_ )
             {
                 rj = col_prim[cb + m];
                 rm = col_mask[cb + m];
-                const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj );
+                const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj + 6 * half );
                 #pragma unroll
-                for( int h = 0; h < 6; ++h ) { const double2 t = xj[h]; rx[2 * h] = t.x; rx[2 * h + 1] = t.y; }
+                for( int h = 0; h < 3; ++h ) { const double2 t = xj[h]; rx[2 * h] = t.x; rx[2 * h + 1] = t.y; }
             }
             else { rm = 0u; }
         };
         auto lstore = [&]( int buf )
         {
-            sj[buf][lane] = rj;
-            sm[buf][lane] = rm;
+            if( half == 0 ) { sj[buf][lc] = rj; sm[buf][lc] = rm; }
             #pragma unroll
-            for( int k = 0; k < 12; ++k ) { sx[buf][k][lane] = rx[k]; }
+            for( int k = 0; k < 6; ++k ) { sx[buf][6 * half + k][lc] = rx[k]; }
         };
         gload( 0 );
         lstore( 0 );
         __syncthreads();
         int buf = 0;
-        for( int64_t base = 0; base < M_; base += CH )
+        for( int64_t base = 0; base < M_; base += CS )
         {
-            const bool more = base + CH < M_;
-            if( more ) { gload( base + CH ); }
-            const int cnt_c = (int)( (M_ - base < CH) ? (M_ - base) : CH );
+            const bool more = base + CS < M_;
+            if( more ) { gload( base + CS ); }
+            const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {
                 for( int mm = c; mm < cnt_c; mm += nS )
@@ -318,20 +319,19 @@ namespace rp
             __syncthreads();
             buf ^= 1;
         }
+        // row sums over the lanes ii, ii + nI, ... in that order (every lane takes part in the shuffles)
+        double s[12];
         #pragma unroll
-        for( int k = 0; k < 12; ++k ) { red[lane][k] = acc[k]; }
-        __syncthreads();
+        for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
+        for( int cc = 0; cc < nS; ++cc )
+        {
+            const int src = ( lane % nI ) + cc * nI;
+            #pragma unroll
+            for( int k = 0; k < 12; ++k ) { s[k] += __shfl( acc[k], src, 64 ); }
+        }
         if( lane < nI )
         {
             const int64_t i = ib + ii;
-            double s[12];
-            #pragma unroll
-            for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
-            for( int cc = 0; cc < nS; ++cc )
-            {
-                #pragma unroll
-                for( int k = 0; k < 12; ++k ) { s[k] += red[lane + cc * nI][k]; }
-            }
             const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
             const double * xi = xin + 12 * i;
             #pragma unroll
