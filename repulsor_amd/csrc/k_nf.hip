@@ -302,11 +302,32 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {
-                for( int mm = c; mm < cnt_c; mm += nS )
+                // two value pairs in flight before the (masked) products; every tile slot exists, masked
+                // pairs add 0, the products are summed in column order as one at a time
+                const VT * vrow = vals + vb + base * nI + ii;
+                int mm = c;
+                for( ; mm + nS < cnt_c; mm += 2 * nS )
+                {
+                    const VT av0 = vrow[(int64_t)mm * nI];
+                    const VT av1 = vrow[(int64_t)( mm + nS ) * nI];
+                    #pragma unroll
+                    for( int u = 0; u < 2; ++u )
+                    {
+                        const int m2 = mm + u * nS;
+                        const VT av = u ? av1 : av0;
+                        const bool on = ( sm[buf][m2] >> ii ) & 1u;
+                        const double a0 = on ? (double)av.x : 0.0, a1 = on ? (double)av.y : 0.0;
+                        #pragma unroll
+                        for( int k = 0; k < 3; ++k ) { acc[k] += a0 * sx[buf][k][m2]; }
+                        #pragma unroll
+                        for( int k = 3; k < 12; ++k ) { acc[k] += a1 * sx[buf][k][m2]; }
+                    }
+                }
+                for( ; mm < cnt_c; mm += nS )
                 {
                     if( (sm[buf][mm] >> ii) & 1u )
                     {
-                        const VT av = vals[vb + (base + mm) * nI + ii];
+                        const VT av = vrow[(int64_t)mm * nI];
                         const double a0 = (double)av.x, a1 = (double)av.y;
                         #pragma unroll
                         for( int k = 0; k < 3; ++k ) { acc[k] += a0 * sx[buf][k][mm]; }

@@ -194,11 +194,28 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
             if( active )
             {
-                for( int mm = c; mm < cnt_c; mm += nS )
+                // four value loads in flight before the (masked) products; every tile slot exists, masked
+                // pairs add 0, the products are summed in column order as one at a time
+                const VT * vrow = vals + vb + base * nI + ii;
+                int mm = c;
+                for( ; mm + 3 * nS < cnt_c; mm += 4 * nS )
+                {
+                    double v[4];
+                    #pragma unroll
+                    for( int u = 0; u < 4; ++u ) { v[u] = (double)vrow[(int64_t)( mm + u * nS ) * nI]; }
+                    #pragma unroll
+                    for( int u = 0; u < 4; ++u )
+                    {
+                        const int m2 = mm + u * nS;
+                        const double vv = ( ( sm[buf][m2] >> ii ) & 1u ) ? v[u] : 0.0;
+                        for( int k = 0; k < 3; ++k ) { acc[k] += vv * sx[buf][k][m2]; }
+                    }
+                }
+                for( ; mm < cnt_c; mm += nS )
                 {
                     if( (sm[buf][mm] >> ii) & 1u )
                     {
-                        const double v = (double)vals[vb + (base + mm) * nI + ii];
+                        const double v = (double)vrow[(int64_t)mm * nI];
                         for( int k = 0; k < 3; ++k ) { acc[k] += v * sx[buf][k][mm]; }
                     }
                 }
