@@ -302,19 +302,20 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {
-                // two value pairs in flight before the (masked) products; every tile slot exists, masked
+                // four value pairs in flight before the (masked) products; every tile slot exists, masked
                 // pairs add 0, the products are summed in column order as one at a time
                 const VT * vrow = vals + vb + base * nI + ii;
                 int mm = c;
-                for( ; mm + nS < cnt_c; mm += 2 * nS )
+                for( ; mm + 3 * nS < cnt_c; mm += 4 * nS )
                 {
-                    const VT av0 = vrow[(int64_t)mm * nI];
-                    const VT av1 = vrow[(int64_t)( mm + nS ) * nI];
+                    VT av4[4];
                     #pragma unroll
-                    for( int u = 0; u < 2; ++u )
+                    for( int u = 0; u < 4; ++u ) { av4[u] = vrow[(int64_t)( mm + u * nS ) * nI]; }
+                    #pragma unroll
+                    for( int u = 0; u < 4; ++u )
                     {
                         const int m2 = mm + u * nS;
-                        const VT av = u ? av1 : av0;
+                        const VT av = av4[u];
                         const bool on = ( sm[buf][m2] >> ii ) & 1u;
                         const double a0 = on ? (double)av.x : 0.0, a1 = on ? (double)av.y : 0.0;
                         #pragma unroll

@@ -194,17 +194,17 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
             if( active )
             {
-                // four value loads in flight before the (masked) products; every tile slot exists, masked
+                // eight value loads in flight before the (masked) products; every tile slot exists, masked
                 // pairs add 0, the products are summed in column order as one at a time
                 const VT * vrow = vals + vb + base * nI + ii;
                 int mm = c;
-                for( ; mm + 3 * nS < cnt_c; mm += 4 * nS )
+                for( ; mm + 7 * nS < cnt_c; mm += 8 * nS )
                 {
-                    double v[4];
+                    double v[8];
                     #pragma unroll
-                    for( int u = 0; u < 4; ++u ) { v[u] = (double)vrow[(int64_t)( mm + u * nS ) * nI]; }
+                    for( int u = 0; u < 8; ++u ) { v[u] = (double)vrow[(int64_t)( mm + u * nS ) * nI]; }
                     #pragma unroll
-                    for( int u = 0; u < 4; ++u )
+                    for( int u = 0; u < 8; ++u )
                     {
                         const int m2 = mm + u * nS;
                         const double vv = ( ( sm[buf][m2] >> ii ) & 1u ) ? v[u] : 0.0;
