@@ -208,6 +208,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-solve", action="store_true", help="time energy+grad only")
     ap.add_argument("--host-tree", action="store_true", help="build the cluster tree and block cluster tree on the host")
+    ap.add_argument("--pc-precision", default="mixed", choices=["fp64", "mixed"],
+                    help="pseudo-Laplacian values inside the preconditioner (the metric keeps --metric-precision)")
     ap.add_argument("--pc-weight", type=float, default=-1.0,
                     help="diagonal weight w of P = P0 + w diag(A)^-1 (0: reference P0, <0: estimated per state)")
     ap.add_argument("--plain-pc-max-iter", type=int, default=4000,
@@ -244,6 +246,7 @@ def main():
     E = R.TangentPointEnergy0(wl.q, wl.p, precision=args.precision)
     A = R.TangentPointMetric0(wl.q, wl.p, precision=args.metric_precision)
     A.SetPreconditionerDiagonalWeight(args.pc_weight)
+    A.SetPreconditionerPrecision(args.pc_precision)
     if learned:
         A.SetLearnedPreconditioner(path=learned, bf16=True)
 
@@ -416,6 +419,7 @@ def main():
             "theta_far": wl.far_theta, "theta_near": wl.near_theta, "leaf_size": wl.leaf,
             "near_nnz": b["near_nnz"], "far_nnz": b["far_nnz"], "very_near_nnz": b["verynear_nnz"],
             "precision": args.precision, "metric_precision": args.metric_precision,
+            "preconditioner_precision": args.pc_precision,
             "step": ("energy + differential (device-resident) + metric solve to |B - A X|/|B| <= "
                      f"{args.tol:g} per column ({args.solver})") if not args.no_solve
                     else "energy + differential (device-resident)",

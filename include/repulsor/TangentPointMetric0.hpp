@@ -21,6 +21,7 @@ namespace Repulsor
     protected:
         double q, p, s;
         Precision precision = Precision::FP64;
+        int pc_precision = -1;   // precision of the pseudo-Laplacian values inside the preconditioner; -1: as precision
 
     public:
         explicit TangentPointMetric0( const double p_ ) : q( p_ ), p( 2 * p_ ), s( (2 * p_ - DOM_DIM) / p_ ) {}
@@ -28,6 +29,13 @@ namespace Repulsor
 
         void SetPrecision( Precision prec ) { precision = prec; }
         Precision GetPrecision() const { return precision; }
+
+        // Storage and pair arithmetic of the pseudo-Laplacian inside the preconditioner (no reference
+        // counterpart). PCG and GMRES need only a fixed symmetric positive definite preconditioner, so FP32
+        // values there leave the operator, its FP64 residual and the converged solution's accuracy as they
+        // are; the metric itself keeps SetPrecision.
+        void SetPreconditionerPrecision( Precision prec ) { pc_precision = (int)prec; }
+        Precision GetPreconditionerPrecision() const { return pc_precision < 0 ? precision : (Precision)pc_precision; }
 
         std::string ClassName() const
         {
@@ -132,10 +140,11 @@ namespace Repulsor
             {
                 rp_ctx * c = M.Device();
                 char key[96];
-                std::snprintf( key, sizeof(key), "pl:%.17g:%d", 2.0 - s, (int)precision );
+                const int pl_prec = (int)GetPreconditionerPrecision();
+                std::snprintf( key, sizeof(key), "pl:%.17g:%d", 2.0 - s, pl_prec );
                 if( M.device_state.count( key ) ) { return true; }
                 ScopedTimer timer( "PseudoLaplacian::MetricValues" );
-                if( !Check( rp_pl_compute( c, 2.0 - s, (int)precision ), ClassName() + "::PseudoLaplacianValues" ) ) { return false; }
+                if( !Check( rp_pl_compute( c, 2.0 - s, pl_prec ), ClassName() + "::PseudoLaplacianValues" ) ) { return false; }
                 for( auto k = M.device_state.begin(); k != M.device_state.end(); )
                 {
                     if( k->first.rfind( "pl:", 0 ) == 0 ) { k = M.device_state.erase(k); } else { ++k; }
@@ -200,6 +209,7 @@ namespace Repulsor
                 if( !X0.empty() ) { std::copy( X0.begin(), X0.end(), X ); }
                 TangentPointMetric0 plain( q, p );
                 plain.SetPrecision( precision );
+                plain.pc_precision = pc_precision;
                 plain.SetPreconditionerDiagonalWeight( -1.0 );
                 const bool ok = plain.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance, device_ptrs, solver_flags );
                 iter = it + plain.Solver_IterationCount();

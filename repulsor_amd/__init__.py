@@ -105,6 +105,7 @@ _host_sigs = {
     "rpm_pl_multiply": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_h1_solve_device": (C.c_int, [_vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_op_set_pc_weight": (None, [_vp, C.c_double]),
+    "rpm_op_set_pc_precision": (None, [_vp, C.c_int]),
     "rpm_op_set_learned": (None, [_vp, C.c_int, C.POINTER(C.c_float), C.c_int]),
     "rpm_learned_feature_count": (C.c_int, []),
     "rpm_learned_features": (C.c_int, [_vp, C.POINTER(C.c_float)]),
@@ -507,6 +508,11 @@ class TangentPointMetric0(_Op):
         assert Y.dtype == np.float64 and Y.flags.c_contiguous
         _check(H.rpm_metric_precondition(self._h, M._h, alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1], nrhs, 0))
         return Y
+
+    def SetPreconditionerPrecision(self, precision):
+        """Precision ("fp64" or "mixed") of the pseudo-Laplacian inside the preconditioner; the metric
+        operator keeps its own precision (no reference counterpart)."""
+        H.rpm_op_set_pc_precision(self._h, PRECISIONS[precision])
 
     def SetPreconditionerDiagonalWeight(self, w):
         """P = P0 + w diag(A)^-1: w = 0 the reference P0 (default), w > 0 fixed, w < 0 estimated per state

@@ -23,6 +23,7 @@ namespace
     {
         double q = 6, p = 12;
         int precision = 0;
+        int pc_precision = -1;
         double pc_weight = 0.0;
         std::vector<float> lp_params;
         int lp_nfeat = 0;
@@ -52,6 +53,7 @@ namespace
     {
         TangentPointMetric0<Mesh> A( o->q, o->p );
         A.SetPrecision( (Precision)o->precision );
+        if( o->pc_precision >= 0 ) { A.SetPreconditionerPrecision( (Precision)o->pc_precision ); }
         A.SetPreconditionerDiagonalWeight( o->pc_weight );
         if( !o->lp_params.empty() ) { A.SetLearnedPreconditioner( o->lp_params, o->lp_nfeat, o->lp_bf16 ); }
         return A;
@@ -274,6 +276,9 @@ void * rpm_op_new( double q, double p, int precision )
 }
 
 void rpm_op_free( void * o ) { delete (OpHandle *)o; }
+
+// Precision of the pseudo-Laplacian inside the metric's preconditioner (-1: the operator's precision).
+void rpm_op_set_pc_precision( void * o, int precision ) { ((OpHandle *)o)->pc_precision = precision; }
 
 // Diagonal weight of the metric's preconditioner P = P0 + w diag(A)^-1 (0: reference P0, < 0: estimated).
 void rpm_op_set_pc_weight( void * o, double w ) { ((OpHandle *)o)->pc_weight = w; }

@@ -243,3 +243,39 @@ def test_solve_after_semistatic_update():
     assert rel(B2, Bo) <= 1e-10
     true = np.linalg.norm(Bo - S.A(X2), axis=0) / np.linalg.norm(Bo, axis=0)
     assert np.all(true <= 1.5e-6), true
+
+
+@pytest.mark.parametrize("name", ["torus_2400", "sphere_1280"])
+def test_solve_fp32_preconditioner(name):
+    """FP32 pseudo-Laplacian values inside P = P0 + w diag(A)^-1, FP64 metric: the preconditioner
+    product stays within FP32 accuracy of the oracle's, and the solve reaches the oracle-measured FP64
+    residual with the iteration count of the FP64 preconditioner (at most one more)."""
+    wl, M = _mesh(name)
+    print(f"case {name}", flush=True)
+    E = R.TangentPointEnergy0(wl.q, wl.p)
+    B = E.Differential(M)
+    A64 = R.TangentPointMetric0(wl.q, wl.p)
+    A64.SetPreconditionerDiagonalWeight(-1.0)
+    X64 = np.zeros_like(B)
+    A64.Solve(M, 1.0, B, 0.0, X64, max_iter=2000, tolerance=1e-6)
+    it64 = A64.Solver_IterationCount()
+    A = R.TangentPointMetric0(wl.q, wl.p)
+    A.SetPreconditionerDiagonalWeight(-1.0)
+    A.SetPreconditionerPrecision("mixed")
+    X = np.zeros_like(B)
+    A.Solve(M, 1.0, B, 0.0, X, max_iter=2000, tolerance=1e-6)
+    it = A.Solver_IterationCount()
+    w = A.PreconditionerWeightInUse(M)
+    S = OracleSolve(M, wl.q, wl.p)
+    S.with_diagonal_term(w, S.diagonal())
+    Xr = random_field(M.V, seed=19)
+    Y = np.zeros_like(Xr)
+    A.MultiplyPreconditioner(M, 1.0, Xr, 0.0, Y)
+    err = rel(Y, S.P(Xr))
+    _, Bo, _, _ = oracle_energy_diff(M, wl.q, wl.p)
+    true = np.linalg.norm(Bo - S.A(X), axis=0) / np.linalg.norm(Bo, axis=0)
+    print(f"FP32 pseudo-Laplacian: preconditioner rel err {err:.2e}, iterations {it} (FP64 {it64}), "
+          f"residual {true.max():.2e}", flush=True)
+    assert err <= 1e-5
+    assert np.all(true <= 1.5e-6), true
+    assert it <= it64 + 1
