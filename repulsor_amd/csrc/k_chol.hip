@@ -15,8 +15,8 @@
 //   backward (root to leaves): x_s = inv(L_ss)^T y_s - M^T x(R_s[w:]), i.e. for column c the dot product
 //       of the contiguous block column c (rows c..r-1) with v = [y_s; -x(R_s[w:])].
 //
-// Levels whose supernodes have at most 64 rows run one wavefront per supernode (a lane per front row
-// forward, a lane per row of v and a wavefront sum per column backward); larger levels run workgroups
+// Levels whose supernodes have at most 64 rows run a group of 16, 32 or 64 lanes per supernode (a lane
+// per front row forward, a lane per row of v and a group sum per column backward); larger levels run workgroups
 // over tasks (supernode, block of 64 front rows) forward, 4 or 16 waves splitting the columns, and
 // (supernode, group of columns) backward, a group of lanes per column. Every sum has a fixed order, so
 // the solve is deterministic and uses no atomics.
@@ -103,14 +103,16 @@ namespace rp
         for( int k = 0; k < 3; ++k ) { bp[3 * j + k] = alpha * B[ldB * o + k]; }
     }
 
-    // Forward, levels with r <= 64: one wavefront per supernode, a lane per front row.
+    // Forward, levels with r <= G: a group of G lanes per supernode (64 / G supernodes per wavefront),
+    // a lane per front row.
+    template<int G>
     __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const int32_t * nodes, int64_t count )
     {
-        __shared__ double fr_all[4][3 * 64];
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int64_t t = (int64_t)blockIdx.x * 4 + wv;
+        __shared__ double fr_all[3 * 256];
+        const int lane = threadIdx.x % G;
+        const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
-        double * fr = fr_all[wv];
+        double * fr = fr_all + 3 * ( threadIdx.x - lane );
         const int s = nodes[t];
         const int f = F.sn_begin[s];
         const int w = F.sn_begin[s + 1] - f;
@@ -194,12 +196,13 @@ namespace rp
     }
 
     // ---------------------------------------------------------------------------------------------
-    // Backward, levels with r <= 64: one wavefront per supernode, a lane per row of v; the column sums
-    // are wavefront reductions.
+    // Backward, levels with r <= G: a group of G lanes per supernode, a lane per row of v; the column
+    // sums are group reductions.
+    template<int G>
     __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const int32_t * nodes, int64_t count )
     {
-        const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int64_t t = (int64_t)blockIdx.x * 4 + wv;
+        const int lane = threadIdx.x % G;
+        const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
         const int s = nodes[t];
         const int f = F.sn_begin[s];
@@ -219,7 +222,7 @@ namespace rp
         {
             const double l = ( lane >= c && lane < r ) ? Ls[(int64_t)c * r + lane] : 0.0;
             double p[3];
-            for( int k = 0; k < 3; ++k ) { p[k] = group_sum( l * v[k], 64 ); }
+            for( int k = 0; k < 3; ++k ) { p[k] = group_sum( l * v[k], G ); }
             if( lane == c ) { for( int k = 0; k < 3; ++k ) { mine[k] = p[k]; } }
         }
         if( lane < w ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = mine[k]; } }
@@ -297,10 +300,12 @@ namespace rp
         hipLaunchKernelGGL( k_chol_gather, dim3( (unsigned)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, n, perm, alpha, B, ldB, bp );
     }
 
-    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s )
+    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, int rmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        hipLaunchKernelGGL( k_chol_fwd_small, dim3( (unsigned)( ( count + 3 ) / 4 ) ), dim3( 256 ), 0, s, F, nodes, count );
+        if( rmax <= 16 )      { hipLaunchKernelGGL( k_chol_fwd_small<16>, dim3( (unsigned)( ( count + 15 ) / 16 ) ), dim3( 256 ), 0, s, F, nodes, count ); }
+        else if( rmax <= 32 ) { hipLaunchKernelGGL( k_chol_fwd_small<32>, dim3( (unsigned)( ( count + 7 ) / 8 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
+        else                  { hipLaunchKernelGGL( k_chol_fwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
     }
 
     void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, bool wide, hipStream_t s )
@@ -310,10 +315,12 @@ namespace rp
         else       { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
     }
 
-    void launch_chol_bwd_small( const CholDev & F, const int32_t * nodes, int64_t count, hipStream_t s )
+    void launch_chol_bwd_small( const CholDev & F, const int32_t * nodes, int64_t count, int rmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        hipLaunchKernelGGL( k_chol_bwd_small, dim3( (unsigned)( ( count + 3 ) / 4 ) ), dim3( 256 ), 0, s, F, nodes, count );
+        if( rmax <= 16 )      { hipLaunchKernelGGL( k_chol_bwd_small<16>, dim3( (unsigned)( ( count + 15 ) / 16 ) ), dim3( 256 ), 0, s, F, nodes, count ); }
+        else if( rmax <= 32 ) { hipLaunchKernelGGL( k_chol_bwd_small<32>, dim3( (unsigned)( ( count + 7 ) / 8 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
+        else                  { hipLaunchKernelGGL( k_chol_bwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
     }
 
     void launch_chol_bwd_big( const CholDev & F, const int2 * tasks, int64_t count, hipStream_t s )

@@ -76,7 +76,7 @@ namespace rp
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_fwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->stream );
+                launch_chol_fwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
             }
             else
             {
@@ -89,7 +89,7 @@ namespace rp
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_bwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->stream );
+                launch_chol_bwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
             }
             else
             {
