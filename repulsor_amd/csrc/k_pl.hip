@@ -261,12 +261,12 @@ namespace rp
     __global__ __launch_bounds__(256) void k_ff_pl_spmm( int64_t C, const int64_t * rp, const int32_t * ci,
         const VT * vals, const double * diag, const double * xin, double * yout )
     {
-        const int w = threadIdx.x >> 6;
-        const int lane = threadIdx.x & 63;
-        const int64_t i = (int64_t)blockIdx.x * 4 + w;
+        constexpr int G = 16;   // lanes per cluster row (rows hold about two dozen entries)
+        const int lane = threadIdx.x % G;
+        const int64_t i = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( i >= C ) { return; }
         double acc[3] = { 0, 0, 0 };
-        for( int64_t k = rp[i] + lane; k < rp[i + 1]; k += 64 )
+        for( int64_t k = rp[i] + lane; k < rp[i + 1]; k += G )
         {
             const double v = (double)vals[k];
             const double * xj = xin + 3 * (int64_t)ci[k];
@@ -275,7 +275,7 @@ namespace rp
         for( int h = 0; h < 3; ++h )
         {
             #pragma unroll
-            for( int off = 32; off > 0; off >>= 1 ) { acc[h] += __shfl_xor( acc[h], off, 64 ); }
+            for( int off = G / 2; off > 0; off >>= 1 ) { acc[h] += __shfl_xor( acc[h], off, 64 ); }
         }
         if( lane == 0 )
         {
@@ -337,7 +337,7 @@ namespace rp
     void launch_pl_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {
         if( !C ) { return; }
-        const dim3 grid( (unsigned)((C + 3) / 4) );
+        const dim3 grid( (unsigned)((C + 15) / 16) );
         if( mixed ) { hipLaunchKernelGGL( (k_ff_pl_spmm<float>), grid, dim3(256), 0, s, C, rp, ci, (const float *)vals, diag, xin, yout ); }
         else        { hipLaunchKernelGGL( (k_ff_pl_spmm<double>), grid, dim3(256), 0, s, C, rp, ci, (const double *)vals, diag, xin, yout ); }
     }

@@ -249,6 +249,47 @@ namespace rp
         C_in[W * (int64_t)c + h] = C_in[W * (int64_t)C_left[c] + h] + C_in[W * (int64_t)C_right[c] + h];
     }
 
+    // Three depth levels of PercolateUp in one launch: one thread per (internal node c at the band's top
+    // depth, component h) computes C_in = C_in[left] + C_in[right] for c's internal descendants two and
+    // one levels down (band size 3, fewer at the root end), then for c: the same sums as one level per
+    // launch. Children below the band were done by the previous (deeper) band.
+    __global__ void k_up_band( int64_t n, int W, int band, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in )
+    {
+        const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t q = g / W;
+        const int h = (int)(g % W);
+        if( q >= n ) { return; }
+        const int32_t c = nodes[q];
+        int32_t l1[2], l2[4];
+        int n1 = 0, n2 = 0;
+        if( band >= 2 )
+        {
+            const int32_t a = C_left[c], b = C_right[c];
+            if( C_left[a] >= 0 ) { l1[n1++] = a; }
+            if( C_left[b] >= 0 ) { l1[n1++] = b; }
+            if( band >= 3 )
+            {
+                for( int u = 0; u < n1; ++u )
+                {
+                    const int32_t x = C_left[l1[u]], y = C_right[l1[u]];
+                    if( C_left[x] >= 0 ) { l2[n2++] = x; }
+                    if( C_left[y] >= 0 ) { l2[n2++] = y; }
+                }
+            }
+        }
+        for( int u = 0; u < n2; ++u )
+        {
+            const int32_t x = l2[u];
+            C_in[W * (int64_t)x + h] = C_in[W * (int64_t)C_left[x] + h] + C_in[W * (int64_t)C_right[x] + h];
+        }
+        for( int u = 0; u < n1; ++u )
+        {
+            const int32_t x = l1[u];
+            C_in[W * (int64_t)x + h] = C_in[W * (int64_t)C_left[x] + h] + C_in[W * (int64_t)C_right[x] + h];
+        }
+        C_in[W * (int64_t)c + h] = C_in[W * (int64_t)C_left[c] + h] + C_in[W * (int64_t)C_right[c] + h];
+    }
+
     template<typename T>
     __device__ __forceinline__ T wave_sum64( T v )
     {
@@ -257,30 +298,41 @@ namespace rp
         return v;
     }
 
+    // Far-field block SpMM: a group of FF_G lanes per cluster row (rows hold about two dozen entries),
+    // group sums by lane shuffles, then the diagonal block.
+    constexpr int FF_G = 16;
+
     template<typename VT>
     __global__ __launch_bounds__(256) void k_ff_spmm( int64_t C, const int64_t * rp, const int32_t * ci,
         const VT * vals, const double * diag, const double * xin, double * yout )
     {
-        const int w = threadIdx.x >> 6;
-        const int lane = threadIdx.x & 63;
-        const int64_t i = (int64_t)blockIdx.x * 4 + w;
+        const int lane = threadIdx.x % FF_G;
+        const int64_t i = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / FF_G;
         if( i >= C ) { return; }
         double acc[12];
         #pragma unroll
         for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
-        for( int64_t k = rp[i] + lane; k < rp[i + 1]; k += 64 )
+        const int64_t e = rp[i + 1];
+        for( int64_t k = rp[i] + lane; k < e; k += FF_G )
         {
             const int32_t j = ci[k];
             const VT av = vals[k];
             const double a0 = (double)av.x, a1 = (double)av.y;
-            const double * xj = xin + 12 * (int64_t)j;
+            const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)j );
+            double x[12];
             #pragma unroll
-            for( int h = 0; h < 3; ++h ) { acc[h] += a0 * xj[h]; }
+            for( int h = 0; h < 6; ++h ) { const double2 t = xj[h]; x[2 * h] = t.x; x[2 * h + 1] = t.y; }
             #pragma unroll
-            for( int h = 3; h < 12; ++h ) { acc[h] += a1 * xj[h]; }
+            for( int h = 0; h < 3; ++h ) { acc[h] += a0 * x[h]; }
+            #pragma unroll
+            for( int h = 3; h < 12; ++h ) { acc[h] += a1 * x[h]; }
         }
         #pragma unroll
-        for( int k = 0; k < 12; ++k ) { acc[k] = wave_sum64( acc[k] ); }
+        for( int k = 0; k < 12; ++k )
+        {
+            #pragma unroll
+            for( int off = FF_G / 2; off > 0; off >>= 1 ) { acc[k] += __shfl_xor( acc[k], off, 64 ); }
+        }
         if( lane == 0 )
         {
             const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
@@ -379,6 +431,22 @@ namespace rp
     void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s, int W )
     { if( n ) { hipLaunchKernelGGL( k_up_level, grid1(W * n), dim3(256), 0, s, n, W, nodes, C_left, C_right, C_in ); } }
 
+    void launch_up_tree( const std::vector<int32_t> & level_ptr, const int32_t * level_nodes, const int32_t * C_left, const int32_t * C_right,
+                         double * C_in, hipStream_t s, int W )
+    {
+        // level index l holds the internal clusters of depth maxd - l; bands of three levels end at l = 2, 5, ...
+        const int nl = (int)level_ptr.size() - 1;
+        int prev = -1;
+        for( int l = 0; l < nl; ++l )
+        {
+            if( l % 3 != 2 && l != nl - 1 ) { continue; }
+            const int64_t b = level_ptr[(size_t)l], e = level_ptr[(size_t)l + 1];
+            const int band = l - prev;
+            prev = l;
+            if( e > b ) { hipLaunchKernelGGL( k_up_band, grid1(W * ( e - b )), dim3(256), 0, s, e - b, W, band, level_nodes + b, C_left, C_right, C_in ); }
+        }
+    }
+
     void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s )
     { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<3>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
 
@@ -388,7 +456,7 @@ namespace rp
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {
         if( !C ) { return; }
-        const dim3 grid( (unsigned)((C + 3) / 4) );
+        const dim3 grid( (unsigned)((C + 256 / FF_G - 1) / (256 / FF_G)) );
         if( mixed ) { hipLaunchKernelGGL( (k_ff_spmm<float2>), grid, dim3(256), 0, s, C, rp, ci, (const float2 *)vals, diag, xin, yout ); }
         else        { hipLaunchKernelGGL( (k_ff_spmm<double2>), grid, dim3(256), 0, s, C, rp, ci, (const double2 *)vals, diag, xin, yout ); }
     }

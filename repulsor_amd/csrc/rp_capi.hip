@@ -35,11 +35,7 @@ namespace rp
         if( do_ff )
         {
             launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream );
-            for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
-            {
-                const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
-                launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream );
-            }
+            launch_up_tree( ctx->h_level_ptr, ctx->level_nodes.get(), ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream, 12 );
             // Far field first, so that the near-field SpMM epilogue can add the percolated cluster
             // output of its leaf and apply the post-operator (S_out = per-vertex-slot contributions).
             launch_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->ff_vals.get(), ctx->ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );

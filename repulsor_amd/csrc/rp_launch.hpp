@@ -59,6 +59,9 @@ namespace rp
     void launch_pre( int size, int64_t T, const int32_t * cols, const double * vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
     void launch_up_leaves( int64_t L, const int32_t * leaf_cluster, const int32_t * C_begin, const int32_t * C_end, const double * P_in, double * C_in, hipStream_t s, int W = 12 );
     void launch_up_level( int64_t n, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in, hipStream_t s, int W = 12 );
+    // the whole up-sweep (all depth levels, deepest first) in launches of three levels each
+    void launch_up_tree( const std::vector<int32_t> & level_ptr, const int32_t * level_nodes, const int32_t * C_left, const int32_t * C_right,
+                         double * C_in, hipStream_t s, int W );
     void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s );
     void launch_nf_spmm_post( bool mixed, int size, int64_t T, int64_t L, const int32_t * lb, const int32_t * le,
                               const int64_t * cp, const int32_t * cpr, const uint32_t * cm, const int64_t * vp,

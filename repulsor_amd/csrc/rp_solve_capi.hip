@@ -26,11 +26,7 @@ namespace rp
         if( do_ff )
         {
             launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream, 3 );
-            for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
-            {
-                const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
-                launch_up_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream, 3 );
-            }
+            launch_up_tree( ctx->h_level_ptr, ctx->level_nodes.get(), ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream, 3 );
             launch_pl_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->pl_ff_vals.get(), ctx->pl_ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
             launch_leaf_out3( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
         }
@@ -1049,11 +1045,7 @@ int rp_pc_learn_step( rp_ctx * ctx, float lr, int step, float * params, float * 
         ctx->lp_gp.require( (size_t)ctx->T ); ctx->lp_gc.require( (size_t)ctx->C );
         launch_grad_prim( ctx->T, ctx->size, ctx->pre_cols.get(), ctx->vp_ptr.get(), gacc1, ctx->lp_gp.get(), s );
         launch_up_leaves( ctx->L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->lp_gp.get(), ctx->lp_gc.get(), s, 1 );
-        for( size_t d = 0; d + 1 < ctx->h_level_ptr.size(); ++d )
-        {
-            const int64_t b = ctx->h_level_ptr[d], en = ctx->h_level_ptr[d + 1];
-            launch_up_level( en - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), ctx->lp_gc.get(), s, 1 );
-        }
+        launch_up_tree( ctx->h_level_ptr, ctx->level_nodes.get(), ctx->C_left.get(), ctx->C_right.get(), ctx->lp_gc.get(), s, 1 );
         launch_scale1( ctx->C, inv, ctx->lp_gc.get(), s );                                        // mean over samples
         ctx->lp_part.require( (size_t)( mlp_backward_blocks( ctx->C ) * NP ) );
         ctx->lp_grad.require( (size_t)( NP + 1 ) );
