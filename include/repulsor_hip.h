@@ -240,8 +240,9 @@ int rp_multiply_preconditioner( rp_ctx * ctx, double alpha, const double * X, in
                                 int nrhs, int ptr_flags );
 
 /* Preconditioner used by rp_multiply_preconditioner and rp_solve: P = P0 + w diag(A)^-1.
- * w = 0 (default): the reference preconditioner P0 alone. w > 0: fixed weight. w < 0: w is estimated
- * once per metric/factor state as lambda_max(P0 A) / 4 (power iteration, timed as slot 10).
+ * w = 0 (default): the reference preconditioner P0 alone. w > 0: fixed weight. w < 0: |w| times an
+ * estimate lambda_max(P0 A) / 4 made once per metric/factor state (power iteration, timed as slot 10);
+ * w = -1 is the estimate itself.
  * The diagonal term is an addition of this library: P0 under-weights the roughest modes, so the
  * condition number of P0 A grows under refinement; the additive diagonal (Jacobi) term bounds it.
  * No reference counterpart. */

@@ -185,9 +185,12 @@ namespace rp
     // setting if positive, else the estimate) by exp(learned log w).
     static void update_weight( rp_ctx * ctx )
     {
+        // setting > 0: fixed weight; setting < 0: |setting| times the estimate; 0: P0 alone (a learned model
+        // then starts from the estimate)
+        const double set = ctx->pc_weight_setting;
         double w;
-        if( ctx->lp_enabled ) { w = ( ctx->pc_weight_setting > 0.0 ? ctx->pc_weight_setting : estimated_weight( ctx ) ) * std::exp( ctx->lp_log_w ); }
-        else { w = ctx->pc_weight_setting >= 0.0 ? ctx->pc_weight_setting : estimated_weight( ctx ); }
+        if( ctx->lp_enabled ) { w = ( set > 0.0 ? set : ( set < 0.0 ? -set : 1.0 ) * estimated_weight( ctx ) ) * std::exp( ctx->lp_log_w ); }
+        else { w = set >= 0.0 ? set : -set * estimated_weight( ctx ); }
         ctx->pc_weight = w;
         if( w > 0.0 ) { ensure_metric_diag( ctx ); }
     }
