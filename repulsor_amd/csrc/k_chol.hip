@@ -44,17 +44,15 @@ namespace rp
     // ---------------------------------------------------------------------------------------------
     // M = L_off inv(L_ss) per supernode (tasks: supernode, first update row); the top block is copied.
     // One lane per update row, columns split over the 4 waves.
-    __global__ __launch_bounds__(256) void k_chol_form_m( CholDev F, const int2 * tasks, const double * Lin, double * Lout )
+    __global__ __launch_bounds__(256) void k_chol_form_m( const CholTask * tasks, const double * Lin, double * Lout )
     {
-        const int2 tk = tasks[blockIdx.x];
-        const int s = tk.x;
-        const int w = F.sn_begin[s + 1] - F.sn_begin[s];
-        const int r = (int)( F.row_ptr[s + 1] - F.row_ptr[s] );
-        const int64_t vb = F.val_ptr[s];
+        const CholTask tk = tasks[blockIdx.x];
+        const int w = tk.w, r = tk.r;
+        const int64_t vb = tk.vp;
         const double * Li = Lin + vb;
         double * Lo = Lout + vb;
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-        if( tk.y == 0 )
+        if( tk.k == 0 )
         {
             // copy inv(L_ss) (the whole top w x w block, column by column)
             for( int64_t e = threadIdx.x; e < (int64_t)w * w; e += 256 )
@@ -63,7 +61,7 @@ namespace rp
                 Lo[(int64_t)c * r + a] = Li[(int64_t)c * r + a];
             }
         }
-        const int q = w + tk.y + lane;   // front row
+        const int q = w + tk.k + lane;   // front row
         if( q >= r ) { return; }
         const int cw = ( w + 3 ) / 4;
         const int cb = wv * cw, ce = min( w, cb + cw );
@@ -106,18 +104,16 @@ namespace rp
     // Forward, levels with r <= G: a group of G lanes per supernode (64 / G supernodes per wavefront),
     // a lane per front row.
     template<int G>
-    __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const int32_t * nodes, int64_t count )
+    __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const CholTask * desc, int64_t count )
     {
         __shared__ double fr_all[3 * 256];
         const int lane = threadIdx.x % G;
         const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
         double * fr = fr_all + 3 * ( threadIdx.x - lane );
-        const int s = nodes[t];
-        const int f = F.sn_begin[s];
-        const int w = F.sn_begin[s + 1] - f;
-        const int64_t rb = F.row_ptr[s];
-        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const CholTask d = desc[t];
+        const int f = d.f, w = d.w, r = d.r;
+        const int64_t rb = d.rb;
         double v[3] = { 0.0, 0.0, 0.0 };
         if( lane < r ) { chol_front_row( F, rb, f, w, lane, v ); }
         for( int k = 0; k < 3; ++k ) { fr[3 * lane + k] = v[k]; }
@@ -125,7 +121,7 @@ namespace rp
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
         if( lane >= r ) { return; }
-        const double * Ls = F.vals + F.val_ptr[s];
+        const double * Ls = F.vals + d.vp;
         const int a = lane;
         const int ce = ( a < w ) ? a + 1 : w;
         double acc[3] = { 0.0, 0.0, 0.0 };
@@ -143,17 +139,14 @@ namespace rp
     // Forward, larger levels: one workgroup of NW waves per task (supernode, 64 front rows); the waves
     // split the columns and add their partial sums in LDS.
     template<int NW>
-    __global__ __launch_bounds__(64 * NW) void k_chol_fwd_big( CholDev F, const int2 * tasks )
+    __global__ __launch_bounds__(64 * NW) void k_chol_fwd_big( CholDev F, const CholTask * tasks )
     {
         constexpr int NT = 64 * NW;
         __shared__ double zt[3 * CHOL_MAX_WIDTH];
         __shared__ double part[NW * 64 * 3];
-        const int2 tk = tasks[blockIdx.x];
-        const int s = tk.x, row0 = tk.y;
-        const int f = F.sn_begin[s];
-        const int w = F.sn_begin[s + 1] - f;
-        const int64_t rb = F.row_ptr[s];
-        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const CholTask tk = tasks[blockIdx.x];
+        const int row0 = tk.k, f = tk.f, w = tk.w, r = tk.r;
+        const int64_t rb = tk.rb;
         const int tid = threadIdx.x;
         for( int a = tid; a < w; a += NT )
         {
@@ -166,7 +159,7 @@ namespace rp
         double vb[3] = { 0.0, 0.0, 0.0 };
         if( wv == 0 && a >= w && a < r ) { chol_front_row( F, rb, f, w, a, vb ); }
         __syncthreads();
-        const double * Ls = F.vals + F.val_ptr[s];
+        const double * Ls = F.vals + tk.vp;
         const int cw = ( w + NW - 1 ) / NW;
         double acc[3] = { 0.0, 0.0, 0.0 };
         if( a < r )
@@ -199,16 +192,14 @@ namespace rp
     // Backward, levels with r <= G: a group of G lanes per supernode, a lane per row of v; the column
     // sums are group reductions.
     template<int G>
-    __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const int32_t * nodes, int64_t count )
+    __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count )
     {
         const int lane = threadIdx.x % G;
         const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
-        const int s = nodes[t];
-        const int f = F.sn_begin[s];
-        const int w = F.sn_begin[s + 1] - f;
-        const int64_t rb = F.row_ptr[s];
-        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const CholTask d = desc[t];
+        const int f = d.f, w = d.w, r = d.r;
+        const int64_t rb = d.rb;
         double v[3] = { 0.0, 0.0, 0.0 };
         if( lane < w ) { for( int k = 0; k < 3; ++k ) { v[k] = F.z[3 * ( rb + lane ) + k]; } }
         else if( lane < r )
@@ -216,7 +207,7 @@ namespace rp
             const int64_t j = F.rows[rb + lane];
             for( int k = 0; k < 3; ++k ) { v[k] = -F.xp[3 * j + k]; }
         }
-        const double * Ls = F.vals + F.val_ptr[s];
+        const double * Ls = F.vals + d.vp;
         double mine[3] = { 0.0, 0.0, 0.0 };
         for( int c = 0; c < w; ++c )
         {
@@ -230,23 +221,20 @@ namespace rp
 
     // Backward, larger levels: one workgroup per task (supernode, group of columns), a group of G lanes
     // per column; v staged in LDS in chunks of CHOL_QC rows.
-    __global__ __launch_bounds__(256) void k_chol_bwd_big( CholDev F, const int2 * tasks )
+    __global__ __launch_bounds__(256) void k_chol_bwd_big( CholDev F, const CholTask * tasks )
     {
         __shared__ double v[3 * CHOL_QC];
-        const int2 tk = tasks[blockIdx.x];
-        const int s = tk.x;
-        const int f = F.sn_begin[s];
-        const int w = F.sn_begin[s + 1] - f;
-        const int64_t rb = F.row_ptr[s];
-        const int r = (int)( F.row_ptr[s + 1] - rb );
+        const CholTask tk = tasks[blockIdx.x];
+        const int f = tk.f, w = tk.w, r = tk.r;
+        const int64_t rb = tk.rb;
         const double * z = F.z + 3 * rb;
         const int32_t * R = F.rows + rb;
         const int G = chol_pow2_at_least( r < 64 ? r : 64 );
         const int grp = threadIdx.x / G, gl = threadIdx.x % G;
-        const int c = tk.y + grp;
-        const double * col = F.vals + F.val_ptr[s] + (int64_t)c * r;
+        const int c = tk.k + grp;
+        const double * col = F.vals + tk.vp + (int64_t)c * r;
         double acc[3] = { 0.0, 0.0, 0.0 };
-        for( int i0 = tk.y; i0 < r; i0 += CHOL_QC )
+        for( int i0 = tk.k; i0 < r; i0 += CHOL_QC )
         {
             const int in = min( CHOL_QC, r - i0 );
             __syncthreads();
@@ -288,10 +276,10 @@ namespace rp
         }
     }
 
-    void launch_chol_form_m( const CholDev & F, const int2 * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s )
+    void launch_chol_form_m( const CholTask * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        hipLaunchKernelGGL( k_chol_form_m, dim3( (unsigned)count ), dim3( 256 ), 0, s, F, tasks, Lin, Lout );
+        hipLaunchKernelGGL( k_chol_form_m, dim3( (unsigned)count ), dim3( 256 ), 0, s, tasks, Lin, Lout );
     }
 
     void launch_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp, hipStream_t s )
@@ -300,7 +288,7 @@ namespace rp
         hipLaunchKernelGGL( k_chol_gather, dim3( (unsigned)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, n, perm, alpha, B, ldB, bp );
     }
 
-    void launch_chol_fwd_small( const CholDev & F, const int32_t * nodes, int64_t count, int rmax, hipStream_t s )
+    void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
         if( rmax <= 16 )      { hipLaunchKernelGGL( k_chol_fwd_small<16>, dim3( (unsigned)( ( count + 15 ) / 16 ) ), dim3( 256 ), 0, s, F, nodes, count ); }
@@ -308,14 +296,14 @@ namespace rp
         else                  { hipLaunchKernelGGL( k_chol_fwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
     }
 
-    void launch_chol_fwd_big( const CholDev & F, const int2 * tasks, int64_t count, bool wide, hipStream_t s )
+    void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s )
     {
         if( count <= 0 ) { return; }
         if( wide ) { hipLaunchKernelGGL( (k_chol_fwd_big<16>), dim3( (unsigned)count ), dim3( 1024 ), 0, s, F, tasks ); }
         else       { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
     }
 
-    void launch_chol_bwd_small( const CholDev & F, const int32_t * nodes, int64_t count, int rmax, hipStream_t s )
+    void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
         if( rmax <= 16 )      { hipLaunchKernelGGL( k_chol_bwd_small<16>, dim3( (unsigned)( ( count + 15 ) / 16 ) ), dim3( 256 ), 0, s, F, nodes, count ); }
@@ -323,7 +311,7 @@ namespace rp
         else                  { hipLaunchKernelGGL( k_chol_bwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
     }
 
-    void launch_chol_bwd_big( const CholDev & F, const int2 * tasks, int64_t count, hipStream_t s )
+    void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, hipStream_t s )
     {
         if( count <= 0 ) { return; }
         hipLaunchKernelGGL( k_chol_bwd_big, dim3( (unsigned)count ), dim3( 256 ), 0, s, F, tasks );

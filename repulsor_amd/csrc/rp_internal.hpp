@@ -113,6 +113,14 @@ namespace rp
         double ms[16] = {};
     };
 
+    // One supernode's block of the H^1 factor, or one task on it: first column f, width w, rows r, first
+    // front row (or column) k of the task, offsets of its row set and of its values.
+    struct CholTask
+    {
+        int32_t f, w, r, k;
+        int64_t rb, vp;
+    };
+
     // Cluster tree and block cluster tree built on the device (k_build.hip), kept until downloaded.
     struct BuiltTopology
     {
@@ -260,7 +268,8 @@ struct rp_ctx
     std::vector<int>     h1_level_wmax;
     std::vector<int>     h1_level_rmax;
     std::vector<char>    h1_level_small;           // levels run a wavefront per supernode
-    rp::DBuf<int2>       h1_tasks;                 // forward, backward and M-forming tasks
+    rp::DBuf<rp::CholTask> h1_tasks;               // forward, backward and M-forming tasks
+    rp::DBuf<rp::CholTask> h1_sdesc;               // per supernode, in level order
     std::vector<int64_t> h1_task_ptr[3];           // per kind (forward, backward, forming M): level offsets into h1_tasks
     int64_t h1_n = 0, h1_ns = 0;
     bool have_h1 = false;

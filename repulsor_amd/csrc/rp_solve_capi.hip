@@ -65,14 +65,14 @@ namespace rp
     {
         const CholDev F = chol_dev( ctx );
         const size_t nl = ctx->h1_level_small.size();
-        const int2 * tasks = ctx->h1_tasks.get();
+        const CholTask * tasks = ctx->h1_tasks.get();
         launch_chol_gather( ctx->h1_n, ctx->h1_perm.get(), alpha, Bd, ldb, ctx->h1_bp.get(), ctx->stream );
         for( size_t l = 0; l < nl; ++l )
         {
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_fwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
+                launch_chol_fwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
             }
             else
             {
@@ -85,7 +85,7 @@ namespace rp
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_bwd_small( F, ctx->h1_level_nodes.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
+                launch_chol_bwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
             }
             else
             {
@@ -716,12 +716,21 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         // columns as groups of G = min(64, r rounded up to a power of two) lanes fit in 256 threads).
         // A third list, (supernode, 64 update rows) over all supernodes, forms M = L_off inv(L_ss).
         std::vector<char> small( (size_t)nl, 0 );
-        std::vector<int2> tasks;
+        std::vector<CholTask> tasks;
         std::vector<int64_t> tptr[3];
         for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
         auto group_of = []( int64_t len ) { int g = 1; while( g < len && g < 64 ) { g <<= 1; } return g; };
+        auto task = [&]( int32_t s, int32_t k )
         {
-            std::vector<int2> tk[3];
+            CholTask t;
+            t.f = sn_begin[s]; t.w = sn_begin[s + 1] - sn_begin[s]; t.r = (int32_t)( row_ptr[s + 1] - row_ptr[s] ); t.k = k;
+            t.rb = row_ptr[s]; t.vp = val_ptr[s];
+            return t;
+        };
+        std::vector<CholTask> sdesc( (size_t)ns );
+        for( size_t k = 0; k < lnodes.size(); ++k ) { sdesc[k] = task( lnodes[k], 0 ); }
+        {
+            std::vector<CholTask> tk[3];
             for( int32_t l = 0; l < nl; ++l )
             {
                 small[(size_t)l] = rmax[(size_t)l] <= CHOL_SMALL_ROWS;
@@ -730,11 +739,11 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                     const int32_t s = lnodes[(size_t)k];
                     const int32_t w = sn_begin[s + 1] - sn_begin[s];
                     const int32_t r = (int32_t)( row_ptr[s + 1] - row_ptr[s] );
-                    for( int32_t q0 = 0; q0 < std::max( r - w, 1 ); q0 += 64 ) { tk[2].push_back( make_int2( s, q0 ) ); }
+                    for( int32_t q0 = 0; q0 < std::max( r - w, 1 ); q0 += 64 ) { tk[2].push_back( task( s, q0 ) ); }
                     if( small[(size_t)l] ) { continue; }
-                    for( int32_t a0 = 0; a0 < r; a0 += CHOL_FWD_ROWS ) { tk[0].push_back( make_int2( s, a0 ) ); }
+                    for( int32_t a0 = 0; a0 < r; a0 += CHOL_FWD_ROWS ) { tk[0].push_back( task( s, a0 ) ); }
                     const int per = 256 / group_of( r );
-                    for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( make_int2( s, c0 ) ); }
+                    for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( task( s, c0 ) ); }
                 }
                 for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
             }
@@ -747,6 +756,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
             }
         }
         if( !tasks.empty() ) { ctx->h1_tasks.upload( tasks.data(), tasks.size(), ctx->stream ); }
+        ctx->h1_sdesc.upload( sdesc.data(), sdesc.size(), ctx->stream );
         ctx->h1_sn_begin.upload( sn_begin, (size_t)ns + 1, ctx->stream );
         ctx->h1_row_ptr.upload( row_ptr, (size_t)ns + 1, ctx->stream );
         ctx->h1_rows.upload( rows, (size_t)nrows, ctx->stream );
@@ -756,10 +766,8 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
             rp::DBuf<double> lin;
             lin.upload( vals, (size_t)val_ptr[ns], ctx->stream );
             ctx->h1_vals.require( (size_t)val_ptr[ns] );
-            CholDev F{};
-            F.sn_begin = ctx->h1_sn_begin.get(); F.row_ptr = ctx->h1_row_ptr.get(); F.val_ptr = ctx->h1_val_ptr.get();
             const int64_t mb = tptr[2][0], me = tptr[2][(size_t)nl];
-            launch_chol_form_m( F, ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
+            launch_chol_form_m( ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
             RP_HIP( hipGetLastError() );
             RP_HIP( hipStreamSynchronize( ctx->stream ) );
         }
