@@ -257,16 +257,22 @@ namespace rp
         if( I < 0 ) { return; }
         const int ib = leaf_begin[I];
         const int nI = leaf_end[I] - ib;
-        const int nS = 64 / nI;
-        const int ii = lane % nI;
-        const int c  = lane / nI;
-        const bool active = lane < nS * nI;
+        // a lane takes two rows of the leaf, p and p + h, for one column of the chunk at a time: every
+        // column block read from LDS serves two pairs (the LDS reads, not the HBM stream, bound the
+        // one-row form)
+        const int h = ( nI + 1 ) >> 1;
+        const int nS = 64 / h;
+        const int p = lane % h;
+        const int c = lane / h;
+        const bool active = lane < nS * h;
+        const int r1 = p + h;
+        const bool two = r1 < nI;
         const int64_t cb = col_ptr[I];
         const int64_t M_ = col_ptr[I + 1] - cb;
         const int64_t vb = val_ptr[I];
-        double acc[12];
+        double acc0[12], acc1[12];
         #pragma unroll
-        for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
+        for( int k = 0; k < 12; ++k ) { acc0[k] = 0.0; acc1[k] = 0.0; }
         // lane l loads half of the input block of column l mod 32 of the chunk (doubles 6 (l / 32) ..)
         double rx[6];
         int32_t rj = 0;
@@ -281,7 +287,7 @@ namespace rp
                 rm = col_mask[cb + m];
                 const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj + 6 * half );
                 #pragma unroll
-                for( int h = 0; h < 3; ++h ) { const double2 t = xj[h]; rx[2 * h] = t.x; rx[2 * h + 1] = t.y; }
+                for( int q = 0; q < 3; ++q ) { const double2 t = xj[q]; rx[2 * q] = t.x; rx[2 * q + 1] = t.y; }
             }
             else { rm = 0u; }
         };
@@ -302,38 +308,38 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {
-                // four value pairs in flight before the (masked) products; every tile slot exists, masked
-                // pairs add 0, the products are summed in column order as one at a time
-                const VT * vrow = vals + vb + base * nI + ii;
+                // the value pairs of two columns (four slots) are loaded before the (masked) products; every
+                // tile slot exists, masked pairs add 0, each row sums its products in column order
+                const VT * vrow = vals + vb + base * nI + p;
                 int mm = c;
-                for( ; mm + 3 * nS < cnt_c; mm += 4 * nS )
+                for( ; mm < cnt_c; mm += 2 * nS )
                 {
-                    VT av4[4];
-                    #pragma unroll
-                    for( int u = 0; u < 4; ++u ) { av4[u] = vrow[(int64_t)( mm + u * nS ) * nI]; }
-                    #pragma unroll
-                    for( int u = 0; u < 4; ++u )
+                    const bool sec = mm + nS < cnt_c;
+                    VT a00 = {}, a01 = {}, a10 = {}, a11 = {};   // [column u][row]
+                    a00 = vrow[(int64_t)mm * nI];
+                    if( two ) { a01 = vrow[(int64_t)mm * nI + h]; }
+                    if( sec )
                     {
-                        const int m2 = mm + u * nS;
-                        const VT av = av4[u];
-                        const bool on = ( sm[buf][m2] >> ii ) & 1u;
-                        const double a0 = on ? (double)av.x : 0.0, a1 = on ? (double)av.y : 0.0;
-                        #pragma unroll
-                        for( int k = 0; k < 3; ++k ) { acc[k] += a0 * sx[buf][k][m2]; }
-                        #pragma unroll
-                        for( int k = 3; k < 12; ++k ) { acc[k] += a1 * sx[buf][k][m2]; }
+                        a10 = vrow[(int64_t)( mm + nS ) * nI];
+                        if( two ) { a11 = vrow[(int64_t)( mm + nS ) * nI + h]; }
                     }
-                }
-                for( ; mm < cnt_c; mm += nS )
-                {
-                    if( (sm[buf][mm] >> ii) & 1u )
+                    #pragma unroll
+                    for( int u = 0; u < 2; ++u )
                     {
-                        const VT av = vrow[(int64_t)mm * nI];
-                        const double a0 = (double)av.x, a1 = (double)av.y;
+                        if( u == 1 && !sec ) { break; }
+                        const int m2 = mm + u * nS;
+                        const uint32_t msk = sm[buf][m2];
+                        const VT va = u ? a10 : a00, vb2 = u ? a11 : a01;
+                        const bool on0 = ( msk >> p ) & 1u, on1 = two && ( ( msk >> r1 ) & 1u );
+                        const double b0 = on0 ? (double)va.x : 0.0,  b1 = on0 ? (double)va.y : 0.0;
+                        const double e0 = on1 ? (double)vb2.x : 0.0, e1 = on1 ? (double)vb2.y : 0.0;
                         #pragma unroll
-                        for( int k = 0; k < 3; ++k ) { acc[k] += a0 * sx[buf][k][mm]; }
-                        #pragma unroll
-                        for( int k = 3; k < 12; ++k ) { acc[k] += a1 * sx[buf][k][mm]; }
+                        for( int k = 0; k < 12; ++k )
+                        {
+                            const double x = sx[buf][k][m2];
+                            acc0[k] += ( k < 3 ? b0 : b1 ) * x;
+                            acc1[k] += ( k < 3 ? e0 : e1 ) * x;
+                        }
                     }
                 }
             }
@@ -341,19 +347,25 @@ namespace rp
             __syncthreads();
             buf ^= 1;
         }
-        // row sums over the lanes ii, ii + nI, ... in that order (every lane takes part in the shuffles)
+        // row sums over the column groups of each row pair, in group order (every lane takes part in the
+        // shuffles); lane q < nI then holds row q: its own first row if q < h, else the second row of q - h
         double s[12];
         #pragma unroll
-        for( int k = 0; k < 12; ++k ) { s[k] = 0.0; }
-        for( int cc = 0; cc < nS; ++cc )
+        for( int k = 0; k < 12; ++k )
         {
-            const int src = ( lane % nI ) + cc * nI;
-            #pragma unroll
-            for( int k = 0; k < 12; ++k ) { s[k] += __shfl( acc[k], src, 64 ); }
+            double s0 = 0.0, s1 = 0.0;
+            for( int cc = 0; cc < nS; ++cc )
+            {
+                const int src = p + cc * h;
+                s0 += __shfl( acc0[k], src, 64 );
+                s1 += __shfl( acc1[k], src, 64 );
+            }
+            const double t = __shfl( s1, ( lane >= h ) ? lane - h : lane, 64 );
+            s[k] = ( lane < h ) ? s0 : t;
         }
         if( lane < nI )
         {
-            const int64_t i = ib + ii;
+            const int64_t i = ib + lane;
             const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
             const double * xi = xin + 12 * i;
             #pragma unroll
