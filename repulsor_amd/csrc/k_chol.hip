@@ -25,7 +25,9 @@
 
 namespace rp
 {
-    constexpr int CHOL_QC = 512;   // rows of v staged in LDS per chunk of the backward tasks
+    constexpr int CHOL_QC = 512;   // rows of v staged in LDS per chunk of the backward tasks (rows beyond CHOL_BWD_ONE)
+    constexpr int CHOL_BWD_ONE = 2560;   // backward tasks of supernodes with at most this many rows stage v at once
+    constexpr int CHOL_PF = 32;    // block values a lane loads ahead of the front merge (forward) or the staging of v (backward)
 
     __device__ __forceinline__ int chol_pow2_at_least( int x )
     {
@@ -148,30 +150,37 @@ namespace rp
         const int row0 = tk.k, f = tk.f, w = tk.w, r = tk.r;
         const int64_t rb = tk.rb;
         const int tid = threadIdx.x;
-        for( int a = tid; a < w; a += NT )
-        {
-            double v[3];
-            chol_front_row( F, rb, f, w, a, v );
-            for( int k = 0; k < 3; ++k ) { zt[3 * a + k] = v[k]; }
-        }
         const int lane = tid & 63, wv = tid >> 6;
         const int a = row0 + lane;
+        // the lane's block values do not depend on the front: their loads go out before the merge
+        const double * Ls = F.vals + tk.vp;
+        const int cw = ( w + NW - 1 ) / NW;
+        const int cb = wv * cw;
+        const int ce = ( a < r ) ? min( ( a < w ) ? a + 1 : w, cb + cw ) : cb;
+        constexpr int PF = ( NW == 16 ) ? CHOL_PF : 1;   // prefetch on the wide (latency-bound) levels only
+        double lp[PF];
+        #pragma unroll
+        for( int u = 0; u < PF; ++u ) { lp[u] = ( cb + u < ce ) ? Ls[(int64_t)( cb + u ) * r + a] : 0.0; }
+        for( int a2 = tid; a2 < w; a2 += NT )
+        {
+            double v[3];
+            chol_front_row( F, rb, f, w, a2, v );
+            for( int k = 0; k < 3; ++k ) { zt[3 * a2 + k] = v[k]; }
+        }
         double vb[3] = { 0.0, 0.0, 0.0 };
         if( wv == 0 && a >= w && a < r ) { chol_front_row( F, rb, f, w, a, vb ); }
         __syncthreads();
-        const double * Ls = F.vals + tk.vp;
-        const int cw = ( w + NW - 1 ) / NW;
         double acc[3] = { 0.0, 0.0, 0.0 };
-        if( a < r )
+        #pragma unroll
+        for( int u = 0; u < PF; ++u )
         {
-            const int cb = wv * cw;
-            const int ce = min( ( a < w ) ? a + 1 : w, cb + cw );
-            #pragma unroll 8
-            for( int c = cb; c < ce; ++c )
-            {
-                const double l = Ls[(int64_t)c * r + a];
-                for( int k = 0; k < 3; ++k ) { acc[k] += l * zt[3 * c + k]; }
-            }
+            if( cb + u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * zt[3 * ( cb + u ) + k]; } }
+        }
+        #pragma unroll 8
+        for( int c = cb + PF; c < ce; ++c )
+        {
+            const double l = Ls[(int64_t)c * r + a];
+            for( int k = 0; k < 3; ++k ) { acc[k] += l * zt[3 * c + k]; }
         }
         for( int k = 0; k < 3; ++k ) { part[( wv * 64 + lane ) * 3 + k] = acc[k]; }
         __syncthreads();
@@ -220,10 +229,13 @@ namespace rp
     }
 
     // Backward, larger levels: one workgroup per task (supernode, group of columns), a group of G lanes
-    // per column; v staged in LDS in chunks of CHOL_QC rows.
+    // per column. On the wide levels (few tasks, latency-bound) supernodes with at most CHOL_BWD_ONE rows
+    // stage all of v in LDS (sized per launch) and load the lane's block values before staging; otherwise v
+    // is staged in chunks of CHOL_QC rows.
+    template<bool WIDE>
     __global__ __launch_bounds__(256) void k_chol_bwd_big( CholDev F, const CholTask * tasks )
     {
-        __shared__ double v[3 * CHOL_QC];
+        extern __shared__ double v[];
         const CholTask tk = tasks[blockIdx.x];
         const int f = tk.f, w = tk.w, r = tk.r;
         const int64_t rb = tk.rb;
@@ -234,29 +246,66 @@ namespace rp
         const int c = tk.k + grp;
         const double * col = F.vals + tk.vp + (int64_t)c * r;
         double acc[3] = { 0.0, 0.0, 0.0 };
-        for( int i0 = tk.k; i0 < r; i0 += CHOL_QC )
+        if( WIDE && r <= CHOL_BWD_ONE )
         {
-            const int in = min( CHOL_QC, r - i0 );
-            __syncthreads();
-            for( int i = threadIdx.x; i < in; i += 256 )
+            const int i0 = tk.k;          // no column of the task needs rows above its first column
+            const int ib = ( c < w ) ? c : r;
+            double lp[CHOL_PF];
+            #pragma unroll
+            for( int u = 0; u < CHOL_PF; ++u )
             {
-                const int ii = i0 + i;
-                if( ii < w ) { for( int k = 0; k < 3; ++k ) { v[3 * i + k] = z[3 * ii + k]; } }
+                const int i = ib + gl + u * G;
+                lp[u] = ( i < r ) ? col[i] : 0.0;
+            }
+            for( int i = i0 + threadIdx.x; i < r; i += 256 )
+            {
+                const int q = i - i0;
+                if( i < w ) { for( int k = 0; k < 3; ++k ) { v[3 * q + k] = z[3 * i + k]; } }
                 else
                 {
-                    const int64_t j = R[ii];
-                    for( int k = 0; k < 3; ++k ) { v[3 * i + k] = -F.xp[3 * j + k]; }
+                    const int64_t j = R[i];
+                    for( int k = 0; k < 3; ++k ) { v[3 * q + k] = -F.xp[3 * j + k]; }
                 }
             }
             __syncthreads();
-            if( c < w )
+            #pragma unroll
+            for( int u = 0; u < CHOL_PF; ++u )
             {
-                const int ib = max( c, i0 );
-                #pragma unroll 8
-                for( int i = ib + gl; i < i0 + in; i += G )
+                const int i = ib + gl + u * G;
+                if( i < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * v[3 * ( i - i0 ) + k]; } }
+            }
+            for( int i = ib + gl + CHOL_PF * G; i < r; i += G )
+            {
+                const double l = col[i];
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * v[3 * ( i - i0 ) + k]; }
+            }
+        }
+        else
+        {
+            for( int i0 = tk.k; i0 < r; i0 += CHOL_QC )
+            {
+                const int in = min( CHOL_QC, r - i0 );
+                __syncthreads();
+                for( int i = threadIdx.x; i < in; i += 256 )
                 {
-                    const double l = col[i];
-                    for( int k = 0; k < 3; ++k ) { acc[k] += l * v[3 * ( i - i0 ) + k]; }
+                    const int ii = i0 + i;
+                    if( ii < w ) { for( int k = 0; k < 3; ++k ) { v[3 * i + k] = z[3 * ii + k]; } }
+                    else
+                    {
+                        const int64_t j = R[ii];
+                        for( int k = 0; k < 3; ++k ) { v[3 * i + k] = -F.xp[3 * j + k]; }
+                    }
+                }
+                __syncthreads();
+                if( c < w )
+                {
+                    const int ib = max( c, i0 );
+                    #pragma unroll 8
+                    for( int i = ib + gl; i < i0 + in; i += G )
+                    {
+                        const double l = col[i];
+                        for( int k = 0; k < 3; ++k ) { acc[k] += l * v[3 * ( i - i0 ) + k]; }
+                    }
                 }
             }
         }
@@ -311,10 +360,15 @@ namespace rp
         else                  { hipLaunchKernelGGL( k_chol_bwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
     }
 
-    void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, hipStream_t s )
+    void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        hipLaunchKernelGGL( k_chol_bwd_big, dim3( (unsigned)count ), dim3( 256 ), 0, s, F, tasks );
+        if( wide )
+        {
+            const size_t shm = sizeof(double) * 3 * (size_t)( rmax <= CHOL_BWD_ONE ? rmax : CHOL_QC );
+            hipLaunchKernelGGL( k_chol_bwd_big<true>, dim3( (unsigned)count ), dim3( 256 ), shm, s, F, tasks );
+        }
+        else { hipLaunchKernelGGL( k_chol_bwd_big<false>, dim3( (unsigned)count ), dim3( 256 ), sizeof(double) * 3 * CHOL_QC, s, F, tasks ); }
     }
 
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s )

@@ -107,7 +107,7 @@ namespace rp
     void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s );
     void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s );
     void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s );
-    void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, hipStream_t s );
+    void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s );
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
 
     // Krylov vector kernels (k_solve.hip)

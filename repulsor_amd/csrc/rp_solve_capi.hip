@@ -90,7 +90,7 @@ namespace rp
             else
             {
                 const int64_t tb = ctx->h1_task_ptr[1][l], te = ctx->h1_task_ptr[1][l + 1];
-                launch_chol_bwd_big( F, tasks + tb, te - tb, ctx->stream );
+                launch_chol_bwd_big( F, tasks + tb, te - tb, ctx->h1_level_rmax[l], ctx->h1_level_wmax[l] > 128, ctx->stream );
             }
         }
         launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream );
