@@ -116,6 +116,14 @@ namespace rp
         const CholTask d = desc[t];
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
+        const double * Ls = F.vals + d.vp;
+        const int a = lane;
+        const int ce = ( a < r ) ? ( ( a < w ) ? a + 1 : w ) : 0;
+        // the lane's first block values go out before the front merge
+        constexpr int PFS = 16;
+        double lp[PFS];
+        #pragma unroll
+        for( int u = 0; u < PFS; ++u ) { lp[u] = ( u < ce ) ? Ls[(int64_t)u * r + a] : 0.0; }
         double v[3] = { 0.0, 0.0, 0.0 };
         if( lane < r ) { chol_front_row( F, rb, f, w, lane, v ); }
         for( int k = 0; k < 3; ++k ) { fr[3 * lane + k] = v[k]; }
@@ -123,14 +131,15 @@ namespace rp
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
         if( lane >= r ) { return; }
-        const double * Ls = F.vals + d.vp;
-        const int a = lane;
-        const int ce = ( a < w ) ? a + 1 : w;
         double acc[3] = { 0.0, 0.0, 0.0 };
-        #pragma unroll 8
-        for( int c = 0; c < w; ++c )
+        #pragma unroll
+        for( int u = 0; u < PFS; ++u )
         {
-            const double l = ( c < ce ) ? Ls[(int64_t)c * r + a] : 0.0;
+            if( u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * fr[3 * u + k]; } }
+        }
+        for( int c = PFS; c < ce; ++c )
+        {
+            const double l = Ls[(int64_t)c * r + a];
             for( int k = 0; k < 3; ++k ) { acc[k] += l * fr[3 * c + k]; }
         }
         double * z = F.z + 3 * rb;
@@ -209,6 +218,12 @@ namespace rp
         const CholTask d = desc[t];
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
+        const double * Ls = F.vals + d.vp;
+        // the lane's first block values go out before v
+        constexpr int PFS = 16;
+        double lp[PFS];
+        #pragma unroll
+        for( int u = 0; u < PFS; ++u ) { lp[u] = ( u < w && lane >= u && lane < r ) ? Ls[(int64_t)u * r + lane] : 0.0; }
         double v[3] = { 0.0, 0.0, 0.0 };
         if( lane < w ) { for( int k = 0; k < 3; ++k ) { v[k] = F.z[3 * ( rb + lane ) + k]; } }
         else if( lane < r )
@@ -216,11 +231,13 @@ namespace rp
             const int64_t j = F.rows[rb + lane];
             for( int k = 0; k < 3; ++k ) { v[k] = -F.xp[3 * j + k]; }
         }
-        const double * Ls = F.vals + d.vp;
         double mine[3] = { 0.0, 0.0, 0.0 };
         for( int c = 0; c < w; ++c )
         {
-            const double l = ( lane >= c && lane < r ) ? Ls[(int64_t)c * r + lane] : 0.0;
+            double l = 0.0;
+            #pragma unroll
+            for( int u = 0; u < PFS; ++u ) { l = ( c == u ) ? lp[u] : l; }
+            if( c >= PFS && lane >= c && lane < r ) { l = Ls[(int64_t)c * r + lane]; }
             double p[3];
             for( int k = 0; k < 3; ++k ) { p[k] = group_sum( l * v[k], G ); }
             if( lane == c ) { for( int k = 0; k < 3; ++k ) { mine[k] = p[k]; } }
