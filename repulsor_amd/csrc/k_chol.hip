@@ -207,42 +207,48 @@ namespace rp
     }
 
     // ---------------------------------------------------------------------------------------------
-    // Backward, levels with r <= G: a group of G lanes per supernode, a lane per row of v; the column
-    // sums are group reductions.
+    // Backward, levels with r <= G: a group of G lanes per supernode; the lanes stage v in LDS, then lane
+    // c < w forms x_c as the dot product of block column c (rows c..r-1) with v, its first values loaded
+    // before the staging.
     template<int G>
     __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count )
     {
+        __shared__ double v_all[3 * 256];
         const int lane = threadIdx.x % G;
         const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
+        double * vs = v_all + 3 * ( threadIdx.x - lane );
         const CholTask d = desc[t];
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
-        const double * Ls = F.vals + d.vp;
-        // the lane's first block values go out before v
+        const double * col = F.vals + d.vp + (int64_t)lane * r;
         constexpr int PFS = 16;
         double lp[PFS];
         #pragma unroll
-        for( int u = 0; u < PFS; ++u ) { lp[u] = ( u < w && lane >= u && lane < r ) ? Ls[(int64_t)u * r + lane] : 0.0; }
-        double v[3] = { 0.0, 0.0, 0.0 };
-        if( lane < w ) { for( int k = 0; k < 3; ++k ) { v[k] = F.z[3 * ( rb + lane ) + k]; } }
+        for( int u = 0; u < PFS; ++u ) { lp[u] = ( lane < w && lane + u < r ) ? col[lane + u] : 0.0; }
+        if( lane < w ) { for( int k = 0; k < 3; ++k ) { vs[3 * lane + k] = F.z[3 * ( rb + lane ) + k]; } }
         else if( lane < r )
         {
             const int64_t j = F.rows[rb + lane];
-            for( int k = 0; k < 3; ++k ) { v[k] = -F.xp[3 * j + k]; }
+            for( int k = 0; k < 3; ++k ) { vs[3 * lane + k] = -F.xp[3 * j + k]; }
         }
-        double mine[3] = { 0.0, 0.0, 0.0 };
-        for( int c = 0; c < w; ++c )
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        if( lane >= w ) { return; }
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        #pragma unroll
+        for( int u = 0; u < PFS; ++u )
         {
-            double l = 0.0;
-            #pragma unroll
-            for( int u = 0; u < PFS; ++u ) { l = ( c == u ) ? lp[u] : l; }
-            if( c >= PFS && lane >= c && lane < r ) { l = Ls[(int64_t)c * r + lane]; }
-            double p[3];
-            for( int k = 0; k < 3; ++k ) { p[k] = group_sum( l * v[k], G ); }
-            if( lane == c ) { for( int k = 0; k < 3; ++k ) { mine[k] = p[k]; } }
+            const int i = lane + u;
+            if( i < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * i + k]; } }
         }
-        if( lane < w ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = mine[k]; } }
+        for( int i = lane + PFS; i < r; ++i )
+        {
+            const double l = col[i];
+            for( int k = 0; k < 3; ++k ) { acc[k] += l * vs[3 * i + k]; }
+        }
+        for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = acc[k]; }
     }
 
     // Backward, larger levels: one workgroup per task (supernode, group of columns), a group of G lanes
