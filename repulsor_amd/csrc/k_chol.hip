@@ -166,7 +166,7 @@ namespace rp
         const int cw = ( w + NW - 1 ) / NW;
         const int cb = wv * cw;
         const int ce = ( a < r ) ? min( ( a < w ) ? a + 1 : w, cb + cw ) : cb;
-        constexpr int PF = ( NW == 16 ) ? CHOL_PF : 1;   // prefetch on the wide (latency-bound) levels only
+        constexpr int PF = ( NW == 16 ) ? CHOL_PF : 8;   // deep prefetch on the wide (latency-bound) levels
         double lp[PF];
         #pragma unroll
         for( int u = 0; u < PF; ++u ) { lp[u] = ( cb + u < ce ) ? Ls[(int64_t)( cb + u ) * r + a] : 0.0; }
@@ -243,6 +243,7 @@ namespace rp
             const int i = lane + u;
             if( i < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * i + k]; } }
         }
+        #pragma unroll 8
         for( int i = lane + PFS; i < r; ++i )
         {
             const double l = col[i];
