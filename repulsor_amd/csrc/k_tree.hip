@@ -373,10 +373,22 @@ namespace rp
         const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if( v >= V ) { return; }
         double s0 = 0, s1 = 0, s2 = 0;
-        for( int64_t k = vp_ptr[v]; k < vp_ptr[v + 1]; ++k )
+        const int64_t kb = vp_ptr[v], ke = vp_ptr[v + 1];
+        // up to eight entries (a vertex's valence is about six) are gathered at once, then summed in order
+        for( int64_t k0 = kb; k0 < ke; k0 += 8 )
         {
-            const double * so = S_out + 3 * (int64_t)vp_entry[k];
-            s0 += so[0]; s1 += so[1]; s2 += so[2];
+            int32_t e[8];
+            #pragma unroll
+            for( int u = 0; u < 8; ++u ) { e[u] = ( k0 + u < ke ) ? vp_entry[k0 + u] : -1; }
+            double t[8][3];
+            #pragma unroll
+            for( int u = 0; u < 8; ++u )
+            {
+                if( e[u] >= 0 ) { const double * so = S_out + 3 * (int64_t)e[u]; t[u][0] = so[0]; t[u][1] = so[1]; t[u][2] = so[2]; }
+                else { t[u][0] = t[u][1] = t[u][2] = 0.0; }
+            }
+            #pragma unroll
+            for( int u = 0; u < 8; ++u ) { if( e[u] >= 0 ) { s0 += t[u][0]; s1 += t[u][1]; s2 += t[u][2]; } }
         }
         double * y = Y + ldY * v;
         if( beta == 0.0 ) { y[0] = alpha * s0; y[1] = alpha * s1; y[2] = alpha * s2; }
