@@ -208,8 +208,9 @@ namespace rp
 
     // ---------------------------------------------------------------------------------------------
     // Backward, levels with r <= G: a group of G lanes per supernode; the lanes stage v in LDS, then lane
-    // c < w forms x_c as the dot product of block column c (rows c..r-1) with v, its first values loaded
-    // before the staging.
+    // c < w forms x_c = sum over rows i >= c of L(i, c) v_i, reading the transposed copy of the block
+    // (row-major, so the lanes of a row read consecutive addresses), its first rows loaded before the
+    // staging.
     template<int G>
     __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count )
     {
@@ -221,11 +222,11 @@ namespace rp
         const CholTask d = desc[t];
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
-        const double * col = F.vals + d.vp + (int64_t)lane * r;
+        const double * Lt = F.vals_t + d.vp;   // Lt[i * w + c] = L(i, c)
         constexpr int PFS = 16;
         double lp[PFS];
         #pragma unroll
-        for( int u = 0; u < PFS; ++u ) { lp[u] = ( lane < w && lane + u < r ) ? col[lane + u] : 0.0; }
+        for( int u = 0; u < PFS; ++u ) { lp[u] = ( lane < w && u >= lane && u < r ) ? Lt[(int64_t)u * w + lane] : 0.0; }
         if( lane < w ) { for( int k = 0; k < 3; ++k ) { vs[3 * lane + k] = F.z[3 * ( rb + lane ) + k]; } }
         else if( lane < r )
         {
@@ -240,16 +241,29 @@ namespace rp
         #pragma unroll
         for( int u = 0; u < PFS; ++u )
         {
-            const int i = lane + u;
-            if( i < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * i + k]; } }
+            if( u >= lane && u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * u + k]; } }
         }
         #pragma unroll 8
-        for( int i = lane + PFS; i < r; ++i )
+        for( int i = ( lane > PFS ? lane : PFS ); i < r; ++i )
         {
-            const double l = col[i];
+            const double l = Lt[(int64_t)i * w + lane];
             for( int k = 0; k < 3; ++k ) { acc[k] += l * vs[3 * i + k]; }
         }
         for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = acc[k]; }
+    }
+
+    // Transposed (row-major) copy of the blocks of the small levels: Lt[i * w + c] = L(i, c).
+    __global__ __launch_bounds__(256) void k_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t )
+    {
+        const int64_t t = blockIdx.x;
+        if( t >= count ) { return; }
+        const CholTask d = desc[t];
+        const int w = d.w, r = d.r;
+        for( int e = threadIdx.x; e < w * r; e += 256 )
+        {
+            const int i = e / w, c = e - i * w;
+            vals_t[d.vp + e] = vals[d.vp + (int64_t)c * r + i];
+        }
     }
 
     // Backward, larger levels: one workgroup per task (supernode, group of columns), a group of G lanes
@@ -393,6 +407,12 @@ namespace rp
             hipLaunchKernelGGL( k_chol_bwd_big<true>, dim3( (unsigned)count ), dim3( 256 ), shm, s, F, tasks );
         }
         else { hipLaunchKernelGGL( k_chol_bwd_big<false>, dim3( (unsigned)count ), dim3( 256 ), sizeof(double) * 3 * CHOL_QC, s, F, tasks ); }
+    }
+
+    void launch_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t, hipStream_t s )
+    {
+        if( count <= 0 ) { return; }
+        hipLaunchKernelGGL( k_chol_transpose, dim3( (unsigned)count ), dim3( 256 ), 0, s, desc, count, vals, vals_t );
     }
 
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s )

@@ -263,6 +263,7 @@ struct rp_ctx
     rp::DBuf<int32_t> h1_sn_begin, h1_rows, h1_kid_ptr, h1_kids, h1_cmap, h1_perm, h1_level_nodes;
     rp::DBuf<int64_t> h1_row_ptr, h1_val_ptr;
     rp::DBuf<double>  h1_vals, h1_z, h1_xp, h1_bp;  // h1_vals: inv(L_ss) over M = L_off inv(L_ss) per supernode
+    rp::DBuf<double>  h1_vals_t;                     // row-major copies of the small levels' blocks
     rp::DBuf<int32_t> h1_pull_ptr, h1_pull_src;      // per front row: the child update rows that map to it
     std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
     std::vector<int>     h1_level_wmax;

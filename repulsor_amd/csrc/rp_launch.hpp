@@ -89,6 +89,7 @@ namespace rp
         const int32_t * rows;       // row sets, sorted, own columns first
         const int64_t * val_ptr;    // ns + 1
         const double  * vals;       // column-major r x w blocks: inv(L_ss) on top, M = L_off inv(L_ss) below
+        const double  * vals_t;     // row-major copies of the blocks of the small levels (same offsets)
         const int32_t * kid_ptr;    // ns + 1
         const int32_t * kids;       // children in increasing order
         const int32_t * cmap;       // per row entry: position in the parent's row list
@@ -108,6 +109,7 @@ namespace rp
     void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s );
     void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s );
     void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s );
+    void launch_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t, hipStream_t s );
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
 
     // Krylov vector kernels (k_solve.hip)

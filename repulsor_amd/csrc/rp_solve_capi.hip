@@ -53,7 +53,7 @@ namespace rp
     {
         CholDev F{};
         F.sn_begin = ctx->h1_sn_begin.get(); F.row_ptr = ctx->h1_row_ptr.get(); F.rows = ctx->h1_rows.get();
-        F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get();
+        F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get(); F.vals_t = ctx->h1_vals_t.get();
         F.kid_ptr = ctx->h1_kid_ptr.get(); F.kids = ctx->h1_kids.get(); F.cmap = ctx->h1_cmap.get();
         F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get();
         F.pull_ptr = ctx->h1_pull_ptr.get(); F.pull_src = ctx->h1_pull_src.get(); F.bp = ctx->h1_bp.get();
@@ -768,6 +768,20 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
             ctx->h1_vals.require( (size_t)val_ptr[ns] );
             const int64_t mb = tptr[2][0], me = tptr[2][(size_t)nl];
             launch_chol_form_m( ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
+            RP_HIP( hipGetLastError() );
+            // row-major copies of the blocks of every small level (same offsets as in h1_vals)
+            int64_t extent = 1;
+            for( int32_t l = 0; l < nl; ++l )
+            {
+                if( !small[(size_t)l] ) { continue; }
+                for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k ) { extent = std::max<int64_t>( extent, sdesc[(size_t)k].vp + (int64_t)sdesc[(size_t)k].w * sdesc[(size_t)k].r ); }
+            }
+            ctx->h1_vals_t.require( (size_t)extent );
+            for( int32_t l = 0; l < nl; ++l )
+            {
+                if( !small[(size_t)l] ) { continue; }
+                launch_chol_transpose( ctx->h1_sdesc.get() + lptr[(size_t)l], lptr[(size_t)l + 1] - lptr[(size_t)l], ctx->h1_vals.get(), ctx->h1_vals_t.get(), ctx->stream );
+            }
             RP_HIP( hipGetLastError() );
             RP_HIP( hipStreamSynchronize( ctx->stream ) );
         }
