@@ -279,3 +279,20 @@ def test_solve_fp32_preconditioner(name):
     assert err <= 1e-5
     assert np.all(true <= 1.5e-6), true
     assert it <= it64 + 1
+
+
+def test_preconditioner_weight_factor():
+    """SetPreconditionerDiagonalWeight(-f) uses f times the estimated weight (f = 1: the estimate)."""
+    wl, M = _mesh("sphere_1280")
+    E = R.TangentPointEnergy0(wl.q, wl.p)
+    B = E.Differential(M)
+    ws = {}
+    for f in (1.0, 2.5):
+        A = R.TangentPointMetric0(wl.q, wl.p)
+        A.SetPreconditionerDiagonalWeight(-f)
+        X = np.zeros_like(B)
+        A.Solve(M, 1.0, B, 0.0, X, max_iter=500, tolerance=1e-6)
+        ws[f] = A.PreconditionerWeightInUse(M)
+    print(f"weights {ws}", flush=True)
+    assert ws[1.0] > 0
+    assert abs(ws[2.5] - 2.5 * ws[1.0]) <= 1e-12 * ws[2.5]
