@@ -101,17 +101,19 @@ namespace rp
             {
                 rj = A.col_prim[cb + m];
                 rm = A.col_mask[cb + m];
-                rd[0] = (R)A.a[rj];
+                // the column's record (a, x, P): five 16-byte loads from one place
+                const double2 * q = reinterpret_cast<const double2 *>( A.rec + 10 * (int64_t)rj );
+                const double2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+                rd[0] = (R)q0.x;
                 if constexpr ( MIXED )
                 {
-                    rd[1] = (float)(A.x[rj] - o0); rd[2] = (float)(A.x[T + rj] - o1); rd[3] = (float)(A.x[2 * T + rj] - o2);
+                    rd[1] = (float)(q0.y - o0); rd[2] = (float)(q1.x - o1); rd[3] = (float)(q1.y - o2);
                 }
                 else
                 {
-                    rd[1] = A.x[rj]; rd[2] = A.x[T + rj]; rd[3] = A.x[2 * T + rj];
+                    rd[1] = q0.y; rd[2] = q1.x; rd[3] = q1.y;
                 }
-                #pragma unroll
-                for( int k = 0; k < 6; ++k ) { rd[4 + k] = (R)A.P[k * T + rj]; }
+                rd[4] = (R)q2.x; rd[5] = (R)q2.y; rd[6] = (R)q3.x; rd[7] = (R)q3.y; rd[8] = (R)q4.x; rd[9] = (R)q4.y;
             }
             else
             {

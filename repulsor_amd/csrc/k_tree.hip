@@ -17,20 +17,26 @@
 namespace rp
 {
     // ---------------------------------------------------------------- data preparation
-    __global__ void k_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P )
+    // Primitive data in structure-of-arrays form (a, x, P) for the row side of the pair kernels, and
+    // the same ten values as one record per primitive (rec: a, x, P) for their column gathers.
+    __global__ void k_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, double * rec )
     {
         const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if( i >= T ) { return; }
         const double * X = near + (int64_t)nd * i;
         const double s = 1.0 / size;
-        a[i] = X[0];
+        double r[10];
+        r[0] = X[0];
+        a[i] = r[0];
         for( int k = 0; k < 3; ++k )
         {
             double xk = s * X[1 + k];
             for( int h = 1; h < size; ++h ) { xk += s * X[1 + 3 * h + k]; }
             x[k * T + i] = xk;
+            r[1 + k] = xk;
         }
-        for( int k = 0; k < 6; ++k ) { P[k * T + i] = X[1 + 3 * size + k]; }
+        for( int k = 0; k < 6; ++k ) { P[k * T + i] = X[1 + 3 * size + k]; r[4 + k] = X[1 + 3 * size + k]; }
+        for( int k = 0; k < 10; ++k ) { rec[10 * i + k] = r[k]; }
     }
 
     __global__ void k_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P )
@@ -404,8 +410,8 @@ namespace rp
     // ---------------------------------------------------------------- host launchers
     static inline dim3 grid1( int64_t n, int b = 256 ) { return dim3( (unsigned)std::max<int64_t>( 1, (n + b - 1) / b ) ); }
 
-    void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, hipStream_t s )
-    { if( T ) { hipLaunchKernelGGL( k_prep_prims, grid1(T), dim3(256), 0, s, T, nd, size, near, a, x, P ); } }
+    void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, double * rec, hipStream_t s )
+    { if( T ) { hipLaunchKernelGGL( k_prep_prims, grid1(T), dim3(256), 0, s, T, nd, size, near, a, x, P, rec ); } }
 
     void launch_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P, hipStream_t s )
     { if( C ) { hipLaunchKernelGGL( k_prep_clusters, grid1(C), dim3(256), 0, s, C, far, a, x, P ); } }

@@ -445,8 +445,8 @@ int rp_upload_data( rp_ctx * ctx, const double * V_coords, const double * P_near
         for( int64_t k = 0; k < 3 * ctx->V; ++k ) { require( std::isfinite( V_coords[k] ), "vertex coordinates contain NaN or Inf" ); }
         ctx->coords.upload( V_coords, (size_t)(3 * ctx->V), ctx->stream );
         ctx->stage.upload( P_near, (size_t)(ctx->nd * T), ctx->stream );
-        ctx->p_a.require( (size_t)T ); ctx->p_x.require( (size_t)(3 * T) ); ctx->p_P.require( (size_t)(6 * T) );
-        launch_prep_prims( T, ctx->nd, ctx->size, ctx->stage.get(), ctx->p_a.get(), ctx->p_x.get(), ctx->p_P.get(), ctx->stream );
+        ctx->p_a.require( (size_t)T ); ctx->p_x.require( (size_t)(3 * T) ); ctx->p_P.require( (size_t)(6 * T) ); ctx->p_rec.require( (size_t)(10 * T) );
+        launch_prep_prims( T, ctx->nd, ctx->size, ctx->stage.get(), ctx->p_a.get(), ctx->p_x.get(), ctx->p_P.get(), ctx->p_rec.get(), ctx->stream );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->stage.upload( C_far, (size_t)(10 * C), ctx->stream );
         ctx->c_a.require( (size_t)C ); ctx->c_x.require( (size_t)(3 * C) ); ctx->c_P.require( (size_t)(6 * C) );
@@ -496,8 +496,8 @@ int rp_update_coordinates( rp_ctx * ctx, const double * V_coords, int64_t ldV, i
             const int64_t b = ctx->h_level_ptr[d], e = ctx->h_level_ptr[d + 1];
             launch_cluster_level( e - b, ctx->level_nodes.get() + b, ctx->C_left.get(), ctx->C_right.get(), cfar, s );
         }
-        ctx->p_a.require( (size_t)T ); ctx->p_x.require( (size_t)(3 * T) ); ctx->p_P.require( (size_t)(6 * T) );
-        launch_prep_prims( T, ctx->nd, ctx->size, near, ctx->p_a.get(), ctx->p_x.get(), ctx->p_P.get(), s );
+        ctx->p_a.require( (size_t)T ); ctx->p_x.require( (size_t)(3 * T) ); ctx->p_P.require( (size_t)(6 * T) ); ctx->p_rec.require( (size_t)(10 * T) );
+        launch_prep_prims( T, ctx->nd, ctx->size, near, ctx->p_a.get(), ctx->p_x.get(), ctx->p_P.get(), ctx->p_rec.get(), s );
         ctx->c_a.require( (size_t)C ); ctx->c_x.require( (size_t)(3 * C) ); ctx->c_P.require( (size_t)(6 * C) );
         launch_prep_clusters( C, cfar, ctx->c_a.get(), ctx->c_x.get(), ctx->c_P.get(), s );
         ctx->leaf_origin.require( (size_t)(3 * L) );
@@ -676,7 +676,7 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
             A.L = L; A.T = T;
             A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
             A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
-            A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.P = ctx->p_P.get(); A.origin = ctx->leaf_origin.get();
+            A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.P = ctx->p_P.get(); A.rec = ctx->p_rec.get(); A.origin = ctx->leaf_origin.get();
             A.scale = 1.0 / ctx->size; A.size = ctx->size; A.ex = ex;
             A.dx = D ? ctx->nf_dx.get() : nullptr;
             A.epart = ctx->partials.get();

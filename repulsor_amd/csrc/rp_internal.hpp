@@ -225,6 +225,7 @@ struct rp_ctx
 
     // data (tree order, SoA)
     rp::DBuf<double> p_a, p_x, p_P;      // T, 3T, 6T
+    rp::DBuf<double> p_rec;              // T x 10 records (a, x, P)
     rp::DBuf<double> c_a, c_x, c_P;      // C, 3C, 6C
     rp::DBuf<double> leaf_origin;        // 3L
     rp::DBuf<double> stage;              // upload staging; after rp_update_coordinates: the near rows T x ND

@@ -18,6 +18,7 @@ namespace rp
         const double   * a;       // T
         const double   * x;       // 3 x T
         const double   * P;       // 6 x T
+        const double   * rec;     // T x 10 records (a, x, P): the column gathers of the tangent-point kernels
         const double   * origin;  // 3 x L (mixed precision local origins)
         double           scale;   // 1/(n+1)
         int              size;    // n+1
@@ -48,7 +49,7 @@ namespace rp
     constexpr int64_t REDUCE_BLOCKS = 1024;
     void launch_reduce_sum( const double * part, int64_t n, double * out, hipStream_t s );   // out: 1 + REDUCE_BLOCKS
 
-    void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, hipStream_t s );
+    void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, double * rec, hipStream_t s );
     void launch_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P, hipStream_t s );
     void launch_leaf_origin( int64_t L, int64_t C, const int32_t * leaf_cluster, const double * cx, double * origin, hipStream_t s );
     void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * ff_dx, double * leaf_far, hipStream_t s );
