@@ -52,7 +52,7 @@ def spmm_algorithmic_bytes(b, precision):
     index + row mask, 8 B per tile column entry), and per primitive the input block (12 doubles), the
     diagonal (2 doubles), the post-operator weights (4 x 3 doubles) and the output (3 x 3 doubles),
     plus the far-field total of every leaf (12 doubles)."""
-    val_bytes = 16 if precision == "fp64" else 8
+    val_bytes = 8 if precision == "mixed" else 16
     return (val_bytes * b["near_nnz"] + 8 * b["tile_cols"]
             + b["T"] * (96 + 16 + 96 + 72) + 96 * b["L"])
 
@@ -193,8 +193,9 @@ def main():
     ap.add_argument("--workload", default="sphere_1310720")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
                     help="energy + differential: FP64, or FP32 pair arithmetic with FP64 accumulation")
-    ap.add_argument("--metric-precision", default="fp64", choices=["fp64", "mixed"],
-                    help="metric blocks of the solve (FP64 keeps the final residual the FP64 one)")
+    ap.add_argument("--metric-precision", default="fp64", choices=["fp64", "mixed", "mixed-refined"],
+                    help="metric blocks of the solve (FP64 keeps the final residual the FP64 one; mixed-refined: FP64 "
+                         "values plus FP32 copies for the CG iterations, convergence judged by the FP64 residual)")
     ap.add_argument("--learned", default="",
                     help="learned preconditioner model for the timed solves (default: the two-level form; the packaged "
                          "model is timed once among the comparison solves)")
@@ -315,10 +316,14 @@ def main():
     X = np.random.default_rng(11).standard_normal((M.V, 3))
     Y = np.zeros_like(X)
     A.MultiplyMetric(M, 1.0, X, 0.0, Y)
-    mm_ms, spmm_ms, pcl_ms, h1l_ms = [], [], [], []
+    mm_ms, spmm_ms, pcl_ms, h1l_ms, spmm32_ms = [], [], [], [], []
+    refined = args.metric_precision == "mixed-refined"
     for _ in range(max(3, args.steps)):
         A.MultiplyMetric(M, 1.0, X, 0.0, Y)
         mm_ms.append(M.last_timing(2)); spmm_ms.append(M.last_timing(6))
+        if refined:
+            A.MultiplyMetricInner(M, 1.0, X, 0.0, Y)
+            spmm32_ms.append(M.last_timing(6))
         A.MultiplyPreconditioner(M, 1.0, X, 0.0, Y)
         pcl_ms.append(M.last_timing(7))
         M.h1_solve(X)
@@ -329,9 +334,17 @@ def main():
     # descent direction -X (staged in HBM, device-pointer entry), max_time such that the fastest vertex
     # moves one mean edge length
     safe = None
+    fp64_check = None
     if not args.no_solve:
         Xh = np.zeros((M.V, 3))
         assert hip.hipMemcpy(Xh.ctypes.data, Xd, M.V * 3 * 8, 2) == 0
+        # the last solve's residual recomputed with the FP64 metric product (independent of the solver's
+        # own bookkeeping; with --metric-precision mixed the product is the FP32-valued one)
+        Bh = np.zeros((M.V, 3))
+        assert hip.hipMemcpy(Bh.ctypes.data, Yd, M.V * 3 * 8, 2) == 0
+        AX = np.zeros_like(Xh)
+        A.MultiplyMetric(M, 1.0, Xh, 0.0, AX)
+        fp64_check = [float(x) for x in np.linalg.norm(Bh - AX, axis=0) / np.linalg.norm(Bh, axis=0)]
         Dh = np.ascontiguousarray(-Xh)
         Dd = C.c_void_p()
         assert hip.hipMalloc(C.byref(Dd), M.V * 3 * 8) == 0
@@ -399,7 +412,7 @@ def main():
     spmm_bytes = spmm_algorithmic_bytes(b, args.metric_precision)
     spmm_s = float(np.mean(spmm_ms)) * 1e-3
     spmm_gbs = spmm_bytes / spmm_s / 1e9
-    traffic = MEASURED_SPMM_TRAFFIC.get((wl.name, args.metric_precision))
+    traffic = MEASURED_SPMM_TRAFFIC.get((wl.name, "fp64" if refined else args.metric_precision))
     result = {
         "metric": "energy+grad ms & metric-solve ms to 1e-6 residual, learned vs plain PC, 1 GPU",
         "value": round(ms_per_step, 3),
@@ -438,11 +451,15 @@ def main():
         "multiply_metric_ms": round(float(np.mean(mm_ms)), 3),
         "near_spmm_ms": round(float(np.mean(spmm_ms)), 3),
         "near_spmm_GBs_algorithmic": round(spmm_gbs, 1),
+        "near_spmm_fp32_copies_ms": round(float(np.mean(spmm32_ms)), 3) if refined else None,
+        "near_spmm_fp32_copies_GBs_algorithmic": round(spmm_algorithmic_bytes(b, "mixed") / (float(np.mean(spmm32_ms)) * 1e-3) / 1e9, 1)
+                                                 if refined else None,
         "multiply_preconditioner_ms": round(pc_ms, 3),
         "h1_solve_ms": round(h1_ms, 3),
         "solve_ms": None if args.no_solve else round(float(np.mean(sv_ms)), 2),
         "solve_iterations": None if args.no_solve else solve_iters,
         "solve_rel_residuals": None if args.no_solve else [float(x) for x in rel],
+        "solve_rel_residuals_fp64_check": fp64_check,
         "solver": args.solver,
         "pc_weight": pc_weight,
         "pc_setup_ms": round(pc_setup_dev_ms, 3),

@@ -13,11 +13,21 @@ namespace Repulsor
 
     // Floating-point mode of the GPU pair kernels. FP64 is the reference arithmetic;
     // Mixed evaluates the pair arithmetic in FP32 and accumulates in FP64.
+    // MixedRefined (metric only; energy and differential compute as FP64): FP64 metric values plus
+    // FP32 copies of the near-field values; the CG iterations of Solve multiply with the copies and
+    // judge convergence by the FP64 residual, replacing the residual and continuing on the FP64
+    // values where needed. MultiplyMetric uses the FP64 values.
     enum class Precision : int
     {
-        FP64  = 0,
-        Mixed = 1
+        FP64         = 0,
+        Mixed        = 1,
+        MixedRefined = 2
     };
+
+    inline const char * PrecisionTag( Precision prec )
+    {
+        return prec == Precision::Mixed ? "[mixed]" : prec == Precision::MixedRefined ? "[mixed-refined]" : "";
+    }
 
     struct ClusterTreeSettings
     {

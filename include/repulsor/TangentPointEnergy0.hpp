@@ -31,7 +31,7 @@ namespace Repulsor
         std::string ClassName() const
         {
             char buf[128];
-            std::snprintf( buf, sizeof(buf), "TangentPointEnergy0<%d,3>(%g,%g)%s", DOM_DIM, q, p, precision == Precision::Mixed ? "[mixed]" : "" );
+            std::snprintf( buf, sizeof(buf), "TangentPointEnergy0<%d,3>(%g,%g)%s", DOM_DIM, q, p, PrecisionTag( precision ) );
             return std::string(buf);
         }
 

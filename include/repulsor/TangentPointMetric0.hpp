@@ -33,14 +33,22 @@ namespace Repulsor
         // Storage and pair arithmetic of the pseudo-Laplacian inside the preconditioner (no reference
         // counterpart). PCG and GMRES need only a fixed symmetric positive definite preconditioner, so FP32
         // values there leave the operator, its FP64 residual and the converged solution's accuracy as they
-        // are; the metric itself keeps SetPrecision.
-        void SetPreconditionerPrecision( Precision prec ) { pc_precision = (int)prec; }
-        Precision GetPreconditionerPrecision() const { return pc_precision < 0 ? precision : (Precision)pc_precision; }
+        // are; the metric itself keeps SetPrecision. Unset, it follows SetPrecision, with MixedRefined
+        // (FP32 products inside the CG iterations) giving Mixed.
+        void SetPreconditionerPrecision( Precision prec )
+        {
+            pc_precision = prec == Precision::MixedRefined ? (int)Precision::Mixed : (int)prec;
+        }
+        Precision GetPreconditionerPrecision() const
+        {
+            if( pc_precision >= 0 ) { return (Precision)pc_precision; }
+            return precision == Precision::MixedRefined ? Precision::Mixed : precision;
+        }
 
         std::string ClassName() const
         {
             char buf[128];
-            std::snprintf( buf, sizeof(buf), "TangentPointMetric0<%d,3>(%g,%g,%g)%s", DOM_DIM, q, p, s, precision == Precision::Mixed ? "[mixed]" : "" );
+            std::snprintf( buf, sizeof(buf), "TangentPointMetric0<%d,3>(%g,%g,%g)%s", DOM_DIM, q, p, s, PrecisionTag( precision ) );
             return std::string(buf);
         }
 
@@ -110,6 +118,17 @@ namespace Repulsor
             return Check( rp_multiply_metric( M.Device(), alpha, X, ldX, beta, Y, ldY, nrhs,
                                               VF_flag, NF_flag, FF_flag, device_ptrs ? RP_DEVICE_PTR : 0 ),
                           ClassName() + "::MultiplyMetric" );
+        }
+
+        // Y = alpha * A32 X + beta * Y with the FP32 copies of a MixedRefined metric: the product the
+        // CG iterations of Solve use (no reference counterpart).
+        bool MultiplyMetricInner( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,
+                                  const double beta, double * Y, const LInt ldY, const bool device_ptrs = false ) const
+        {
+            if( !MetricValues( M ) ) { return false; }
+            return Check( rp_multiply_metric( M.Device(), alpha, X, ldX, beta, Y, ldY, AMB_DIM,
+                                              1, RP_NF_FP32_COPIES, 1, device_ptrs ? RP_DEVICE_PTR : 0 ),
+                          ClassName() + "::MultiplyMetricInner" );
         }
 
         // Vector overload (MetricBase.hpp:46-59): X, Y are V x 3.

@@ -38,6 +38,12 @@ extern "C" {
 /* precision */
 #define RP_FP64          0
 #define RP_MIXED         1
+/* RP_MIXED_REFINED (metric only): FP64 pair arithmetic and FP64 metric values as RP_FP64, plus FP32
+ * copies of the near-field metric values with a diagonal summed from those copies. rp_multiply_metric
+ * uses the FP64 values; the PCG iterations of rp_solve use the FP32 copies until their recursive
+ * residual meets the tolerance, then replace the residual by the FP64 one and, where that is still
+ * above the tolerance, continue on the FP64 values. No reference counterpart. */
+#define RP_MIXED_REFINED 2
 
 typedef struct rp_ctx rp_ctx;
 
@@ -160,7 +166,8 @@ int rp_download_state_rows( rp_ctx * ctx, double * P_near, double * C_far, doubl
 
 /* TangentPointEnergy0 kernels over the very-near, near and far fields (flags: RP_ENERGY | RP_DIFF |
  * RP_METRIC). energy_out may be NULL. RP_METRIC stores the metric blocks on the device for
- * rp_multiply_metric. The very-near field is computed in FP64 at either precision; its energy terms
+ * rp_multiply_metric. precision: RP_FP64, RP_MIXED, or RP_MIXED_REFINED (computes like RP_FP64 and
+ * with RP_METRIC also keeps the FP32 copies). The very-near field is computed in FP64 at every precision; its energy terms
  * are a E b per adaptive evaluation without the quadrature weight, as the reference sums them
  * (TP0_Kernel_VF.hpp:273), while its derivative and metric terms carry the weight. A very-near
  * traversal that exceeds 2^24 evaluations on one level-1 sub-pair stops with RP_EUNSUPPORTED.
@@ -193,7 +200,10 @@ int rp_download_vf_metric( rp_ctx * ctx, double * vf_vals, double * vf_diag );
 int rp_vf_stats( rp_ctx * ctx, int64_t * evaluations, int * max_level );
 
 /* Y = alpha * A X + beta * Y, X and Y V x nrhs (nrhs must be 3), ld >= 3.
+ * nf_flag = RP_NF_FP32_COPIES (metric assembled at RP_MIXED_REFINED): the near field multiplies with
+ * the FP32 copies, the product the CG iterations of rp_solve use (no reference counterpart).
  * Replaces: MetricDimRestricted::MultiplyMetric (reference src/Metrics/MetricDimRestricted.hpp:109-139). */
+#define RP_NF_FP32_COPIES 2
 int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ldX,
                         double beta, double * Y, int64_t ldY, int nrhs,
                         int vf_flag, int nf_flag, int ff_flag, int ptr_flags );
@@ -286,7 +296,10 @@ int rp_pc_learn_step( rp_ctx * ctx, float lr, int step, float * params, float * 
 /* X = alpha * A^+ B + beta * X on the three columns of B (zero initial guess).
  * Default: preconditioned conjugate gradients, stopping when every column has
  * |B_k - A Z_k|_2 / |B_k|_2 <= tolerance (recursive residual) or, with RP_SOLVE_PC_NORM, the same ratio
- * in the preconditioner norm |r|_P = sqrt(<r, P r>).
+ * in the preconditioner norm |r|_P = sqrt(<r, P r>). With a metric assembled at RP_MIXED_REFINED the
+ * iterations multiply with the FP32 copies until that test passes, then the residual is replaced by
+ * B - A X with the FP64 metric; the solve ends when the replaced residual passes (the reported measure
+ * is then the FP64 one), else it restarts CG from X on the FP64 metric.
  * RP_SOLVE_GMRES: left-preconditioned restarted GMRES(100), stopping when |P (B_k - A Z_k)|_2 / |P B_k|_2
  * <= tolerance (the reference's algorithm and measure).
  * After max_iter iterations without convergence: RP_ESTAGNATION, X still written. iterations and

@@ -21,7 +21,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 
 RP_OK, RP_EINVAL, RP_EHIP, RP_ENOMEM, RP_ESTAGNATION, RP_EUNSUPPORTED, RP_ENOTREADY = range(7)
 RP_SOLVE_NO_PRECONDITIONER, RP_SOLVE_PC_NORM, RP_SOLVE_GMRES, RP_SOLVE_RECORD = 1, 2, 4, 8
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed-refined": 2}
 
 
 class RepulsorError(RuntimeError):
@@ -477,6 +477,16 @@ class TangentPointMetric0(_Op):
         st = H.rpm_metric_multiply(self._h, M._h, alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1],
                                    nrhs, int(VF), int(NF), int(FF), 0)
         _check(st)
+        return Y
+
+    def MultiplyMetricInner(self, M: SimplicialMesh, alpha, X, beta, Y):
+        """Y = alpha A32 X + beta Y with the FP32 copies of a "mixed-refined" metric (the product the CG
+        iterations of Solve use; no reference counterpart)."""
+        M.build()
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        assert Y.dtype == np.float64 and Y.flags.c_contiguous
+        _check(H.rpm_metric_multiply(self._h, M._h, alpha, _p(X), X.shape[1], beta, _p(Y), Y.shape[1],
+                                     3, 1, 2, 1, 0))
         return Y
 
     def MetricNorm(self, M: SimplicialMesh, X):

@@ -428,6 +428,7 @@ int rpm_metric_multiply( void * op, void * h, double alpha, const double * X, in
             g_err = "nrhs must be 3";
             return RP_EINVAL;
         }
+        if( nf == RP_NF_FP32_COPIES ) { return A.MultiplyMetricInner( M, alpha, X, ldX, beta, Y, ldY, device_ptr ) ? RP_OK : RP_EINVAL; }
         return A.MultiplyMetric( M, alpha, X, ldX, beta, Y, ldY, nrhs, vf, nf, ff, device_ptr ) ? RP_OK : RP_EINVAL;
     });
 }

@@ -44,10 +44,12 @@ namespace rp
     __global__ __launch_bounds__(64) void k_nf( NFArgs A )
     {
         using R = typename std::conditional<MIXED, float, double>::type;
-        union Smem { Chunk<R> ch; double red[64][NACC]; };
+        // FP64 metric: two more sums, the diagonal of the FP32 copies (13, 14)
+        constexpr int NA = ( M && !MIXED ) ? NACC + 2 : NACC;
+        union Smem { Chunk<R> ch; double red[64][NA]; };
         __shared__ Smem smem;
         Chunk<R> & sh = smem.ch;
-        double (&red)[64][NACC] = smem.red;
+        double (&red)[64][NA] = smem.red;
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( A.L );
         if( I < 0 ) { return; }
@@ -84,10 +86,11 @@ namespace rp
         const int64_t M_ = A.col_ptr[I + 1] - cb;
         const int64_t vb = M ? A.val_ptr[I] : 0;
 
-        double acc[NACC];
-        R part[NACC];
+        const bool copies = M && !MIXED && A.vals32 != nullptr;
+        double acc[NA];
+        R part[NA];
         #pragma unroll
-        for( int k = 0; k < NACC; ++k ) { acc[k] = 0.0; part[k] = R(0); }
+        for( int k = 0; k < NA; ++k ) { acc[k] = 0.0; part[k] = R(0); }
         int cnt = 0;
 
         // register staging of one column per lane
@@ -173,6 +176,16 @@ namespace rp
                             const R boa = b * inv_a;
                             part[10] -= boa * t.a0;
                             part[11] -= boa * t.a1;
+                            if constexpr ( !MIXED )
+                            {
+                                if( copies )
+                                {
+                                    const float2 v32 = make_float2( (float)t.a0, (float)t.a1 );
+                                    A.vals32[slot] = v32;
+                                    part[NA - 2] -= boa * (double)v32.x;
+                                    part[NA - 1] -= boa * (double)v32.y;
+                                }
+                            }
                         }
                         if constexpr ( MIXED )
                         {
@@ -188,7 +201,11 @@ namespace rp
                     {
                         const int64_t slot = vb + (base + mm) * nI + ii;
                         if constexpr ( MIXED ) { reinterpret_cast<float2 *>(A.vals)[slot] = make_float2( 0.0f, 0.0f ); }
-                        else                   { reinterpret_cast<double2 *>(A.vals)[slot] = make_double2( 0.0, 0.0 ); }
+                        else
+                        {
+                            reinterpret_cast<double2 *>(A.vals)[slot] = make_double2( 0.0, 0.0 );
+                            if( copies ) { A.vals32[slot] = make_float2( 0.0f, 0.0f ); }
+                        }
                     }
                 }
             }
@@ -197,9 +214,9 @@ namespace rp
             buf ^= 1;
         }
         #pragma unroll
-        for( int k = 0; k < NACC; ++k ) { acc[k] += (double)part[k]; }
+        for( int k = 0; k < NA; ++k ) { acc[k] += (double)part[k]; }
         #pragma unroll
-        for( int k = 0; k < NACC; ++k ) { red[lane][k] = acc[k]; }
+        for( int k = 0; k < NA; ++k ) { red[lane][k] = acc[k]; }
         __syncthreads();
         if( lane < nI )
         {
@@ -226,6 +243,16 @@ namespace rp
                 A.diag[2 * i]     = s[10];
                 A.diag[2 * i + 1] = s[11];
             }
+            if constexpr ( M && !MIXED )
+            {
+                if( copies )
+                {
+                    double d0 = 0.0, d1 = 0.0;
+                    for( int cc = 0; cc < nS; ++cc ) { d0 += red[lane + cc * nI][NA - 2]; d1 += red[lane + cc * nI][NA - 1]; }
+                    A.diag32[2 * i]     = d0;
+                    A.diag32[2 * i + 1] = d1;
+                }
+            }
         }
         if constexpr ( E )
         {
@@ -243,7 +270,7 @@ namespace rp
     // Epilogue (post-operator, ClusterTree::Post): ytot_i = y_i + far-field leaf total of leaf(i), then
     // S_out[i][s][c] = sum_r pre_vals[i][r][s] ytot_i[r][c] for the (n+1) vertex slots s.
     template<typename VT, int SIZE>
-    __global__ __launch_bounds__(64) void k_nf_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
+    __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_nf_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
         const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
         const VT * vals, const double * diag, const double * xin,
         const double * leaf_out, int use_far, const double * pre_vals, double * S_out )
@@ -252,8 +279,6 @@ namespace rp
         // 24 leaves share a CU
         constexpr int CS = 32;
         __shared__ double sx[2][12][CS];
-        __shared__ int32_t sj[2][CS];
-        __shared__ uint32_t sm[2][CS];
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( L );
         if( I < 0 ) { return; }
@@ -277,25 +302,20 @@ namespace rp
         for( int k = 0; k < 12; ++k ) { acc0[k] = 0.0; acc1[k] = 0.0; }
         // lane l loads half of the input block of column l mod 32 of the chunk (doubles 6 (l / 32) ..)
         double rx[6];
-        int32_t rj = 0;
-        uint32_t rm = 0;
         const int half = lane >> 5, lc = lane & ( CS - 1 );
         auto gload = [&]( int64_t base )
         {
             const int64_t m = base + lc;
             if( m < M_ )
             {
-                rj = col_prim[cb + m];
-                rm = col_mask[cb + m];
+                const int32_t rj = col_prim[cb + m];
                 const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj + 6 * half );
                 #pragma unroll
                 for( int q = 0; q < 3; ++q ) { const double2 t = xj[q]; rx[2 * q] = t.x; rx[2 * q + 1] = t.y; }
             }
-            else { rm = 0u; }
         };
         auto lstore = [&]( int buf )
         {
-            if( half == 0 ) { sj[buf][lc] = rj; sm[buf][lc] = rm; }
             #pragma unroll
             for( int k = 0; k < 6; ++k ) { sx[buf][6 * half + k][lc] = rx[k]; }
         };
@@ -310,31 +330,33 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {
-                // the value pairs of two columns (four slots) are loaded before the (masked) products; every
-                // tile slot exists, masked pairs add 0, each row sums its products in column order
+                // the value pairs of U columns (2U slots) are loaded before the products: 64 bytes in flight
+                // per lane for double2 (U = 2) and for float2 (U = 4); every tile slot exists and the
+                // assembly stores 0 in the slots of masked pairs, so the products need no mask; each row
+                // sums its products in column order
+                constexpr int U = sizeof(VT) == 16 ? 2 : 4;
                 const VT * vrow = vals + vb + base * nI + p;
-                int mm = c;
-                for( ; mm < cnt_c; mm += 2 * nS )
+                for( int mm = c; mm < cnt_c; mm += U * nS )
                 {
-                    const bool sec = mm + nS < cnt_c;
-                    VT a00 = {}, a01 = {}, a10 = {}, a11 = {};   // [column u][row]
-                    a00 = vrow[(int64_t)mm * nI];
-                    if( two ) { a01 = vrow[(int64_t)mm * nI + h]; }
-                    if( sec )
+                    VT a0[U], a1[U];   // [column u], rows p and p + h
+                    #pragma unroll
+                    for( int u = 0; u < U; ++u )
                     {
-                        a10 = vrow[(int64_t)( mm + nS ) * nI];
-                        if( two ) { a11 = vrow[(int64_t)( mm + nS ) * nI + h]; }
+                        const int m2 = mm + u * nS;
+                        a0[u] = VT{}; a1[u] = VT{};
+                        if( m2 < cnt_c )
+                        {
+                            a0[u] = vrow[(int64_t)m2 * nI];
+                            if( two ) { a1[u] = vrow[(int64_t)m2 * nI + h]; }
+                        }
                     }
                     #pragma unroll
-                    for( int u = 0; u < 2; ++u )
+                    for( int u = 0; u < U; ++u )
                     {
-                        if( u == 1 && !sec ) { break; }
                         const int m2 = mm + u * nS;
-                        const uint32_t msk = sm[buf][m2];
-                        const VT va = u ? a10 : a00, vb2 = u ? a11 : a01;
-                        const bool on0 = ( msk >> p ) & 1u, on1 = two && ( ( msk >> r1 ) & 1u );
-                        const double b0 = on0 ? (double)va.x : 0.0,  b1 = on0 ? (double)va.y : 0.0;
-                        const double e0 = on1 ? (double)vb2.x : 0.0, e1 = on1 ? (double)vb2.y : 0.0;
+                        if( m2 >= cnt_c ) { break; }
+                        const double b0 = (double)a0[u].x, b1 = (double)a0[u].y;
+                        const double e0 = (double)a1[u].x, e1 = (double)a1[u].y;
                         #pragma unroll
                         for( int k = 0; k < 12; ++k )
                         {

@@ -53,7 +53,9 @@ namespace rp
             {
                 const int32_t j = A.col_prim[cb + m];
                 rm = A.col_mask[cb + m];
-                rd[0] = A.a[j]; rd[1] = A.x[j]; rd[2] = A.x[T + j]; rd[3] = A.x[2 * T + j];
+                const double2 * q = reinterpret_cast<const double2 *>( A.rec + 10 * (int64_t)j );
+                const double2 q0 = q[0], q1 = q[1];
+                rd[0] = q0.x; rd[1] = q0.y; rd[2] = q1.x; rd[3] = q1.y;
             }
             else { rm = 0u; }
         };

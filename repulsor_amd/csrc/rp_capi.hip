@@ -20,12 +20,15 @@ namespace rp
 
     // Y = alpha A X + beta Y on device vectors (V x 3, leading dimensions ldx, ldy): pre-operator,
     // cluster sums, far-field SpMM and down-sweep, near-field SpMM with fused post epilogue, vertex gather.
+    // inner32: with a metric assembled at RP_MIXED_REFINED, the near field multiplies with the FP32
+    // copies and their diagonal (far and very-near fields stay FP64).
     // Reference: MetricDimRestricted::MultiplyMetric (src/Metrics/MetricDimRestricted.hpp:109-139).
     void metric_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy,
-                       bool nf_flag, bool ff_flag, bool time_nf, bool vf_flag )
+                       bool nf_flag, bool ff_flag, bool time_nf, bool vf_flag, bool inner32 )
     {
         const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
         const bool mixed = ctx->metric_precision == RP_MIXED;
+        const bool use32 = inner32 && ctx->metric_precision == RP_MIXED_REFINED;
         ctx->P_in.require( (size_t)(12 * T) ); ctx->P_out.require( (size_t)(12 * T) );
         ctx->C_in.require( (size_t)(12 * C) ); ctx->C_out.require( (size_t)(12 * C) );
         ctx->leaf_out.require( (size_t)(12 * L) );
@@ -45,8 +48,9 @@ namespace rp
         if( time_nf ) { RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) ); }
         if( do_nf )
         {
-            launch_nf_spmm_post( mixed, ctx->size, T, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
-                                 ctx->col_mask.get(), ctx->val_ptr.get(), ctx->nf_vals.get(), ctx->nf_diag.get(), ctx->P_in.get(),
+            launch_nf_spmm_post( mixed || use32, ctx->size, T, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
+                                 ctx->col_mask.get(), ctx->val_ptr.get(), use32 ? ctx->nf_vals32.get() : ctx->nf_vals.get(),
+                                 use32 ? ctx->nf_diag32.get() : ctx->nf_diag.get(), ctx->P_in.get(),
                                  ctx->leaf_out.get(), do_ff ? 1 : 0, ctx->pre_vals.get(), S_out, ctx->stream );
         }
         else
@@ -630,7 +634,7 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
     {
         require( ctx != nullptr && ctx->have_data && ctx->have_near && ctx->have_far, "mesh, tree, patterns and data must be uploaded", RP_ENOTREADY );
         require( (flags & (RP_ENERGY | RP_DIFF | RP_METRIC)) != 0, "no work requested" );
-        require( precision == RP_FP64 || precision == RP_MIXED, "unknown precision" );
+        require( precision == RP_FP64 || precision == RP_MIXED || precision == RP_MIXED_REFINED, "unknown precision" );
         require( q >= 0, "Tangent-point kernels not well-defined for q < 0." );
         require( q == 0 || q >= 2, "Tangent-point kernels not implement for 0 < q < 2." );
         RP_HIP( hipSetDevice( ctx->device ) );
@@ -642,6 +646,7 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         const int kind = exponent_kind( q, p );
         const bool mixed = ( precision == RP_MIXED );
         const bool E = flags & RP_ENERGY, D = flags & RP_DIFF, M = flags & RP_METRIC;
+        const bool copies = M && precision == RP_MIXED_REFINED;
 
         const bool have_vf = ctx->vf_nnz > 0;
         const int64_t R = have_vf ? ctx->vf_nrows : 0;
@@ -652,6 +657,11 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         {
             ctx->nf_vals.require( (size_t)std::max<int64_t>( 1, 2 * ctx->nf_slots ) );
             ctx->nf_diag.require( (size_t)(2 * T) );
+            if( copies )
+            {
+                ctx->nf_vals32.require( (size_t)std::max<int64_t>( 1, ctx->nf_slots ) );
+                ctx->nf_diag32.require( (size_t)(2 * T) );
+            }
             ctx->ff_vals.require( (size_t)std::max<int64_t>( 1, 2 * ctx->far_nnz ) );
             ctx->ff_diag.require( (size_t)(2 * C) );
         }
@@ -663,6 +673,7 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         {
             if( D ) { launch_fill( ctx->nf_dx.get(), 10 * T, 0.0, ctx->stream ); }
             if( M ) { launch_fill( ctx->nf_diag.get(), 2 * T, 0.0, ctx->stream ); }
+            if( copies ) { launch_fill( ctx->nf_diag32.get(), 2 * T, 0.0, ctx->stream ); }
         }
         if( !have_ff )
         {
@@ -682,6 +693,8 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
             A.epart = ctx->partials.get();
             A.vals = M ? (void *)ctx->nf_vals.get() : nullptr;
             A.diag = M ? ctx->nf_diag.get() : nullptr;
+            A.vals32 = copies ? reinterpret_cast<float2 *>( ctx->nf_vals32.get() ) : nullptr;
+            A.diag32 = copies ? ctx->nf_diag32.get() : nullptr;
             launch_nf( flags, kind, mixed, A, ctx->stream );
             RP_HIP( hipGetLastError() );
         }
@@ -891,6 +904,8 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
         require( nrhs == 3, "nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
         require( ctx->have_metric, "metric values not assembled (call rp_tp0_compute with RP_METRIC)", RP_ENOTREADY );
         require( X != nullptr && Y != nullptr && ldX >= 3 && ldY >= 3, "X and Y must be V x 3 with ld >= 3" );
+        require( nf_flag != RP_NF_FP32_COPIES || ctx->metric_precision == RP_MIXED_REFINED,
+                 "RP_NF_FP32_COPIES needs a metric assembled at RP_MIXED_REFINED", RP_ENOTREADY );
         (void)vf_flag;
         RP_HIP( hipSetDevice( ctx->device ) );
         const int64_t V = ctx->V;
@@ -910,7 +925,7 @@ int rp_multiply_metric( rp_ctx * ctx, double alpha, const double * X, int64_t ld
             Yd = ctx->Yd.get(); ldy = 3;
         }
         EventScope ev( ctx, 2 );
-        metric_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy, nf_flag != 0, ff_flag != 0, true, vf_flag != 0 );
+        metric_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy, nf_flag != 0, ff_flag != 0, true, vf_flag != 0, nf_flag == RP_NF_FP32_COPIES );
         ev.stop();
         RP_HIP( hipGetLastError() );
         if( !dev )

@@ -15,7 +15,7 @@ namespace rp
 
     // Y = alpha A X + beta Y with the tangent-point metric A on device vectors (defined in rp_capi.hip).
     void metric_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy,
-                       bool nf_flag, bool ff_flag, bool time_nf, bool vf_flag = true );
+                       bool nf_flag, bool ff_flag, bool time_nf, bool vf_flag = true, bool inner32 = false );
 
     struct InvalidArg { std::string msg; int status; };
 

@@ -245,6 +245,8 @@ struct rp_ctx
     // metric
     rp::DBuf<double> nf_vals;            // 2 x nf_slots (fp64) or packed floats
     rp::DBuf<double> nf_diag;            // 2T
+    rp::DBuf<double> nf_vals32;          // RP_MIXED_REFINED: nf_slots float2 copies of nf_vals
+    rp::DBuf<double> nf_diag32;          // RP_MIXED_REFINED: 2T, the diagonal summed from the copies
     rp::DBuf<double> ff_vals;            // 2 x far_nnz
     rp::DBuf<double> ff_diag;            // 2C
     bool have_metric = false;

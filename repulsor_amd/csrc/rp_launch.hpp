@@ -27,6 +27,8 @@ namespace rp
         double         * epart;   // L
         void           * vals;    // 2 x slots (double or float)
         double         * diag;    // 2 x T
+        float2         * vals32;  // FP64 metric only, may be NULL: FP32 copies of vals
+        double         * diag32;  // with vals32: 2 x T, the diagonal summed from the copies
     };
 
     struct FFArgs

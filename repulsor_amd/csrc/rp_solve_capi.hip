@@ -262,6 +262,10 @@ namespace rp
     // Preconditioned conjugate gradients on the three columns (independent scalars per column, zero
     // start). Stopping measure: |r|_2 / |b|_2, or with pc_norm the preconditioner norm
     // sqrt(<r, P r>) / sqrt(<b, P b>). b, x: device V x 3 (ld 3). Returns the iteration count.
+    // Metric assembled at RP_MIXED_REFINED: the products A p use the FP32 copies of the near-field
+    // values until the recursive residual passes; then r = b - A x with the FP64 values replaces it
+    // (residual replacement, as in mixed-precision iterative refinement). If a column is still above
+    // the tolerance, CG restarts from x (p = P r) and continues on the FP64 values.
     static int pcg( rp_ctx * ctx, const double * b, double * x, int max_iter, double tol, bool use_pc, bool pc_norm,
                     std::vector<double> & rel, bool & done, bool record )
     {
@@ -318,9 +322,38 @@ namespace rp
         done = true;   // nothing to do when every column of B vanishes
         for( int k = 0; k < 3; ++k ) { if( h_ref[k] > 0 ) { done = false; } }
         int cur = 0;
+        bool inner32 = ctx->metric_precision == RP_MIXED_REFINED;
+        // r = b - A x on the FP64 values; a new measure per column; false if a column is still above
+        // the tolerance (then z = P r and rho[cur] = <r, z> are set for the restart)
+        auto replace_residual = [&]() -> bool
+        {
+            launch_copy3( V, b, 3, r, 3, s );
+            metric_apply( ctx, -1.0, x, 3, 1.0, r, 3, true, true, false );
+            if( !pc_norm )
+            {
+                launch_dot3( V, r, r, part, rr, s );
+                RP_HIP( hipMemcpyAsync( h_m, rr, sizeof(h_m), hipMemcpyDeviceToHost, s ) );
+                launch_cg_check( rr, bb, tol2, active, s );
+                RP_HIP( hipStreamSynchronize( s ) );
+                if( all_done( h_m ) ) { return true; }
+                precondition( r, z );
+                launch_dot3( V, r, z, part, rho[cur], s );
+            }
+            else
+            {
+                precondition( r, z );
+                launch_dot3( V, r, z, part, rho[cur], s );
+                RP_HIP( hipMemcpyAsync( h_m, rho[cur], sizeof(h_m), hipMemcpyDeviceToHost, s ) );
+                launch_cg_check( rho[cur], rho0, tol2, active, s );
+                RP_HIP( hipStreamSynchronize( s ) );
+                if( all_done( h_m ) ) { return true; }
+            }
+            launch_copy3( V, z, 3, p, 3, s );
+            return false;
+        };
         while( !done && it < max_iter )
         {
-            metric_apply( ctx, 1.0, p, 3, 0.0, Ap, 3, true, true, false );
+            metric_apply( ctx, 1.0, p, 3, 0.0, Ap, 3, true, true, false, true, inner32 );
             launch_dot3( V, p, Ap, part, pAp, s );
             launch_cg_update( V, x, r, p, Ap, rho[cur], pAp, active, part, rr, s );
             keep( r, x );
@@ -331,6 +364,13 @@ namespace rp
                 RP_HIP( hipStreamSynchronize( s ) );
                 ++it;
                 done = all_done( h_m );
+                if( done && inner32 )
+                {
+                    inner32 = false;
+                    done = replace_residual();
+                    if( done || it >= max_iter ) { break; }
+                    continue;
+                }
                 if( done || it >= max_iter ) { break; }
                 precondition( r, z );
                 launch_dot3( V, r, z, part, rho[cur ^ 1], s );
@@ -344,10 +384,22 @@ namespace rp
                 RP_HIP( hipStreamSynchronize( s ) );
                 ++it;
                 done = all_done( h_m );
+                if( done && inner32 )
+                {
+                    inner32 = false;
+                    done = replace_residual();
+                    if( done || it >= max_iter ) { break; }
+                    continue;
+                }
                 if( done || it >= max_iter ) { break; }
             }
             launch_cg_dir( V, p, z, rho[cur ^ 1], rho[cur], active, s );
             cur ^= 1;
+        }
+        if( inner32 && !done )
+        {
+            // iteration cap reached on the FP32 copies: judge and report the FP64 residual
+            done = replace_residual();
         }
         return it;
     }
@@ -501,7 +553,7 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision )
         A.L = L; A.T = T;
         A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
         A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
-        A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.ex = ex;
+        A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.rec = ctx->p_rec.get(); A.ex = ex;
         A.vals = ctx->pl_nf_vals.get(); A.diag = ctx->pl_nf_diag.get();
         FFArgs B{};
         B.C = C; B.rp = ctx->far_rp.get(); B.ci = ctx->far_ci.get();

@@ -73,3 +73,61 @@ def test_mixed_solve(name):
     true = np.linalg.norm(B - S.A(X), axis=0) / np.linalg.norm(B, axis=0)
     print(f"case {name}: mixed solve {A.Solver_IterationCount()} iterations, FP64 residual {true.max():.2e}", flush=True)
     assert np.all(true <= 3e-6), true
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_refined_metric_copies(name):
+    """"mixed-refined": the FP64 values are those of the FP64 metric (same kernel, bit for bit); the
+    FP32 copies and their diagonal give a product within FP32 rounding of the FP64 one, also on a
+    smooth field (the vertex coordinates), where a diagonal inconsistent with the copies would show."""
+    wl, M = _mesh(name)
+    A64 = R.TangentPointMetric0(wl.q, wl.p)
+    Ar = R.TangentPointMetric0(wl.q, wl.p, precision="mixed-refined")
+    rng = np.random.default_rng(5)
+    for label, X in (("random", random_field(M.V, seed=11)),
+                     ("coordinates", M.VertexCoordinates() + 1e-3 * rng.standard_normal((M.V, 3)))):
+        Y64 = np.zeros_like(X); A64.MultiplyMetric(M, 1.0, X, 0.0, Y64)
+        Yr = np.zeros_like(X); Ar.MultiplyMetric(M, 1.0, X, 0.0, Yr)
+        assert np.array_equal(Yr, Y64), label
+        Yi = np.zeros_like(X); Ar.MultiplyMetricInner(M, 1.0, X, 0.0, Yi)
+        e_i = rel(Yi, Y64)
+        e_o = rel(Y64, oracle_multiply(M, wl.q, wl.p, X))
+        print(f"case {name} {label}: FP32-copy product rel err {e_i:.2e} (FP64 vs oracle {e_o:.2e})", flush=True)
+        assert e_o <= 1e-10
+        assert 0 < e_i <= 1e-6
+
+
+def test_refined_copies_need_refined_metric():
+    wl, M = _mesh("torus_2400")
+    A = R.TangentPointMetric0(wl.q, wl.p)
+    X = random_field(M.V, seed=3)
+    with pytest.raises(RuntimeError, match="RP_MIXED_REFINED"):
+        A.MultiplyMetricInner(M, 1.0, X, 0.0, np.zeros_like(X))
+
+
+@pytest.mark.parametrize("name", ["torus_2400", "sphere_1280"])
+@pytest.mark.parametrize("pc_norm", [False, True])
+def test_refined_solve(name, pc_norm):
+    """CG on the FP32 copies, residual replaced by the FP64 one: the FP64 oracle residual meets the
+    tolerance, as in the FP64 solve, and the reported residual is the FP64 one."""
+    wl, M = _mesh(name)
+    E = R.TangentPointEnergy0(wl.q, wl.p)
+    B = E.Differential(M)
+    S = OracleSolve(M, wl.q, wl.p)
+    its = {}
+    for prec in ("fp64", "mixed-refined"):
+        A = R.TangentPointMetric0(wl.q, wl.p, precision=prec)
+        A.SetPreconditionerDiagonalWeight(-1.0)
+        X = np.zeros_like(B)
+        A.Solve(M, 1.0, B, 0.0, X, max_iter=500, tolerance=1e-6, measure="pc" if pc_norm else "l2")
+        its[prec] = A.Solver_IterationCount()
+        true = np.linalg.norm(B - S.A(X), axis=0) / np.linalg.norm(B, axis=0)
+        rep = np.asarray(A.Solver_RelativeResiduals())
+        print(f"case {name} pc_norm={pc_norm} {prec}: {its[prec]} iterations, FP64 residual {true.max():.2e}, "
+              f"reported {rep.max():.2e}", flush=True)
+        if not pc_norm:
+            assert np.all(true <= 1e-6), true
+            assert np.allclose(rep, true, rtol=1e-3, atol=0), (rep, true)
+        else:
+            assert np.all(rep <= 1e-6), rep
+    assert its["mixed-refined"] <= its["fp64"] + 4
