@@ -156,7 +156,6 @@ namespace rp
         const double * pre_vals, double * S_out )
     {
         __shared__ double sx[2][3][PL_CH];
-        __shared__ uint32_t sm[2][PL_CH];
         __shared__ double red[64][3];
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( L );
@@ -171,20 +170,17 @@ namespace rp
         const int64_t M_ = col_ptr[I + 1] - cb;
         const int64_t vb = val_ptr[I];
         double acc[3] = { 0, 0, 0 };
-        double rx[3];
-        uint32_t rm = 0;
+        double rx[3] = { 0, 0, 0 };
         auto gload = [&]( int64_t base )
         {
             const int64_t m = base + lane;
             if( m < M_ )
             {
                 const int32_t j = col_prim[cb + m];
-                rm = col_mask[cb + m];
                 for( int k = 0; k < 3; ++k ) { rx[k] = xin[3 * (int64_t)j + k]; }
             }
-            else { rm = 0u; }
         };
-        auto lstore = [&]( int buf ) { sm[buf][lane] = rm; for( int k = 0; k < 3; ++k ) { sx[buf][k][lane] = rx[k]; } };
+        auto lstore = [&]( int buf ) { for( int k = 0; k < 3; ++k ) { sx[buf][k][lane] = rx[k]; } };
         gload( 0 );
         lstore( 0 );
         __syncthreads();
@@ -196,8 +192,9 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
             if( active )
             {
-                // eight value loads in flight before the (masked) products; every tile slot exists, masked
-                // pairs add 0, the products are summed in column order as one at a time
+                // eight value loads in flight before the products; every tile slot exists and the assembly
+                // stores 0 in the slots of masked pairs, so the products need no mask; the products are
+                // summed in column order as one at a time
                 const VT * vrow = vals + vb + base * nI + ii;
                 int mm = c;
                 for( ; mm + 7 * nS < cnt_c; mm += 8 * nS )
@@ -209,17 +206,13 @@ namespace rp
                     for( int u = 0; u < 8; ++u )
                     {
                         const int m2 = mm + u * nS;
-                        const double vv = ( ( sm[buf][m2] >> ii ) & 1u ) ? v[u] : 0.0;
-                        for( int k = 0; k < 3; ++k ) { acc[k] += vv * sx[buf][k][m2]; }
+                        for( int k = 0; k < 3; ++k ) { acc[k] += v[u] * sx[buf][k][m2]; }
                     }
                 }
                 for( ; mm < cnt_c; mm += nS )
                 {
-                    if( (sm[buf][mm] >> ii) & 1u )
-                    {
-                        const double v = (double)vrow[(int64_t)mm * nI];
-                        for( int k = 0; k < 3; ++k ) { acc[k] += v * sx[buf][k][mm]; }
-                    }
+                    const double v = (double)vrow[(int64_t)mm * nI];
+                    for( int k = 0; k < 3; ++k ) { acc[k] += v * sx[buf][k][mm]; }
                 }
             }
             if( more ) { lstore( buf ^ 1 ); }
