@@ -103,48 +103,59 @@ namespace rp
         for( int k = 0; k < 3; ++k ) { bp[3 * j + k] = alpha * B[ldB * o + k]; }
     }
 
-    // Forward, levels with r <= G: a group of G lanes per supernode (64 / G supernodes per wavefront),
-    // a lane per front row.
+    // Forward, small levels (at most CHOL_SMALL_ROWS rows per supernode): a group of G lanes per
+    // supernode (64 / G supernodes per wavefront), lane a taking the front rows a, a + G, ...; G follows
+    // the level's typical front, not its largest, so that few lanes idle.
     template<int G>
-    __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const CholTask * desc, int64_t count )
+    __global__ __launch_bounds__(256) void k_chol_fwd_small( CholDev F, const CholTask * desc, int64_t count, int rs )
     {
-        __shared__ double fr_all[3 * 256];
+        extern __shared__ double fr_all[];   // (256 / G) x 3 rs, rs = the level's largest row count
         const int lane = threadIdx.x % G;
         const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
-        double * fr = fr_all + 3 * ( threadIdx.x - lane );
+        double * fr = fr_all + 3 * rs * (int)( threadIdx.x / G );
         const CholTask d = desc[t];
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
         const double * Ls = F.vals + d.vp;
-        const int a = lane;
-        const int ce = ( a < r ) ? ( ( a < w ) ? a + 1 : w ) : 0;
-        // the lane's first block values go out before the front merge
+        // the first row's first block values go out before the front merge
         constexpr int PFS = 16;
+        const int ce0 = ( lane < r ) ? ( ( lane < w ) ? lane + 1 : w ) : 0;
         double lp[PFS];
         #pragma unroll
-        for( int u = 0; u < PFS; ++u ) { lp[u] = ( u < ce ) ? Ls[(int64_t)u * r + a] : 0.0; }
-        double v[3] = { 0.0, 0.0, 0.0 };
-        if( lane < r ) { chol_front_row( F, rb, f, w, lane, v ); }
-        for( int k = 0; k < 3; ++k ) { fr[3 * lane + k] = v[k]; }
+        for( int u = 0; u < PFS; ++u ) { lp[u] = ( u < ce0 ) ? Ls[(int64_t)u * r + lane] : 0.0; }
+        for( int a = lane; a < r; a += G )
+        {
+            double v[3];
+            chol_front_row( F, rb, f, w, a, v );
+            for( int k = 0; k < 3; ++k ) { fr[3 * a + k] = v[k]; }
+        }
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-        if( lane >= r ) { return; }
-        double acc[3] = { 0.0, 0.0, 0.0 };
-        #pragma unroll
-        for( int u = 0; u < PFS; ++u )
-        {
-            if( u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * fr[3 * u + k]; } }
-        }
-        for( int c = PFS; c < ce; ++c )
-        {
-            const double l = Ls[(int64_t)c * r + a];
-            for( int k = 0; k < 3; ++k ) { acc[k] += l * fr[3 * c + k]; }
-        }
         double * z = F.z + 3 * rb;
-        if( a < w ) { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = acc[k]; } }
-        else        { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = v[k] - acc[k]; } }
+        for( int a = lane; a < r; a += G )
+        {
+            const int ce = ( a < w ) ? a + 1 : w;
+            double acc[3] = { 0.0, 0.0, 0.0 };
+            int c = 0;
+            if( a == lane )
+            {
+                #pragma unroll
+                for( int u = 0; u < PFS; ++u )
+                {
+                    if( u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * fr[3 * u + k]; } }
+                }
+                c = PFS;
+            }
+            for( ; c < ce; ++c )
+            {
+                const double l = Ls[(int64_t)c * r + a];
+                for( int k = 0; k < 3; ++k ) { acc[k] += l * fr[3 * c + k]; }
+            }
+            if( a < w ) { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = acc[k]; } }
+            else        { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = fr[3 * a + k] - acc[k]; } }
+        }
     }
 
     // Forward, larger levels: one workgroup of NW waves per task (supernode, 64 front rows); the waves
@@ -207,18 +218,18 @@ namespace rp
     }
 
     // ---------------------------------------------------------------------------------------------
-    // Backward, levels with r <= G: a group of G lanes per supernode; the lanes stage v in LDS, then lane
-    // c < w forms x_c = sum over rows i >= c of L(i, c) v_i, reading the transposed copy of the block
-    // (row-major, so the lanes of a row read consecutive addresses), its first rows loaded before the
-    // staging.
+    // Backward, small levels: a group of G lanes per supernode (G at least the level's widest supernode);
+    // the lanes stage v in LDS (rows lane, lane + G, ...), then lane c < w forms
+    // x_c = sum over rows i >= c of L(i, c) v_i, reading the transposed copy of the block (row-major, so
+    // the lanes of a row read consecutive addresses), its first rows loaded before the staging.
     template<int G>
-    __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count )
+    __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count, int rs )
     {
-        __shared__ double v_all[3 * 256];
+        extern __shared__ double v_all[];   // (256 / G) x 3 rs, rs = the level's largest row count
         const int lane = threadIdx.x % G;
         const int64_t t = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( t >= count ) { return; }
-        double * vs = v_all + 3 * ( threadIdx.x - lane );
+        double * vs = v_all + 3 * rs * (int)( threadIdx.x / G );
         const CholTask d = desc[t];
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
@@ -227,11 +238,14 @@ namespace rp
         double lp[PFS];
         #pragma unroll
         for( int u = 0; u < PFS; ++u ) { lp[u] = ( lane < w && u >= lane && u < r ) ? Lt[(int64_t)u * w + lane] : 0.0; }
-        if( lane < w ) { for( int k = 0; k < 3; ++k ) { vs[3 * lane + k] = F.z[3 * ( rb + lane ) + k]; } }
-        else if( lane < r )
+        for( int i = lane; i < r; i += G )
         {
-            const int64_t j = F.rows[rb + lane];
-            for( int k = 0; k < 3; ++k ) { vs[3 * lane + k] = -F.xp[3 * j + k]; }
+            if( i < w ) { for( int k = 0; k < 3; ++k ) { vs[3 * i + k] = F.z[3 * ( rb + i ) + k]; } }
+            else
+            {
+                const int64_t j = F.rows[rb + i];
+                for( int k = 0; k < 3; ++k ) { vs[3 * i + k] = -F.xp[3 * j + k]; }
+            }
         }
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
         __builtin_amdgcn_wave_barrier();
@@ -375,12 +389,15 @@ namespace rp
         hipLaunchKernelGGL( k_chol_gather, dim3( (unsigned)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, n, perm, alpha, B, ldB, bp );
     }
 
-    void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s )
+    void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        if( rmax <= 16 )      { hipLaunchKernelGGL( k_chol_fwd_small<16>, dim3( (unsigned)( ( count + 15 ) / 16 ) ), dim3( 256 ), 0, s, F, nodes, count ); }
-        else if( rmax <= 32 ) { hipLaunchKernelGGL( k_chol_fwd_small<32>, dim3( (unsigned)( ( count + 7 ) / 8 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
-        else                  { hipLaunchKernelGGL( k_chol_fwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
+        const unsigned nb = (unsigned)( ( count * G + 255 ) / 256 );
+        const size_t shm = sizeof(double) * 3 * (size_t)rmax * (size_t)( 256 / G );
+        if( G <= 8 )       { hipLaunchKernelGGL( k_chol_fwd_small<8>,  dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
+        else if( G <= 16 ) { hipLaunchKernelGGL( k_chol_fwd_small<16>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
+        else if( G <= 32 ) { hipLaunchKernelGGL( k_chol_fwd_small<32>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
+        else               { hipLaunchKernelGGL( k_chol_fwd_small<64>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
     }
 
     void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s )
@@ -390,12 +407,15 @@ namespace rp
         else       { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
     }
 
-    void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int rmax, hipStream_t s )
+    void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        if( rmax <= 16 )      { hipLaunchKernelGGL( k_chol_bwd_small<16>, dim3( (unsigned)( ( count + 15 ) / 16 ) ), dim3( 256 ), 0, s, F, nodes, count ); }
-        else if( rmax <= 32 ) { hipLaunchKernelGGL( k_chol_bwd_small<32>, dim3( (unsigned)( ( count + 7 ) / 8 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
-        else                  { hipLaunchKernelGGL( k_chol_bwd_small<64>, dim3( (unsigned)( ( count + 3 ) / 4 ) ),   dim3( 256 ), 0, s, F, nodes, count ); }
+        const unsigned nb = (unsigned)( ( count * G + 255 ) / 256 );
+        const size_t shm = sizeof(double) * 3 * (size_t)rmax * (size_t)( 256 / G );
+        if( G <= 8 )       { hipLaunchKernelGGL( k_chol_bwd_small<8>,  dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
+        else if( G <= 16 ) { hipLaunchKernelGGL( k_chol_bwd_small<16>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
+        else if( G <= 32 ) { hipLaunchKernelGGL( k_chol_bwd_small<32>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
+        else               { hipLaunchKernelGGL( k_chol_bwd_small<64>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
     }
 
     void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s )

@@ -271,6 +271,7 @@ struct rp_ctx
     std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
     std::vector<int>     h1_level_wmax;
     std::vector<int>     h1_level_rmax;
+    std::vector<int>     h1_level_g;       // small levels: lanes per supernode
     std::vector<char>    h1_level_small;           // levels run a wavefront per supernode
     rp::DBuf<rp::CholTask> h1_tasks;               // forward, backward and M-forming tasks
     rp::DBuf<rp::CholTask> h1_sdesc;               // per supernode, in level order

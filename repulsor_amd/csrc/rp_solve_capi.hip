@@ -72,7 +72,7 @@ namespace rp
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_fwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
+                launch_chol_fwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_g[l], ctx->h1_level_rmax[l], ctx->stream );
             }
             else
             {
@@ -85,7 +85,7 @@ namespace rp
             if( ctx->h1_level_small[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
-                launch_chol_bwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_rmax[l], ctx->stream );
+                launch_chol_bwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_g[l], ctx->h1_level_rmax[l], ctx->stream );
             }
             else
             {
@@ -752,7 +752,8 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         for( auto h : height ) { ++lptr[(size_t)h + 1]; }
         for( int32_t l = 0; l < nl; ++l ) { lptr[(size_t)l + 1] += lptr[(size_t)l]; }
         std::vector<int32_t> lnodes( (size_t)ns );
-        std::vector<int> wmax( (size_t)nl, 1 ), rmax( (size_t)nl, 1 );
+        std::vector<int> wmax( (size_t)nl, 1 ), rmax( (size_t)nl, 1 ), gsz( (size_t)nl, 64 );
+        std::vector<int64_t> rsum( (size_t)nl, 0 );
         {
             std::vector<int64_t> pos( lptr.begin(), lptr.end() - 1 );
             for( int32_t s = 0; s < ns; ++s )
@@ -761,9 +762,20 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 lnodes[(size_t)pos[(size_t)h]++] = s;
                 wmax[(size_t)h] = std::max( wmax[(size_t)h], sn_begin[s + 1] - sn_begin[s] );
                 rmax[(size_t)h] = std::max( rmax[(size_t)h], (int)( row_ptr[s + 1] - row_ptr[s] ) );
+                rsum[(size_t)h] += row_ptr[s + 1] - row_ptr[s];
+            }
+            // small levels: G lanes per supernode, the power of two covering the level's mean row count and
+            // its widest supernode (one lane per column backward), at least 8; rows beyond G loop
+            for( int32_t l = 0; l < nl; ++l )
+            {
+                const int64_t cnt = lptr[(size_t)l + 1] - lptr[(size_t)l];
+                const int64_t need = std::max<int64_t>( { (int64_t)8, (int64_t)wmax[(size_t)l], cnt > 0 ? ( rsum[(size_t)l] + cnt - 1 ) / cnt : 1 } );
+                int g = 8;
+                while( g < need && g < 64 ) { g <<= 1; }
+                gsz[(size_t)l] = g;
             }
         }
-        // Levels whose supernodes all have at most CHOL_SMALL_ROWS rows run a wavefront per supernode;
+        // Levels whose supernodes all have at most CHOL_SMALL_ROWS rows run a lane group per supernode;
         // the others run task lists: forward (supernode, 64 front rows), backward (supernode, as many
         // columns as groups of G = min(64, r rounded up to a power of two) lanes fit in 256 threads).
         // A third list, (supernode, 64 update rows) over all supernodes, forms M = L_off inv(L_ss).
@@ -851,6 +863,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_level_ptr = lptr;
         ctx->h1_level_wmax = wmax;
         ctx->h1_level_rmax = rmax;
+        ctx->h1_level_g = gsz;
         ctx->h1_level_small = small;
         for( int kind = 0; kind < 3; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
         ctx->h1_n = n;
