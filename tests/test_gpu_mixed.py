@@ -136,3 +136,25 @@ def test_refined_solve(name, pc_norm):
         else:
             assert np.all(rep <= 1e-6), rep
     assert its["mixed-refined"] <= its["fp64"] + 4
+
+
+def test_refined_solve_iteration_cap():
+    """Iteration cap reached while CG still multiplies with the FP32 copies: the solve reports
+    stagnation, and the residual it reports is the FP64 one of the iterate it returns."""
+    wl, M = _mesh("torus_2400")
+    E = R.TangentPointEnergy0(wl.q, wl.p)
+    B = E.Differential(M)
+    A = R.TangentPointMetric0(wl.q, wl.p, precision="mixed-refined")
+    A.SetPreconditionerDiagonalWeight(-1.0)
+    X = np.zeros_like(B)
+    A.Solve(M, 1.0, B, 0.0, X, max_iter=3, tolerance=1e-6, raise_on_stagnation=False)
+    S = OracleSolve(M, wl.q, wl.p)
+    true = np.linalg.norm(B - S.A(X), axis=0) / np.linalg.norm(B, axis=0)
+    rep = np.asarray(A.Solver_RelativeResiduals())
+    print(f"capped refined solve: {A.Solver_IterationCount()} iterations, FP64 residual {true.max():.2e}, "
+          f"reported {rep.max():.2e}", flush=True)
+    assert A.Solver_IterationCount() == 3
+    assert np.all(true > 1e-6)
+    assert np.allclose(rep, true, rtol=1e-6, atol=0), (rep, true)
+    with pytest.raises(R.RepulsorError):
+        A.Solve(M, 1.0, B, 0.0, X, max_iter=3, tolerance=1e-6)
