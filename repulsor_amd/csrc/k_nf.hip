@@ -278,7 +278,9 @@ namespace rp
         // 32 columns per chunk and the row sums by lane shuffles: 6.5 KB of LDS per wavefront, so that
         // 24 leaves share a CU
         constexpr int CS = 32;
-        __shared__ double sx[2][12][CS];
+        // input blocks column by column (96 bytes each, 16-byte aligned): a lane reads a column's block
+        // with six 16-byte loads; the distinct columns of a wavefront start on distinct bank groups
+        __shared__ double2 sx[2][CS][6];
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( L );
         if( I < 0 ) { return; }
@@ -317,7 +319,7 @@ namespace rp
         auto lstore = [&]( int buf )
         {
             #pragma unroll
-            for( int k = 0; k < 6; ++k ) { sx[buf][6 * half + k][lc] = rx[k]; }
+            for( int q = 0; q < 3; ++q ) { sx[buf][lc][3 * half + q] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
         };
         gload( 0 );
         lstore( 0 );
@@ -358,11 +360,14 @@ namespace rp
                         const double b0 = (double)a0[u].x, b1 = (double)a0[u].y;
                         const double e0 = (double)a1[u].x, e1 = (double)a1[u].y;
                         #pragma unroll
-                        for( int k = 0; k < 12; ++k )
+                        for( int q = 0; q < 6; ++q )
                         {
-                            const double x = sx[buf][k][m2];
-                            acc0[k] += ( k < 3 ? b0 : b1 ) * x;
-                            acc1[k] += ( k < 3 ? e0 : e1 ) * x;
+                            const double2 x2 = sx[buf][m2][q];
+                            const int k = 2 * q;
+                            acc0[k]     += ( k < 3 ? b0 : b1 ) * x2.x;
+                            acc1[k]     += ( k < 3 ? e0 : e1 ) * x2.x;
+                            acc0[k + 1] += ( k + 1 < 3 ? b0 : b1 ) * x2.y;
+                            acc1[k + 1] += ( k + 1 < 3 ? e0 : e1 ) * x2.y;
                         }
                     }
                 }
