@@ -281,11 +281,11 @@ namespace rp
     }
 
     // Backward, larger levels: one workgroup per task (supernode, group of columns), a group of G lanes
-    // per column. On the wide levels (few tasks, latency-bound) supernodes with at most CHOL_BWD_ONE rows
-    // stage all of v in LDS (sized per launch) and load the lane's block values before staging; otherwise v
-    // is staged in chunks of CHOL_QC rows.
+    // per column. On the wide levels (few tasks, latency-bound) a task whose rows of v (r - k) fit the
+    // lds_rows rows of LDS the launch allocated stages all of v at once and loads the lane's block values
+    // before staging; otherwise v is staged in chunks of CHOL_QC rows (lds_rows >= CHOL_QC then).
     template<bool WIDE>
-    __global__ __launch_bounds__(256) void k_chol_bwd_big( CholDev F, const CholTask * tasks )
+    __global__ __launch_bounds__(256) void k_chol_bwd_big( CholDev F, const CholTask * tasks, int lds_rows )
     {
         extern __shared__ double v[];
         const CholTask tk = tasks[blockIdx.x];
@@ -298,7 +298,7 @@ namespace rp
         const int c = tk.k + grp;
         const double * col = F.vals + tk.vp + (int64_t)c * r;
         double acc[3] = { 0.0, 0.0, 0.0 };
-        if( WIDE && r <= CHOL_BWD_ONE )
+        if( WIDE && r - tk.k <= lds_rows )
         {
             const int i0 = tk.k;          // no column of the task needs rows above its first column
             const int ib = ( c < w ) ? c : r;
@@ -423,10 +423,14 @@ namespace rp
         if( count <= 0 ) { return; }
         if( wide )
         {
-            const size_t shm = sizeof(double) * 3 * (size_t)( rmax <= CHOL_BWD_ONE ? rmax : CHOL_QC );
-            hipLaunchKernelGGL( k_chol_bwd_big<true>, dim3( (unsigned)count ), dim3( 256 ), shm, s, F, tasks );
+            // one rule for launch and kernel: a task stages v at once iff r - k <= lds_rows; levels with
+            // longer supernodes get CHOL_BWD_ONE rows, enough for the chunks of CHOL_QC rows of the others
+            const int lds_rows = rmax <= CHOL_BWD_ONE ? std::max( rmax, 1 ) : CHOL_BWD_ONE;
+            static_assert( CHOL_BWD_ONE >= CHOL_QC, "the staging buffer must hold a chunk" );
+            const size_t shm = sizeof(double) * 3 * (size_t)lds_rows;
+            hipLaunchKernelGGL( k_chol_bwd_big<true>, dim3( (unsigned)count ), dim3( 256 ), shm, s, F, tasks, lds_rows );
         }
-        else { hipLaunchKernelGGL( k_chol_bwd_big<false>, dim3( (unsigned)count ), dim3( 256 ), sizeof(double) * 3 * CHOL_QC, s, F, tasks ); }
+        else { hipLaunchKernelGGL( k_chol_bwd_big<false>, dim3( (unsigned)count ), dim3( 256 ), sizeof(double) * 3 * CHOL_QC, s, F, tasks, CHOL_QC ); }
     }
 
     void launch_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t, hipStream_t s )
