@@ -174,6 +174,20 @@ int rp_download_state_rows( rp_ctx * ctx, double * P_near, double * C_far, doubl
  * Replaces: TP0_Traversor<...>::Compute (reference src/Energies/TangentPointEnergy0/TP0_Traversor.hpp:129-253). */
 int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precision, double * energy_out );
 
+/* Doubles of device scratch that rp_tp0_energy_noalloc can use (16 MiB per device). */
+#define RP_NOALLOC_SCRATCH_DOUBLES (2 * 1024 * 1024)
+
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the energy of rp_tp0_compute(ctx,
+ * RP_ENERGY, q, p, precision, energy_out), bit for bit, computed by the same kernels. It calls no hipMalloc or
+ * hipFree: its energy partials (one per leaf, cluster and very-near row), reduction blocks and very-near
+ * per-entry records (24 doubles per entry) live in a static device array of RP_NOALLOC_SCRATCH_DOUBLES
+ * doubles that each device holds until the process ends. A mesh that needs more returns RP_EINVAL with the
+ * size in the message (1.31M triangles with split_threshold 16 need about 0.34M doubles). All calls on
+ * one device share that array: calls that overlap on one device race. energy_out is a host pointer, or a
+ * device pointer with RP_DEVICE_PTR. The context must hold the uploaded mesh, tree, patterns and data. */
+int rp_tp0_energy_noalloc( rp_ctx * ctx, double q, double p, int precision, double * energy_out, int ptr_flags );
+
 /* Moment derivatives of the last RP_DIFF compute: P_D_near T x ND, P_D_far T x 10, original order.
  * Replaces: ClusterTree::CollectDerivatives (reference src/ClusterTree.hpp:1115-1208). */
 int rp_download_derivatives( rp_ctx * ctx, double * P_D_near, double * P_D_far );

@@ -5,9 +5,39 @@
 #include "rp_launch.hpp"
 
 #include <cstring>
+#include <mutex>
 
 namespace rp
 {
+    // Scratch of rp_tp0_energy_noalloc: one array per device (each device holds its own copy until the
+    // process ends). Host code never passes the array's name to a kernel or a copy: it uses the address
+    // that hipGetSymbolAddress returns on each device (scratch_address), looked up once per device when
+    // a context is created there. Every call on a device shares the array: overlapping calls on one
+    // device race.
+    constexpr int RP_SCRATCH_DEVICES = 64;
+    static __device__ __align__(256) double scratch[RP_NOALLOC_SCRATCH_DOUBLES];
+    static std::mutex scratch_mutex;
+    static double * scratch_ptr[RP_SCRATCH_DEVICES] = {};
+
+    // Looks up (once) the address of `scratch` on the current device `device`.
+    static void scratch_setup( int device )
+    {
+        if( device < 0 || device >= RP_SCRATCH_DEVICES ) { return; }
+        std::lock_guard<std::mutex> lock( scratch_mutex );
+        if( scratch_ptr[device] ) { return; }
+        void * p = nullptr;
+        RP_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL(scratch) ) );
+        require( p != nullptr, "hipGetSymbolAddress returned no address for the scratch array", RP_EHIP );
+        scratch_ptr[device] = static_cast<double *>( p );
+    }
+
+    static double * scratch_address( int device )
+    {
+        if( device < 0 || device >= RP_SCRATCH_DEVICES ) { return nullptr; }
+        std::lock_guard<std::mutex> lock( scratch_mutex );
+        return scratch_ptr[device];
+    }
+
     static thread_local std::string g_last_error;
     void set_error( const std::string & msg ) { g_last_error = msg; }
 
@@ -93,6 +123,7 @@ int rp_ctx_create( int device, rp_ctx ** out )
         RP_HIP( hipGetDeviceCount( &count ) );
         require( device >= 0 && device < count, "no such HIP device", RP_EHIP );
         RP_HIP( hipSetDevice( device ) );
+        scratch_setup( device );
         rp_ctx * c = new rp_ctx();
         c->device = device;
         try
@@ -763,6 +794,94 @@ int rp_tp0_compute( rp_ctx * ctx, unsigned flags, double q, double p, int precis
         if( energy_out ) { *energy_out = E ? en : 0.0; }
         if( D ) { ctx->have_diff = true; ctx->have_vf_dh = have_vf; }
         if( M ) { ctx->have_metric = true; ctx->have_vf_metric = have_vf; ctx->metric_precision = precision; ++ctx->metric_version; }
+    });
+}
+
+int rp_tp0_energy_noalloc( rp_ctx * ctx, double q, double p, int precision, double * energy_out, int ptr_flags )
+{
+    return guarded( "rp_tp0_energy_noalloc", [&]()
+    {
+        require( ctx != nullptr && ctx->have_data && ctx->have_near && ctx->have_far, "mesh, tree, patterns and data must be uploaded", RP_ENOTREADY );
+        require( precision == RP_FP64 || precision == RP_MIXED || precision == RP_MIXED_REFINED, "unknown precision" );
+        require( q >= 0, "Tangent-point kernels not well-defined for q < 0." );
+        require( q == 0 || q >= 2, "Tangent-point kernels not implement for 0 < q < 2." );
+        require( energy_out != nullptr, "energy_out is NULL" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        double * base = scratch_address( ctx->device );
+        require( base != nullptr, "no scratch array on this device (device index beyond 63)", RP_EUNSUPPORTED );
+        const int n = ctx->dom_dim;
+        const int64_t T = ctx->T, C = ctx->C, L = ctx->L;
+        const bool have_vf = ctx->vf_nnz > 0;
+        const int64_t R = have_vf ? ctx->vf_nrows : 0;
+        const int64_t nnz = have_vf ? ctx->vf_nnz : 0;
+        // scratch layout, each part starting on a 256-byte boundary: energy partials (leaves, clusters,
+        // very-near rows), the reduction (1 + REDUCE_BLOCKS) and the very-near statistics (3), the
+        // very-near per-entry records (VF_REC per entry)
+        auto up32 = []( int64_t x ) { return ( x + 31 ) / 32 * 32; };
+        const int64_t o_scalar = up32( L + C + R );
+        const int64_t o_rec = o_scalar + up32( 1 + REDUCE_BLOCKS + 3 );
+        const int64_t need = o_rec + VF_REC * nnz;
+        require( need <= RP_NOALLOC_SCRATCH_DOUBLES, "mesh needs " + std::to_string( need ) + " doubles of scratch; rp_tp0_energy_noalloc holds at most " +
+                 std::to_string( (long long)RP_NOALLOC_SCRATCH_DOUBLES ) + " (use rp_tp0_compute)" );
+        double * partials = base;
+        double * scalar = base + o_scalar;
+        double * vf_stat = scalar + 1 + REDUCE_BLOCKS;
+        double * rec = base + o_rec;
+        const double s = (p - n) / q;
+        const double s_exp = -0.5 * ( 2.0 * (s - 1.0) + n );
+        const Expo ex = make_expo( q, p, n, s_exp );
+        const int kind = exponent_kind( q, p );
+        const bool mixed = ( precision == RP_MIXED );
+        // the array keeps the last call's contents: the partials start at zero
+        launch_fill( partials, L + C, 0.0, ctx->stream );
+        if( ctx->nf_cols > 0 )
+        {
+            NFArgs A{};
+            A.L = L; A.T = T;
+            A.leaf_begin = ctx->leaf_begin.get(); A.leaf_end = ctx->leaf_end.get();
+            A.col_ptr = ctx->col_ptr.get(); A.col_prim = ctx->col_prim.get(); A.col_mask = ctx->col_mask.get(); A.val_ptr = ctx->val_ptr.get();
+            A.a = ctx->p_a.get(); A.x = ctx->p_x.get(); A.P = ctx->p_P.get(); A.rec = ctx->p_rec.get(); A.origin = ctx->leaf_origin.get();
+            A.scale = 1.0 / ctx->size; A.size = ctx->size; A.ex = ex;
+            A.epart = partials;
+            launch_nf( RP_ENERGY, kind, mixed, A, ctx->stream );
+            RP_HIP( hipGetLastError() );
+        }
+        if( ctx->far_nnz > 0 )
+        {
+            FFArgs B{};
+            B.C = C; B.rp = ctx->far_rp.get(); B.ci = ctx->far_ci.get();
+            B.a = ctx->c_a.get(); B.x = ctx->c_x.get(); B.P = ctx->c_P.get(); B.ex = ex;
+            B.epart = partials + L;
+            launch_ff( RP_ENERGY, kind, mixed, B, ctx->stream );
+            RP_HIP( hipGetLastError() );
+        }
+        if( have_vf )
+        {
+            VFArgs A{};
+            A.nnz = nnz; A.T = T; A.row = ctx->vf_row.get(); A.col = ctx->vf_ci.get();
+            A.P_ordering = ctx->P_ordering.get(); A.simplices = ctx->simplices.get(); A.coords = ctx->coords.get();
+            A.a = ctx->p_a.get(); A.P = ctx->p_P.get();
+            A.theta2 = ctx->vf_theta * ctx->vf_theta; A.max_ref = ctx->vf_max_ref; A.ex = ex;
+            A.energy = true; A.diff = false; A.metric = false; A.pl_sexp = 0.0;
+            A.cap = VF_LANE_EVALUATION_LIMIT; A.rec = rec;
+            launch_vf_pairs( n, 0, A, ctx->stream );
+            VFRowArgs W{};
+            W.nrows = R; W.T = T; W.rows = ctx->vf_rows.get(); W.rp = ctx->vf_rp.get(); W.dg = ctx->vf_dg.get(); W.tr = ctx->vf_tr.get();
+            W.rec = rec;
+            W.epart = partials + L + C;
+            launch_vf_rows( n, 0, W, ctx->stream );
+            launch_vf_stats( nnz, rec, vf_stat, ctx->stream );
+            RP_HIP( hipGetLastError() );
+        }
+        launch_reduce_sum( partials, L + C + R, scalar, ctx->stream );
+        RP_HIP( hipGetLastError() );
+        const hipMemcpyKind dir = ( ptr_flags & RP_DEVICE_PTR ) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        RP_HIP( hipMemcpyAsync( energy_out, scalar, sizeof(double), dir, ctx->stream ) );
+        double st[3] = { 0, 0, 0 };
+        if( have_vf ) { RP_HIP( hipMemcpyAsync( st, vf_stat, sizeof(st), hipMemcpyDeviceToHost, ctx->stream ) ); }
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        require( st[2] == 0.0, "very-near-field subdivision exceeded 2^24 evaluations on one level-1 sub-pair "
+                 "(primitives almost touching); refine the mesh or lower adaptivity_settings.max_refinement", RP_EUNSUPPORTED );
     });
 }
 
