@@ -154,7 +154,7 @@ namespace rp
                         #pragma unroll
                         for( int k = 0; k < 6; ++k ) { Q[k] = sh.d[buf][4 + k][mm]; }
                         PairTerms<R> t;
-                        pair_terms<R,KIND,E,D,M>( A.ex, v, P, Q, t );
+                        pair_terms<R,KIND,E,D,M,true>( A.ex, v, P, Q, t );
                         if constexpr ( E ) { if( j > i ) { part[12] += a * t.E * b; } }
                         if constexpr ( D )
                         {

@@ -14,6 +14,8 @@
 
 #include "rp_internal.hpp"
 
+#include <type_traits>
+
 namespace rp
 {
     // Leaf of a leaf-tile workgroup. Workgroups are dealt to the 8 XCDs round robin (workgroup b runs on
@@ -80,7 +82,9 @@ namespace rp
         R a0, a1;   // metric block
     };
 
-    template<typename R, int KIND, bool NEED_E, bool NEED_D, bool NEED_M>
+    // PROJ: P and Q are orthogonal projectors (primitive rows; cluster rows hold area-weighted averages of
+    // projectors, which are not).
+    template<typename R, int KIND, bool NEED_E, bool NEED_D, bool NEED_M, bool PROJ = false>
     __device__ __forceinline__ void pair_terms( const Expo & ex, const R v[3], const R P[6], const R Q[6], PairTerms<R> & t )
     {
         const R Pv0 = P[0] * v[0] + P[1] * v[1] + P[2] * v[2];
@@ -90,8 +94,21 @@ namespace rp
         const R Qv1 = Q[1] * v[0] + Q[3] * v[1] + Q[4] * v[2];
         const R Qv2 = Q[2] * v[0] + Q[4] * v[1] + Q[5] * v[2];
         const R r2  = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        const R phi = v[0] * Pv0 + v[1] * Pv1 + v[2] * Pv2;
-        const R psi = v[0] * Qv0 + v[1] * Qv1 + v[2] * Qv2;
+        // phi = <v, P v>. For an orthogonal projector (P = P^T P) <v, P v> = |P v|^2. FP64 keeps the
+        // reference's form; FP32 on projectors uses |P v|^2: for close pairs v lies almost in the tangent
+        // space, and <v, P v> then sums terms of size |v|^2 to a result of size |v|^2 theta^2 (relative
+        // rounding about eps / theta^2), while |P v|^2 squares components of size |v| theta (eps / theta).
+        R phi, psi;
+        if constexpr ( PROJ && std::is_same<R, float>::value )
+        {
+            phi = Pv0 * Pv0 + Pv1 * Pv1 + Pv2 * Pv2;
+            psi = Qv0 * Qv0 + Qv1 * Qv1 + Qv2 * Qv2;
+        }
+        else
+        {
+            phi = v[0] * Pv0 + v[1] * Pv1 + v[2] * Pv2;
+            psi = v[0] * Qv0 + v[1] * Qv1 + v[2] * Qv2;
+        }
 
         if constexpr ( NEED_E || NEED_D )
         {

@@ -2,9 +2,11 @@
 against the FP64 oracle: energy, differential, metric product, pseudo-Laplacian product, and a
 solve whose residual is measured with the FP64 oracle metric.
 
-Stated errors (relative, 2-norm; measured on MI355X in brackets): energy <= 1e-7 [0.9-4.8e-9],
-differential <= 1e-5 for surfaces [0.9-1.8e-6] and <= 2e-4 for curves [7.2e-5], metric product
-<= 1e-6 [4.5-8.3e-8], pseudo-Laplacian product <= 1e-6 [1.4-2.5e-8]. The mixed solve stops at
+Stated errors (relative, 2-norm; measured on MI355X in brackets): energy <= 1e-7 [0.2-4.2e-9],
+differential <= 1e-5 for surfaces [0.7-1.4e-6] and <= 2e-4 for curves [7.2e-5], metric product
+<= 1e-6 [4.5-8.3e-8], pseudo-Laplacian product <= 1e-6 [1.4-2.5e-8]. The near-field FP32 kernel forms
+<v, P v> as |P v|^2 (P a projector), which keeps the rounding of close, almost tangential pairs at
+about eps / theta instead of eps / theta^2; larger meshes are covered in test_gpu_fullsize.py. The mixed solve stops at
 1e-6 on its FP32-valued operator; its residual against the FP64 operator is <= 3e-6 [1.2-1.7e-6].
 
 "mixed-refined" (FP64 metric plus FP32 copies of the near values for the CG iterations): the FP64

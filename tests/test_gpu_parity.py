@@ -124,3 +124,42 @@ def test_repeatability_bitwise():
     Y2, e2 = E.Differential(M, return_energy=True)
     assert e1 == e2
     assert np.array_equal(Y1, Y2)
+
+
+# ------------------------------------------------------------------------------------------------ real exponents
+# The run-time exponent kernel (KIND_GENERIC: real q/2 and p/2 through pow, q = 0) against the oracle,
+# which restates the reference's Power as binary powering for integer and std::pow for real exponents
+# (TP0_Traversor.hpp:169-245). Tolerances as above: the device pow and the host std::pow differ by ulps.
+GENERIC = [
+    ("torus_2400", 5.5, 11.3, True),       # real q/2 and p/2, metric s_exp through pow
+    ("trefoil_3000", 3.0, 5.5, True),       # curve, odd q
+    ("torus_2400", 0.0, 4.5, False),        # q = 0: E = 2 r^-p, no projector terms (metric undefined: s = inf)
+]
+
+
+@pytest.mark.parametrize("name,q,p,metric", GENERIC)
+def test_generic_exponents(name, q, p, metric):
+    wl, M = _mesh(name)
+    print(f"case {name} q={q} p={p}", flush=True)
+    E = R.TangentPointEnergy0(q, p)
+    Y, en = E.Differential(M, return_energy=True)
+    en_ref, Y_ref, _, _ = oracle_energy_diff(M, q, p)
+    assert abs(en - en_ref) <= 1e-10 * abs(en_ref)
+    assert rel(Y, Y_ref) <= 1e-10
+    if metric:
+        A = R.TangentPointMetric0(q, p)
+        X = random_field(M.V, seed=21)
+        Z = np.zeros_like(X)
+        A.MultiplyMetric(M, 1.0, X, 0.0, Z)
+        assert rel(Z, oracle_multiply(M, q, p, X)) <= 1e-10
+
+
+def test_single_parameter_constructor_odd_p():
+    """TangentPointEnergy0(p) means q = p, p = 2p (TangentPointEnergy0.hpp:35-39): p = 3 gives q/2 = 1.5."""
+    wl, M = _mesh("sphere_1280")
+    E = R.TangentPointEnergy0(3.0)
+    assert (E.q, E.p) == (3.0, 6.0)
+    Y, en = E.Differential(M, return_energy=True)
+    en_ref, Y_ref, _, _ = oracle_energy_diff(M, 3.0, 6.0)
+    assert abs(en - en_ref) <= 1e-10 * abs(en_ref)
+    assert rel(Y, Y_ref) <= 1e-10
