@@ -4,11 +4,16 @@
 
 #include "Common.hpp"
 #include "Settings.hpp"
+#include "Tensors.hpp"
 #include "MeshGenerators.hpp"
 #include "SimplexData.hpp"
 #include "PrimitiveDistance.hpp"
 #include "ClusterTree.hpp"
 #include "BlockClusterTree.hpp"
+#include "SimplicialMeshBase.hpp"
 #include "SimplicialMesh.hpp"
+#include "SimplicialMesh_Factory.hpp"
+#include "EnergyBase.hpp"
+#include "MetricBase.hpp"
 #include "TangentPointEnergy0.hpp"
 #include "TangentPointMetric0.hpp"

@@ -7,6 +7,7 @@
 
 #include "Common.hpp"
 #include "Settings.hpp"
+#include "SimplicialMeshBase.hpp"
 #include "SimplexData.hpp"
 #include "ClusterTree.hpp"
 #include "BlockClusterTree.hpp"
@@ -44,8 +45,13 @@ namespace Repulsor
         return true;
     }
 
-    template<int DOM_DIM_, int AMB_DIM_ = 3>
-    class SimplicialMesh
+    // SimplicialMesh<DOM_DIM,AMB_DIM,Real,Int,LInt,SReal,ExtReal> (reference src/SimplicialMesh.hpp:13-869)
+    // for curves (DOM_DIM 1) and surfaces (DOM_DIM 2) in R^3. The template parameters after AMB_DIM are the
+    // interface types of SimplicialMeshBase (defaults: double, int32, int64); internal arrays use int32
+    // indices and int64 offsets.
+    template<int DOM_DIM_, int AMB_DIM_ = 3, typename Real_ = double, typename Int_ = int32_t, typename LInt_ = int64_t,
+             typename SReal_ = Real_, typename ExtReal_ = Real_>
+    class SimplicialMesh : public SimplicialMeshBase<Real_,Int_,LInt_,SReal_,ExtReal_>
     {
         static_assert( AMB_DIM_ == 3, "only meshes in R^3 are supported" );
         static_assert( DOM_DIM_ == 1 || DOM_DIM_ == 2, "only curves and surfaces are supported" );
@@ -57,19 +63,26 @@ namespace Repulsor
         static constexpr int NEAR_DIM = SimplexData<DOM_DIM>::NEAR_DIM;
         static constexpr int FAR_DIM  = SimplexData<DOM_DIM>::FAR_DIM;
 
+        using Base_T            = SimplicialMeshBase<Real_,Int_,LInt_,SReal_,ExtReal_>;
+        using ExtInt_T          = Int_;
+        using TangentVector_T   = typename Base_T::TangentVector_T;
+        using CotangentVector_T = typename Base_T::CotangentVector_T;
+
         using Tree_T = ClusterTree<DOM_DIM>;
         using BCT_T  = BlockClusterTree<DOM_DIM>;
 
-        ClusterTreeSettings      cluster_tree_settings;
-        BlockClusterTreeSettings block_cluster_tree_settings;
-        AdaptivitySettings       adaptivity_settings;
+        using Base_T::cluster_tree_settings;
+        using Base_T::block_cluster_tree_settings;
+        using Base_T::adaptivity_settings;
+        using Base_T::scalar_cache;
+        using Base_T::tensor_cache;
 
     protected:
         int threads = 1;
         int device  = 0;
-        mutable std::vector<double> V_coords;        // host copy (refreshed from the device when stale)
+        mutable Tensor2<double,Int_> V_coords;        // host copy (refreshed from the device when stale)
         std::vector<double> V_coords_frozen, V_charges;
-        std::vector<Int>    simplices;
+        Tensor2<Int_,Int_>  simplices;
         bool have_charges = false;
 
         mutable std::unique_ptr<Tree_T> tree;
@@ -86,35 +99,75 @@ namespace Repulsor
         mutable bool rows_stale   = false;
 
     public:
-        // Row-major V x 3 coordinates and T x (n+1) simplices, as the reference's pointer constructor
-        // (src/SimplicialMesh.hpp:143-155).
+        // Row-major V x 3 coordinates and T x (n+1) simplices, optional vertex charges.
         template<typename ExtReal, typename ExtInt>
         SimplicialMesh( const ExtReal * V_coords_, const LInt vertex_count_,
                         const ExtInt * simplices_, const LInt simplex_count_,
                         const LInt thread_count_ = 1, const double * vertex_charges_ = nullptr )
-        :   threads( (int)std::max<LInt>( 1, thread_count_ ) )
+        :   Base_T( (Int_)std::max<LInt>( 1, thread_count_ ) ), threads( (int)std::max<LInt>( 1, thread_count_ ) )
         {
-            V_coords.assign( (size_t)(3 * vertex_count_), 0.0 );
-            for( LInt k = 0; k < 3 * vertex_count_; ++k ) { V_coords[(size_t)k] = (double)V_coords_[k]; }
-            V_coords_frozen = V_coords;
-            simplices.assign( (size_t)(SIZE * simplex_count_), 0 );
-            for( LInt k = 0; k < SIZE * simplex_count_; ++k ) { simplices[(size_t)k] = (Int)simplices_[k]; }
+            Read( V_coords_, vertex_count_, false, simplices_, simplex_count_, false );
             if( vertex_charges_ )
             {
                 have_charges = true;
                 V_charges.assign( vertex_charges_, vertex_charges_ + vertex_count_ );
             }
+        }
+
+        // The reference's constructor (src/SimplicialMesh.hpp:143-185, called by SimplicialMesh_Factory::Make):
+        // coordinates V x AMB_DIM and simplices T x (DOM_DIM+1), each row-major or, with the ColMajor flag,
+        // column-major (AMB_DIM x V, (DOM_DIM+1) x T).
+        template<typename ExtReal, typename ExtInt>
+        SimplicialMesh( const ExtReal * V_coords_, const Int_ vertex_count_, const bool V_ColMajorQ,
+                        const ExtInt * simplices_, const Int_ simplex_count_, const bool S_ColMajorQ,
+                        const Int_ thread_count_ = 1 )
+        :   Base_T( std::max<Int_>( 1, thread_count_ ) ), threads( (int)std::max<Int_>( 1, thread_count_ ) )
+        {
+            Read( V_coords_, vertex_count_, V_ColMajorQ, simplices_, simplex_count_, S_ColMajorQ );
+        }
+
+    private:
+        template<typename ExtReal, typename ExtInt>
+        void Read( const ExtReal * V_, const LInt V, const bool V_transp, const ExtInt * S_, const LInt T, const bool S_transp )
+        {
+            if( V <= 0 || T <= 0 || !V_ || !S_ ) { throw std::invalid_argument( "SimplicialMesh: empty mesh" ); }
+            V_coords = Tensor2<double,Int_>( (Int_)V, (Int_)AMB_DIM );
+            for( LInt v = 0; v < V; ++v )
+            {
+                for( int k = 0; k < AMB_DIM; ++k ) { V_coords( (Int_)v, k ) = (double)( V_transp ? V_[k * V + v] : V_[AMB_DIM * v + k] ); }
+            }
+            V_coords_frozen.assign( V_coords.begin(), V_coords.end() );
+            simplices = Tensor2<Int_,Int_>( (Int_)T, (Int_)SIZE );
+            for( LInt t = 0; t < T; ++t )
+            {
+                for( int k = 0; k < SIZE; ++k )
+                {
+                    const LInt idx = (LInt)( S_transp ? S_[k * T + t] : S_[SIZE * t + k] );
+                    if( idx < 0 || idx >= V ) { throw std::invalid_argument( "SimplicialMesh: simplex vertex index out of range" ); }
+                    simplices( (Int_)t, k ) = (Int_)idx;
+                }
+            }
             cluster_tree_settings.thread_count = threads;
         }
 
-        LInt VertexCount()  const { return (LInt)V_coords.size() / 3; }
-        LInt SimplexCount() const { return (LInt)simplices.size() / SIZE; }
-        int  AmbDim()       const { return AMB_DIM; }
-        int  DomDim()       const { return DOM_DIM; }
-        int  ThreadCount()  const { return threads; }
+    public:
+        Int_ VertexCount()  const override { return V_coords.Dim(0); }
+        Int_ SimplexCount() const override { return simplices.Dim(0); }
+        Int_ AmbDim()       const override { return AMB_DIM; }
+        Int_ DomDim()       const override { return DOM_DIM; }
+        Int_ FarDim()       const override { return FAR_DIM; }
+        Int_ NearDim()      const override { return NEAR_DIM; }
+        Int_ ThreadCount()  const override { return threads; }
         void SetDevice( int d ) { device = d; }
 
-        const std::vector<double> & VertexCoordinates() const { RequireHostCoordinates(); return V_coords; }
+        cref<Tensor2<double,Int_>> VertexCoordinates() const override { RequireHostCoordinates(); return V_coords; }
+        cref<Tensor2<Int_,Int_>> Simplices() const override { return simplices; }
+
+        static std::string className()
+        {
+            return "SimplicialMesh<" + std::to_string( DOM_DIM ) + "," + std::to_string( AMB_DIM ) + ",double,int32,LInt,double,double>";
+        }
+        std::string ClassName() const override { return className(); }
 
         // Host copy of the coordinates of the current state.
         void RequireHostCoordinates() const
@@ -126,7 +179,6 @@ namespace Repulsor
             }
             coords_stale = false;
         }
-        const std::vector<Int>    & Simplices()         const { return simplices; }
         const double * VertexCharges() const { return have_charges ? V_charges.data() : nullptr; }
 
         // ---------------------------------------------------------------- persistent cache
@@ -236,7 +288,14 @@ namespace Repulsor
         // (H^1 metric, learned-preconditioner features, read-outs). Without it, the coordinates are
         // copied and, once the device is set up, sent to the device, which computes the per-state data
         // itself (15.7 MB of coordinates instead of 168 MB of rows at 1.31M triangles).
-        void SemiStaticUpdate( const double * V_new, const bool transp = false, const bool device_ptr = false )
+        // The reference's form: errors are printed, not thrown.
+        void SemiStaticUpdate( const double * V_new, const bool transp = false ) const override
+        {
+            try { SemiStaticUpdate( V_new, transp, false ); }
+            catch( const std::exception & e ) { eprint( ClassName() + "::SemiStaticUpdate: " + e.what() ); }
+        }
+
+        void SemiStaticUpdate( const double * V_new, const bool transp, const bool device_ptr ) const
         {
             ScopedTimer timer("SimplicialMesh::SemiStaticUpdate");
             const LInt V = VertexCount();
@@ -255,7 +314,7 @@ namespace Repulsor
                 dev_data = true;
                 return;
             }
-            if( transp ) { for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { V_coords[(size_t)(3 * v + k)] = V_new[k * V + v]; } } }
+            if( transp ) { for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { V_coords( (Int_)v, k ) = V_new[k * V + v]; } } }
             else { std::copy( V_new, V_new + 3 * V, V_coords.begin() ); }
             coords_stale = false;
             rows_stale = (bool)tree;
@@ -275,8 +334,16 @@ namespace Repulsor
         // Largest t in [0, max_time] such that moving every vertex by t vecs causes no collision between
         // simplices without a common vertex (SimplicialMesh::MaximumSafeStepSize, src/SimplicialMesh.hpp:441-468).
         // vecs: V x 3 (3 x V with transp), host memory, or V x 3 row-major in HBM with device_ptr.
+        // The reference's form: errors are printed, not thrown, and give 0.
         double MaximumSafeStepSize( const double * vecs, const double max_time, const double TOL = 0.0625,
-                                    const bool transp = false, const bool device_ptr = false ) const
+                                    const bool transp = false ) override
+        {
+            try { return static_cast<const SimplicialMesh &>( *this ).MaximumSafeStepSize( vecs, max_time, TOL, transp, false ); }
+            catch( const std::exception & e ) { eprint( ClassName() + "::MaximumSafeStepSize: " + e.what() ); return 0.0; }
+        }
+
+        double MaximumSafeStepSize( const double * vecs, const double max_time, const double TOL,
+                                    const bool transp, const bool device_ptr ) const
         {
             ScopedTimer timer("SimplicialMesh::MaximumSafeStepSize");
             rp_ctx * c = Device();
@@ -306,17 +373,22 @@ namespace Repulsor
 
     public:
         // ---------------------------------------------------------------- value cache
-        mutable std::map<std::string,double> scalar_cache;
-        mutable std::map<std::string,std::vector<double>> vector_cache;
         mutable std::map<std::string,int> device_state;   // e.g. which metric the device holds
 
-        void ClearCache() const
+        void ClearCache() const override
         {
-            scalar_cache.clear();
-            vector_cache.clear();
+            Base_T::ClearCache();
             device_state.clear();
             h1.reset();
             h1_factor.reset();
+        }
+
+        // Y = alpha * dE + beta * Y for the differential last computed on the device
+        // (SimplicialMesh::Assemble_ClusterTree_Derivatives, src/SimplicialMesh.hpp:651-675).
+        void Assemble_ClusterTree_Derivatives( const double alpha, const double beta, double * Y, const Int_ ldY ) const override
+        {
+            if( !dev ) { eprint( ClassName() + "::Assemble_ClusterTree_Derivatives: no differential computed. Doing nothing." ); return; }
+            Check( rp_assemble_vertex_gradient( dev->get(), alpha, beta, Y, ldY, 0 ), ClassName() + "::Assemble_ClusterTree_Derivatives" );
         }
 
         // ---------------------------------------------------------------- H^1 metric and its factor
@@ -328,11 +400,23 @@ namespace Repulsor
         mutable std::unique_ptr<SupernodalFactor> h1_factor;
         mutable std::unique_ptr<std::vector<Int>> nd_ordering;   // persistent (depends on the tree only)
 
+        mutable Int_ h1_solver_id = 0;
+
     public:
-        double H1StiffnessWeight() const { return h1_stiffness_weight; }
-        double H1MassWeight()      const { return h1_mass_weight; }
-        void SetH1StiffnessWeight( double w ) { h1_stiffness_weight = w; h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
-        void SetH1MassWeight( double w )      { h1_mass_weight = w;      h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
+        double H1StiffnessWeight() const override { return h1_stiffness_weight; }
+        double H1MassWeight()      const override { return h1_mass_weight; }
+        void SetH1StiffnessWeight( const double w ) override { h1_stiffness_weight = w; h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
+        void SetH1MassWeight( const double w ) override      { h1_mass_weight = w;      h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
+
+        // Ordering of the H^1 Cholesky factorisation (SimplicialMesh::SetH1SolverID, src/SimplicialMesh/H1Metric.hpp:174-257):
+        // 0 = nested dissection on the cluster tree, the reference's default and the only ordering this
+        // library builds; other IDs are refused with an error message and leave the ID unchanged.
+        Int_ H1SolverID() const override { return h1_solver_id; }
+        void SetH1SolverID( const Int_ ID ) const override
+        {
+            if( ID != 0 ) { eprint( ClassName() + "::SetH1SolverID: solver ID " + std::to_string( ID ) + " is not available; only 0 (nested dissection) is. Doing nothing." ); return; }
+            h1_solver_id = ID;
+        }
 
         const CSRMatrix & H1Metric() const
         {

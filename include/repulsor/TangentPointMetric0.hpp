@@ -1,9 +1,12 @@
 #pragma once
-// TangentPointMetric0 with the reference's public surface
-// (src/Metrics/TangentPointMetric0.hpp:29-126, src/Metrics/MetricBase.hpp:34-134,
-//  src/Metrics/MetricDimRestricted.hpp:28-297): MetricValues, MultiplyMetric (pointer and
-// vector overloads), MetricNorm. The metric blocks are assembled and applied on the GPU.
+// TangentPointMetric0 with the reference's public surface (src/Metrics/TangentPointMetric0.hpp:29-126 on
+// MetricDimRestricted / MetricBase, src/Metrics/MetricDimRestricted.hpp:28-297, src/Metrics/MetricBase.hpp:34-134):
+// MetricValues, MultiplyMetric (pointer and tensor forms), MultiplyPreconditioner, Solve (by default the
+// reference's left-preconditioned GMRES with P0), MetricNorm, PreconditionerNorm, reachable through
+// MetricBase<MeshBase> pointers made by TangentPointMetric0_Factory. The metric blocks are assembled and
+// applied on the GPU.
 
+#include "MetricBase.hpp"
 #include "SimplicialMesh.hpp"
 #include "LearnedPreconditioner.hpp"
 
@@ -11,12 +14,32 @@
 
 namespace Repulsor
 {
+    // Krylov method of TangentPointMetric0::Solve (no reference counterpart: the reference runs GMRES).
+    enum class SolverMethod : int
+    {
+        GMRES = 0,   // left-preconditioned restarted GMRES(100), stopping on |P r|_2 / |P b|_2 (the reference's)
+        PCG   = 1    // preconditioned conjugate gradients, stopping on |B - A X|_2 / |B|_2 per column
+    };
+
     template<typename Mesh_T>
-    class TangentPointMetric0
+    class TangentPointMetric0 : public MetricDimRestricted<Mesh_T, OperatorType::MixedOrder>
     {
     public:
         static constexpr int DOM_DIM = Mesh_T::DOM_DIM;
         static constexpr int AMB_DIM = 3;
+
+        using Base_T           = MetricDimRestricted<Mesh_T, OperatorType::MixedOrder>;
+        using MeshBase_T       = typename Base_T::MeshBase_T;
+        using ValueContainer_T = typename Base_T::ValueContainer_T;
+        using SolverVector_T   = typename Base_T::SolverVector_T;
+        using ExtReal          = typename Base_T::ExtReal;
+        using Real             = typename Base_T::Real;
+        using Int              = typename Base_T::Int;
+
+        using Base_T::MetricValues;
+        using Base_T::MultiplyMetric;
+        using Base_T::MultiplyPreconditioner;
+        using Base_T::Solve;
 
     protected:
         double q, p, s;
@@ -45,15 +68,21 @@ namespace Repulsor
             return precision == Precision::MixedRefined ? Precision::Mixed : precision;
         }
 
-        std::string ClassName() const
+        std::string ClassName() const override
         {
             char buf[128];
             std::snprintf( buf, sizeof(buf), "TangentPointMetric0<%d,3>(%g,%g,%g)%s", DOM_DIM, q, p, s, PrecisionTag( precision ) );
             return std::string(buf);
         }
 
-        // Assembles the metric blocks on the device (cached until SemiStaticUpdate).
-        bool MetricValues( const Mesh_T & M ) const
+        // Krylov method and stopping measure of Solve without explicit solver flags (default: GMRES, as the
+        // reference).
+        void SetSolverMethod( const SolverMethod m ) { solver_method = m; }
+        SolverMethod GetSolverMethod() const { return solver_method; }
+
+        // Assembles the metric blocks on the device (cached until SemiStaticUpdate); the reference's
+        // MetricValues(M) assembles and caches them in the mesh (MetricDimRestricted.hpp:28-50).
+        bool AssembleMetric( const Mesh_T & M ) const
         {
             try
             {
@@ -78,7 +107,7 @@ namespace Repulsor
         // NF diagonal (2 per primitive), FF values over the far CSR, FF diagonal.
         bool DownloadMetricValues( const Mesh_T & M, double * nf_csr, double * nf_diag, double * ff_vals, double * ff_diag ) const
         {
-            if( !MetricValues( M ) ) { return false; }
+            if( !AssembleMetric( M ) ) { return false; }
             const auto & B = M.GetBlockClusterTree();
             std::vector<double> tile( (size_t)std::max<LInt>( 1, 2 * B.tiles.val_ptr.back() ) );
             if( !Check( rp_download_metric( M.Device(), nf_csr ? tile.data() : nullptr, nf_diag, ff_vals, ff_diag ), ClassName() ) ) { return false; }
@@ -86,11 +115,31 @@ namespace Repulsor
             return true;
         }
 
+        // Host copy of all metric values in the reference's layout (MetricValues through the base class).
+        bool DownloadMetricValues( const Mesh_T & M, ValueContainer_T & out ) const override
+        {
+            try
+            {
+                const auto & B = M.GetBlockClusterTree();
+                const LInt T = M.SimplexCount(), C = M.GetClusterTree().ClusterCount();
+                out.NF.assign( (size_t)( 2 * B.NearCSR().NonzeroCount() ), 0.0 );
+                out.NF_diag.assign( (size_t)( 2 * T ), 0.0 );
+                out.FF.assign( (size_t)( 2 * B.far.NonzeroCount() ), 0.0 );
+                out.FF_diag.assign( (size_t)( 2 * C ), 0.0 );
+                if( !DownloadMetricValues( M, out.NF.data(), out.NF_diag.data(), out.FF.data(), out.FF_diag.data() ) ) { return false; }
+                const LInt nvf = B.VeryNearCSR().NonzeroCount();
+                out.VF.assign( (size_t)( 2 * nvf ), 0.0 );
+                out.VF_diag.assign( (size_t)( 2 * T ), 0.0 );
+                return nvf == 0 || DownloadVeryNearMetricValues( M, out.VF.data(), out.VF_diag.data() );
+            }
+            catch( const std::exception & e ) { eprint( ClassName() + "::MetricValues: " + e.what() ); return false; }
+        }
+
         // Very-near-field metric values: vf_vals (a0, a1) per entry of the VeryNear pattern (CSR order),
         // vf_diag T x 2 in tree order (TP0_Traversor::VF_Compute, TP0_Traversor.hpp:280-308).
         bool DownloadVeryNearMetricValues( const Mesh_T & M, double * vf_vals, double * vf_diag ) const
         {
-            if( !MetricValues( M ) ) { return false; }
+            if( !AssembleMetric( M ) ) { return false; }
             return Check( rp_download_vf_metric( M.Device(), vf_vals, vf_diag ), ClassName() );
         }
 
@@ -103,17 +152,17 @@ namespace Repulsor
 
         // Y = alpha * A X + beta * Y (MetricDimRestricted::MultiplyMetric, MetricDimRestricted.hpp:109-139).
         // nrhs must equal 3; otherwise an error is printed and Y is left untouched.
-        bool MultiplyMetric( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,
-                             const double beta, double * Y, const LInt ldY, const int nrhs,
+        bool MultiplyMetric( const Mesh_T & M, const double alpha, const double * X, const Int ldX,
+                             const double beta, double * Y, const Int ldY, const Int nrhs,
                              const bool VF_flag = true, const bool NF_flag = true, const bool FF_flag = true,
-                             const bool device_ptrs = false ) const
+                             const bool device_ptrs = false ) const override
         {
             if( nrhs != AMB_DIM )
             {
                 eprint( ClassName() + "::MultiplyMetric: nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
                 return false;
             }
-            if( !MetricValues( M ) ) { return false; }
+            if( !AssembleMetric( M ) ) { return false; }
             ScopedTimer timer( "TangentPointMetric0::MultiplyMetric" );
             return Check( rp_multiply_metric( M.Device(), alpha, X, ldX, beta, Y, ldY, nrhs,
                                               VF_flag, NF_flag, FF_flag, device_ptrs ? RP_DEVICE_PTR : 0 ),
@@ -122,10 +171,10 @@ namespace Repulsor
 
         // Y = alpha * A32 X + beta * Y with the FP32 copies of a MixedRefined metric: the product the
         // CG iterations of Solve use (no reference counterpart).
-        bool MultiplyMetricInner( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,
-                                  const double beta, double * Y, const LInt ldY, const bool device_ptrs = false ) const
+        bool MultiplyMetricInner( const Mesh_T & M, const double alpha, const double * X, const Int ldX,
+                                  const double beta, double * Y, const Int ldY, const bool device_ptrs = false ) const
         {
-            if( !MetricValues( M ) ) { return false; }
+            if( !AssembleMetric( M ) ) { return false; }
             return Check( rp_multiply_metric( M.Device(), alpha, X, ldX, beta, Y, ldY, AMB_DIM,
                                               1, RP_NF_FP32_COPIES, 1, device_ptrs ? RP_DEVICE_PTR : 0 ),
                           ClassName() + "::MultiplyMetricInner" );
@@ -136,18 +185,6 @@ namespace Repulsor
                              const double beta, std::vector<double> & Y ) const
         {
             return MultiplyMetric( M, alpha, X.data(), 3, beta, Y.data(), 3, 3 );
-        }
-
-        // sqrt(<X, A X>) (MetricBase.hpp:81-100).
-        double MetricNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs ) const
-        {
-            return QuadraticNorm( M, X, ldX, nrhs, false );
-        }
-
-        // sqrt(<X, P X>) (MetricBase.hpp:102-121).
-        double PreconditionerNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs ) const
-        {
-            return QuadraticNorm( M, X, ldX, nrhs, true );
         }
 
         // Pseudo-Laplacian L_{2-s} of the preconditioner (the reference's member
@@ -176,9 +213,9 @@ namespace Repulsor
 
         // Y = alpha * H^-1 L_{2-s} H^-1 X + beta * Y (TangentPointMetric0::multiply_preconditioner,
         // TangentPointMetric0.hpp:88-126; nrhs check MetricDimRestricted.hpp:173-195).
-        bool MultiplyPreconditioner( const Mesh_T & M, const double alpha, const double * X, const LInt ldX,
-                                     const double beta, double * Y, const LInt ldY, const int nrhs,
-                                     const bool device_ptrs = false ) const
+        bool MultiplyPreconditioner( const Mesh_T & M, const double alpha, const double * X, const Int ldX,
+                                     const double beta, double * Y, const Int ldY, const Int nrhs,
+                                     const bool device_ptrs = false ) const override
         {
             if( nrhs != AMB_DIM )
             {
@@ -191,55 +228,62 @@ namespace Repulsor
                           ClassName() + "::MultiplyPreconditioner" );
         }
 
-        // X = alpha * A^+ B + beta * X with P0 on the GPU (MetricDimRestricted::Solve,
-        // MetricDimRestricted.hpp:232-289). solver_flags (RP_SOLVE_*): default preconditioned conjugate
-        // gradients with the l2 residual; RP_SOLVE_PC_NORM measures in the preconditioner norm;
-        // RP_SOLVE_GMRES runs the reference's left-preconditioned GMRES. The iteration count and the
-        // per-column relative residuals are kept for Solver_IterationCount / Solver_RelativeResiduals.
-        bool Solve( const Mesh_T & M, const double alpha, const double * B, const LInt ldB,
-                    const double beta, double * X, const LInt ldX, const int nrhs,
-                    const int max_iter, const double tolerance, const bool device_ptrs = false,
-                    const int solver_flags = 0 ) const
+        // X = alpha * A^{-1} B + beta * X on the GPU (MetricDimRestricted::Solve, MetricDimRestricted.hpp:232-289).
+        // solver_flags (RP_SOLVE_*); -1 takes the object's SolverMethod: GMRES (default, the reference's
+        // left-preconditioned GMRES, stopping on |P r|_2 / |P b|_2) or PCG (stopping on |B - A X|_2 / |B|_2 per
+        // column). The preconditioner is P0 = H^-1 L_{2-s} H^-1 unless SetPreconditionerDiagonalWeight or a
+        // learned model is set. Returns true iff the solve met the tolerance; on max_iter without convergence
+        // it prints a warning, keeps the last iterate in X and returns false (Solver_Status: RP_ESTAGNATION).
+        // The iteration count and per-column relative residuals are kept for Solver_IterationCount /
+        // Solver_RelativeResiduals.
+        bool Solve( const Mesh_T & M, const double alpha, const double * B, const Int ldB,
+                    const double beta, double * X, const Int ldX, const Int nrhs,
+                    const Int max_iter, const double tolerance, const bool device_ptrs = false,
+                    const int solver_flags = -1 ) const override
         {
-            const bool use_preconditioner = !( solver_flags & RP_SOLVE_NO_PRECONDITIONER );
+            const int flags = solver_flags >= 0 ? solver_flags : ( solver_method == SolverMethod::GMRES ? RP_SOLVE_GMRES : 0 );
+            const bool use_preconditioner = !( flags & RP_SOLVE_NO_PRECONDITIONER );
+            status = RP_EINVAL;
             if( nrhs != AMB_DIM )
             {
                 eprint( ClassName() + "::Solve: nrhs = " + std::to_string(nrhs) + " != 3 = AMB_DIM. The current implementation only accepts nrhs = AMB_DIM. Doing nothing." );
                 return false;
             }
-            if( !MetricValues( M ) ) { return false; }
+            if( !AssembleMetric( M ) ) { return false; }
             if( use_preconditioner && !PreparePreconditioner( M ) ) { return false; }
             ScopedTimer timer( "TangentPointMetric0::Solve" );
             const bool learned = use_preconditioner && !lp_params.empty();
             const bool can_fall_back = learned && ( beta == 0.0 || !device_ptrs );
             std::vector<double> X0;
-            if( can_fall_back && beta != 0.0 ) { X0.assign( X, X + ldX * ( M.VertexCount() - 1 ) + 3 ); }
+            if( can_fall_back && beta != 0.0 ) { X0.assign( X, X + (size_t)ldX * (size_t)( M.VertexCount() - 1 ) + 3 ); }
             int it = 0;
             double rel[3] = { 0, 0, 0 };
             const int st = rp_solve( M.Device(), alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance,
-                                     solver_flags, device_ptrs ? RP_DEVICE_PTR : 0, &it, rel );
-            iter = it;
-            rel_residuals.assign( rel, rel + 3 );
+                                     flags, device_ptrs ? RP_DEVICE_PTR : 0, &it, rel );
+            SetRecord( it, rel );
             fell_back = false;
+            status = st;
             if( st == RP_ESTAGNATION && can_fall_back )
             {
-                // the reference preconditioner (with the estimated diagonal term) is the fallback
+                // the two-level preconditioner the learned scalings wrap is the fallback
                 wprint( ClassName() + "::Solve: learned preconditioner stagnated; falling back to P0 + w diag(A)^-1" );
                 if( !X0.empty() ) { std::copy( X0.begin(), X0.end(), X ); }
                 TangentPointMetric0 plain( q, p );
                 plain.SetPrecision( precision );
                 plain.pc_precision = pc_precision;
+                plain.solver_method = solver_method;
                 plain.SetPreconditionerDiagonalWeight( -1.0 );
                 const bool ok = plain.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance, device_ptrs, solver_flags );
                 iter = it + plain.Solver_IterationCount();
                 rel_residuals = plain.Solver_RelativeResiduals();
+                status = plain.Solver_Status();
                 fell_back = true;
                 return ok;
             }
             if( st == RP_ESTAGNATION )
             {
                 wprint( ClassName() + "::Solve: " + rp_last_error() );
-                return true;
+                return false;
             }
             return Check( st, ClassName() + "::Solve" );
         }
@@ -247,13 +291,23 @@ namespace Repulsor
         // Whether the last Solve had to fall back from the learned preconditioner.
         bool Solver_FellBack() const { return fell_back; }
 
-        int Solver_IterationCount() const { return iter; }
-        std::vector<double> Solver_RelativeResiduals() const { return rel_residuals; }
+        // Status of the last Solve: RP_OK (converged), RP_ESTAGNATION (max_iter reached; X holds the last
+        // iterate) or the error status of a failed call.
+        int Solver_Status() const { return status; }
 
     protected:
-        mutable int iter = 0;
-        mutable std::vector<double> rel_residuals;
+        using Base_T::iter;
+        using Base_T::rel_residuals;
         mutable bool fell_back = false;
+        mutable int status = RP_OK;
+        SolverMethod solver_method = SolverMethod::GMRES;
+
+        void SetRecord( const int it, const double rel[3] ) const
+        {
+            iter = it;
+            rel_residuals = SolverVector_T( 3 );
+            for( int k = 0; k < 3; ++k ) { rel_residuals[k] = rel[k]; }
+        }
 
         double pc_diagonal_weight = 0.0;
         std::vector<float> lp_params;      // learned preconditioner model (empty: none)
@@ -273,7 +327,7 @@ namespace Repulsor
             catch( const std::exception & e ) { eprint( ClassName() + "::MultiplyPreconditioner: " + e.what() ); return false; }
             if( !PseudoLaplacianValues( M ) ) { return false; }
             const bool learned = !lp_params.empty();
-            if( ( learned || pc_diagonal_weight != 0.0 ) && !MetricValues( M ) ) { return false; }
+            if( ( learned || pc_diagonal_weight != 0.0 ) && !AssembleMetric( M ) ) { return false; }
             rp_ctx * c = M.Device();
             // the device holds at most one learned model: switch it on or off as this object needs
             std::string have;
@@ -341,8 +395,6 @@ namespace Repulsor
             return Check( rp_pc_download_scaling( M.Device(), sqrt_d ), ClassName() + "::LearnedScaling" );
         }
 
-    protected:
-
     public:
         // Preconditioner P = P0 + w diag(A)^-1 (w = 0: the reference's P0; w < 0: w estimated per state
         // as lambda_max(P0 A) / 4). The diagonal term has no counterpart in the reference: P0 alone
@@ -361,23 +413,60 @@ namespace Repulsor
         // Exact diagonal of the assembled metric (V values).
         bool MetricDiagonal( const Mesh_T & M, double * diag ) const
         {
-            if( !MetricValues( M ) ) { return false; }
+            if( !AssembleMetric( M ) ) { return false; }
             return Check( rp_download_metric_diagonal( M.Device(), diag ), ClassName() + "::MetricDiagonal" );
         }
 
-    protected:
+    };
 
-        double QuadraticNorm( const Mesh_T & M, const double * X, const LInt ldX, const int nrhs, const bool pc ) const
+    // TangentPointMetric0_Factory<MeshBase_T, MIN_DOM_DIM, MAX_DOM_DIM, MIN_AMB_DIM, MAX_AMB_DIM>::Make(dom_dim,
+    // amb_dim, q, p) (reference src/Energies/TangentPointEnergy/TP_Factory.hpp:1-80, instantiated for
+    // TangentPointMetric0 at src/Metrics/TangentPointMetric0.hpp:156): a MetricBase pointer to
+    // TangentPointMetric0<SimplicialMesh<dom_dim,amb_dim,...>>, or nullptr with an error message for
+    // dimensions outside the factory's range or not built here (curves and surfaces in R^3).
+    template<typename MeshBase_T, int ...N> class TangentPointMetric0_Factory {};
+
+    template<typename Real, typename Int, typename LInt, typename SReal, typename ExtReal,
+             int MIN_DOM_DIM_, int MAX_DOM_DIM_, int MIN_AMB_DIM_, int MAX_AMB_DIM_>
+    class TangentPointMetric0_Factory<SimplicialMeshBase<Real,Int,LInt,SReal,ExtReal>, MIN_DOM_DIM_, MAX_DOM_DIM_, MIN_AMB_DIM_, MAX_AMB_DIM_>
+    {
+    public:
+        using MeshBase_T = SimplicialMeshBase<Real,Int,LInt,SReal,ExtReal>;
+        using Root_T     = MetricBase<MeshBase_T>;
+
+        static constexpr int MIN_AMB_DIM = std::max( 1, MIN_AMB_DIM_ );
+        static constexpr int MAX_AMB_DIM = std::max( 1, MAX_AMB_DIM_ );
+        static constexpr int MIN_DOM_DIM = std::max( 0, MIN_DOM_DIM_ );
+        static constexpr int MAX_DOM_DIM = std::min( MAX_AMB_DIM - 1, MAX_DOM_DIM_ );
+
+        [[nodiscard]] std::unique_ptr<Root_T> Make( const Int dom_dim, const Int amb_dim, const Real q, const Real p ) const
         {
-            const LInt V = M.VertexCount();
-            std::vector<double> Xc( (size_t)(3 * V) ), Yc( (size_t)(3 * V), 0.0 );
-            for( LInt v = 0; v < V; ++v ) { for( int k = 0; k < 3; ++k ) { Xc[(size_t)(3 * v + k)] = X[ldX * v + k]; } }
-            const bool ok = pc ? MultiplyPreconditioner( M, 1.0, Xc.data(), 3, 0.0, Yc.data(), 3, nrhs )
-                               : MultiplyMetric( M, 1.0, Xc.data(), 3, 0.0, Yc.data(), 3, nrhs );
-            if( !ok ) { return 0; }
-            double s2 = 0;
-            for( size_t k = 0; k < Xc.size(); ++k ) { s2 += Xc[k] * Yc[k]; }
-            return std::sqrt( s2 );
+            if( dom_dim < MIN_DOM_DIM || dom_dim > MAX_DOM_DIM ) { eprint( ClassName() + "::Make: dom_dim is out of range." ); return Error( dom_dim, amb_dim ); }
+            if( amb_dim < MIN_AMB_DIM || amb_dim > MAX_AMB_DIM ) { eprint( ClassName() + "::Make: amb_dim is out of range." ); return Error( dom_dim, amb_dim ); }
+            if( dom_dim >= amb_dim || amb_dim != 3 ) { return Error( dom_dim, amb_dim ); }
+            if constexpr ( MIN_DOM_DIM <= 2 && 2 <= MAX_DOM_DIM )
+            {
+                if( dom_dim == 2 ) { return std::unique_ptr<Root_T>( new TangentPointMetric0<SimplicialMesh<2,3,Real,Int,LInt,SReal,ExtReal>>( q, p ) ); }
+            }
+            if constexpr ( MIN_DOM_DIM <= 1 && 1 <= MAX_DOM_DIM )
+            {
+                if( dom_dim == 1 ) { return std::unique_ptr<Root_T>( new TangentPointMetric0<SimplicialMesh<1,3,Real,Int,LInt,SReal,ExtReal>>( q, p ) ); }
+            }
+            return Error( dom_dim, amb_dim );
+        }
+
+        std::string ClassName() const
+        {
+            return "TangentPointMetric0_Factory<" + std::to_string( MIN_DOM_DIM ) + "," + std::to_string( MAX_DOM_DIM ) + ","
+                 + std::to_string( MIN_AMB_DIM ) + "," + std::to_string( MAX_AMB_DIM ) + ">";
+        }
+
+    private:
+        std::unique_ptr<Root_T> Error( const Int dom_dim, const Int amb_dim ) const
+        {
+            eprint( ClassName() + " cannot create TangentPointMetric0 with domain dimension " + std::to_string( dom_dim )
+                  + " and ambient dimension " + std::to_string( amb_dim ) + "." );
+            return nullptr;
         }
     };
 

@@ -439,7 +439,7 @@ int rpm_metric_assemble( void * op, void * h )
     {
         using Mesh = std::decay_t<decltype(M)>;
         auto A = make_metric<Mesh>( (OpHandle *)op );
-        return A.MetricValues( M ) ? RP_OK : RP_EINVAL;
+        return A.AssembleMetric( M ) ? RP_OK : RP_EINVAL;
     });
 }
 
@@ -466,8 +466,9 @@ int rpm_metric_solve( void * op, void * h, double alpha, const double * B, int64
     {
         using Mesh = std::decay_t<decltype(M)>;
         auto A = make_metric<Mesh>( (OpHandle *)op );
-        const bool ok = A.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tol, device_ptr, solver_flags );
-        if( !ok ) { if( nrhs != 3 ) { g_err = "nrhs must be 3"; } return RP_EINVAL; }
+        (void)A.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tol, device_ptr, solver_flags );
+        const int st = A.Solver_Status();
+        if( st != RP_OK && st != RP_ESTAGNATION ) { if( nrhs != 3 ) { g_err = "nrhs must be 3"; } return st; }
         const auto rr = A.Solver_RelativeResiduals();
         if( iters ) { iters[0] = A.Solver_IterationCount(); iters[1] = A.Solver_FellBack() ? 1 : 0; }
         bool conv = true;
