@@ -66,112 +66,145 @@ def _hip():
     return lib
 
 
-def cpu_baseline(M, wl, target_pairs, solve_iters=0):
-    """Oracle restatement (-O3 -ffast-math build, 1 thread) on a bounded sample of the same workload:
-    the near-field rows [0, R) holding about `target_pairs` upper pairs, the complete far field.
-    Returns the extrapolated energy+grad ms (plus, with solve_iters, the metric solve) for the whole
-    workload."""
+def host_cpu_info():
+    """Cores this process may use (its CPU share on the GPU box) and the CPU model."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(M, wl, threads, solve_iters=0):
+    """The oracle restatement (-O3 -ffast-math build) timed on full passes of the same workload, on
+    `threads` host threads (rows split by cost, per-thread accumulators, as the reference's
+    FMM_Traversor / AddReduce) and on one thread: energy + differential (near and far pair kernels,
+    derivative collection, chain rule, assembly), one MultiplyMetric and one preconditioner application
+    (pseudo-Laplacian product between two host H^1 solves), all complete, nothing sampled. The CPU step is
+    energy+grad + iterations x (MultiplyMetric + preconditioner), with the iteration count of the GPU's
+    solve (same method). Metric and pseudo-Laplacian assembly (per state, like the GPU's) are timed on
+    `threads` threads and reported apart. The derivative collection/chain rule and the H^1 solves run
+    on one thread in every column."""
     import oracle as O
     t_start = time.perf_counter()
-    f = O.fast_tp0()
     b = M.build()
     td = M.tree_data()
-    # choose R from the per-row counts of the tile structure: upper pairs of row i ~ half its row
-    T = b["T"]
-    near_upper_total = b["near_nnz"] // 2
-    R = max(1, int(T * min(1.0, target_pairs / max(1, near_upper_total))))
-    nrp, nci, ndg = M.pattern("near", row_end=R)
-    sample_pairs = int(np.sum(nrp[1:R + 1] - ndg[:R]))
-    frp, fci, fdg = M.pattern("far")
-    P_near, C_far = O.c64(td["P_near"]), O.c64(td["C_far"])
-    Cn = C_far.shape[0]
-    nd = P_near.shape[1]
-    PDn = np.zeros((T, nd)); CDf = np.zeros((Cn, 10))
-    d = C.POINTER(C.c_double); i32 = C.POINTER(C.c_int32); i64 = C.POINTER(C.c_int64)
-    zf = np.zeros((1, 2))
-
-    def run(np_rp, np_ci, np_dg, fp_rp, fp_ci, fp_dg):
-        t0 = time.perf_counter()
-        f(M.dom_dim, T, Cn, P_near.ctypes.data_as(d), C_far.ctypes.data_as(d),
-          np_rp.ctypes.data_as(i64), np_ci.ctypes.data_as(i32), np_dg.ctypes.data_as(i64),
-          fp_rp.ctypes.data_as(i64), fp_ci.ctypes.data_as(i32), fp_dg.ctypes.data_as(i64),
-          wl.q, wl.p, 3, PDn.ctypes.data_as(d), CDf.ctypes.data_as(d), zf.ctypes.data_as(d), zf.ctypes.data_as(d),
-          zf.ctypes.data_as(d), zf.ctypes.data_as(d))
-        return time.perf_counter() - t0
-
-    empty_rp = np.zeros(Cn + 1, dtype=np.int64); empty_ci = np.zeros(1, dtype=np.int32); empty_dg = np.zeros(Cn, dtype=np.int64)
-    t_near = run(O.c64i(nrp), O.c32(nci), O.c64i(ndg), empty_rp, empty_ci, empty_dg)
-    nrp0 = np.zeros(T + 1, dtype=np.int64); ndg0 = np.zeros(T, dtype=np.int64)
-    t_far = run(nrp0, empty_ci, ndg0, O.c64i(frp), O.c32(fci), O.c64i(fdg))
-    # collection + chain rule + assembly (complete)
     tr = M.tree()
-    t0 = time.perf_counter()
-    O.differential(M.dom_dim, M.VertexCoordinates(), M.simplices, tr, PDn, CDf)
-    t_chain = time.perf_counter() - t0
-    eg_s = t_near * (near_upper_total / max(1, sample_pairs)) + t_far + t_chain
-    sampled_s = t_near + t_far + t_chain
+    n = M.dom_dim
+    npat = M.pattern("near")
+    fpat = M.pattern("far")
+    P_near, C_far = O.c64(td["P_near"]), O.c64(td["C_far"])
+    s_pl = 2.0 - (wl.p - n) / wl.q
+    Xv = np.random.default_rng(11).standard_normal((M.V, 3))
+    Yv = np.zeros_like(Xv)
+
+    def timed(fn):
+        t0 = time.perf_counter(); r = fn(); return time.perf_counter() - t0, r
+
+    def eg(th):
+        o = O.tp0_compute(n, P_near, C_far, npat, fpat, wl.q, wl.p, 3, impl=O.fast_mt("orc_tp0_compute", th))
+        O.differential(n, M.VertexCoordinates(), M.simplices, tr, o["P_D_near_tree"], o["C_D_far"])
+        return o["energy"]
+
+    t_asm, mv = timed(lambda: O.tp0_compute(n, P_near, C_far, npat, fpat, wl.q, wl.p, 4, impl=O.fast_mt("orc_tp0_compute", threads)))
+    t_plasm, pl = timed(lambda: O.pl_compute(n, P_near, C_far, npat, fpat, s_pl, impl=O.fast_mt("orc_pl_compute", threads)))
+    M.h1_solve_host(Xv)                          # factorisation (per state, not timed)
+
+    def mm(th):
+        return O.multiply_metric(n, M.V, tr, td["pre_cols"], td["pre_vals"], npat, mv["NF"], mv["NF_diag"], fpat, mv["FF"],
+                                 mv["FF_diag"], 1.0, Xv, 0.0, Yv, impl=O.fast_mt("orc_multiply_metric", th))
+
+    def pc(th):
+        Z, _ = M.h1_solve_host(Xv)
+        W = np.zeros_like(Z)
+        O.pl_multiply(n, M.V, tr, td["pre_cols"], td["pre_vals"], npat, pl, fpat, 1.0, Z, 0.0, W,
+                      impl=O.fast_mt("orc_pl_multiply", th))
+        return M.h1_solve_host(W)[0]
+
+    cols = {}
+    for th in sorted({threads, 1}, reverse=True):
+        t_eg, _ = timed(lambda: eg(th))
+        t_mm = min(timed(lambda: mm(th))[0] for _ in range(2))
+        t_pc = min(timed(lambda: pc(th))[0] for _ in range(2))
+        step = t_eg + solve_iters * (t_mm + t_pc)
+        cols[th] = {"cores": th, "energy_grad_ms": round(t_eg * 1e3, 1), "multiply_metric_ms": round(t_mm * 1e3, 1),
+                    "preconditioner_ms": round(t_pc * 1e3, 1), "step_ms": round(step * 1e3, 1),
+                    "pairs_per_s": round((b["near_nnz"] + b["far_nnz"]) / 2 / t_eg, 1)}
+    allc = cols[threads]
     out = {
-        "value": round(eg_s * 1e3, 1), "unit": "ms (energy+grad, extrapolated)", "cores": 1, "kind": "port",
-        "sample": (f"near-field rows [0,{R}) = {sample_pairs} of {near_upper_total} upper pairs timed and scaled; "
-                   f"far field ({b['far_nnz'] // 2} pairs), derivative collection, chain rule and assembly complete; "
-                   f"oracle -O3 -ffast-math build"),
-        "pairs_per_s": round(sample_pairs / t_near, 1),
-        "energy_grad_ms": round(eg_s * 1e3, 1),
+        "value": allc["step_ms"], "unit": "ms (energy+grad + iterations x (MultiplyMetric + preconditioner), full passes)",
+        "cores": threads, "kind": "port",
+        "sample": (f"full: all {b['near_nnz'] // 2} near and {b['far_nnz'] // 2} far upper pairs, complete MultiplyMetric "
+                   f"and preconditioner applications on the same mesh; solve = {solve_iters} iterations (the GPU solve's "
+                   f"count) x (MultiplyMetric + preconditioner); oracle -O3 -ffast-math build, std::threads"),
+        "energy_grad_ms": allc["energy_grad_ms"], "multiply_metric_ms": allc["multiply_metric_ms"],
+        "preconditioner_ms": allc["preconditioner_ms"], "iterations": solve_iters,
+        "one_core": cols[1],
+        "metric_assembly_ms": round(t_asm * 1e3, 1), "pl_assembly_ms": round(t_plasm * 1e3, 1),
     }
-    if solve_iters:
-        # One PCG iteration on the CPU: metric product A p and preconditioner (two H^1 solves around
-        # the pseudo-Laplacian product), same near rows sample, scaled; far field, tree passes and
-        # the pre/post operators complete. Iteration count taken from the GPU solve (same method).
-        s_pl = 2.0 - (wl.p - M.dom_dim) / wl.q
-        near_s = (nrp, nci, ndg)
-        empty_near = (nrp0, empty_ci, ndg0)
-        far_full = (frp, fci, fdg)
-        empty_far = (empty_rp, empty_ci, empty_dg)
-        mv_near = O.tp0_compute(M.dom_dim, P_near, C_far, near_s, empty_far, wl.q, wl.p, 4, impl=O.fast("orc_tp0_compute"))
-        mv_far = O.tp0_compute(M.dom_dim, P_near, C_far, empty_near, far_full, wl.q, wl.p, 4, impl=O.fast("orc_tp0_compute"))
-        pl_near = O.pl_compute(M.dom_dim, P_near, C_far, near_s, empty_far, s_pl, impl=O.fast("orc_pl_compute"))
-        pl_far = O.pl_compute(M.dom_dim, P_near, C_far, empty_near, far_full, s_pl, impl=O.fast("orc_pl_compute"))
-        Xv = np.random.default_rng(11).standard_normal((M.V, 3))
-        Yv = np.zeros_like(Xv)
-        mm, plm = O.fast("orc_multiply_metric"), O.fast("orc_pl_multiply")
+    out["cpu_s"] = round(time.perf_counter() - t_start, 1)
+    return out
 
-        def timed(fn):
-            t0 = time.perf_counter(); fn(); return time.perf_counter() - t0
 
-        t_a_near = timed(lambda: O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], near_s, mv_near["NF"],
-                                                   mv_near["NF_diag"], empty_far, np.zeros((1, 2)), mv_near["FF_diag"],
-                                                   1.0, Xv, 0.0, Yv, True, False, impl=mm))
-        t_a_far = timed(lambda: O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, np.zeros((1, 2)),
-                                                  mv_far["NF_diag"], far_full, mv_far["FF"], mv_far["FF_diag"],
-                                                  1.0, Xv, 0.0, Yv, False, True, impl=mm))
-        t_a_base = timed(lambda: O.multiply_metric(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, np.zeros((1, 2)),
-                                                   mv_far["NF_diag"], empty_far, np.zeros((1, 2)), mv_far["FF_diag"],
-                                                   1.0, Xv, 0.0, Yv, False, False, impl=mm))
-        pl_ne = {"NF": pl_near["NF"], "NF_diag": pl_near["NF_diag"], "FF": np.zeros(1), "FF_diag": pl_far["FF_diag"]}
-        pl_fa = {"NF": np.zeros(1), "NF_diag": pl_near["NF_diag"], "FF": pl_far["FF"], "FF_diag": pl_far["FF_diag"]}
-        t_l_near = timed(lambda: O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], near_s, pl_ne, empty_far,
-                                               1.0, Xv, 0.0, Yv, True, False, impl=plm))
-        t_l_far = timed(lambda: O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, pl_fa, far_full,
-                                              1.0, Xv, 0.0, Yv, False, True, impl=plm))
-        t_l_base = timed(lambda: O.pl_multiply(M.dom_dim, M.V, tr, td["pre_cols"], td["pre_vals"], empty_near, pl_fa, empty_far,
-                                               1.0, Xv, 0.0, Yv, False, False, impl=plm))
-        M.h1_solve_host(Xv)                      # factorisation (per state, not timed)
-        t_h = timed(lambda: M.h1_solve_host(Xv))
-        nnz_scale = b["near_nnz"] / max(1, int(nrp[-1]))
-        t_a = t_a_base + (t_a_near - t_a_base) * nnz_scale + (t_a_far - t_a_base)
-        t_l = t_l_base + (t_l_near - t_l_base) * nnz_scale + (t_l_far - t_l_base)
-        t_it = t_a + t_l + 2.0 * t_h
-        total_s = eg_s + solve_iters * t_it
-        out.update({
-            "value": round(total_s * 1e3, 1), "unit": "ms (energy+grad + metric solve, extrapolated)",
-            "solve_ms": round(solve_iters * t_it * 1e3, 1), "iteration_ms": round(t_it * 1e3, 1),
-            "multiply_metric_ms": round(t_a * 1e3, 1), "pseudo_laplacian_ms": round(t_l * 1e3, 1),
-            "h1_solve_ms": round(t_h * 1e3, 1), "iterations": solve_iters,
-            "sample": out["sample"] + (f"; solve: one metric product and one pseudo-Laplacian product on the same near rows "
-                                       f"({int(nrp[-1])} of {b['near_nnz']} near entries, scaled) with far field and tree passes "
-                                       f"complete, one host supernodal H^1 solve (1 thread), times {solve_iters} iterations"),
-        })
-    out["sampled_cpu_s"] = round(time.perf_counter() - t_start, 1)
+def tree_pass_bytes(b, n):
+    """Compulsory bytes of the parts of one MultiplyMetric outside the near-field SpMM (pre-operator,
+    leaf and up-sweep sums, far-field SpMM, leaf path sums, vertex gather of the post-operator):
+    pre_cols (4 B) and pre_vals (4 x 8 B) per simplex vertex, X (24 B per vertex), P_in written (96 B per
+    primitive) and read by the leaf sums (96 B), 96 B per leaf written and 288 B per interior cluster for
+    the up-sweep, far-field values (16 B) and columns (4 B) per far nnz with 8 + 96 + 16 + 96 B per cluster
+    (row pointer, input, diagonal, output), 96 B read per cluster and 96 B written per leaf for the path
+    sums, and the post-operator's per-slot inputs (24 B) and entry indices (4 B) per simplex vertex with
+    8 B per vertex pointer and 24 B per output vertex."""
+    T, V, C, L = b["T"], b["V"], b["C"], b["L"]
+    s = n + 1
+    return (T * s * (4 + 32) + 24 * V + 96 * T + 96 * T + 96 * L + 288 * (C - L)
+            + 20 * b["far_nnz"] + C * (8 + 96 + 16 + 96) + 96 * C + 96 * L + T * s * (24 + 4) + V * (8 + 24))
+
+
+def preconditioner_comparison(R, WORKLOADS, name, tol, max_iter, model, threads):
+    """Metric solve A X = dE to |dE - A X|_2 / |dE|_2 <= tol (PCG, FP64 metric) on one workload with the
+    reference preconditioner P0, the two-level P0 + w diag(A)^-1 and the learned model; each solve's
+    residual is recomputed with the FP64 product. Times are device events of the solve (setup excluded)."""
+    wl = WORKLOADS[name]
+    M = wl.mesh(threads=threads)
+    M.cluster_tree_settings.device_build = True
+    b = M.build()
+    G = R.TangentPointEnergy0(wl.q, wl.p).Differential(M)
+    A64 = R.TangentPointMetric0(wl.q, wl.p)
+    out = {"workload": name, "triangles": b["T"], "method": "pcg", "tolerance": tol}
+    for label in ("reference_P0", "two_level", "learned"):
+        S = R.TangentPointMetric0(wl.q, wl.p)
+        if label == "two_level":
+            S.SetPreconditionerDiagonalWeight(-1.0)
+        if label == "learned":
+            if not model or not os.path.exists(model):
+                continue
+            S.SetLearnedPreconditioner(path=model)
+        Xs = np.zeros_like(G)
+        S.Solve(M, 1.0, G, 0.0, Xs, max_iter=max_iter, tolerance=tol, raise_on_stagnation=False)   # setup and warm-up
+        Xs[:] = 0.0
+        S.Solve(M, 1.0, G, 0.0, Xs, max_iter=max_iter, tolerance=tol, raise_on_stagnation=False)
+        ms = M.last_timing(3)
+        AX = np.zeros_like(Xs)
+        A64.MultiplyMetric(M, 1.0, Xs, 0.0, AX)
+        res = np.linalg.norm(G - AX, axis=0) / np.linalg.norm(G, axis=0)
+        out[label] = {"iterations": S.Solver_IterationCount(), "solve_ms": round(ms, 2),
+                      "fp64_residual_max": float(res.max()), "converged": bool(np.all(res <= 1.001 * tol))}
+        if label == "learned":
+            out[label]["fell_back"] = S.Solver_FellBack()
+            out[label]["model"] = os.path.relpath(model, ROOT)
+        print(f"# {name} {label}: {out[label]}", file=sys.stderr, flush=True)
+    if "learned" in out:
+        out["learned_vs_P0_iterations"] = round(out["reference_P0"]["iterations"] / max(1, out["learned"]["iterations"]), 2)
+        out["learned_vs_two_level_iterations"] = round(out["two_level"]["iterations"] / max(1, out["learned"]["iterations"]), 3)
     return out
 
 
@@ -202,7 +235,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the comparison solves (two-level FP64, reference P0 FP64) and the FP64 energy timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-pairs", type=float, default=1.2e8)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="host threads of the CPU baseline (0: this process's CPU share, at most 16)")
     ap.add_argument("--calibrate-copy", type=int, default=0, help="run only the copy calibration kernel (MiB)")
     ap.add_argument("--solver", default="pcg", choices=["pcg", "gmres"])
     ap.add_argument("--max-iter", type=int, default=3000)
@@ -215,6 +248,8 @@ def main():
                     help="diagonal weight w of P = P0 + w diag(A)^-1 (0: reference P0, <0: estimated per state)")
     ap.add_argument("--plain-pc-max-iter", type=int, default=4000,
                     help="iteration cap of the one reported solve with the reference P0 (0 skips it)")
+    ap.add_argument("--pc-compare-workload", default="torus_153600",
+                    help="workload of the preconditioner comparison (reference P0, two-level, learned); '' skips it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -283,6 +318,7 @@ def main():
     dX = C.cast(Xd, C.POINTER(C.c_double))
 
     fell_back = []
+    converged = []
 
     def step():
         _check(H.rpm_energy_differential(E._h, M._h, 1.0, 0.0, dY, 3, _p(en), 1))
@@ -294,10 +330,12 @@ def main():
         if st not in (0, RP_ESTAGNATION):
             _check(st)
         fell_back.append(int(it_c[1]))
+        converged.append(st == 0)
         return t_eg, tn, tf, M.last_timing(3)
 
     for _ in range(args.warmup):
         step()
+    converged.clear()
     nf_ms, ff_ms, eg_ms, sv_ms = [], [], [], []
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
@@ -310,6 +348,10 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     solve_iters = int(it_c[0])
+    stagnated_steps = sum(1 for c in converged if not c)
+    if stagnated_steps:
+        print(f"# WARNING: {stagnated_steps} of {len(converged)} timed solves stopped at max_iter without meeting the tolerance",
+              file=sys.stderr, flush=True)
     print(f"# energy {en[0]:.15e}  step {ms_per_step:.3f} ms (device energy+grad {np.mean(eg_ms):.3f} ms: near {np.mean(nf_ms):.3f}, "
           f"far {np.mean(ff_ms):.3f}; solve {np.mean(sv_ms):.1f} ms in {solve_iters} iterations)", file=sys.stderr, flush=True)
     # operator timings (device events; host staging excluded)
@@ -317,6 +359,20 @@ def main():
     Y = np.zeros_like(X)
     A.MultiplyMetric(M, 1.0, X, 0.0, Y)
     mm_ms, spmm_ms, pcl_ms, h1l_ms, spmm32_ms = [], [], [], [], []
+    # host-pointer (cold) calls: the step's two operations with host memory on both sides (PCIe included)
+    Gh = np.zeros((M.V, 3))
+    cold_eg, cold_sv = [], []
+    if not args.no_solve:
+        for _ in range(2):
+            t0 = time.perf_counter()
+            _check(H.rpm_energy_differential(E._h, M._h, 1.0, 0.0, _p(Gh), 3, _p(en), 0))
+            cold_eg.append((time.perf_counter() - t0) * 1e3)
+            Xc = np.zeros_like(Gh)
+            t0 = time.perf_counter()
+            stc = H.rpm_metric_solve(A._h, M._h, 1.0, _p(Gh), 3, 0.0, _p(Xc), 3, 3, args.max_iter, args.tol, solve_flags, 0, it_c, _p(rel))
+            cold_sv.append((time.perf_counter() - t0) * 1e3)
+            if stc not in (0, RP_ESTAGNATION):
+                _check(stc)
     refined = args.metric_precision == "mixed-refined"
     for _ in range(max(3, args.steps)):
         A.MultiplyMetric(M, 1.0, X, 0.0, Y)
@@ -403,8 +459,20 @@ def main():
             plain["step_ms_is_lower_bound"] = not plain["converged"]
             print(f"# plain FP64 form (reference P0 alone): {plain}", file=sys.stderr, flush=True)
 
+    pc_compare = None
+    if args.pc_compare_workload and not args.no_solve and not args.no_compare:
+        try:
+            pc_compare = preconditioner_comparison(R, WORKLOADS, args.pc_compare_workload, args.tol, args.max_iter,
+                                                   LP.DEFAULT_MODEL, threads)
+        except Exception as e:  # reported, not fatal
+            pc_compare = {"error": str(e)}
+
     pairs_nf = b["near_nnz"] // 2
     pairs_ff = b["far_nnz"] // 2
+    ff_s = float(np.mean(ff_ms)) * 1e-3
+    ff_achieved = pairs_ff * FF_FLOPS_VALUE_DIFF / ff_s / 1e12 if ff_s > 0 else 0.0
+    tree_ms = float(np.mean(mm_ms)) - float(np.mean(spmm_ms))
+    tree_bytes = tree_pass_bytes(b, M.dom_dim)
     nf_flops = pairs_nf * NF_FLOPS_VALUE_DIFF
     nf_avg_s = float(np.mean(nf_ms)) * 1e-3
     achieved = nf_flops / nf_avg_s / 1e12
@@ -472,6 +540,12 @@ def main():
         "solve_iterations": None if args.no_solve else solve_iters,
         "solve_rel_residuals": None if args.no_solve else [float(x) for x in rel],
         "solve_rel_residuals_fp64_check": fp64_check,
+        "solve_stagnated_steps": None if args.no_solve else stagnated_steps,
+        "cold": None if args.no_solve else {
+            "energy_grad_host_ms": round(float(np.median(cold_eg)), 2), "solve_host_ms": round(float(np.median(cold_sv)), 2),
+            "note": "wall time of the host-pointer calls: the differential copied to host memory (D2H), the solve reading B "
+                    "from and writing X to host memory (H2D + D2H)"},
+        "preconditioner_comparison": pc_compare,
         "solver": args.solver,
         "pc_weight": pc_weight,
         "pc_setup_ms": round(pc_setup_dev_ms, 3),
@@ -501,6 +575,17 @@ def main():
             "frac": round(spmm_gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes": int(spmm_bytes),
             "kernel_ms": round(spmm_s * 1e3, 4), "traffic": traffic,
         } if refined else None,
+        "roofline_far_energy": {
+            "kernel": "k_ff (far-field energy + differential, cluster pairs)", "bound": "mfma", "compute_unit": "valu",
+            "achieved": round(ff_achieved, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(ff_achieved / peak, 4),
+            "flops_per_pair": FF_FLOPS_VALUE_DIFF, "pairs": pairs_ff, "kernel_ms": round(ff_s * 1e3, 4),
+        },
+        "tree_passes": {
+            "kernels": "pre-operator, leaf and up-sweep sums, far-field SpMM, leaf path sums, post-operator vertex gather "
+                       "(one MultiplyMetric minus its near-field SpMM)",
+            "bound": "hbm", "ms": round(tree_ms, 4), "algorithmic_bytes": int(tree_bytes),
+            "achieved": round(tree_bytes / (tree_ms * 1e-3) / 1e9, 1) if tree_ms > 0 else None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        },
         "roofline_near_energy": {
             "kernel": "k_nf<Q6P12,E,D> (near-field energy + differential)",
             "bound": "mfma", "compute_unit": "valu",
@@ -511,8 +596,11 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
+        cores, model = host_cpu_info()
+        th = args.cpu_threads or min(16, cores)
         try:
-            result["cpu_baseline"] = cpu_baseline(M, wl, args.cpu_sample_pairs, 0 if args.no_solve else solve_iters)
+            result["cpu_baseline"] = cpu_baseline(M, wl, th, 0 if args.no_solve else solve_iters)
+            result["cpu_baseline"].update({"host_cores_available": cores, "cpu_model": model})
         except Exception as e:  # reported, not fatal
             result["cpu_baseline"] = {"error": str(e)}
     hip.hipFree(Yd)

@@ -45,6 +45,20 @@ def fast(name):
     f.restype = ref.restype
     f.argtypes = ref.argtypes
     return f
+
+
+def fast_mt(name, threads):
+    """The threaded variant (<name>_mt: rows split by cost over `threads` std::threads, per-thread
+    accumulators) of a function of the -O3 -ffast-math build, called with the checker build's signature
+    (CPU baseline timing only)."""
+    global _fast_lib
+    if _fast_lib is None:
+        _fast_lib = _load("liboracle_fast.so")
+    f = getattr(_fast_lib, name + "_mt")
+    ref = getattr(_lib, name)
+    f.restype = ref.restype
+    f.argtypes = list(ref.argtypes) + [C.c_int]
+    return lambda *args: f(*args, int(threads))
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)

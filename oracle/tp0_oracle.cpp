@@ -34,6 +34,10 @@
 //                             src/Metrics/MetricDimRestricted.hpp:109-139, ClusterTree::Pre/Post (lo_pre, lo_post)
 //                             src/ClusterTree.hpp:458-500,770-943
 //
+// The *_mt variants take a thread count: one thread runs the rows in order (what the checker uses), more
+// threads split the rows by cost with per-thread accumulators, as the reference's ParallelDoReduce /
+// AddReduce do (FMM_Traversor.hpp:56-147); they serve as the multi-core CPU baseline.
+//
 // The `Power` of the Tensors library (absent from the reference checkout) is restated as binary
 // powering for integer exponents and std::pow for real exponents.
 
@@ -41,6 +45,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 typedef int32_t Int;
@@ -69,6 +74,49 @@ namespace
     }
 
     int lin_k3( int i, int j ) { if( i > j ) { int t = i; i = j; j = t; } return i * 3 - (i * (i - 1)) / 2 + (j - i); }
+
+    // Static partition of the rows [0, rows) into `parts` contiguous ranges of about equal cost (cost(i) + 1
+    // per row): the reference's JobPointers over the rows' pair counts (FMM_Traversor.hpp:65-69).
+    template<typename Cost>
+    std::vector<LInt> CostPartition( const LInt rows, const int parts, Cost && cost )
+    {
+        std::vector<double> pre( (size_t)rows + 1, 0.0 );
+        for( LInt i = 0; i < rows; ++i ) { pre[(size_t)i + 1] = pre[(size_t)i] + (double)cost( i ) + 1.0; }
+        std::vector<LInt> cut( (size_t)parts + 1, rows );
+        cut[0] = 0;
+        for( int t = 1; t < parts; ++t )
+        {
+            const double target = pre[(size_t)rows] * t / parts;
+            const LInt k = (LInt)( std::lower_bound( pre.begin(), pre.end(), target ) - pre.begin() );
+            cut[(size_t)t] = std::max( cut[(size_t)t - 1], std::min( rows, k ) );
+        }
+        return cut;
+    }
+
+    // Runs f(t, begin, end) for the ranges of cut, one std::thread per range (the caller's thread alone
+    // for a single range).
+    template<typename F>
+    void ParallelRanges( const std::vector<LInt> & cut, F && f )
+    {
+        const int parts = (int)cut.size() - 1;
+        if( parts <= 1 ) { f( 0, cut[0], cut[1] ); return; }
+        std::vector<std::thread> pool;
+        pool.reserve( (size_t)parts );
+        for( int t = 0; t < parts; ++t ) { pool.emplace_back( [&f, &cut, t]() { f( t, cut[(size_t)t], cut[(size_t)t + 1] ); } ); }
+        for( auto & th : pool ) { th.join(); }
+    }
+
+    // out[0..n) += sum over the per-thread buffers bufs[1..] (thread order), split over threads.
+    void ReduceBuffers( double * out, const std::vector<std::vector<double>> & bufs, const LInt n, const int threads )
+    {
+        if( bufs.size() <= 1 || n <= 0 ) { return; }
+        std::vector<LInt> cut( (size_t)threads + 1 );
+        for( int t = 0; t <= threads; ++t ) { cut[(size_t)t] = n * t / threads; }
+        ParallelRanges( cut, [&]( int, LInt b, LInt e )
+        {
+            for( size_t t = 1; t < bufs.size(); ++t ) { for( LInt k = b; k < e; ++k ) { out[k] += bufs[t][(size_t)k]; } }
+        });
+    }
 
     // Exponent specialisation of TP0_Traversor::Compute (TP0_Traversor.hpp:169-245).
     struct Exponents
@@ -252,18 +300,21 @@ void orc_simplex_data( int n, LInt T, const double * V, const Int * S, const dou
 // Outputs (any may be NULL when not requested): P_D_near T x ND, C_D_far C x 10 (both zeroed here),
 // nf_vals N_nf x 2, ff_vals N_ff x 2 (full pattern, lower part filled from the upper part),
 // nf_diag T x 2, ff_diag C x 2. Returns the energy (compensated sum).
-double orc_tp0_compute( int n, LInt T, LInt C, const double * P_near, const double * C_far,
+// threads = 1 runs the rows in order (the checker); more threads split the rows of each pattern by
+// cost, with per-thread energy sums, derivative and diagonal buffers added in thread order at the end
+// (the reference's per-thread buffers and AddReduce, TP0_Traversor.hpp:145-167, 306, 340).
+double orc_tp0_compute_mt( int n, LInt T, LInt C, const double * P_near, const double * C_far,
     const LInt * nrp, const Int * nci, const LInt * ndiag,
     const LInt * frp, const Int * fci, const LInt * fdiag,
     double q, double p, int flags,
-    double * P_D_near, double * C_D_far, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag )
+    double * P_D_near, double * C_D_far, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag, int threads )
 {
     const bool energy = flags & 1, diff = flags & 2, metric = flags & 4;
     const Exponents ex( q, p, n );
     if( ex.q_flag < 0 ) { return std::nan(""); }
+    threads = std::max( 1, threads );
     const int SIZE = n + 1;
     const int ND = 1 + 3 * SIZE + 6;
-    Neumaier en;
     if( diff )
     {
         std::memset( P_D_near, 0, sizeof(double) * (size_t)(ND * T) );
@@ -275,94 +326,126 @@ double orc_tp0_compute( int n, LInt T, LInt C, const double * P_near, const doub
         std::memset( ff_diag, 0, sizeof(double) * (size_t)(2 * C) );
     }
     const double scale = 1.0 / SIZE;
+    std::vector<Neumaier> en( (size_t)threads );
+    // per-thread derivative / diagonal buffers: thread 0 writes the outputs, the others their own
+    auto buffers = [&]( const bool need ) { return std::vector<std::vector<double>>( need ? (size_t)threads : 0 ); };
     // ---- near field ----
     if( nrp[T] > 0 )
     {
-        for( LInt i = 0; i < T; ++i )
+        const std::vector<LInt> cut = CostPartition( T, threads, [&]( LInt i ) { return nrp[i + 1] - ndiag[i]; } );
+        auto dbuf = buffers( diff ), gbuf = buffers( metric );
+        ParallelRanges( cut, [&]( int t, LInt i0, LInt i1 )
         {
-            const LInt kb = ndiag[i], ke = nrp[i + 1];
-            if( ke <= kb ) { continue; }
-            const double * X = P_near + ND * i;
-            const double a = X[0];
-            double x[3];
-            for( int k = 0; k < 3; ++k ) { x[k] = scale * X[1 + k]; for( int kk = 1; kk < SIZE; ++kk ) { x[k] += scale * X[1 + 3 * kk + k]; } }
-            double DX[16] = {0};
-            double ii_block[2] = {0,0};
-            for( LInt k = kb; k < ke; ++k )
+            double * PD = P_D_near;
+            double * DG = nf_diag;
+            if( t > 0 && diff )   { dbuf[(size_t)t].assign( (size_t)(ND * T), 0.0 ); PD = dbuf[(size_t)t].data(); }
+            if( t > 0 && metric ) { gbuf[(size_t)t].assign( (size_t)(2 * T), 0.0 ); DG = gbuf[(size_t)t].data(); }
+            Neumaier e = en[(size_t)t];     // thread-local accumulator, stored once (no false sharing)
+            for( LInt i = i0; i < i1; ++i )
             {
-                const Int j = nci[k];
-                const double * Y = P_near + ND * j;
-                const double b = Y[0];
-                double y[3];
-                for( int kk = 0; kk < 3; ++kk ) { y[kk] = scale * Y[1 + kk]; for( int h = 1; h < SIZE; ++h ) { y[kk] += scale * Y[1 + 3 * h + kk]; } }
-                double DY[16] = {0};
-                const PairOut o = PairCompute( ex, n, energy, diff, metric, a, x, X + 1 + 3 * SIZE, b, y, Y + 1 + 3 * SIZE,
-                                               DX, DY, SIZE, SIZE, scale, scale );
-                if( energy ) { en.add( o.result ); }
-                if( diff ) { for( int h = 0; h < ND; ++h ) { P_D_near[ND * j + h] += DY[h]; } }
-                if( metric )
-                {
-                    nf_vals[2 * k] = o.a0; nf_vals[2 * k + 1] = o.a1;
-                    ii_block[0] -= (b / a) * o.a0;
-                    ii_block[1] -= (b / a) * o.a1;
-                    nf_diag[2 * j]     += -(a / b) * o.a0;
-                    nf_diag[2 * j + 1] += -(a / b) * o.a1;
-                }
-            }
-            if( diff ) { for( int h = 0; h < ND; ++h ) { P_D_near[ND * i + h] += DX[h]; } }
-            if( metric ) { nf_diag[2 * i] += ii_block[0]; nf_diag[2 * i + 1] += ii_block[1]; }
-        }
-        if( metric )
-        {
-            // FillLowerTriangleFromUpperTriangle.
-            for( LInt i = 0; i < T; ++i )
-            {
-                for( LInt k = nrp[i]; k < ndiag[i]; ++k )
+                const LInt kb = ndiag[i], ke = nrp[i + 1];
+                if( ke <= kb ) { continue; }
+                const double * X = P_near + ND * i;
+                const double a = X[0];
+                double x[3];
+                for( int k = 0; k < 3; ++k ) { x[k] = scale * X[1 + k]; for( int kk = 1; kk < SIZE; ++kk ) { x[k] += scale * X[1 + 3 * kk + k]; } }
+                double DX[16] = {0};
+                double ii_block[2] = {0,0};
+                for( LInt k = kb; k < ke; ++k )
                 {
                     const Int j = nci[k];
-                    const Int * b = nci + ndiag[j];
-                    const Int * e = nci + nrp[j + 1];
-                    const Int * f = std::lower_bound( b, e, (Int)i );
-                    const LInt pos = f - nci;
-                    nf_vals[2 * k] = nf_vals[2 * pos]; nf_vals[2 * k + 1] = nf_vals[2 * pos + 1];
+                    const double * Y = P_near + ND * j;
+                    const double b = Y[0];
+                    double y[3];
+                    for( int kk = 0; kk < 3; ++kk ) { y[kk] = scale * Y[1 + kk]; for( int h = 1; h < SIZE; ++h ) { y[kk] += scale * Y[1 + 3 * h + kk]; } }
+                    double DY[16] = {0};
+                    const PairOut o = PairCompute( ex, n, energy, diff, metric, a, x, X + 1 + 3 * SIZE, b, y, Y + 1 + 3 * SIZE,
+                                                   DX, DY, SIZE, SIZE, scale, scale );
+                    if( energy ) { e.add( o.result ); }
+                    if( diff ) { for( int h = 0; h < ND; ++h ) { PD[ND * j + h] += DY[h]; } }
+                    if( metric )
+                    {
+                        nf_vals[2 * k] = o.a0; nf_vals[2 * k + 1] = o.a1;
+                        ii_block[0] -= (b / a) * o.a0;
+                        ii_block[1] -= (b / a) * o.a1;
+                        DG[2 * j]     += -(a / b) * o.a0;
+                        DG[2 * j + 1] += -(a / b) * o.a1;
+                    }
                 }
+                if( diff ) { for( int h = 0; h < ND; ++h ) { PD[ND * i + h] += DX[h]; } }
+                if( metric ) { DG[2 * i] += ii_block[0]; DG[2 * i + 1] += ii_block[1]; }
             }
+            en[(size_t)t] = e;
+        });
+        if( diff ) { ReduceBuffers( P_D_near, dbuf, ND * T, threads ); }
+        if( metric )
+        {
+            ReduceBuffers( nf_diag, gbuf, 2 * T, threads );
+            // FillLowerTriangleFromUpperTriangle.
+            ParallelRanges( CostPartition( T, threads, [&]( LInt i ) { return ndiag[i] - nrp[i]; } ), [&]( int, LInt i0, LInt i1 )
+            {
+                for( LInt i = i0; i < i1; ++i )
+                {
+                    for( LInt k = nrp[i]; k < ndiag[i]; ++k )
+                    {
+                        const Int j = nci[k];
+                        const Int * b = nci + ndiag[j];
+                        const Int * e = nci + nrp[j + 1];
+                        const Int * f = std::lower_bound( b, e, (Int)i );
+                        const LInt pos = f - nci;
+                        nf_vals[2 * k] = nf_vals[2 * pos]; nf_vals[2 * k + 1] = nf_vals[2 * pos + 1];
+                    }
+                }
+            });
         }
     }
     // ---- far field ----
     if( frp[C] > 0 )
     {
-        for( LInt i = 0; i < C; ++i )
+        const std::vector<LInt> cut = CostPartition( C, threads, [&]( LInt i ) { return frp[i + 1] - fdiag[i]; } );
+        auto dbuf = buffers( diff ), gbuf = buffers( metric );
+        ParallelRanges( cut, [&]( int t, LInt i0, LInt i1 )
         {
-            const LInt kb = fdiag[i], ke = frp[i + 1];
-            if( ke <= kb ) { continue; }
-            const double * X = C_far + 10 * i;
-            const double a = X[0];
-            double DX[10] = {0};
-            double ii_block[2] = {0,0};
-            for( LInt k = kb; k < ke; ++k )
+            double * CD = C_D_far;
+            double * DG = ff_diag;
+            if( t > 0 && diff )   { dbuf[(size_t)t].assign( (size_t)(10 * C), 0.0 ); CD = dbuf[(size_t)t].data(); }
+            if( t > 0 && metric ) { gbuf[(size_t)t].assign( (size_t)(2 * C), 0.0 ); DG = gbuf[(size_t)t].data(); }
+            Neumaier e = en[(size_t)t];
+            for( LInt i = i0; i < i1; ++i )
             {
-                const Int j = fci[k];
-                const double * Y = C_far + 10 * j;
-                const double b = Y[0];
-                double DY[10] = {0};
-                const PairOut o = PairCompute( ex, n, energy, diff, metric, a, X + 1, X + 4, b, Y + 1, Y + 4, DX, DY, 1, 1, 1.0, 1.0 );
-                if( energy ) { en.add( o.result ); }
-                if( diff ) { for( int h = 0; h < 10; ++h ) { C_D_far[10 * j + h] += DY[h]; } }
-                if( metric )
+                const LInt kb = fdiag[i], ke = frp[i + 1];
+                if( ke <= kb ) { continue; }
+                const double * X = C_far + 10 * i;
+                const double a = X[0];
+                double DX[10] = {0};
+                double ii_block[2] = {0,0};
+                for( LInt k = kb; k < ke; ++k )
                 {
-                    ff_vals[2 * k] = o.a0; ff_vals[2 * k + 1] = o.a1;
-                    ii_block[0] -= (b / a) * o.a0;
-                    ii_block[1] -= (b / a) * o.a1;
-                    ff_diag[2 * j]     += -(a / b) * o.a0;
-                    ff_diag[2 * j + 1] += -(a / b) * o.a1;
+                    const Int j = fci[k];
+                    const double * Y = C_far + 10 * j;
+                    const double b = Y[0];
+                    double DY[10] = {0};
+                    const PairOut o = PairCompute( ex, n, energy, diff, metric, a, X + 1, X + 4, b, Y + 1, Y + 4, DX, DY, 1, 1, 1.0, 1.0 );
+                    if( energy ) { e.add( o.result ); }
+                    if( diff ) { for( int h = 0; h < 10; ++h ) { CD[10 * j + h] += DY[h]; } }
+                    if( metric )
+                    {
+                        ff_vals[2 * k] = o.a0; ff_vals[2 * k + 1] = o.a1;
+                        ii_block[0] -= (b / a) * o.a0;
+                        ii_block[1] -= (b / a) * o.a1;
+                        DG[2 * j]     += -(a / b) * o.a0;
+                        DG[2 * j + 1] += -(a / b) * o.a1;
+                    }
                 }
+                if( diff ) { for( int h = 0; h < 10; ++h ) { CD[10 * i + h] += DX[h]; } }
+                if( metric ) { DG[2 * i] += ii_block[0]; DG[2 * i + 1] += ii_block[1]; }
             }
-            if( diff ) { for( int h = 0; h < 10; ++h ) { C_D_far[10 * i + h] += DX[h]; } }
-            if( metric ) { ff_diag[2 * i] += ii_block[0]; ff_diag[2 * i + 1] += ii_block[1]; }
-        }
+            en[(size_t)t] = e;
+        });
+        if( diff ) { ReduceBuffers( C_D_far, dbuf, 10 * C, threads ); }
         if( metric )
         {
+            ReduceBuffers( ff_diag, gbuf, 2 * C, threads );
             for( LInt i = 0; i < C; ++i )
             {
                 for( LInt k = frp[i]; k < fdiag[i]; ++k )
@@ -375,7 +458,19 @@ double orc_tp0_compute( int n, LInt T, LInt C, const double * P_near, const doub
             }
         }
     }
-    return en.value();
+    Neumaier total;
+    for( const auto & e : en ) { total.add( e.s ); total.add( e.c ); }
+    return threads == 1 ? en[0].value() : total.value();
+}
+
+double orc_tp0_compute( int n, LInt T, LInt C, const double * P_near, const double * C_far,
+    const LInt * nrp, const Int * nci, const LInt * ndiag,
+    const LInt * frp, const Int * fci, const LInt * fdiag,
+    double q, double p, int flags,
+    double * P_D_near, double * C_D_far, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag )
+{
+    return orc_tp0_compute_mt( n, T, C, P_near, C_far, nrp, nci, ndiag, frp, fci, fdiag, q, p, flags,
+                               P_D_near, C_D_far, nf_vals, ff_vals, nf_diag, ff_diag, 1 );
 }
 
 // CollectFarFieldDerivatives + CollectNearFieldDerivatives. Inputs in tree order; outputs in the
@@ -537,17 +632,21 @@ void orc_assemble( int n, LInt V, LInt T, const Int * S, const double * B, doubl
 // Pre (MixedOrder) -> NF/FF block products -> Post, for 3 right-hand sides.
 // pre_cols T x (n+1), pre_vals T x 4 x (n+1) (tree order; row 0 averaging, rows 1..3 derivative),
 // nf_vals/ff_vals full-pattern (a0,a1), nf_diag T x 2, ff_diag C x 2.
-void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
+// threads = 1 runs in order (the checker); more threads split the pre-operator and the block products by
+// rows (cost-balanced) and the post-operator scatter into per-thread vertex buffers added in thread order.
+void orc_multiply_metric_mt( int n, LInt V, LInt T, LInt C,
     const Int * C_left, const Int * C_right, const Int * C_begin, const Int * C_end,
     const Int * pre_cols, const double * pre_vals,
     const LInt * nrp, const Int * nci, const double * nf_vals, const double * nf_diag,
     const LInt * frp, const Int * fci, const double * ff_vals, const double * ff_diag,
-    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag )
+    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag, int threads )
 {
     const int SIZE = n + 1;
+    threads = std::max( 1, threads );
     std::vector<double> P_in( (size_t)(12 * T), 0.0 ), P_out( (size_t)(12 * T), 0.0 );
     std::vector<double> C_in( (size_t)(12 * C), 0.0 ), C_out( (size_t)(12 * C), 0.0 );
-    for( LInt i = 0; i < T; ++i )
+    const std::vector<LInt> tcut = CostPartition( T, threads, []( LInt ) { return 0; } );
+    ParallelRanges( tcut, [&]( int, LInt i0, LInt i1 ) { for( LInt i = i0; i < i1; ++i )
     {
         for( int r = 0; r < 4; ++r )
         {
@@ -558,7 +657,7 @@ void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
                 P_in[12 * i + 3 * r + k] = s;
             }
         }
-    }
+    } } );
     // PrimitivesToClusters + PercolateUp (children before parents: reverse depth-first order).
     for( LInt c = C; c-- > 0; )
     {
@@ -574,7 +673,8 @@ void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
     auto apply = [&]( const LInt rows, const LInt * rp, const Int * ci, const double * vals, const double * diag,
                       const std::vector<double> & xin, std::vector<double> & yout )
     {
-        for( LInt i = 0; i < rows; ++i )
+        ParallelRanges( CostPartition( rows, threads, [&]( LInt i ) { return rp[i + 1] - rp[i]; } ), [&]( int, LInt i0, LInt i1 ) {
+        for( LInt i = i0; i < i1; ++i )
         {
             double y[12] = {0};
             for( LInt k = rp[i]; k < rp[i + 1]; ++k )
@@ -588,7 +688,7 @@ void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
             for( int h = 0; h < 3; ++h ) { y[h] += diag[2 * i] * x[h]; }
             for( int h = 3; h < 12; ++h ) { y[h] += diag[2 * i + 1] * x[h]; }
             for( int h = 0; h < 12; ++h ) { yout[12 * (size_t)i + h] = y[h]; }
-        }
+        } } );
     };
     if( nf_flag && nrp[T] > 0 ) { apply( T, nrp, nci, nf_vals, nf_diag, P_in, P_out ); }
     if( ff_flag && frp[C] > 0 ) { apply( C, frp, fci, ff_vals, ff_diag, C_in, C_out ); }
@@ -610,23 +710,41 @@ void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
             for( Int i = C_begin[c]; i < C_end[c]; ++i ) { for( int h = 0; h < 12; ++h ) { P_out[12 * (size_t)i + h] += C_out[12 * c + h]; } }
         }
     }
-    std::vector<double> acc( (size_t)(3 * V), 0.0 );
-    for( LInt i = 0; i < T; ++i )
+    std::vector<std::vector<double>> accs( (size_t)threads );
+    ParallelRanges( tcut, [&]( int t, LInt i0, LInt i1 )
     {
-        for( int l = 0; l < SIZE; ++l )
+        std::vector<double> & acc = accs[(size_t)t];
+        acc.assign( (size_t)(3 * V), 0.0 );
+        for( LInt i = i0; i < i1; ++i )
         {
-            const Int v = pre_cols[SIZE * i + l];
-            for( int r = 0; r < 4; ++r )
+            for( int l = 0; l < SIZE; ++l )
             {
-                const double w = pre_vals[4 * SIZE * i + SIZE * r + l];
-                for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += w * P_out[12 * i + 3 * r + k]; }
+                const Int v = pre_cols[SIZE * i + l];
+                for( int r = 0; r < 4; ++r )
+                {
+                    const double w = pre_vals[4 * SIZE * i + SIZE * r + l];
+                    for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += w * P_out[12 * i + 3 * r + k]; }
+                }
             }
         }
-    }
+    });
+    ReduceBuffers( accs[0].data(), accs, 3 * V, threads );
+    const std::vector<double> & acc = accs[0];
     for( LInt v = 0; v < V; ++v )
     {
         for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = alpha * acc[3 * v + k] + (beta == 0.0 ? 0.0 : beta * Y[ldY * v + k]); }
     }
+}
+
+void orc_multiply_metric( int n, LInt V, LInt T, LInt C,
+    const Int * C_left, const Int * C_right, const Int * C_begin, const Int * C_end,
+    const Int * pre_cols, const double * pre_vals,
+    const LInt * nrp, const Int * nci, const double * nf_vals, const double * nf_diag,
+    const LInt * frp, const Int * fci, const double * ff_vals, const double * ff_diag,
+    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag )
+{
+    orc_multiply_metric_mt( n, V, T, C, C_left, C_right, C_begin, C_end, pre_cols, pre_vals, nrp, nci, nf_vals, nf_diag,
+                            frp, fci, ff_vals, ff_diag, alpha, X, ldX, beta, Y, ldY, nf_flag, ff_flag, 1 );
 }
 
 // Brute-force energy: sum over all unordered pairs i < j of a_i a_j E_ij from far rows
@@ -671,11 +789,12 @@ void orc_allpairs_metric( int n, LInt T, const double * P_far, double q, double 
 // _FF.hpp:101-146), row diagonal -= (b/a) val, column diagonal += -(a/b) val; the lower part is
 // filled from the upper part. Near rows use the barycenter of the hull (FMM_Kernel_NF::loadS),
 // far rows the cluster center. nf_vals N_nf, ff_vals N_ff, nf_diag T, ff_diag C.
-void orc_pl_compute( int n, LInt T, LInt C, const double * P_near, const double * C_far,
+void orc_pl_compute_mt( int n, LInt T, LInt C, const double * P_near, const double * C_far,
     const LInt * nrp, const Int * nci, const LInt * ndiag,
     const LInt * frp, const Int * fci, const LInt * fdiag,
-    double s, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag )
+    double s, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag, int threads )
 {
+    threads = std::max( 1, threads );
     const int SIZE = n + 1;
     const int ND = 1 + 3 * SIZE + 6;
     const double scale = 1.0 / SIZE;
@@ -689,34 +808,44 @@ void orc_pl_compute( int n, LInt T, LInt C, const double * P_near, const double 
                     auto && load )
     {
         std::memset( diag, 0, sizeof(double) * (size_t)rows );
-        for( LInt i = 0; i < rows; ++i )
+        std::vector<std::vector<double>> gbuf( (size_t)threads );
+        ParallelRanges( CostPartition( rows, threads, [&]( LInt i ) { return rp[i + 1] - dg[i]; } ), [&]( int t, LInt i0, LInt i1 )
         {
-            double a, x[3];
-            load( i, a, x );
-            double ii_block = 0;
-            for( LInt k = dg[i]; k < rp[i + 1]; ++k )
+            double * D = diag;
+            if( t > 0 ) { gbuf[(size_t)t].assign( (size_t)rows, 0.0 ); D = gbuf[(size_t)t].data(); }
+            for( LInt i = i0; i < i1; ++i )
             {
-                const Int j = ci[k];
-                double b, y[3];
-                load( j, b, y );
-                double r2 = 0;
-                for( int l = 0; l < 3; ++l ) { const double v = y[l] - x[l]; r2 += v * v; }
-                const double val = -power( r2 );
-                vals[k] = val;
-                ii_block -= (b / a) * val;
-                diag[j] += -(a / b) * val;
+                double a, x[3];
+                load( i, a, x );
+                double ii_block = 0;
+                for( LInt k = dg[i]; k < rp[i + 1]; ++k )
+                {
+                    const Int j = ci[k];
+                    double b, y[3];
+                    load( j, b, y );
+                    double r2 = 0;
+                    for( int l = 0; l < 3; ++l ) { const double v = y[l] - x[l]; r2 += v * v; }
+                    const double val = -power( r2 );
+                    vals[k] = val;
+                    ii_block -= (b / a) * val;
+                    D[j] += -(a / b) * val;
+                }
+                D[i] += ii_block;
             }
-            diag[i] += ii_block;
-        }
-        for( LInt i = 0; i < rows; ++i )
+        });
+        ReduceBuffers( diag, gbuf, rows, threads );
+        ParallelRanges( CostPartition( rows, threads, [&]( LInt i ) { return dg[i] - rp[i]; } ), [&]( int, LInt i0, LInt i1 )
         {
-            for( LInt k = rp[i]; k < dg[i]; ++k )
+            for( LInt i = i0; i < i1; ++i )
             {
-                const Int j = ci[k];
-                const Int * f = std::lower_bound( ci + dg[j], ci + rp[j + 1], (Int)i );
-                vals[k] = vals[f - ci];
+                for( LInt k = rp[i]; k < dg[i]; ++k )
+                {
+                    const Int j = ci[k];
+                    const Int * f = std::lower_bound( ci + dg[j], ci + rp[j + 1], (Int)i );
+                    vals[k] = vals[f - ci];
+                }
             }
-        }
+        });
     };
     run( T, nrp, nci, ndiag, nf_vals, nf_diag, [&]( LInt i, double & a, double * x )
     {
@@ -732,28 +861,41 @@ void orc_pl_compute( int n, LInt T, LInt C, const double * P_near, const double 
     });
 }
 
+void orc_pl_compute( int n, LInt T, LInt C, const double * P_near, const double * C_far,
+    const LInt * nrp, const Int * nci, const LInt * ndiag,
+    const LInt * frp, const Int * fci, const LInt * fdiag,
+    double s, double * nf_vals, double * ff_vals, double * nf_diag, double * ff_diag )
+{
+    orc_pl_compute_mt( n, T, C, P_near, C_far, nrp, nci, ndiag, frp, fci, fdiag, s, nf_vals, ff_vals, nf_diag, ff_diag, 1 );
+}
+
 // MultiplyMetric of the pseudo-Laplacian: LowOrder Pre (row 0 of the pre-operator: a/(n+1) per
 // vertex, ClusterTree.hpp:458-500) -> scalar NF/FF block products plus diagonals -> Post
 // (PercolateDown, ClustersToPrimitives, lo_post = lo_pre^T), for 3 right-hand sides.
-void orc_pl_multiply( int n, LInt V, LInt T, LInt C,
+void orc_pl_multiply_mt( int n, LInt V, LInt T, LInt C,
     const Int * C_left, const Int * C_right, const Int * C_begin, const Int * C_end,
     const Int * pre_cols, const double * pre_vals,
     const LInt * nrp, const Int * nci, const double * nf_vals, const double * nf_diag,
     const LInt * frp, const Int * fci, const double * ff_vals, const double * ff_diag,
-    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag )
+    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag, int threads )
 {
     const int SIZE = n + 1;
+    threads = std::max( 1, threads );
     std::vector<double> P_in( (size_t)(3 * T), 0.0 ), P_out( (size_t)(3 * T), 0.0 );
     std::vector<double> C_in( (size_t)(3 * C), 0.0 ), C_out( (size_t)(3 * C), 0.0 );
-    for( LInt i = 0; i < T; ++i )
+    const std::vector<LInt> tcut = CostPartition( T, threads, []( LInt ) { return 0; } );
+    ParallelRanges( tcut, [&]( int, LInt i0, LInt i1 )
     {
-        for( int k = 0; k < 3; ++k )
+        for( LInt i = i0; i < i1; ++i )
         {
-            double s = 0;
-            for( int l = 0; l < SIZE; ++l ) { s += pre_vals[4 * SIZE * i + l] * X[ldX * pre_cols[SIZE * i + l] + k]; }
-            P_in[3 * i + k] = s;
+            for( int k = 0; k < 3; ++k )
+            {
+                double s = 0;
+                for( int l = 0; l < SIZE; ++l ) { s += pre_vals[4 * SIZE * i + l] * X[ldX * pre_cols[SIZE * i + l] + k]; }
+                P_in[3 * i + k] = s;
+            }
         }
-    }
+    });
     for( LInt c = C; c-- > 0; )
     {
         if( C_left[c] < 0 )
@@ -768,16 +910,19 @@ void orc_pl_multiply( int n, LInt V, LInt T, LInt C,
     auto apply = [&]( const LInt rows, const LInt * rp, const Int * ci, const double * vals, const double * diag,
                       const std::vector<double> & xin, std::vector<double> & yout )
     {
-        for( LInt i = 0; i < rows; ++i )
+        ParallelRanges( CostPartition( rows, threads, [&]( LInt i ) { return rp[i + 1] - rp[i]; } ), [&]( int, LInt i0, LInt i1 )
         {
-            double y[3] = {0, 0, 0};
-            for( LInt k = rp[i]; k < rp[i + 1]; ++k )
+            for( LInt i = i0; i < i1; ++i )
             {
-                const double * x = &xin[3 * (size_t)ci[k]];
-                for( int h = 0; h < 3; ++h ) { y[h] += vals[k] * x[h]; }
+                double y[3] = {0, 0, 0};
+                for( LInt k = rp[i]; k < rp[i + 1]; ++k )
+                {
+                    const double * x = &xin[3 * (size_t)ci[k]];
+                    for( int h = 0; h < 3; ++h ) { y[h] += vals[k] * x[h]; }
+                }
+                for( int h = 0; h < 3; ++h ) { yout[3 * (size_t)i + h] = y[h] + diag[i] * xin[3 * (size_t)i + h]; }
             }
-            for( int h = 0; h < 3; ++h ) { yout[3 * (size_t)i + h] = y[h] + diag[i] * xin[3 * (size_t)i + h]; }
-        }
+        });
     };
     if( nf_flag && nrp[T] > 0 ) { apply( T, nrp, nci, nf_vals, nf_diag, P_in, P_out ); }
     if( ff_flag && frp[C] > 0 ) { apply( C, frp, fci, ff_vals, ff_diag, C_in, C_out ); }
@@ -798,20 +943,38 @@ void orc_pl_multiply( int n, LInt V, LInt T, LInt C,
             for( Int i = C_begin[c]; i < C_end[c]; ++i ) { for( int h = 0; h < 3; ++h ) { P_out[3 * (size_t)i + h] += C_out[3 * c + h]; } }
         }
     }
-    std::vector<double> acc( (size_t)(3 * V), 0.0 );
-    for( LInt i = 0; i < T; ++i )
+    std::vector<std::vector<double>> accs( (size_t)threads );
+    ParallelRanges( tcut, [&]( int t, LInt i0, LInt i1 )
     {
-        for( int l = 0; l < SIZE; ++l )
+        std::vector<double> & acc = accs[(size_t)t];
+        acc.assign( (size_t)(3 * V), 0.0 );
+        for( LInt i = i0; i < i1; ++i )
         {
-            const Int v = pre_cols[SIZE * i + l];
-            const double w = pre_vals[4 * SIZE * i + l];
-            for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += w * P_out[3 * i + k]; }
+            for( int l = 0; l < SIZE; ++l )
+            {
+                const Int v = pre_cols[SIZE * i + l];
+                const double w = pre_vals[4 * SIZE * i + l];
+                for( int k = 0; k < 3; ++k ) { acc[3 * (size_t)v + k] += w * P_out[3 * i + k]; }
+            }
         }
-    }
+    });
+    ReduceBuffers( accs[0].data(), accs, 3 * V, threads );
+    const std::vector<double> & acc = accs[0];
     for( LInt v = 0; v < V; ++v )
     {
         for( int k = 0; k < 3; ++k ) { Y[ldY * v + k] = alpha * acc[3 * v + k] + (beta == 0.0 ? 0.0 : beta * Y[ldY * v + k]); }
     }
+}
+
+void orc_pl_multiply( int n, LInt V, LInt T, LInt C,
+    const Int * C_left, const Int * C_right, const Int * C_begin, const Int * C_end,
+    const Int * pre_cols, const double * pre_vals,
+    const LInt * nrp, const Int * nci, const double * nf_vals, const double * nf_diag,
+    const LInt * frp, const Int * fci, const double * ff_vals, const double * ff_diag,
+    double alpha, const double * X, LInt ldX, double beta, double * Y, LInt ldY, int nf_flag, int ff_flag )
+{
+    orc_pl_multiply_mt( n, V, T, C, C_left, C_right, C_begin, C_end, pre_cols, pre_vals, nrp, nci, nf_vals, nf_diag,
+                        frp, fci, ff_vals, ff_diag, alpha, X, ldX, beta, Y, ldY, nf_flag, ff_flag, 1 );
 }
 
 } // extern "C"
