@@ -258,15 +258,23 @@ namespace Repulsor
             if( can_fall_back && beta != 0.0 ) { X0.assign( X, X + (size_t)ldX * (size_t)( M.VertexCount() - 1 ) + 3 ); }
             int it = 0;
             double rel[3] = { 0, 0, 0 };
+            // A learned PCG solve runs under the residual guard: by iteration lp_guard_iter it must have reached
+            // the residual that the base preconditioner reaches by lp_guard_iter / LP_GUARD_SLOWDOWN iterations
+            // (base_rate per iteration); otherwise it stops early and the base preconditioner takes over.
+            const bool guarded = can_fall_back && !( flags & RP_SOLVE_GMRES ) && lp_guard_iter > 0;
+            if( guarded && !Check( rp_set_solve_guard( M.Device(), lp_guard_iter, std::pow( base_rate, lp_guard_iter / LP_GUARD_SLOWDOWN ) ),
+                                   ClassName() + "::Solve" ) ) { return false; }
             const int st = rp_solve( M.Device(), alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tolerance,
                                      flags, device_ptrs ? RP_DEVICE_PTR : 0, &it, rel );
+            const std::string why = st == RP_OK ? std::string() : std::string( rp_last_error() );
+            if( guarded ) { rp_set_solve_guard( M.Device(), 0, 0.0 ); }
             SetRecord( it, rel );
             fell_back = false;
             status = st;
             if( st == RP_ESTAGNATION && can_fall_back )
             {
                 // the two-level preconditioner the learned scalings wrap is the fallback
-                wprint( ClassName() + "::Solve: learned preconditioner stagnated; falling back to P0 + w diag(A)^-1" );
+                wprint( ClassName() + "::Solve: learned preconditioner too slow (" + why + "); falling back to P0 + w diag(A)^-1" );
                 if( !X0.empty() ) { std::copy( X0.begin(), X0.end(), X ); }
                 TangentPointMetric0 plain( q, p );
                 plain.SetPrecision( precision );
@@ -278,6 +286,11 @@ namespace Repulsor
                 rel_residuals = plain.Solver_RelativeResiduals();
                 status = plain.Solver_Status();
                 fell_back = true;
+                // the base preconditioner's measured rate becomes the guard's reference
+                const int pit = plain.Solver_IterationCount();
+                double worst = 0.0;
+                for( Int k = 0; k < rel_residuals.Size(); ++k ) { worst = std::max( worst, rel_residuals[k] ); }
+                if( ok && pit > 0 && worst > 0.0 ) { base_rate = std::pow( worst, 1.0 / pit ); }
                 return ok;
             }
             if( st == RP_ESTAGNATION )
@@ -295,10 +308,27 @@ namespace Repulsor
         // iterate) or the error status of a failed call.
         int Solver_Status() const { return status; }
 
+        // Residual guard of learned solves (no reference counterpart): the iteration at which a learned PCG
+        // solve is compared with the base preconditioner's rate (0 disables the early fallback; the fallback
+        // at max_iter stays), and that reference rate (relative residual reduction per iteration; updated by
+        // every fallback solve).
+        void SetLearnedPreconditionerGuard( const int check_iteration, const double reference_rate = 0.5 )
+        {
+            lp_guard_iter = std::max( 0, check_iteration );
+            base_rate = reference_rate;
+        }
+        int LearnedPreconditionerGuardIteration() const { return lp_guard_iter; }
+        double LearnedPreconditionerReferenceRate() const { return base_rate; }
+
     protected:
         using Base_T::iter;
         using Base_T::rel_residuals;
         mutable bool fell_back = false;
+        // A learned solve may take LP_GUARD_SLOWDOWN times the iterations of the base preconditioner before
+        // the guard hands over.
+        static constexpr double LP_GUARD_SLOWDOWN = 1.5;
+        int lp_guard_iter = 12;
+        mutable double base_rate = 0.5;
         mutable int status = RP_OK;
         SolverMethod solver_method = SolverMethod::GMRES;
 

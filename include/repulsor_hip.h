@@ -324,6 +324,13 @@ int rp_pc_learn_step( rp_ctx * ctx, float lr, int step, float * params, float * 
 int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,
               int nrhs, int max_iter, double tolerance, int solver_flags, int ptr_flags, int * iterations, double * rel_residuals );
 
+/* Residual guard of the PCG in rp_solve (no reference counterpart; the learned preconditioner's early
+ * fallback uses it): at iteration check_iteration a solve whose largest per-column relative residual (the
+ * solve's own measure) is above max_relative_residual stops and returns RP_ESTAGNATION with the iterate
+ * reached, so that the caller can switch preconditioner. check_iteration = 0 disables it (the default);
+ * the setting holds for later solves on the context until changed. GMRES ignores it. */
+int rp_set_solve_guard( rp_ctx * ctx, int check_iteration, double max_relative_residual );
+
 /* Device timing of the last call of each entry point, in milliseconds (HIP events on the
  * context's stream). which: 0 = tp0_compute, 1 = assemble_vertex_gradient, 2 = multiply_metric,
  * 3 = solve, 4 = near kernel, 5 = far kernel, 6 = near SpMM, 7 = multiply_preconditioner,

@@ -105,6 +105,7 @@ _host_sigs = {
     "rpm_pl_multiply": (C.c_int, [_vp, _vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_h1_solve_device": (C.c_int, [_vp, C.c_double, _d, C.c_int64, C.c_double, _d, C.c_int64, C.c_int]),
     "rpm_op_set_pc_weight": (None, [_vp, C.c_double]),
+    "rpm_op_set_lp_guard": (None, [_vp, C.c_int, C.c_double]),
     "rpm_op_set_pc_precision": (None, [_vp, C.c_int]),
     "rpm_op_set_learned": (None, [_vp, C.c_int, C.POINTER(C.c_float), C.c_int]),
     "rpm_learned_feature_count": (C.c_int, []),
@@ -576,6 +577,12 @@ class TangentPointMetric0(_Op):
 
     def Solver_RelativeResiduals(self):
         return getattr(self, "_relres", np.zeros(3))
+
+    def SetLearnedPreconditionerGuard(self, check_iteration=12, reference_rate=0.5):
+        """A learned PCG solve whose largest relative residual at `check_iteration` is above
+        reference_rate ** (check_iteration / 1.5) (the base preconditioner's rate, 1.5 times the iterations)
+        stops and falls back to P0 + w diag(A)^-1; 0 keeps only the fallback at max_iter."""
+        H.rpm_op_set_lp_guard(self._h, int(check_iteration), float(reference_rate))
 
     def Solver_FellBack(self):
         """Whether the last Solve fell back from the learned preconditioner to P0 + w diag(A)^-1."""

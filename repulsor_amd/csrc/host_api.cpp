@@ -28,6 +28,8 @@ namespace
         std::vector<float> lp_params;
         int lp_nfeat = 0;
         bool lp_bf16 = true;
+        int lp_guard_iter = 12;        // residual guard of learned solves (0: fall back only at max_iter)
+        double lp_base_rate = 0.5;     // its reference rate, updated by fallback solves
     };
 
     template<typename F>
@@ -56,6 +58,7 @@ namespace
         if( o->pc_precision >= 0 ) { A.SetPreconditionerPrecision( (Precision)o->pc_precision ); }
         A.SetPreconditionerDiagonalWeight( o->pc_weight );
         if( !o->lp_params.empty() ) { A.SetLearnedPreconditioner( o->lp_params, o->lp_nfeat, o->lp_bf16 ); }
+        A.SetLearnedPreconditionerGuard( o->lp_guard_iter, o->lp_base_rate );
         return A;
     }
 
@@ -283,6 +286,13 @@ void rpm_op_set_pc_precision( void * o, int precision ) { ((OpHandle *)o)->pc_pr
 // Diagonal weight of the metric's preconditioner P = P0 + w diag(A)^-1 (0: reference P0, < 0: estimated).
 void rpm_op_set_pc_weight( void * o, double w ) { ((OpHandle *)o)->pc_weight = w; }
 
+// Residual guard of learned solves: check iteration (0: off) and reference rate per iteration.
+void rpm_op_set_lp_guard( void * o, int check_iteration, double reference_rate )
+{
+    ((OpHandle *)o)->lp_guard_iter = check_iteration;
+    ((OpHandle *)o)->lp_base_rate = reference_rate;
+}
+
 // Learned preconditioner model of the operator (nfeat = 0 removes it).
 void rpm_op_set_learned( void * o, int nfeat, const float * params, int bf16 )
 {
@@ -467,6 +477,7 @@ int rpm_metric_solve( void * op, void * h, double alpha, const double * B, int64
         using Mesh = std::decay_t<decltype(M)>;
         auto A = make_metric<Mesh>( (OpHandle *)op );
         (void)A.Solve( M, alpha, B, ldB, beta, X, ldX, nrhs, max_iter, tol, device_ptr, solver_flags );
+        ((OpHandle *)op)->lp_base_rate = A.LearnedPreconditionerReferenceRate();
         const int st = A.Solver_Status();
         if( st != RP_OK && st != RP_ESTAGNATION ) { if( nrhs != 3 ) { g_err = "nrhs must be 3"; } return st; }
         const auto rr = A.Solver_RelativeResiduals();

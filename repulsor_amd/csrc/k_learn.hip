@@ -1,6 +1,8 @@
 // Learned preconditioner P = D^{1/2} (P0 + w diag(A)^-1) D^{1/2}: a 2-layer MLP (width 64) per
-// cluster-tree node maps node features to a positive scaling s_c = softplus(o_c); the vertex scaling
-// is d_v = exp( mean over the primitives i containing v of sum_{c on the path leaf(i) -> root} log s_c ).
+// cluster-tree node maps node features to a positive scaling s_c = softplus(o_c); log d_v starts as the
+// mean over the primitives i containing v of sum_{c on the path leaf(i) -> root} log s_c, which is
+// constant per leaf, and is smoothed by LP_SMOOTH_STEPS steps of S = (vertex mean of primitive means) so
+// that d carries no jumps at leaf boundaries; d_v = exp(log d_v).
 // Inference runs with BF16 weights and features (FP32 accumulation); training computes the loss
 // gradient in FP32/FP64 with fixed-order reductions (no atomics).
 
@@ -60,15 +62,55 @@ namespace rp
 
     // sqrt_d[v] = exp( 0.5 * mean over entries of v of leaf_lam[leaf_of_prim[prim]] ), with leaf_lam the
     // root-path sums of lam.
-    __global__ void k_vertex_scale( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_of_prim,
-                                    const double * leaf_lam, double * sqrt_d )
+    // logd[v] = mean over the primitives containing v of the leaf-to-root sum of their leaf.
+    __global__ void k_vertex_logd( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_of_prim,
+                                   const double * leaf_lam, double * logd )
     {
         const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if( v >= V ) { return; }
         double s = 0.0;
         const int64_t b = vp_ptr[v], e = vp_ptr[v + 1];
         for( int64_t k = b; k < e; ++k ) { s += leaf_lam[leaf_of_prim[vp_entry[k] / size]]; }
-        sqrt_d[v] = ( e > b ) ? exp( 0.5 * s / (double)( e - b ) ) : 1.0;
+        logd[v] = ( e > b ) ? s / (double)( e - b ) : 0.0;
+    }
+
+    // Smoothing step S = (vertex mean of primitive values) o (primitive mean of vertex values), first half:
+    // pm[i] = mean over the vertices of primitive i of x.
+    __global__ void k_prim_mean( int64_t T, int size, const int32_t * pre_cols, const double * x, double * pm )
+    {
+        const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( i >= T ) { return; }
+        double s = 0.0;
+        for( int l = 0; l < size; ++l ) { s += x[pre_cols[(int64_t)size * i + l]]; }
+        pm[i] = s / (double)size;
+    }
+
+    // Second half of S: x[v] = mean over the primitives containing v of pm.
+    __global__ void k_vertex_mean( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const double * pm, double * x )
+    {
+        const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( v >= V ) { return; }
+        double s = 0.0;
+        const int64_t b = vp_ptr[v], e = vp_ptr[v + 1];
+        for( int64_t k = b; k < e; ++k ) { s += pm[vp_entry[k] / size]; }
+        x[v] = ( e > b ) ? s / (double)( e - b ) : 0.0;
+    }
+
+    // Transpose of k_prim_mean: g[v] = (1 / size) * sum over the primitives containing v of h.
+    __global__ void k_vertex_sum_prim( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const double * h, double * g )
+    {
+        const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( v >= V ) { return; }
+        double s = 0.0;
+        for( int64_t k = vp_ptr[v]; k < vp_ptr[v + 1]; ++k ) { s += h[vp_entry[k] / size]; }
+        g[v] = s / (double)size;
+    }
+
+    __global__ void k_exp_half( int64_t V, const double * x, double * sqrt_d )
+    {
+        const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( v >= V ) { return; }
+        sqrt_d[v] = exp( 0.5 * x[v] );
     }
 
     // Y = s * X (row scaling by sqrt_d), optionally Y = s * X + beta * Y.
@@ -193,10 +235,28 @@ namespace rp
         else { hipLaunchKernelGGL( k_mlp_infer_f32, g256( C ), dim3(256), 0, s, C, F, feats, p32, lam ); }
     }
 
-    void launch_vertex_scale( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_of_prim,
-                              const double * leaf_lam, double * sqrt_d, hipStream_t s )
+    void launch_vertex_scale( int64_t V, int64_t T, int size, const int32_t * pre_cols, const int64_t * vp_ptr, const int32_t * vp_entry,
+                              const int32_t * leaf_of_prim, const double * leaf_lam, double * logd, double * pm, double * sqrt_d, hipStream_t s )
     {
-        if( V > 0 ) { hipLaunchKernelGGL( k_vertex_scale, g256( V ), dim3(256), 0, s, V, size, vp_ptr, vp_entry, leaf_of_prim, leaf_lam, sqrt_d ); }
+        if( V <= 0 ) { return; }
+        hipLaunchKernelGGL( k_vertex_logd, g256( V ), dim3(256), 0, s, V, size, vp_ptr, vp_entry, leaf_of_prim, leaf_lam, logd );
+        for( int k = 0; k < LP_SMOOTH_STEPS; ++k )
+        {
+            hipLaunchKernelGGL( k_prim_mean, g256( T ), dim3(256), 0, s, T, size, pre_cols, logd, pm );
+            hipLaunchKernelGGL( k_vertex_mean, g256( V ), dim3(256), 0, s, V, size, vp_ptr, vp_entry, pm, logd );
+        }
+        hipLaunchKernelGGL( k_exp_half, g256( V ), dim3(256), 0, s, V, logd, sqrt_d );
+    }
+
+    void launch_smooth_transpose( int64_t V, int64_t T, int size, const int32_t * pre_cols, const int64_t * vp_ptr, const int32_t * vp_entry,
+                                  double * g, double * h, hipStream_t s )
+    {
+        if( V <= 0 ) { return; }
+        for( int k = 0; k < LP_SMOOTH_STEPS; ++k )
+        {
+            launch_grad_prim( T, size, pre_cols, vp_ptr, g, h, s );                                           // transpose of k_vertex_mean
+            hipLaunchKernelGGL( k_vertex_sum_prim, g256( V ), dim3(256), 0, s, V, size, vp_ptr, vp_entry, h, g );   // transpose of k_prim_mean
+        }
     }
 
     void launch_row_scale( int64_t V, const double * sd, const double * X, int64_t ldX, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )

@@ -303,9 +303,14 @@ struct rp_ctx
     double lp_log_w = 0.0;                 // learned log factor on the base diagonal weight
     double lp_alpha = 1.0;                 // optimal global scale of P from the last training step
     bool lp_scale_ready = false;
-    rp::DBuf<double> lp_lam, lp_leaf, lp_sqrtd;     // C, L, V
+    rp::DBuf<double> lp_lam, lp_leaf, lp_sqrtd, lp_logd;     // C, L, V, V
     rp::DBuf<double> lp_t1, lp_t2;                   // 3V work
     rp::DBuf<double> lp_gl, lp_gp, lp_gc, lp_part, lp_grad, lp_acc;   // training
+    // PCG guard (rp_set_solve_guard): at iteration guard_iter (> 0) a solve whose largest relative residual is
+    // above guard_ratio stops with RP_ESTAGNATION; guard_tripped tells the caller why it stopped
+    int guard_iter = 0;
+    double guard_ratio = 0.0;
+    bool guard_tripped = false;
     // PCG history for training pairs: r_k and x_k, k = 0..hist_n-1 (3V each)
     rp::DBuf<double> hist_r, hist_x;
     int hist_n = 0, hist_cap = 0;

@@ -220,8 +220,14 @@ namespace rp
     constexpr int MLP_FMAX = 16;   // maximum feature count
     inline int mlp_param_count( int F ) { return MLP_H * F + 2 * MLP_H + 1; }
     void launch_mlp_infer( bool bf16, int64_t C, int F, const float * feats, const float * p32, void * p16, double * lam, hipStream_t s );
-    void launch_vertex_scale( int64_t V, int size, const int64_t * vp_ptr, const int32_t * vp_entry, const int32_t * leaf_of_prim,
-                              const double * leaf_lam, double * sqrt_d, hipStream_t s );
+    // Smoothing steps of log d (vertex mean of primitive means) after the leaf-to-root sums.
+    constexpr int LP_SMOOTH_STEPS = 16;
+    // sqrt_d = exp(log d / 2) with log d the smoothed vertex means of the leaf sums (logd: V, pm: T work).
+    void launch_vertex_scale( int64_t V, int64_t T, int size, const int32_t * pre_cols, const int64_t * vp_ptr, const int32_t * vp_entry,
+                              const int32_t * leaf_of_prim, const double * leaf_lam, double * logd, double * pm, double * sqrt_d, hipStream_t s );
+    // g <- (S^T)^LP_SMOOTH_STEPS g (h: T work), the adjoint of the smoothing.
+    void launch_smooth_transpose( int64_t V, int64_t T, int size, const int32_t * pre_cols, const int64_t * vp_ptr, const int32_t * vp_entry,
+                                  double * g, double * h, hipStream_t s );
     void launch_row_scale( int64_t V, const double * sd, const double * X, int64_t ldX, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
     void launch_grad_logd( int64_t V, const double * sd, const double * g, const double * q, const double * t, const double * r, double * out, hipStream_t s );
     void launch_grad_prim( int64_t T, int size, const int32_t * pre_cols, const int64_t * vp_ptr, const double * g_logd, double * g_prim, hipStream_t s );

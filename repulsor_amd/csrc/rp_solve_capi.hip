@@ -138,11 +138,12 @@ namespace rp
     {
         const int F = ctx->lp_F;
         ctx->lp_lam.require( (size_t)ctx->C ); ctx->lp_leaf.require( (size_t)ctx->L ); ctx->lp_sqrtd.require( (size_t)ctx->V );
+        ctx->lp_logd.require( (size_t)ctx->V ); ctx->lp_gp.require( (size_t)ctx->T );
         ctx->lp_params16.require( (size_t)mlp_param_count( F ) );
         launch_mlp_infer( bf16, ctx->C, F, ctx->lp_feats.get(), ctx->lp_params.get(), ctx->lp_params16.get(), ctx->lp_lam.get(), ctx->stream );
         launch_leaf_path_sum1( ctx->L, ctx->C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->lp_lam.get(), ctx->lp_leaf.get(), ctx->stream );
-        launch_vertex_scale( ctx->V, ctx->size, ctx->vp_ptr.get(), ctx->vp_entry.get(), ctx->leaf_of_prim.get(), ctx->lp_leaf.get(),
-                             ctx->lp_sqrtd.get(), ctx->stream );
+        launch_vertex_scale( ctx->V, ctx->T, ctx->size, ctx->pre_cols.get(), ctx->vp_ptr.get(), ctx->vp_entry.get(), ctx->leaf_of_prim.get(),
+                             ctx->lp_leaf.get(), ctx->lp_logd.get(), ctx->lp_gp.get(), ctx->lp_sqrtd.get(), ctx->stream );
         RP_HIP( hipGetLastError() );
     }
 
@@ -314,9 +315,19 @@ namespace rp
             for( int k = 0; k < 3; ++k )
             {
                 rel[(size_t)k] = h_ref[k] > 0 ? std::sqrt( std::max( 0.0, m[k] ) / h_ref[k] ) : 0.0;
-                if( m[k] > tol2 * h_ref[k] ) { ok = false; }
+                if( !( m[k] <= tol2 * h_ref[k] ) ) { ok = false; }   // a NaN residual is not converged
             }
             return ok;
+        };
+        // the guard: at iteration guard_iter, stop when the largest relative residual is above guard_ratio
+        ctx->guard_tripped = false;
+        auto guard_hit = [&]( const int k ) -> bool
+        {
+            if( ctx->guard_iter <= 0 || k != ctx->guard_iter ) { return false; }
+            bool over = false;
+            for( int c = 0; c < 3; ++c ) { if( !( rel[(size_t)c] <= ctx->guard_ratio ) ) { over = true; } }   // NaN trips it
+            ctx->guard_tripped = over;
+            return ctx->guard_tripped;
         };
         int it = 0;
         done = true;   // nothing to do when every column of B vanishes
@@ -364,6 +375,7 @@ namespace rp
                 RP_HIP( hipStreamSynchronize( s ) );
                 ++it;
                 done = all_done( h_m );
+                if( !done && guard_hit( it ) ) { break; }
                 if( done && inner32 )
                 {
                     inner32 = false;
@@ -384,6 +396,7 @@ namespace rp
                 RP_HIP( hipStreamSynchronize( s ) );
                 ++it;
                 done = all_done( h_m );
+                if( !done && guard_hit( it ) ) { break; }
                 if( done && inner32 )
                 {
                     inner32 = false;
@@ -490,7 +503,7 @@ namespace rp
                     g[(size_t)(3 * (j + 1) + k)] = -s_ * gj;
                     g[(size_t)(3 * j + k)] = c_ * gj;
                     rel[(size_t)k] = beta0[k] > 0 ? std::abs( g[(size_t)(3 * (j + 1) + k)] ) / beta0[k] : 0.0;
-                    if( rel[(size_t)k] > tol ) { all = false; }
+                    if( !( rel[(size_t)k] <= tol ) ) { all = false; }   // a NaN residual is not converged
                 }
                 if( all ) { converged = true; ++j; break; }
             }
@@ -969,7 +982,23 @@ int rp_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double 
         ctx->solve_relres = rel;
         if( iterations ) { *iterations = it; }
         if( rel_residuals ) { for( int k = 0; k < 3; ++k ) { rel_residuals[k] = rel[(size_t)k]; } }
+        if( !done && ctx->guard_tripped )
+        {
+            throw InvalidArg{ "stopped by the residual guard at iteration " + std::to_string( it ) + " (relative residual above "
+                              + std::to_string( ctx->guard_ratio ) + ")", RP_ESTAGNATION };
+        }
         if( !done ) { throw InvalidArg{ "no convergence to the requested tolerance within max_iter iterations", RP_ESTAGNATION }; }
+    });
+}
+
+int rp_set_solve_guard( rp_ctx * ctx, int check_iteration, double max_relative_residual )
+{
+    return guarded( "rp_set_solve_guard", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require( check_iteration >= 0 && std::isfinite( max_relative_residual ) && max_relative_residual >= 0.0, "bad guard" );
+        ctx->guard_iter = check_iteration;
+        ctx->guard_ratio = max_relative_residual;
     });
 }
 
@@ -1131,8 +1160,10 @@ int rp_pc_learn_step( rp_ctx * ctx, float lr, int step, float * params, float * 
         launch_axpy1( V, -alpha, gacc2, gacc1, s );                      // gacc = alpha^2 gacc1 - alpha gacc2
         const double gw_sum = alpha * alpha * gw1 - alpha * gw2;
         const double inv = 1.0 / used;
-        // dL/dlam per cluster: per primitive, then subtree sums (leaf clusters, then levels)
+        // dL/dlam per cluster: through the smoothing of log d (its adjoint), per primitive, then subtree
+        // sums (leaf clusters, then levels)
         ctx->lp_gp.require( (size_t)ctx->T ); ctx->lp_gc.require( (size_t)ctx->C );
+        launch_smooth_transpose( V, ctx->T, ctx->size, ctx->pre_cols.get(), ctx->vp_ptr.get(), ctx->vp_entry.get(), gacc1, ctx->lp_gp.get(), s );
         launch_grad_prim( ctx->T, ctx->size, ctx->pre_cols.get(), ctx->vp_ptr.get(), gacc1, ctx->lp_gp.get(), s );
         launch_up_leaves( ctx->L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->lp_gp.get(), ctx->lp_gc.get(), s, 1 );
         launch_up_tree( ctx->h_level_ptr, ctx->level_nodes.get(), ctx->C_left.get(), ctx->C_right.get(), ctx->lp_gc.get(), s, 1 );

@@ -3,7 +3,7 @@
 P = D^{1/2} (P0 + w diag(A)^-1) D^{1/2}; a 2-layer MLP of width 64 per cluster-tree node maps the node's
 features (include/repulsor/LearnedPreconditioner.hpp) to s_c = softplus(w2 . tanh(W1 f + b1) + b2);
 log d_v averages, over the primitives containing v, the sums of log s_c along their leaf-to-root
-paths. Parameters (float32): W1 (64 x F, row-major), b1 (64), w2 (64), b2 (1), log w (1); the weight w
+paths, smoothed by 16 steps of vertex means of primitive means. Parameters (float32): W1 (64 x F, row-major), b1 (64), w2 (64), b2 (1), log w (1); the weight w
 is exp(log w) times the base weight (a fixed positive setting, else the estimate lambda_max(P0 A) / 4).
 
 Model file: b"RPLP", int32 F, int32 parameter count, float32 parameters (the C++ reader is
@@ -19,7 +19,7 @@ import numpy as np
 
 HIDDEN = 64
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_MODEL = os.path.join(HERE, "models", "learned_pc_tori.rplp")
+DEFAULT_MODEL = os.path.join(HERE, "models", "learned_pc.rplp")
 
 
 def feature_count() -> int:

@@ -34,7 +34,7 @@ from helpers import oracle_energy_diff, oracle_multiply, rel
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-MODEL = os.path.join(ROOT, "repulsor_amd", "models", "learned_pc_tori.rplp")
+from repulsor_amd.learned_pc import DEFAULT_MODEL as MODEL  # noqa: E402
 
 _meshes = {}
 

@@ -1,11 +1,12 @@
 """Learned preconditioner P = D^{1/2} (P0 + w diag(A)^-1) D^{1/2} on the GPU.
 
-* vertex scalings: the FP32 path equals a numpy evaluation of the same MLP, root-path sums and
-  vertex averages (relative 1e-5); the BF16 inference path agrees to 2e-2;
+* vertex scalings: the FP32 path equals a numpy evaluation of the same MLP, root-path sums, vertex
+  averages and 16 smoothing steps (relative 1e-5); the BF16 inference path agrees to 2e-2;
 * P applied on the GPU equals D^{1/2} (P0 + w J) D^{1/2} composed from the oracle's operators (1e-9)
   and is symmetric;
 * the training gradient of rp_pc_learn_step matches central finite differences of its loss;
-* the solve converges with a model, and a deliberately poor model falls back to P0 + w diag(A)^-1.
+* the solve converges with a model; a deliberately poor model is stopped by the residual guard at
+  iteration 12, long before max_iter, and the solve falls back to P0 + w diag(A)^-1.
 """
 
 import ctypes as C
@@ -61,11 +62,17 @@ def _numpy_sqrt_d(M, params, feats):
         if tr["C_left"][c] < 0:
             leaf_of_prim[tr["C_begin"][c]:tr["C_end"][c]] = c
     td = M.tree_data()
+    cols = np.asarray(td["pre_cols"])
     acc = np.zeros(M.V); cnt = np.zeros(M.V)
-    for i in range(M.T):
-        for v in td["pre_cols"][i]:
-            acc[v] += path[leaf_of_prim[i]]; cnt[v] += 1
-    return np.exp(0.5 * acc / cnt)
+    np.add.at(acc, cols.ravel(), np.repeat(path[leaf_of_prim], cols.shape[1]))
+    np.add.at(cnt, cols.ravel(), 1.0)
+    logd = acc / cnt
+    for _ in range(16):                     # smoothing: vertex mean of primitive means
+        pm = logd[cols].mean(axis=1)
+        acc[:] = 0.0
+        np.add.at(acc, cols.ravel(), np.repeat(pm, cols.shape[1]))
+        logd = acc / cnt
+    return np.exp(0.5 * logd)
 
 
 def test_learned_scaling_matches_numpy():
@@ -189,12 +196,24 @@ def test_learned_solve_and_fallback():
     assert not A.Solver_FellBack()
     true = np.linalg.norm(Bo - S.A(X), axis=0) / np.linalg.norm(Bo, axis=0)
     assert np.all(true <= 1.5e-6), true
-    # a model with wildly varying scalings stagnates within the budget and falls back
+    # a model with wildly varying scalings is stopped by the residual guard at iteration 12 (far below
+    # max_iter) and the solve falls back to the two-level form
+    plain = R.TangentPointMetric0(wl.q, wl.p)
+    plain.SetPreconditionerDiagonalWeight(-1.0)
+    plain.Solve(M, 1.0, B, 0.0, np.zeros_like(B), max_iter=300, tolerance=1e-6)
+    n_plain = plain.Solver_IterationCount()
     bad = _model(F, scale=40.0)
     A.SetLearnedPreconditioner(bad)
     X2 = np.zeros_like(B)
-    A.Solve(M, 1.0, B, 0.0, X2, max_iter=25, tolerance=1e-6, raise_on_stagnation=False)
-    print(f"bad model: fell back {A.Solver_FellBack()}, iterations {A.Solver_IterationCount()}", flush=True)
+    A.Solve(M, 1.0, B, 0.0, X2, max_iter=300, tolerance=1e-6, raise_on_stagnation=False)
+    print(f"bad model: fell back {A.Solver_FellBack()}, iterations {A.Solver_IterationCount()} "
+          f"(two-level alone {n_plain})", flush=True)
     assert A.Solver_FellBack()
+    assert A.Solver_IterationCount() == 12 + n_plain
     true2 = np.linalg.norm(Bo - S.A(X2), axis=0) / np.linalg.norm(Bo, axis=0)
     assert np.all(true2 <= 1.5e-6), true2
+    # with the guard off, the same model falls back only at max_iter
+    A.SetLearnedPreconditionerGuard(0)
+    X3 = np.zeros_like(B)
+    A.Solve(M, 1.0, B, 0.0, X3, max_iter=40, tolerance=1e-6, raise_on_stagnation=False)
+    assert A.Solver_FellBack() and A.Solver_IterationCount() >= 40
