@@ -9,8 +9,10 @@ near theta 10, leaf size 16.
 One step = TangentPointEnergy0 energy + differential (near and far kernels, derivative
 collection, chain rule, vertex assembly) with all inputs resident in HBM and the output in a
 device buffer, followed by the metric solve A X = dE (TangentPointMetric0::Solve, three
-columns, preconditioner P0 = H^-1 L_{2-s} H^-1, stopped at |dE - A X|_2 / |dE|_2 <= 1e-6 per
-column) from and into device buffers. Tree/BCT construction, uploads, the per-state metric and
+columns, PCG with the learned preconditioner D^1/2 (P0 + w diag(A)^-1) D^1/2 around the reference
+P0 = H^-1 L_{2-s} H^-1, stopped at |dE - A X|_2 / |dE|_2 <= 1e-6 per column) from and into device
+buffers. The same right-hand side is also solved with the two-level form, in FP64, and with the
+reference P0 alone, and reported beside the headline. Tree/BCT construction, uploads, the per-state metric and
 pseudo-Laplacian assembly (device) and the H^1 factorisation (host) are timed separately and
 reported, never folded into the step.
 
@@ -229,9 +231,9 @@ def main():
     ap.add_argument("--metric-precision", default="mixed-refined", choices=["fp64", "mixed", "mixed-refined"],
                     help="metric blocks of the solve (FP64 keeps the final residual the FP64 one; mixed-refined: FP64 "
                          "values plus FP32 copies for the CG iterations, convergence judged by the FP64 residual)")
-    ap.add_argument("--learned", default="",
-                    help="learned preconditioner model for the timed solves (default: the two-level form; the packaged "
-                         "model is timed once among the comparison solves)")
+    ap.add_argument("--learned", default="packaged",
+                    help="learned preconditioner model of the timed solves: 'packaged' (repulsor_amd/models/"
+                         "learned_pc.rplp, the default), a model path, or 'none' for the two-level form")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the comparison solves (two-level FP64, reference P0 FP64) and the FP64 energy timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -278,7 +280,7 @@ def main():
     t_upload = time.perf_counter() - t0
 
     from repulsor_amd import learned_pc as LP
-    learned = args.learned
+    learned = {"packaged": LP.DEFAULT_MODEL, "none": "", "": ""}.get(args.learned, args.learned)
     E = R.TangentPointEnergy0(wl.q, wl.p, precision=args.precision)
     A = R.TangentPointMetric0(wl.q, wl.p, precision=args.metric_precision)
     A.SetPreconditionerDiagonalWeight(args.pc_weight)
@@ -420,11 +422,29 @@ def main():
                 "leaf_pairs": ss["leaf_pairs"], "simplex_pairs": ss["pairs"], "hull_tests": ss["hull_tests"]}
         print(f"# MaximumSafeStepSize: {safe}", file=sys.stderr, flush=True)
 
-    # comparison forms on the same right-hand side: the two-level preconditioner without the learned
-    # scalings, and the plain FP64 form (FP64 energy + differential, reference P0 alone; the P0 solve is
-    # capped to about a minute of iterations)
-    plain = two_level = learned_cmp = eg64 = None
+    # comparison forms on the same right-hand side:
+    #  - the two-level preconditioner at the headline's precisions (what the learned scalings buy),
+    #  - FP64 energy + differential and FP64 metric with the headline's preconditioner (what the FP32 /
+    #    mixed-refined arithmetic buys, preconditioner held fixed),
+    #  - the two-level and the learned form in FP64,
+    #  - the plain FP64 form (reference P0 alone; the P0 solve is capped to about a minute of iterations)
+    plain = two_level = learned_cmp = two_level_same = eg64 = None
     headline_learned = bool(learned) and not any(fell_back)
+
+    def compare_solve(precision, model, pc_precision=None):
+        Ac = R.TangentPointMetric0(wl.q, wl.p, precision=precision)
+        Ac.SetPreconditionerDiagonalWeight(args.pc_weight if args.pc_weight != 0 else -1.0)
+        if pc_precision:
+            Ac.SetPreconditionerPrecision(pc_precision)
+        if model:
+            Ac.SetLearnedPreconditioner(path=model, bf16=True)
+        st = H.rpm_metric_solve(Ac._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, args.max_iter, args.tol, solve_flags, 1, it_c, _p(rel))
+        out = {"metric_precision": precision, "solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]),
+               "converged": st == 0, "rel_residuals": [float(x) for x in rel]}
+        if model:
+            out.update({"model": os.path.relpath(model, ROOT), "inference": "bf16", "fell_back": bool(it_c[1])})
+        return out
+
     if not args.no_solve and not args.no_compare:
         E64 = R.TangentPointEnergy0(wl.q, wl.p, precision="fp64")
         t64 = []
@@ -432,19 +452,20 @@ def main():
             _check(H.rpm_energy_differential(E64._h, M._h, 1.0, 0.0, dY, 3, _p(en), 1))
             t64.append(M.last_timing(0) + M.last_timing(1))
         eg64 = float(np.mean(t64))
-        A2 = R.TangentPointMetric0(wl.q, wl.p, precision="fp64")
-        A2.SetPreconditionerDiagonalWeight(-1.0)
-        st = H.rpm_metric_solve(A2._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, args.max_iter, args.tol, solve_flags, 1, it_c, _p(rel))
-        two_level = {"solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]), "converged": st == 0,
-                     "rel_residuals": [float(x) for x in rel]}
+        # every comparison solves against the FP64 differential of the same state
+        if learned:
+            two_level_same = compare_solve(args.metric_precision, None, args.pc_precision)
+            two_level_same["step_ms"] = round(float(np.mean(eg_ms)) + two_level_same["solve_ms"], 1)
+            print(f"# two-level P0 + w diag(A)^-1, headline precisions: {two_level_same}", file=sys.stderr, flush=True)
+        two_level = compare_solve("fp64", None)
+        two_level["energy_grad_fp64_ms"] = round(eg64, 3)
+        two_level["step_ms"] = round(eg64 + two_level["solve_ms"], 1)
         print(f"# two-level P0 + w diag(A)^-1, FP64: {two_level}", file=sys.stderr, flush=True)
-        if not learned and os.path.exists(LP.DEFAULT_MODEL):
-            AL = R.TangentPointMetric0(wl.q, wl.p, precision="fp64")
-            AL.SetLearnedPreconditioner(path=LP.DEFAULT_MODEL, bf16=True)
-            st = H.rpm_metric_solve(AL._h, M._h, 1.0, dY, 3, 0.0, dX, 3, 3, args.max_iter, args.tol, solve_flags, 1, it_c, _p(rel))
-            learned_cmp = {"model": os.path.relpath(LP.DEFAULT_MODEL, ROOT), "inference": "bf16",
-                           "solve_ms": round(M.last_timing(3), 1), "iterations": int(it_c[0]), "converged": st == 0,
-                           "fell_back": bool(it_c[1]), "rel_residuals": [float(x) for x in rel]}
+        model = learned or (LP.DEFAULT_MODEL if os.path.exists(LP.DEFAULT_MODEL) else "")
+        if model:
+            learned_cmp = compare_solve("fp64", model)
+            learned_cmp["energy_grad_fp64_ms"] = round(eg64, 3)
+            learned_cmp["step_ms"] = round(eg64 + learned_cmp["solve_ms"], 1)
             print(f"# learned D^1/2 (P0 + w diag(A)^-1) D^1/2, FP64: {learned_cmp}", file=sys.stderr, flush=True)
         if args.plain_pc_max_iter > 0:
             A0 = R.TangentPointMetric0(wl.q, wl.p, precision="fp64")
@@ -550,8 +571,11 @@ def main():
         "pc_weight": pc_weight,
         "pc_setup_ms": round(pc_setup_dev_ms, 3),
         "solve_learned_fell_back": (not headline_learned) if learned else None,
+        "solve_two_level_same_precision": two_level_same,
         "solve_two_level_fp64": two_level,
         "solve_learned_fp64": learned_cmp,
+        "speedup_vs_fp64_same_preconditioner": (round((learned_cmp if learned else two_level)["step_ms"] / ms_per_step, 2)
+                                                if (learned_cmp if learned else two_level) else None),
         "plain_fp64_form": plain,
         "speedup_vs_plain_fp64_form": (round(plain["step_ms"] / ms_per_step, 2) if plain else None),
         "max_safe_step": safe,
