@@ -302,15 +302,20 @@ namespace rp
         double acc0[12], acc1[12];
         #pragma unroll
         for( int k = 0; k < 12; ++k ) { acc0[k] = 0.0; acc1[k] = 0.0; }
-        // lane l loads half of the input block of column l mod 32 of the chunk (doubles 6 (l / 32) ..)
+        // lane l loads half of the input block of column l mod 32 of the chunk (doubles 6 (l / 32) ..); the
+        // column indices are fetched a chunk before the blocks they address, so the gather of the next
+        // chunk waits for one load, not two
         double rx[6];
         const int half = lane >> 5, lc = lane & ( CS - 1 );
-        auto gload = [&]( int64_t base )
+        auto iload = [&]( int64_t base ) -> int32_t
         {
             const int64_t m = base + lc;
-            if( m < M_ )
+            return ( m < M_ ) ? col_prim[cb + m] : -1;
+        };
+        auto gload = [&]( int32_t rj )
+        {
+            if( rj >= 0 )
             {
-                const int32_t rj = col_prim[cb + m];
                 const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj + 6 * half );
                 #pragma unroll
                 for( int q = 0; q < 3; ++q ) { const double2 t = xj[q]; rx[2 * q] = t.x; rx[2 * q + 1] = t.y; }
@@ -321,14 +326,19 @@ namespace rp
             #pragma unroll
             for( int q = 0; q < 3; ++q ) { sx[buf][lc][3 * half + q] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
         };
-        gload( 0 );
+        int32_t rj_next = iload( CS );
+        gload( iload( 0 ) );
         lstore( 0 );
         __syncthreads();
         int buf = 0;
         for( int64_t base = 0; base < M_; base += CS )
         {
             const bool more = base + CS < M_;
-            if( more ) { gload( base + CS ); }
+            if( more )
+            {
+                gload( rj_next );
+                rj_next = iload( base + 2 * CS );
+            }
             const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {

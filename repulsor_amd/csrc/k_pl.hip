@@ -171,24 +171,30 @@ namespace rp
         const int64_t vb = val_ptr[I];
         double acc[3] = { 0, 0, 0 };
         double rx[3] = { 0, 0, 0 };
-        auto gload = [&]( int64_t base )
+        // column indices are fetched a chunk before the inputs they address
+        auto iload = [&]( int64_t base ) -> int32_t
         {
             const int64_t m = base + lane;
-            if( m < M_ )
-            {
-                const int32_t j = col_prim[cb + m];
-                for( int k = 0; k < 3; ++k ) { rx[k] = xin[3 * (int64_t)j + k]; }
-            }
+            return ( m < M_ ) ? col_prim[cb + m] : -1;
+        };
+        auto gload = [&]( int32_t j )
+        {
+            if( j >= 0 ) { for( int k = 0; k < 3; ++k ) { rx[k] = xin[3 * (int64_t)j + k]; } }
         };
         auto lstore = [&]( int buf ) { for( int k = 0; k < 3; ++k ) { sx[buf][k][lane] = rx[k]; } };
-        gload( 0 );
+        int32_t j_next = iload( PL_CH );
+        gload( iload( 0 ) );
         lstore( 0 );
         __syncthreads();
         int buf = 0;
         for( int64_t base = 0; base < M_; base += PL_CH )
         {
             const bool more = base + PL_CH < M_;
-            if( more ) { gload( base + PL_CH ); }
+            if( more )
+            {
+                gload( j_next );
+                j_next = iload( base + 2 * PL_CH );
+            }
             const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
             if( active )
             {
