@@ -350,6 +350,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     solve_iters = int(it_c[0])
+    solve_rel = [float(x) for x in rel]     # the last timed solve's (the comparison solves reuse rel)
     stagnated_steps = sum(1 for c in converged if not c)
     if stagnated_steps:
         print(f"# WARNING: {stagnated_steps} of {len(converged)} timed solves stopped at max_iter without meeting the tolerance",
@@ -559,7 +560,7 @@ def main():
         "h1_solve_ms": round(h1_ms, 3),
         "solve_ms": None if args.no_solve else round(float(np.mean(sv_ms)), 2),
         "solve_iterations": None if args.no_solve else solve_iters,
-        "solve_rel_residuals": None if args.no_solve else [float(x) for x in rel],
+        "solve_rel_residuals": None if args.no_solve else solve_rel,
         "solve_rel_residuals_fp64_check": fp64_check,
         "solve_stagnated_steps": None if args.no_solve else stagnated_steps,
         "cold": None if args.no_solve else {
