@@ -130,6 +130,8 @@ int rp_ctx_create( int device, rp_ctx ** out )
         {
             RP_HIP( hipStreamCreateWithFlags( &c->stream, hipStreamNonBlocking ) );
             for( int k = 0; k < 16; ++k ) { RP_HIP( hipEventCreate( &c->ev[k][0] ) ); RP_HIP( hipEventCreate( &c->ev[k][1] ) ); }
+            RP_HIP( hipEventCreateWithFlags( &c->ev_check, hipEventDisableTiming ) );
+            RP_HIP( hipHostMalloc( (void **)&c->h_pinned, 16 * sizeof(double), hipHostMallocDefault ) );
         }
         catch( ... ) { delete c; throw; }
         *out = c;
@@ -142,6 +144,8 @@ int rp_ctx_destroy( rp_ctx * ctx )
     (void)hipSetDevice( ctx->device );
     if( ctx->stream ) { (void)hipStreamSynchronize( ctx->stream ); }
     for( int k = 0; k < 16; ++k ) { (void)hipEventDestroy( ctx->ev[k][0] ); (void)hipEventDestroy( ctx->ev[k][1] ); }
+    if( ctx->ev_check ) { (void)hipEventDestroy( ctx->ev_check ); }
+    if( ctx->h_pinned ) { (void)hipHostFree( ctx->h_pinned ); }
     if( ctx->stream ) { (void)hipStreamDestroy( ctx->stream ); }
     delete ctx;
     return RP_OK;

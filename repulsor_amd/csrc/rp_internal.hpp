@@ -161,6 +161,8 @@ struct rp_ctx
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[16][2];
+    hipEvent_t ev_check = nullptr;       // PCG: the residual read-back of an iteration has landed
+    double * h_pinned = nullptr;         // 16 doubles of pinned host memory for the PCG read-backs
 
     // sizes
     int dom_dim = 0, size = 0, nd = 0;

@@ -362,6 +362,14 @@ namespace rp
             launch_copy3( V, z, 3, p, 3, s );
             return false;
         };
+        // While the host waits for an iteration's residual, the stream already runs the next
+        // preconditioner application when the residual history says this iteration will not converge
+        // (the last relative residual times the last reduction ratio still above twice the tolerance):
+        // the device work and its order are the same either way, only the host's wait moves, so the
+        // GPU does not drain and refill once per iteration. A wrong guess costs one unused
+        // preconditioner application.
+        double rel_prev = 1.0, ratio_prev = 1.0;
+        double * h_pin = ctx->h_pinned;
         while( !done && it < max_iter )
         {
             metric_apply( ctx, 1.0, p, 3, 0.0, Ap, 3, true, true, false, true, inner32 );
@@ -370,11 +378,25 @@ namespace rp
             keep( r, x );
             if( !pc_norm )
             {
-                RP_HIP( hipMemcpyAsync( h_m, rr, sizeof(h_m), hipMemcpyDeviceToHost, s ) );
+                RP_HIP( hipMemcpyAsync( h_pin, rr, 3 * sizeof(double), hipMemcpyDeviceToHost, s ) );
                 launch_cg_check( rr, bb, tol2, active, s );
-                RP_HIP( hipStreamSynchronize( s ) );
+                RP_HIP( hipEventRecord( ctx->ev_check, s ) );
                 ++it;
+                const bool ahead = it < max_iter && rel_prev * std::min( 1.0, ratio_prev ) > 2.0 * tol;
+                if( ahead )
+                {
+                    precondition( r, z );
+                    launch_dot3( V, r, z, part, rho[cur ^ 1], s );
+                }
+                RP_HIP( hipEventSynchronize( ctx->ev_check ) );
+                for( int k = 0; k < 3; ++k ) { h_m[k] = h_pin[k]; }
                 done = all_done( h_m );
+                {
+                    double worst = 0.0;
+                    for( int k = 0; k < 3; ++k ) { if( !( rel[(size_t)k] <= tol ) ) { worst = std::max( worst, std::isfinite( rel[(size_t)k] ) ? rel[(size_t)k] : 1.0 ); } }
+                    ratio_prev = rel_prev > 0.0 ? worst / rel_prev : 1.0;
+                    rel_prev = worst;
+                }
                 if( !done && guard_hit( it ) ) { break; }
                 if( done && inner32 )
                 {
@@ -384,8 +406,11 @@ namespace rp
                     continue;
                 }
                 if( done || it >= max_iter ) { break; }
-                precondition( r, z );
-                launch_dot3( V, r, z, part, rho[cur ^ 1], s );
+                if( !ahead )
+                {
+                    precondition( r, z );
+                    launch_dot3( V, r, z, part, rho[cur ^ 1], s );
+                }
             }
             else
             {
