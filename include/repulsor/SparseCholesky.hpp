@@ -6,7 +6,8 @@
 //
 // Algorithm (textbook, e.g. Davis, "Direct Methods for Sparse Linear Systems", 2006):
 //   elimination tree (Liu) -> postorder -> column counts by row-subtree traversal ->
-//   fundamental supernodes with relaxed amalgamation of small chains -> supernode row sets by
+//   fundamental supernodes, amalgamated along chains (a column whose only child is the previous
+//   column joins its supernode, explicit zeros where the structures differ) -> supernode row sets by
 //   merging the children's update rows -> multifrontal numeric factorisation (dense partial
 //   Cholesky of each front, extend-add of the children's update matrices).
 // For the triangular solves on the GPU every supernode keeps the inverse of its dense diagonal
@@ -95,7 +96,8 @@ namespace Repulsor
     // A: symmetric CSR (full pattern, with diagonal), perm0: fill-reducing ordering new -> old.
     inline SupernodalFactor SparseCholeskyFactor( const Int n, const std::vector<LInt> & rp, const std::vector<Int> & ci,
                                                   const std::vector<double> & av, const std::vector<Int> & perm0,
-                                                  const int threads = 1, const Int relax_cols = 16 )
+                                                  const int threads = 1, const Int relax_cols = 16, const Int max_width = 512,
+                                                  const bool chain_merge = true )
     {
         ScopedTimer timer("SparseCholesky::Factor");
         SupernodalFactor F;
@@ -175,7 +177,12 @@ namespace Repulsor
             const Int width = j - sb.back();
             // relaxed: also merge a short chain link when the column count grows by at most 2
             const bool relaxed = parent[(size_t)(j - 1)] == j && nchild[(size_t)j] == 1 && width < relax_cols && cc[(size_t)(j - 1)] <= cc[(size_t)j] + 3;
-            if( !( fundamental || relaxed ) || width >= 512 ) { sb.push_back( j ); }
+            // chain: a column whose only child is the previous column joins it whatever the fill; a chain
+            // of supernodes is a chain of dependent elimination levels, one launch per level and sweep
+            // on the GPU, and merging them costs about 1 % more stored values on nested-dissection H^1
+            // factors while cutting the levels by a third
+            const bool chain = chain_merge && parent[(size_t)(j - 1)] == j && nchild[(size_t)j] == 1;
+            if( !( fundamental || relaxed || chain ) || width >= max_width ) { sb.push_back( j ); }
         }
         sb.push_back( n );
         const Int ns = (Int)sb.size() - 1;
