@@ -96,7 +96,7 @@ namespace Repulsor
     // A: symmetric CSR (full pattern, with diagonal), perm0: fill-reducing ordering new -> old.
     inline SupernodalFactor SparseCholeskyFactor( const Int n, const std::vector<LInt> & rp, const std::vector<Int> & ci,
                                                   const std::vector<double> & av, const std::vector<Int> & perm0,
-                                                  const int threads = 1, const Int relax_cols = 16, const Int max_width = 512,
+                                                  const int threads = 1, const Int relax_cols = 16, const Int max_width = 2048,
                                                   const bool chain_merge = true )
     {
         ScopedTimer timer("SparseCholesky::Factor");

@@ -159,12 +159,13 @@ namespace rp
     }
 
     // Forward, larger levels: one workgroup of NW waves per task (supernode, 64 front rows); the waves
-    // split the columns and add their partial sums in LDS.
+    // split the columns and add their partial sums in LDS. NW = 4 serves levels whose supernodes are at
+    // most CHOL_NARROW_WIDTH columns wide (the launch picks NW by the level's widest supernode).
     template<int NW>
     __global__ __launch_bounds__(64 * NW) void k_chol_fwd_big( CholDev F, const CholTask * tasks )
     {
         constexpr int NT = 64 * NW;
-        __shared__ double zt[3 * CHOL_MAX_WIDTH];
+        __shared__ double zt[3 * ( NW == 16 ? CHOL_MAX_WIDTH : CHOL_NARROW_WIDTH )];
         __shared__ double part[NW * 64 * 3];
         const CholTask tk = tasks[blockIdx.x];
         const int row0 = tk.k, f = tk.f, w = tk.w, r = tk.r;
@@ -403,6 +404,7 @@ namespace rp
     void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s )
     {
         if( count <= 0 ) { return; }
+        // wide: the level's widest supernode has more than CHOL_NARROW_WIDTH columns
         if( wide ) { hipLaunchKernelGGL( (k_chol_fwd_big<16>), dim3( (unsigned)count ), dim3( 1024 ), 0, s, F, tasks ); }
         else       { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
     }

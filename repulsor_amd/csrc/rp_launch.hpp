@@ -103,7 +103,8 @@ namespace rp
         double        * z;          // 3 x rows: front vectors
         double        * xp;         // 3 x n: solution in the new numbering
     };
-    constexpr int CHOL_MAX_WIDTH = 512;   // columns per supernode
+    constexpr int CHOL_MAX_WIDTH = 2048;  // columns per supernode
+    constexpr int CHOL_NARROW_WIDTH = 128; // levels up to this width run the 4-wave forward (k_chol_fwd_big<4>)
     constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a wavefront per supernode
     constexpr int CHOL_FWD_ROWS = 64;     // front rows per forward task (one per lane)
     void launch_chol_form_m( const CholTask * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s );

@@ -77,7 +77,7 @@ namespace rp
             else
             {
                 const int64_t tb = ctx->h1_task_ptr[0][l], te = ctx->h1_task_ptr[0][l + 1];
-                launch_chol_fwd_big( F, tasks + tb, te - tb, ctx->h1_level_wmax[l] > 128, ctx->stream );
+                launch_chol_fwd_big( F, tasks + tb, te - tb, ctx->h1_level_wmax[l] > CHOL_NARROW_WIDTH, ctx->stream );
             }
         }
         for( size_t l = nl; l-- > 0; )
@@ -90,7 +90,7 @@ namespace rp
             else
             {
                 const int64_t tb = ctx->h1_task_ptr[1][l], te = ctx->h1_task_ptr[1][l + 1];
-                launch_chol_bwd_big( F, tasks + tb, te - tb, ctx->h1_level_rmax[l], ctx->h1_level_wmax[l] > 128, ctx->stream );
+                launch_chol_bwd_big( F, tasks + tb, te - tb, ctx->h1_level_rmax[l], ctx->h1_level_wmax[l] > CHOL_NARROW_WIDTH, ctx->stream );
             }
         }
         launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream );
@@ -722,7 +722,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         {
             const int32_t w = sn_begin[s + 1] - sn_begin[s];
             const int64_t r = row_ptr[s + 1] - row_ptr[s];
-            require( w >= 1 && w <= 512, "supernode widths must lie in 1..512", RP_EUNSUPPORTED );
+            require( w >= 1 && w <= CHOL_MAX_WIDTH, "supernode widths must lie in 1..2048", RP_EUNSUPPORTED );
             require( r >= w && r <= n, "bad supernode row count" );
             require( val_ptr[s + 1] - val_ptr[s] == r * w, "val_ptr must hold r x w blocks" );
             const int32_t * R = rows + row_ptr[s];

@@ -1,10 +1,12 @@
-"""GPU triangular solves with a hand-built supernodal factor (rp_upload_h1_factor + rp_h1_solve through
+"""GPU triangular solves with hand-built supernodal factors (rp_upload_h1_factor + rp_h1_solve through
 the C-ABI), against a dense solve of H = L L^T.
 
-The factor is shaped so that its lowest level is a wide level (supernodes wider than 128 columns)
-whose longest supernode has more than 2560 rows while another has between 512 and 2560: the backward
-kernel of such a level must size its staging buffer for the tasks that stage v at once and chunk the
-others. One supernode's tasks fall on both sides of that rule (r - k above and below 2560).
+* The first factor is shaped so that its lowest level is a wide level (supernodes wider than 128
+  columns) whose longest supernode has more than 2560 rows while another has between 512 and 2560: the
+  backward kernel of such a level must size its staging buffer for the tasks that stage v at once and
+  chunk the others. One supernode's tasks fall on both sides of that rule (r - k above and below 2560).
+* The second has a root supernode of 1536 columns (supernodes may be up to 2048 wide, as chains of the
+  elimination tree are amalgamated) above two children of 300 and 364 columns.
 """
 
 import ctypes as C
@@ -29,9 +31,9 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
-def _factor():
-    """Supernodes (first column, width, update-row ranges, parent) in postorder; 3016 columns."""
-    sn = [
+FACTORS = {
+    # supernodes (first column, width, update-row ranges, parent) in postorder, and the column count
+    "wide_level_long_and_medium": ([
         (0,    256, [(700, 3016)],              2),   # r = 256 + 2316 = 2572: tasks with k < 12 chunk, the rest stage at once
         (256,  200, [(456, 968), (968, 1268)],  2),   # r = 1012, on the same level
         (456,  512, [(968, 3016)],              3),   # r = 2560
@@ -39,8 +41,17 @@ def _factor():
         (1480, 512, [(1992, 3016)],             5),
         (1992, 512, [(2504, 3016)],             6),
         (2504, 512, [],                         -1),
-    ]
-    n = 3016
+    ], 3016),
+    "root_1536_columns": ([
+        (0,    300, [(664, 2200)],              2),   # r = 1836
+        (300,  364, [(664, 1800)],              2),   # r = 1500
+        (664,  1536, [],                        -1),
+    ], 2200),
+}
+
+
+def _factor(name):
+    sn, n = FACTORS[name]
     rng = np.random.default_rng(21)
     L = np.zeros((n, n))
     sn_begin, row_ptr, rows, val_ptr, vals, parent = [0], [0], [], [0], [], []
@@ -64,8 +75,9 @@ def _factor():
             np.array(val_ptr, np.int64), np.concatenate(vals), np.array(parent, np.int32))
 
 
-def test_h1_solve_wide_level_with_long_and_medium_supernodes(require_gpu):
-    n, L, sn_begin, row_ptr, rows, val_ptr, vals, parent = _factor()
+@pytest.mark.parametrize("name", list(FACTORS))
+def test_h1_solve_hand_built_factor(name, require_gpu):
+    n, L, sn_begin, row_ptr, rows, val_ptr, vals, parent = _factor(name)
     ns = len(parent)
     rng = np.random.default_rng(22)
     perm = rng.permutation(n).astype(np.int32)        # new -> old
