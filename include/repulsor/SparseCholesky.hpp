@@ -372,17 +372,39 @@ namespace Repulsor
             double * Ls = &F.vals[(size_t)F.val_ptr[(size_t)s]];
             for( Int c = 0; c < w; ++c ) { for( Int q = 0; q < r; ++q ) { Ls[(size_t)c * r + q] = (q >= c) ? Fm[(size_t)c * r + q] : 0.0; } }
             {
-                std::vector<double> inv( (size_t)w * w, 0.0 );   // column-major inverse of the lower diagonal block
-                for( Int c = 0; c < w; ++c )
+                // column-major inverse of the lower diagonal block, column c by forward substitution on
+                // e_c in column (axpy) order; wide blocks split their columns over the threads in chunks
+                // of equal work (column c costs about (w - c)^2 / 2)
+                std::vector<double> inv( (size_t)w * w, 0.0 );
+                auto inv_cols = [&]( Int cb, Int ce )
                 {
-                    inv[(size_t)c * w + c] = 1.0 / Ls[(size_t)c * r + c];
-                    for( Int a = c + 1; a < w; ++a )
+                    for( Int c = cb; c < ce; ++c )
                     {
-                        double acc = 0;
-                        for( Int b = c; b < a; ++b ) { acc += Ls[(size_t)b * r + a] * inv[(size_t)c * w + b]; }
-                        inv[(size_t)c * w + a] = -acc / Ls[(size_t)a * r + a];
+                        double * x = &inv[(size_t)c * w];
+                        x[c] = 1.0;
+                        for( Int b = c; b < w; ++b )
+                        {
+                            const double * lb = &Ls[(size_t)b * r];
+                            const double xb = x[b] / lb[b];
+                            x[b] = xb;
+                            for( Int a = b + 1; a < w; ++a ) { x[a] -= lb[a] * xb; }
+                        }
                     }
+                };
+                if( w > 256 && threads > 1 )
+                {
+                    const int tc = threads;
+                    std::vector<Int> cut( (size_t)tc + 1, w );
+                    cut[0] = 0;
+                    const double tot = (double)w * w * w / 6.0;
+                    for( int q = 1; q < tc; ++q )
+                    {
+                        const double target = tot * ( 1.0 - (double)q / tc );   // work of the columns >= cut is ~ (w - cut)^3 / 6
+                        cut[(size_t)q] = std::max( cut[(size_t)q - 1], (Int)( w - std::cbrt( 6.0 * target ) ) );
+                    }
+                    ParallelDo( tc, [&]( int q ) { inv_cols( cut[(size_t)q], cut[(size_t)q + 1] ); } );
                 }
+                else { inv_cols( 0, w ); }
                 for( Int c = 0; c < w; ++c ) { for( Int a = 0; a < w; ++a ) { Ls[(size_t)c * r + a] = inv[(size_t)c * w + a]; } }
             }
             const Int u = r - w;
