@@ -244,8 +244,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-solve", action="store_true", help="time energy+grad only")
     ap.add_argument("--host-tree", action="store_true", help="build the cluster tree and block cluster tree on the host")
-    ap.add_argument("--pc-precision", default="mixed", choices=["fp64", "mixed"],
-                    help="pseudo-Laplacian values inside the preconditioner (the metric keeps --metric-precision)")
+    ap.add_argument("--pc-precision", default="bf16", choices=["fp64", "mixed", "bf16"],
+                    help="pseudo-Laplacian values inside the preconditioner: FP64, FP32, or FP32 arithmetic stored in "
+                         "BF16 (the metric keeps --metric-precision; the solve is judged by the FP64 residual)")
     ap.add_argument("--pc-weight", type=float, default=-1.0,
                     help="diagonal weight w of P = P0 + w diag(A)^-1 (0: reference P0, <0: estimated per state)")
     ap.add_argument("--plain-pc-max-iter", type=int, default=4000,

@@ -21,7 +21,8 @@ namespace Repulsor
     {
         FP64         = 0,
         Mixed        = 1,
-        MixedRefined = 2
+        MixedRefined = 2,
+        PseudoLaplacianBF16 = 3   // preconditioner only: near pseudo-Laplacian values stored in BF16
     };
 
     inline const char * PrecisionTag( Precision prec )

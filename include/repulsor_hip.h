@@ -44,6 +44,11 @@ extern "C" {
  * residual meets the tolerance, then replace the residual by the FP64 one and, where that is still
  * above the tolerance, continue on the FP64 values. No reference counterpart. */
 #define RP_MIXED_REFINED 2
+/* RP_PL_BF16 (rp_pl_compute only; no reference counterpart): the pseudo-Laplacian inside the
+ * preconditioner with its near-field values computed in FP32 and stored in BF16 (the diagonal summed
+ * from the rounded values, so constants stay in its kernel), the far field in FP32. The preconditioner
+ * may be inexact; the solve is judged by the metric's residual. */
+#define RP_PL_BF16       3
 
 typedef struct rp_ctx rp_ctx;
 

@@ -21,7 +21,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 
 RP_OK, RP_EINVAL, RP_EHIP, RP_ENOMEM, RP_ESTAGNATION, RP_EUNSUPPORTED, RP_ENOTREADY = range(7)
 RP_SOLVE_NO_PRECONDITIONER, RP_SOLVE_PC_NORM, RP_SOLVE_GMRES, RP_SOLVE_RECORD = 1, 2, 4, 8
-PRECISIONS = {"fp64": 0, "mixed": 1, "mixed-refined": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1, "mixed-refined": 2, "bf16": 3}
 
 
 class RepulsorError(RuntimeError):
@@ -521,8 +521,8 @@ class TangentPointMetric0(_Op):
         return Y
 
     def SetPreconditionerPrecision(self, precision):
-        """Precision ("fp64" or "mixed") of the pseudo-Laplacian inside the preconditioner; the metric
-        operator keeps its own precision (no reference counterpart)."""
+        """Precision ("fp64", "mixed" or "bf16": FP32 values stored in BF16) of the pseudo-Laplacian
+        inside the preconditioner; the metric operator keeps its own precision (no reference counterpart)."""
         H.rpm_op_set_pc_precision(self._h, PRECISIONS[precision])
 
     def SetPreconditionerDiagonalWeight(self, w):

@@ -12,17 +12,26 @@
 #include "k_pair.hpp"
 #include "rp_launch.hpp"
 
+#include <hip/hip_bf16.h>
 #include <type_traits>
 
 namespace rp
 {
     constexpr int PL_CH = 64;
 
+    // near-field value storage of the pseudo-Laplacian: FP64, FP32, or BF16 (the value computed in FP32
+    // and rounded to nearest even; the diagonal is summed from the rounded values, so the stored operator
+    // is the pseudo-Laplacian of the rounded weights and annihilates constants as the exact one does)
+    constexpr int PL_F64 = 0, PL_F32 = 1, PL_BF16 = 2;
+    __device__ __forceinline__ double pl_val( double x ) { return x; }
+    __device__ __forceinline__ double pl_val( float x ) { return (double)x; }
+    __device__ __forceinline__ double pl_val( __hip_bfloat16 x ) { return (double)__bfloat162float( x ); }
+
     // ------------------------------------------------------------------ near-field assembly
-    template<bool MIXED>
+    template<int VM>
     __global__ __launch_bounds__(64) void k_nf_pl_assemble( NFArgs A )
     {
-        using R = typename std::conditional<MIXED, float, double>::type;
+        using R = typename std::conditional<VM == PL_F64, double, float>::type;
         __shared__ double sy[2][4][PL_CH];   // b, y0, y1, y2
         __shared__ uint32_t sm[2][PL_CH];
         __shared__ double red[64];
@@ -80,14 +89,22 @@ namespace rp
                         const R v0 = (R)(sy[buf][1][mm] - x0), v1 = (R)(sy[buf][2][mm] - x1), v2 = (R)(sy[buf][3][mm] - x2);
                         const R r2 = v0 * v0 + v1 * v1 + v2 * v2;
                         const R val = -metric_power<R>( A.ex, r2 );
-                        if constexpr ( MIXED ) { reinterpret_cast<float *>(A.vals)[slot] = val; }
-                        else                   { reinterpret_cast<double *>(A.vals)[slot] = val; }
-                        dacc -= ( b * inv_a ) * (double)val;
+                        double stored;
+                        if constexpr ( VM == PL_BF16 )
+                        {
+                            const __hip_bfloat16 h = __float2bfloat16( val );
+                            reinterpret_cast<__hip_bfloat16 *>(A.vals)[slot] = h;
+                            stored = pl_val( h );
+                        }
+                        else if constexpr ( VM == PL_F32 ) { reinterpret_cast<float *>(A.vals)[slot] = val; stored = (double)val; }
+                        else                               { reinterpret_cast<double *>(A.vals)[slot] = val; stored = val; }
+                        dacc -= ( b * inv_a ) * stored;
                     }
                     else
                     {
-                        if constexpr ( MIXED ) { reinterpret_cast<float *>(A.vals)[slot] = 0.0f; }
-                        else                   { reinterpret_cast<double *>(A.vals)[slot] = 0.0; }
+                        if constexpr ( VM == PL_BF16 )     { reinterpret_cast<__hip_bfloat16 *>(A.vals)[slot] = __float2bfloat16( 0.0f ); }
+                        else if constexpr ( VM == PL_F32 ) { reinterpret_cast<float *>(A.vals)[slot] = 0.0f; }
+                        else                               { reinterpret_cast<double *>(A.vals)[slot] = 0.0; }
                     }
                 }
             }
@@ -207,7 +224,7 @@ namespace rp
                 {
                     double v[8];
                     #pragma unroll
-                    for( int u = 0; u < 8; ++u ) { v[u] = (double)vrow[(int64_t)( mm + u * nS ) * nI]; }
+                    for( int u = 0; u < 8; ++u ) { v[u] = pl_val( vrow[(int64_t)( mm + u * nS ) * nI] ); }
                     #pragma unroll
                     for( int u = 0; u < 8; ++u )
                     {
@@ -217,7 +234,7 @@ namespace rp
                 }
                 for( ; mm < cnt_c; mm += nS )
                 {
-                    const double v = (double)vrow[(int64_t)mm * nI];
+                    const double v = pl_val( vrow[(int64_t)mm * nI] );
                     for( int k = 0; k < 3; ++k ) { acc[k] += v * sx[buf][k][mm]; }
                 }
             }
@@ -286,12 +303,14 @@ namespace rp
     }
 
     // ------------------------------------------------------------------ launchers
-    void launch_pl_assemble( bool mixed, const NFArgs & A, const FFArgs & B, bool do_nf, bool do_ff, hipStream_t s )
+    void launch_pl_assemble( int precision, const NFArgs & A, const FFArgs & B, bool do_nf, bool do_ff, hipStream_t s )
     {
+        const bool mixed = precision != RP_FP64;   // the far field: FP32 unless FP64
         if( do_nf && A.L )
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_assemble<true>), leaf_grid( A.L ), dim3(64), 0, s, A ); }
-            else        { hipLaunchKernelGGL( (k_nf_pl_assemble<false>), leaf_grid( A.L ), dim3(64), 0, s, A ); }
+            if( precision == RP_PL_BF16 ) { hipLaunchKernelGGL( (k_nf_pl_assemble<PL_BF16>), leaf_grid( A.L ), dim3(64), 0, s, A ); }
+            else if( mixed )              { hipLaunchKernelGGL( (k_nf_pl_assemble<PL_F32>),  leaf_grid( A.L ), dim3(64), 0, s, A ); }
+            else                          { hipLaunchKernelGGL( (k_nf_pl_assemble<PL_F64>),  leaf_grid( A.L ), dim3(64), 0, s, A ); }
         }
         if( do_ff && B.C )
         {
@@ -309,7 +328,7 @@ namespace rp
         else            { hipLaunchKernelGGL( (k_pl_pre<2>), grid, dim3(256), 0, s, T, cols, pre_vals, X, ldX, P_in ); }
     }
 
-    void launch_pl_nf_spmm( bool mixed, int size, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp,
+    void launch_pl_nf_spmm( int precision, int size, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp,
                             const int32_t * cpr, const uint32_t * cm, const int64_t * vp, const void * vals, const double * diag,
                             const double * xin, const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s )
     {
@@ -317,12 +336,14 @@ namespace rp
         const dim3 grid = leaf_grid( L ), block( 64 );
         if( size == 3 )
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( precision == RP_PL_BF16 ) { hipLaunchKernelGGL( (k_nf_pl_spmm<__hip_bfloat16,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const __hip_bfloat16 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else if( precision == RP_MIXED ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
             else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
         else
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( precision == RP_PL_BF16 ) { hipLaunchKernelGGL( (k_nf_pl_spmm<__hip_bfloat16,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const __hip_bfloat16 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            else if( precision == RP_MIXED ) { hipLaunchKernelGGL( (k_nf_pl_spmm<float,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
             else        { hipLaunchKernelGGL( (k_nf_pl_spmm<double,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
     }

@@ -76,9 +76,9 @@ namespace rp
     void launch_fill( double * p, int64_t n, double v, hipStream_t s );
 
     // pseudo-Laplacian (k_pl.hip)
-    void launch_pl_assemble( bool mixed, const NFArgs & A, const FFArgs & B, bool do_nf, bool do_ff, hipStream_t s );
+    void launch_pl_assemble( int precision, const NFArgs & A, const FFArgs & B, bool do_nf, bool do_ff, hipStream_t s );
     void launch_pl_pre( int size, int64_t T, const int32_t * cols, const double * pre_vals, const double * X, int64_t ldX, double * P_in, hipStream_t s );
-    void launch_pl_nf_spmm( bool mixed, int size, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp,
+    void launch_pl_nf_spmm( int precision, int size, int64_t L, const int32_t * lb, const int32_t * le, const int64_t * cp,
                             const int32_t * cpr, const uint32_t * cm, const int64_t * vp, const void * vals, const double * diag,
                             const double * xin, const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s );
     void launch_pl_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s );

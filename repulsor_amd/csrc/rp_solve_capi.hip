@@ -7,6 +7,7 @@
 #include "rp_launch.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace rp
 {
@@ -16,7 +17,7 @@ namespace rp
     static void pl_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy, bool nf_flag, bool ff_flag )
     {
         const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
-        const bool mixed = ctx->pl_precision == RP_MIXED;
+        const bool mixed = ctx->pl_precision != RP_FP64;   // the far field: FP32 unless FP64
         ctx->P_in.require( (size_t)(12 * T) ); ctx->P_out.require( (size_t)(12 * T) );
         ctx->C_in.require( (size_t)(12 * C) ); ctx->C_out.require( (size_t)(12 * C) );
         ctx->leaf_out.require( (size_t)(12 * L) );
@@ -33,7 +34,7 @@ namespace rp
         double * S_out = ctx->P_out.get();   // T x (n+1) x 3
         if( do_nf )
         {
-            launch_pl_nf_spmm( mixed, ctx->size, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
+            launch_pl_nf_spmm( ctx->pl_precision, ctx->size, L, ctx->leaf_begin.get(), ctx->leaf_end.get(), ctx->col_ptr.get(), ctx->col_prim.get(),
                                ctx->col_mask.get(), ctx->val_ptr.get(), ctx->pl_nf_vals.get(), ctx->pl_nf_diag.get(), ctx->P_in.get(),
                                ctx->leaf_out.get(), do_ff ? 1 : 0, ctx->pre_vals.get(), S_out, ctx->stream );
         }
@@ -571,7 +572,7 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision )
     return guarded( "rp_pl_compute", [&]()
     {
         require( ctx != nullptr && ctx->have_data && ctx->have_near && ctx->have_far, "mesh, tree, patterns and data must be uploaded", RP_ENOTREADY );
-        require( precision == RP_FP64 || precision == RP_MIXED, "unknown precision" );
+        require( precision == RP_FP64 || precision == RP_MIXED || precision == RP_PL_BF16, "unknown precision" );
         require( std::isfinite( s ), "s must be finite" );
         RP_HIP( hipSetDevice( ctx->device ) );
         const int n = ctx->dom_dim;
@@ -597,7 +598,7 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision )
         B.C = C; B.rp = ctx->far_rp.get(); B.ci = ctx->far_ci.get();
         B.a = ctx->c_a.get(); B.x = ctx->c_x.get(); B.ex = ex;
         B.vals = ctx->pl_ff_vals.get(); B.diag = ctx->pl_ff_diag.get();
-        launch_pl_assemble( mixed, A, B, have_nf, have_ff, ctx->stream );
+        launch_pl_assemble( precision, A, B, have_nf, have_ff, ctx->stream );
         ctx->have_pl_vf = false;
         double st[3] = { 0, 0, 0 };
         if( ctx->vf_nnz > 0 )
@@ -662,18 +663,31 @@ int rp_download_pl( rp_ctx * ctx, double * nf_vals, double * nf_diag, double * f
     {
         require( ctx != nullptr && ctx->have_pl, "no pseudo-Laplacian computed", RP_ENOTREADY );
         RP_HIP( hipSetDevice( ctx->device ) );
-        const bool mixed = ctx->pl_precision == RP_MIXED;
-        auto get_vals = [&]( const DBuf<double> & buf, int64_t cnt, double * out )
+        // bytes per stored value: FP64 8, FP32 4, BF16 2 (near field only; its far field is FP32)
+        auto get_vals = [&]( const DBuf<double> & buf, int64_t cnt, double * out, int width )
         {
             if( !out || cnt == 0 ) { return; }
-            if( !mixed ) { buf.download( out, (size_t)cnt, ctx->stream ); RP_HIP( hipStreamSynchronize( ctx->stream ) ); return; }
-            std::vector<float> f( (size_t)cnt );
-            RP_HIP( hipMemcpyAsync( f.data(), buf.get(), f.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream ) );
+            if( width == 8 ) { buf.download( out, (size_t)cnt, ctx->stream ); RP_HIP( hipStreamSynchronize( ctx->stream ) ); return; }
+            std::vector<uint8_t> raw( (size_t)cnt * (size_t)width );
+            RP_HIP( hipMemcpyAsync( raw.data(), buf.get(), raw.size(), hipMemcpyDeviceToHost, ctx->stream ) );
             RP_HIP( hipStreamSynchronize( ctx->stream ) );
-            for( size_t k = 0; k < f.size(); ++k ) { out[k] = f[k]; }
+            for( size_t k = 0; k < (size_t)cnt; ++k )
+            {
+                float f = 0.0f;
+                if( width == 4 ) { std::memcpy( &f, raw.data() + 4 * k, 4 ); }
+                else
+                {
+                    uint16_t h = 0;
+                    std::memcpy( &h, raw.data() + 2 * k, 2 );
+                    const uint32_t bits = (uint32_t)h << 16;   // BF16 is the top half of an FP32
+                    std::memcpy( &f, &bits, 4 );
+                }
+                out[k] = f;
+            }
         };
-        get_vals( ctx->pl_nf_vals, ctx->nf_cols > 0 ? ctx->nf_slots : 0, nf_vals );
-        get_vals( ctx->pl_ff_vals, ctx->far_nnz, ff_vals );
+        const int p = ctx->pl_precision;
+        get_vals( ctx->pl_nf_vals, ctx->nf_cols > 0 ? ctx->nf_slots : 0, nf_vals, p == RP_FP64 ? 8 : p == RP_MIXED ? 4 : 2 );
+        get_vals( ctx->pl_ff_vals, ctx->far_nnz, ff_vals, p == RP_FP64 ? 8 : 4 );
         if( nf_diag ) { ctx->pl_nf_diag.download( nf_diag, (size_t)ctx->T, ctx->stream ); }
         if( ff_diag ) { ctx->pl_ff_diag.download( ff_diag, (size_t)ctx->C, ctx->stream ); }
         RP_HIP( hipStreamSynchronize( ctx->stream ) );

@@ -46,6 +46,34 @@ def test_pseudo_laplacian_values(name):
             assert np.max(np.abs(g[k] - o[k])) <= 1e-12 * np.max(np.abs(o[k])), k
 
 
+@pytest.mark.parametrize("name", ["torus_2400", "sphere_1280"])
+def test_pseudo_laplacian_bf16_values(name):
+    """BF16 storage of the near-field pseudo-Laplacian: every stored value is a BF16 number within half a
+    BF16 unit (2^-8 relative) of the oracle's FP64 value (plus the FP32 arithmetic's 1e-6), and the
+    diagonals come from the stored values, so the operator still annihilates constants (L 1 = 0 to
+    rounding) while a random field gives an O(1) product."""
+    wl, M = _mesh(name)
+    print(f"case {name}", flush=True)
+    A = R.TangentPointMetric0(wl.q, wl.p)
+    A.SetPreconditionerPrecision("bf16")
+    g = A.PseudoLaplacianValues(M)
+    tr, td, npat, fpat = oracle_state(M)
+    o = O.pl_compute(M.dom_dim, td["P_near"], td["C_far"], npat, fpat, _s_pl(wl, M))
+    nf, onf = g["NF"], o["NF"]
+    bits = nf.astype(np.float32).view(np.uint32)
+    assert np.all(bits & 0xFFFF == 0), "near values are not BF16 numbers"
+    assert np.all(np.abs(nf - onf) <= (2.0 ** -8 + 1e-6) * np.abs(onf) + 1e-300)
+    print(f"max relative deviation {np.max(np.abs(nf - onf) / np.maximum(np.abs(onf), 1e-300)):.2e}", flush=True)
+    ones = np.ones((M.V, 3))
+    Y1 = np.zeros_like(ones)
+    A.MultiplyPseudoLaplacian(M, 1.0, ones, 0.0, Y1)
+    Xr = random_field(M.V, seed=31)
+    Yr = np.zeros_like(Xr)
+    A.MultiplyPseudoLaplacian(M, 1.0, Xr, 0.0, Yr)
+    print(f"|L 1| / |L x| = {np.linalg.norm(Y1) / np.linalg.norm(Yr):.2e}", flush=True)
+    assert np.linalg.norm(Y1) <= 1e-12 * np.linalg.norm(Yr)
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_pseudo_laplacian_multiply(name):
     wl, M = _mesh(name)
@@ -246,12 +274,14 @@ def test_solve_after_semistatic_update():
 
 
 @pytest.mark.parametrize("name", ["torus_2400", "sphere_1280"])
-def test_solve_fp32_preconditioner(name):
-    """FP32 pseudo-Laplacian values inside P = P0 + w diag(A)^-1, FP64 metric: the preconditioner
-    product stays within FP32 accuracy of the oracle's, and the solve reaches the oracle-measured FP64
-    residual with the iteration count of the FP64 preconditioner (at most one more)."""
+@pytest.mark.parametrize("pc_prec,pc_tol", [("mixed", 1e-5), ("bf16", 5e-3)])
+def test_solve_fp32_preconditioner(name, pc_prec, pc_tol):
+    """FP32 (or BF16-stored) pseudo-Laplacian values inside P = P0 + w diag(A)^-1, FP64 metric: the
+    preconditioner product stays within the storage's accuracy of the oracle's (FP32: 1e-5; BF16, 8
+    significant bits: 5e-3), and the solve reaches the oracle-measured FP64 residual with the iteration
+    count of the FP64 preconditioner (at most one more)."""
     wl, M = _mesh(name)
-    print(f"case {name}", flush=True)
+    print(f"case {name}, pseudo-Laplacian {pc_prec}", flush=True)
     E = R.TangentPointEnergy0(wl.q, wl.p)
     B = E.Differential(M)
     A64 = R.TangentPointMetric0(wl.q, wl.p)
@@ -261,7 +291,7 @@ def test_solve_fp32_preconditioner(name):
     it64 = A64.Solver_IterationCount()
     A = R.TangentPointMetric0(wl.q, wl.p)
     A.SetPreconditionerDiagonalWeight(-1.0)
-    A.SetPreconditionerPrecision("mixed")
+    A.SetPreconditionerPrecision(pc_prec)
     X = np.zeros_like(B)
     A.Solve(M, 1.0, B, 0.0, X, max_iter=2000, tolerance=1e-6)
     it = A.Solver_IterationCount()
@@ -274,9 +304,9 @@ def test_solve_fp32_preconditioner(name):
     err = rel(Y, S.P(Xr))
     _, Bo, _, _ = oracle_energy_diff(M, wl.q, wl.p)
     true = np.linalg.norm(Bo - S.A(X), axis=0) / np.linalg.norm(Bo, axis=0)
-    print(f"FP32 pseudo-Laplacian: preconditioner rel err {err:.2e}, iterations {it} (FP64 {it64}), "
+    print(f"{pc_prec} pseudo-Laplacian: preconditioner rel err {err:.2e}, iterations {it} (FP64 {it64}), "
           f"residual {true.max():.2e}", flush=True)
-    assert err <= 1e-5
+    assert err <= pc_tol
     assert np.all(true <= 1.5e-6), true
     assert it <= it64 + 1
 
