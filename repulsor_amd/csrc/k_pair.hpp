@@ -23,7 +23,6 @@ namespace rp
     // contiguous run of leaves, whose column lists overlap, so the gathered column vectors are reused
     // in that XCD's L2. The grid is padded to a multiple of 8 (launch with leaf_grid(L)); returns -1 for
     // the padding workgroups.
-    constexpr int NUM_XCD = 8;
     __device__ __forceinline__ int64_t xcd_leaf( int64_t L )
     {
         const int64_t per = gridDim.x / NUM_XCD;

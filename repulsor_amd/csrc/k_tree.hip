@@ -312,8 +312,13 @@ namespace rp
     __global__ __launch_bounds__(256) void k_ff_spmm( int64_t C, const int64_t * rp, const int32_t * ci,
         const VT * vals, const double * diag, const double * xin, double * yout )
     {
+        // workgroups dealt to the XCDs round robin; each XCD takes one contiguous run of cluster rows
+        // (subtrees of the cluster tree, whose far partners overlap), so the gathered input blocks are
+        // reused in that XCD's L2
+        const int64_t per = gridDim.x / NUM_XCD;
+        const int64_t blk = (int64_t)( blockIdx.x % NUM_XCD ) * per + blockIdx.x / NUM_XCD;
         const int lane = threadIdx.x % FF_G;
-        const int64_t i = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / FF_G;
+        const int64_t i = ( blk * 256 + threadIdx.x ) / FF_G;
         if( i >= C ) { return; }
         double acc[12];
         #pragma unroll
@@ -474,7 +479,8 @@ namespace rp
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {
         if( !C ) { return; }
-        const dim3 grid( (unsigned)((C + 256 / FF_G - 1) / (256 / FF_G)) );
+        const int64_t blocks = (C + 256 / FF_G - 1) / (256 / FF_G);
+        const dim3 grid( (unsigned)( ( ( blocks + NUM_XCD - 1 ) / NUM_XCD ) * NUM_XCD ) );
         if( mixed ) { hipLaunchKernelGGL( (k_ff_spmm<float2>), grid, dim3(256), 0, s, C, rp, ci, (const float2 *)vals, diag, xin, yout ); }
         else        { hipLaunchKernelGGL( (k_ff_spmm<double2>), grid, dim3(256), 0, s, C, rp, ci, (const double2 *)vals, diag, xin, yout ); }
     }

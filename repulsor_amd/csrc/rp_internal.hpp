@@ -115,6 +115,8 @@ namespace rp
 
     // One supernode's block of the H^1 factor, or one task on it: first column f, width w, rows r, first
     // front row (or column) k of the task, offsets of its row set and of its values.
+    constexpr int NUM_XCD = 8;   // accelerator complex dies of an MI355X, each with its own L2
+
     struct CholTask
     {
         int32_t f, w, r, k;
