@@ -15,23 +15,26 @@ namespace rp
     __device__ __forceinline__ float softplus_f( float x ) { return x > 20.f ? x : fmaxf( log1pf( expf( x ) ), 1e-30f ); }
     __device__ __forceinline__ float sigmoid_f( float x ) { return 1.f / ( 1.f + expf( -x ) ); }
 
-    // lam[c] = log softplus( w2 . tanh( W1 f_c + b1 ) + b2 ), weights in BF16.
-    __global__ __launch_bounds__(256) void k_mlp_infer_bf16( int64_t C, int F, const float * feats, const __hip_bfloat16 * p, double * lam )
+    // lam[c] = log softplus( w2 . tanh( W1 f_c + b1 ) + b2 ): the hidden layer (W1, b1 and the features)
+    // in BF16, the output layer (w2, b2) in FP32. b2 carries the scalings' common offset (softplus^-1(1)
+    // for the identity); rounded to BF16 it would shift every node's log s by about 1e-3, which sums to
+    // a depth-dependent scaling of a few percent along the root paths.
+    __global__ __launch_bounds__(256) void k_mlp_infer_bf16( int64_t C, int F, const float * feats, const __hip_bfloat16 * p, const float * p32, double * lam )
     {
         const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if( c >= C ) { return; }
         const __hip_bfloat16 * W1 = p;                     // MLP_H x F
         const __hip_bfloat16 * b1 = W1 + MLP_H * F;
-        const __hip_bfloat16 * w2 = b1 + MLP_H;
-        const __hip_bfloat16 * b2 = w2 + MLP_H;
+        const float * w2 = p32 + MLP_H * F + MLP_H;
+        const float * b2 = w2 + MLP_H;
         float f[MLP_FMAX];
         for( int k = 0; k < F; ++k ) { f[k] = __bfloat162float( __float2bfloat16( feats[(int64_t)F * c + k] ) ); }
-        float o = __bfloat162float( b2[0] );
+        float o = b2[0];
         for( int j = 0; j < MLP_H; ++j )
         {
             float z = __bfloat162float( b1[j] );
             for( int k = 0; k < F; ++k ) { z += __bfloat162float( W1[j * F + k] ) * f[k]; }
-            o += __bfloat162float( w2[j] ) * tanhf( z );
+            o += w2[j] * tanhf( z );
         }
         lam[c] = (double)logf( softplus_f( o ) );
     }
@@ -230,7 +233,7 @@ namespace rp
         {
             const int NP = MLP_H * F + 2 * MLP_H + 1;
             hipLaunchKernelGGL( k_to_bf16, g256( NP ), dim3(256), 0, s, NP, p32, (__hip_bfloat16 *)p16 );
-            hipLaunchKernelGGL( k_mlp_infer_bf16, g256( C ), dim3(256), 0, s, C, F, feats, (const __hip_bfloat16 *)p16, lam );
+            hipLaunchKernelGGL( k_mlp_infer_bf16, g256( C ), dim3(256), 0, s, C, F, feats, (const __hip_bfloat16 *)p16, p32, lam );
         }
         else { hipLaunchKernelGGL( k_mlp_infer_f32, g256( C ), dim3(256), 0, s, C, F, feats, p32, lam ); }
     }

@@ -8,7 +8,9 @@ is computed, PCG with the current model solves A X = B to a tight tolerance whil
 and one Adam step is taken on the pairs (r_k, e_k = x_last - x_k) with the loss
 min_alpha mean_k |alpha P r_k - e_k|_A^2 / |e_k|_A^2 (one minus the mean squared A-cosine of P r_k and
 e_k after the best common scale; PCG does not see the scale of P), gradient by the library's HIP
-kernels (rp_pc_learn_step). The model is written to repulsor_amd/models/learned_pc.rplp.
+kernels (rp_pc_learn_step). The trained model is written to repulsor_amd/models/learned_pc_trained.rplp
+and the packaged one, its output layer shrunk toward the identity by learned_pc.PACKAGED_SHRINK, to
+repulsor_amd/models/learned_pc.rplp.
 
 Usage: python -m repulsor_amd.train_learned_pc [--meshes 192] [--lr 3e-3] [--start K] [--count N]
        [--checkpoint state.npz]
@@ -63,7 +65,8 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--record-tol", type=float, default=1e-9)
-    ap.add_argument("--out", default=LP.DEFAULT_MODEL)
+    ap.add_argument("--out", default=LP.TRAINED_MODEL)
+    ap.add_argument("--packaged", default=LP.DEFAULT_MODEL)
     ap.add_argument("--start", type=int, default=0)
     ap.add_argument("--count", type=int, default=-1)
     ap.add_argument("--checkpoint", default="")
@@ -99,7 +102,8 @@ def main():
         if args.checkpoint:
             np.savez(args.checkpoint, params=params, m=m, v=v, step=k + 1)
     LP.save_model(args.out, params, F)
-    print(f"wrote {args.out}", flush=True)
+    LP.save_model(args.packaged, LP.shrink_output_layer(params, F, LP.PACKAGED_SHRINK), F)
+    print(f"wrote {args.out} and {args.packaged}", flush=True)
 
 
 if __name__ == "__main__":
