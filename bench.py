@@ -507,7 +507,7 @@ def main():
     # the roofline line follows the SpMM the solve launches most: with mixed-refined the CG iterations
     # multiply with the FP32 copies (one FP64 product per solve replaces the residual)
     if refined:
-        rl_kernel = ("k_nf_spmm<float2> (near-field SpMM over the FP32 copies of the metric values, FP64 "
+        rl_kernel = ("k_nf_spmm_rows (near-field SpMM over the FP32 copies of the metric values, FP64 "
                      "accumulation, fused post-operator epilogue; one per CG iteration)")
         rl_bytes = spmm_algorithmic_bytes(b, "mixed")
         rl_s = float(np.mean(spmm32_ms)) * 1e-3

@@ -269,6 +269,9 @@ namespace rp
     // y_i = sum_j diag(a0, a1, a1, a1) x_j + diag_i x_i, x and y 4 x 3 per primitive (AoS, 12 doubles).
     // Epilogue (post-operator, ClusterTree::Post): ytot_i = y_i + far-field leaf total of leaf(i), then
     // S_out[i][s][c] = sum_r pre_vals[i][r][s] ytot_i[r][c] for the (n+1) vertex slots s.
+    // This form (two rows per lane) serves the FP64 values (VT = double2); the FP32 copies run
+    // k_nf_spmm_rows below, where one row per lane measured faster (1.47 -> 1.40 ms at 1.31M
+    // triangles) while for the FP64 values it made no difference.
     template<typename VT, int SIZE>
     __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_nf_spmm( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
         const int64_t * col_ptr, const int32_t * col_prim, const uint32_t * col_mask, const int64_t * val_ptr,
@@ -287,8 +290,7 @@ namespace rp
         const int ib = leaf_begin[I];
         const int nI = leaf_end[I] - ib;
         // a lane takes two rows of the leaf, p and p + h, for one column of the chunk at a time: every
-        // column block read from LDS serves two pairs (the LDS reads, not the HBM stream, bound the
-        // one-row form)
+        // column block read from LDS serves two pairs
         const int h = ( nI + 1 ) >> 1;
         const int nS = 64 / h;
         const int p = lane % h;
@@ -433,6 +435,145 @@ namespace rp
         }
     }
 
+    // The same product over the FP32 copies of the values (the CG iterations of RP_MIXED_REFINED), one
+    // row per lane: half the accumulators of the two-row form, so that 5 instead of 4 wavefronts share
+    // a SIMD (92 registers), and 8 columns of values (64 bytes) in flight per lane. Each lane sums its
+    // row's products in column order; the row sums over the column groups follow group order.
+    template<int SIZE>
+    __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_nf_spmm_rows( int64_t L, const int32_t * leaf_begin, const int32_t * leaf_end,
+        const int64_t * col_ptr, const int32_t * col_prim, const int64_t * val_ptr,
+        const float2 * vals, const double * diag, const double * xin,
+        const double * leaf_out, int use_far, const double * pre_vals, double * S_out )
+    {
+        constexpr int CS = 32;
+        constexpr int U = 8;
+        __shared__ double2 sx[2][CS][6];
+        const int lane = threadIdx.x;
+        const int64_t I = xcd_leaf( L );
+        if( I < 0 ) { return; }
+        const int ib = leaf_begin[I];
+        const int nI = leaf_end[I] - ib;
+        const int nS = 64 / nI;
+        const int p = lane % nI;
+        const int c = lane / nI;
+        const bool active = lane < nS * nI;
+        const int64_t cb = col_ptr[I];
+        const int64_t M_ = col_ptr[I + 1] - cb;
+        const int64_t vb = val_ptr[I];
+        double acc[12];
+        #pragma unroll
+        for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
+        // input-block staging as in k_nf_spmm: lane l gathers half of the block of column l mod 32, the
+        // column indices a chunk ahead
+        double rx[6];
+        const int half = lane >> 5, lc = lane & ( CS - 1 );
+        auto iload = [&]( int64_t base ) -> int32_t
+        {
+            const int64_t m = base + lc;
+            return ( m < M_ ) ? col_prim[cb + m] : -1;
+        };
+        auto gload = [&]( int32_t rj )
+        {
+            if( rj >= 0 )
+            {
+                const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)rj + 6 * half );
+                #pragma unroll
+                for( int q = 0; q < 3; ++q ) { const double2 t = xj[q]; rx[2 * q] = t.x; rx[2 * q + 1] = t.y; }
+            }
+        };
+        auto lstore = [&]( int buf )
+        {
+            #pragma unroll
+            for( int q = 0; q < 3; ++q ) { sx[buf][lc][3 * half + q] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
+        };
+        int32_t rj_next = iload( CS );
+        gload( iload( 0 ) );
+        lstore( 0 );
+        __syncthreads();
+        int buf = 0;
+        for( int64_t base = 0; base < M_; base += CS )
+        {
+            const bool more = base + CS < M_;
+            if( more )
+            {
+                gload( rj_next );
+                rj_next = iload( base + 2 * CS );
+            }
+            const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
+            if( active )
+            {
+                const float2 * vrow = vals + vb + base * nI + p;
+                for( int mm = c; mm < cnt_c; mm += U * nS )
+                {
+                    float2 a[U];
+                    #pragma unroll
+                    for( int u = 0; u < U; ++u )
+                    {
+                        const int m2 = mm + u * nS;
+                        a[u] = float2{};
+                        if( m2 < cnt_c ) { a[u] = vrow[(int64_t)m2 * nI]; }
+                    }
+                    #pragma unroll
+                    for( int u = 0; u < U; ++u )
+                    {
+                        const int m2 = mm + u * nS;
+                        if( m2 >= cnt_c ) { break; }
+                        const double b0 = (double)a[u].x, b1 = (double)a[u].y;
+                        #pragma unroll
+                        for( int q = 0; q < 6; ++q )
+                        {
+                            const double2 x2 = sx[buf][m2][q];
+                            const int k = 2 * q;
+                            acc[k]     += ( k < 3 ? b0 : b1 ) * x2.x;
+                            acc[k + 1] += ( k + 1 < 3 ? b0 : b1 ) * x2.y;
+                        }
+                    }
+                }
+            }
+            if( more ) { lstore( buf ^ 1 ); }
+            __syncthreads();
+            buf ^= 1;
+        }
+        // row sums over the column groups in group order (every lane takes part); lane q < nI holds row q
+        double s[12];
+        #pragma unroll
+        for( int k = 0; k < 12; ++k )
+        {
+            double s0 = 0.0;
+            for( int cc = 0; cc < nS; ++cc ) { s0 += __shfl( acc[k], p + cc * nI, 64 ); }
+            s[k] = s0;
+        }
+        if( lane < nI )
+        {
+            const int64_t i = ib + lane;
+            const double d0 = diag[2 * i], d1 = diag[2 * i + 1];
+            const double * xi = xin + 12 * i;
+            #pragma unroll
+            for( int k = 0; k < 3; ++k )  { s[k] += d0 * xi[k]; }
+            #pragma unroll
+            for( int k = 3; k < 12; ++k ) { s[k] += d1 * xi[k]; }
+            if( use_far )
+            {
+                const double * lo = leaf_out + 12 * I;
+                #pragma unroll
+                for( int k = 0; k < 12; ++k ) { s[k] = s[k] + lo[k]; }
+            }
+            const double * pv = pre_vals + 4 * SIZE * i;
+            #pragma unroll
+            for( int sl = 0; sl < SIZE; ++sl )
+            {
+                #pragma unroll
+                for( int cc = 0; cc < 3; ++cc )
+                {
+                    double t = 0.0;
+                    #pragma unroll
+                    for( int r = 0; r < 4; ++r ) { t += pv[SIZE * r + sl] * s[3 * r + cc]; }
+                    S_out[(SIZE * i + sl) * 3 + cc] = t;
+                }
+            }
+        }
+    }
+
     // ------------------------------------------------------------------ host launchers
     template<bool E, bool D, bool M>
     static void launch_nf_flags( int kind, bool mixed, const NFArgs & A, hipStream_t s )
@@ -474,12 +615,12 @@ namespace rp
         const dim3 grid = leaf_grid( L ), block( 64 );
         if( size == 3 )
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm_rows<3>), grid, block, 0, s, L, lb, le, cp, cpr, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
             else        { hipLaunchKernelGGL( (k_nf_spmm<double2,3>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
         else
         {
-            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm<float2,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
+            if( mixed ) { hipLaunchKernelGGL( (k_nf_spmm_rows<2>), grid, block, 0, s, L, lb, le, cp, cpr, vp, (const float2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
             else        { hipLaunchKernelGGL( (k_nf_spmm<double2,2>), grid, block, 0, s, L, lb, le, cp, cpr, cm, vp, (const double2 *)vals, diag, xin, leaf_out, use_far, pre_vals, S_out ); }
         }
     }
