@@ -45,7 +45,7 @@ PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per k_nf_spmm launch measured with rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by
 # the counter's reading for a copy kernel of known size (profiles/k_nf_spmm_sphere_1310720_traffic.md).
-MEASURED_SPMM_TRAFFIC = {("sphere_1310720", "fp64"): 11.138e9, ("sphere_1310720", "fp32-copies"): 5.879e9}
+MEASURED_SPMM_TRAFFIC = {("sphere_1310720", "fp64"): 11.137e9, ("sphere_1310720", "fp32-copies"): 5.918e9}
 
 
 def spmm_algorithmic_bytes(b, precision):
