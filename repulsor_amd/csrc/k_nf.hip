@@ -281,9 +281,10 @@ namespace rp
         // 32 columns per chunk and the row sums by lane shuffles: 6.5 KB of LDS per wavefront, so that
         // 24 leaves share a CU
         constexpr int CS = 32;
-        // input blocks column by column (96 bytes each, 16-byte aligned): a lane reads a column's block
-        // with six 16-byte loads; the distinct columns of a wavefront start on distinct bank groups
-        __shared__ double2 sx[2][CS][6];
+        // input blocks piece by piece, sx[buffer][16-byte piece][column]: a lane reads a column's block
+        // with six 16-byte loads, the columns a read instruction addresses are consecutive 16-byte
+        // slots, and each gather lane writes into a contiguous run (no LDS bank conflicts)
+        __shared__ double2 sx[2][6][CS];
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( L );
         if( I < 0 ) { return; }
@@ -326,7 +327,7 @@ namespace rp
         auto lstore = [&]( int buf )
         {
             #pragma unroll
-            for( int q = 0; q < 3; ++q ) { sx[buf][lc][3 * half + q] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
+            for( int q = 0; q < 3; ++q ) { sx[buf][3 * half + q][lc] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
         };
         int32_t rj_next = iload( CS );
         gload( iload( 0 ) );
@@ -374,7 +375,7 @@ namespace rp
                         #pragma unroll
                         for( int q = 0; q < 6; ++q )
                         {
-                            const double2 x2 = sx[buf][m2][q];
+                            const double2 x2 = sx[buf][q][m2];
                             const int k = 2 * q;
                             acc0[k]     += ( k < 3 ? b0 : b1 ) * x2.x;
                             acc1[k]     += ( k < 3 ? e0 : e1 ) * x2.x;
@@ -447,7 +448,9 @@ namespace rp
     {
         constexpr int CS = 32;
         constexpr int U = 8;
-        __shared__ double2 sx[2][CS][6];
+        // input blocks piece by piece, sx[buffer][16-byte piece][column]: the columns a read instruction
+        // addresses are consecutive 16-byte slots and each gather lane writes into a contiguous run
+        __shared__ double2 sx[2][6][CS];
         const int lane = threadIdx.x;
         const int64_t I = xcd_leaf( L );
         if( I < 0 ) { return; }
@@ -484,7 +487,7 @@ namespace rp
         auto lstore = [&]( int buf )
         {
             #pragma unroll
-            for( int q = 0; q < 3; ++q ) { sx[buf][lc][3 * half + q] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
+            for( int q = 0; q < 3; ++q ) { sx[buf][3 * half + q][lc] = make_double2( rx[2 * q], rx[2 * q + 1] ); }
         };
         int32_t rj_next = iload( CS );
         gload( iload( 0 ) );
@@ -502,16 +505,20 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < CS) ? (M_ - base) : CS );
             if( active )
             {
-                const float2 * vrow = vals + vb + base * nI + p;
+                // the chunk's values from one wave-uniform base with 32-bit lane offsets (a tile holds at
+                // most 32 rows x its column count of float2 values, far below 2^31 bytes)
+                const char * vchunk = reinterpret_cast<const char *>( vals + vb + base * nI );
+                const uint32_t ostep = (uint32_t)( nS * nI ) * (uint32_t)sizeof(float2);
                 for( int mm = c; mm < cnt_c; mm += U * nS )
                 {
                     float2 a[U];
+                    uint32_t ob = (uint32_t)( mm * nI + p ) * (uint32_t)sizeof(float2);
                     #pragma unroll
                     for( int u = 0; u < U; ++u )
                     {
-                        const int m2 = mm + u * nS;
-                        a[u] = float2{};
-                        if( m2 < cnt_c ) { a[u] = vrow[(int64_t)m2 * nI]; }
+                        // a slot past the chunk's last column is never multiplied, so it needs no value
+                        if( mm + u * nS < cnt_c ) { a[u] = *reinterpret_cast<const float2 *>( vchunk + ob ); }
+                        ob += ostep;
                     }
                     #pragma unroll
                     for( int u = 0; u < U; ++u )
@@ -522,7 +529,7 @@ namespace rp
                         #pragma unroll
                         for( int q = 0; q < 6; ++q )
                         {
-                            const double2 x2 = sx[buf][m2][q];
+                            const double2 x2 = sx[buf][q][m2];
                             const int k = 2 * q;
                             acc[k]     += ( k < 3 ? b0 : b1 ) * x2.x;
                             acc[k + 1] += ( k + 1 < 3 ? b0 : b1 ) * x2.y;
