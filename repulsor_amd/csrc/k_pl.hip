@@ -215,27 +215,32 @@ namespace rp
             const int cnt_c = (int)( (M_ - base < PL_CH) ? (M_ - base) : PL_CH );
             if( active )
             {
-                // eight value loads in flight before the products; every tile slot exists and the assembly
-                // stores 0 in the slots of masked pairs, so the products need no mask; the products are
-                // summed in column order as one at a time
-                const VT * vrow = vals + vb + base * nI + ii;
-                int mm = c;
-                for( ; mm + 7 * nS < cnt_c; mm += 8 * nS )
+                // all of the lane's values of the chunk (UV = 16 slots for 2- and 4-byte values: one batch
+                // when nS >= 4, i.e. leaves of at most 16 primitives) are loaded before the products, so
+                // the chunk waits for one round trip; every tile slot exists and the assembly stores 0 in
+                // the slots of masked pairs, so the products need no mask; the products are summed in
+                // column order, one at a time. Value offsets are 32-bit from the chunk's wave-uniform base.
+                constexpr int UV = sizeof(VT) == 8 ? 8 : 16;
+                const char * vchunk = reinterpret_cast<const char *>( vals + vb + base * nI );
+                const uint32_t ostep = (uint32_t)( nS * nI ) * (uint32_t)sizeof(VT);
+                for( int mm = c; mm < cnt_c; mm += UV * nS )
                 {
-                    double v[8];
+                    VT v[UV];
+                    uint32_t ob = (uint32_t)( mm * nI + ii ) * (uint32_t)sizeof(VT);
                     #pragma unroll
-                    for( int u = 0; u < 8; ++u ) { v[u] = pl_val( vrow[(int64_t)( mm + u * nS ) * nI] ); }
+                    for( int u = 0; u < UV; ++u )
+                    {
+                        if( mm + u * nS < cnt_c ) { v[u] = *reinterpret_cast<const VT *>( vchunk + ob ); }
+                        ob += ostep;
+                    }
                     #pragma unroll
-                    for( int u = 0; u < 8; ++u )
+                    for( int u = 0; u < UV; ++u )
                     {
                         const int m2 = mm + u * nS;
-                        for( int k = 0; k < 3; ++k ) { acc[k] += v[u] * sx[buf][k][m2]; }
+                        if( m2 >= cnt_c ) { break; }
+                        const double vv = pl_val( v[u] );
+                        for( int k = 0; k < 3; ++k ) { acc[k] += vv * sx[buf][k][m2]; }
                     }
-                }
-                for( ; mm < cnt_c; mm += nS )
-                {
-                    const double v = pl_val( vrow[(int64_t)mm * nI] );
-                    for( int k = 0; k < 3; ++k ) { acc[k] += v * sx[buf][k][mm]; }
                 }
             }
             if( more ) { lstore( buf ^ 1 ); }
