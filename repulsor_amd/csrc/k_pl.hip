@@ -289,11 +289,22 @@ namespace rp
         const int64_t i = ( (int64_t)blockIdx.x * 256 + threadIdx.x ) / G;
         if( i >= C ) { return; }
         double acc[3] = { 0, 0, 0 };
-        for( int64_t k = rp[i] + lane; k < rp[i + 1]; k += G )
+        // entries k, k + G, ... in that order, two per step with both loaded before either product
+        const int64_t e = rp[i + 1];
+        for( int64_t k = rp[i] + lane; k < e; k += 2 * G )
         {
-            const double v = (double)vals[k];
-            const double * xj = xin + 3 * (int64_t)ci[k];
-            for( int h = 0; h < 3; ++h ) { acc[h] += v * xj[h]; }
+            const bool two = k + G < e;
+            const int32_t j0 = ci[k];
+            const double v0 = (double)vals[k];
+            int32_t j1 = j0;
+            double v1 = 0.0;
+            if( two ) { j1 = ci[k + G]; v1 = (double)vals[k + G]; }
+            const double * x0 = xin + 3 * (int64_t)j0;
+            const double * x1 = xin + 3 * (int64_t)j1;
+            double y0[3], y1[3];
+            for( int h = 0; h < 3; ++h ) { y0[h] = x0[h]; y1[h] = x1[h]; }
+            for( int h = 0; h < 3; ++h ) { acc[h] += v0 * y0[h]; }
+            if( two ) { for( int h = 0; h < 3; ++h ) { acc[h] += v1 * y1[h]; } }
         }
         for( int h = 0; h < 3; ++h )
         {

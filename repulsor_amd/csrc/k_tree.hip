@@ -323,20 +323,35 @@ namespace rp
         double acc[12];
         #pragma unroll
         for( int k = 0; k < 12; ++k ) { acc[k] = 0.0; }
+        // a lane takes the entries k, k + FF_G, ... of the row in that order, two per step with both
+        // entries' indices, values and input blocks loaded before either product (rows hold about two
+        // dozen entries, so most lanes need one step: one index -> block round trip instead of two)
         const int64_t e = rp[i + 1];
-        for( int64_t k = rp[i] + lane; k < e; k += FF_G )
+        auto fma_block = [&]( const VT av, const double * x )
         {
-            const int32_t j = ci[k];
-            const VT av = vals[k];
             const double a0 = (double)av.x, a1 = (double)av.y;
-            const double2 * xj = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)j );
-            double x[12];
-            #pragma unroll
-            for( int h = 0; h < 6; ++h ) { const double2 t = xj[h]; x[2 * h] = t.x; x[2 * h + 1] = t.y; }
             #pragma unroll
             for( int h = 0; h < 3; ++h ) { acc[h] += a0 * x[h]; }
             #pragma unroll
             for( int h = 3; h < 12; ++h ) { acc[h] += a1 * x[h]; }
+        };
+        for( int64_t k = rp[i] + lane; k < e; k += 2 * FF_G )
+        {
+            const bool two = k + FF_G < e;
+            const int32_t j0 = ci[k];
+            const VT av0 = vals[k];
+            int32_t j1 = j0;
+            VT av1 = av0;
+            if( two ) { j1 = ci[k + FF_G]; av1 = vals[k + FF_G]; }
+            const double2 * x0p = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)j0 );
+            const double2 * x1p = reinterpret_cast<const double2 *>( xin + 12 * (int64_t)j1 );
+            double x0[12], x1[12];
+            #pragma unroll
+            for( int h = 0; h < 6; ++h ) { const double2 t = x0p[h]; x0[2 * h] = t.x; x0[2 * h + 1] = t.y; }
+            #pragma unroll
+            for( int h = 0; h < 6; ++h ) { const double2 t = x1p[h]; x1[2 * h] = t.x; x1[2 * h + 1] = t.y; }
+            fma_block( av0, x0 );
+            if( two ) { fma_block( av1, x1 ); }
         }
         #pragma unroll
         for( int k = 0; k < 12; ++k )
