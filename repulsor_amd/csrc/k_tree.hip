@@ -61,22 +61,24 @@ namespace rp
     // Root-to-leaf sum of the far-field moment derivatives, in the order of the reference's
     // PercolateDown (child = parent + child, parents first). Output 10 x L.
     template<int DIM>
-    __global__ void k_leaf_path_sum( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent,
+    __global__ void k_leaf_path_sum( int64_t L, int64_t C, const int32_t * anc, const int32_t * anc_len,
                                      const double * src, int64_t src_stride_soa, double * dst, int dst_soa )
     {
         const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         if( l >= L ) { return; }
-        int32_t path[96];
-        int depth = 0;
-        int32_t c = leaf_cluster[l];
-        while( c >= 0 && depth < 96 ) { path[depth++] = c; c = parent[c]; }
-        double acc[DIM];
+        // the leaf's path from the ancestor table built at tree upload, anc[d * L + l] = the cluster at
+        // depth d (d = 0: the root): every index load is independent of the sums, so they go out ahead
+        const int n = anc_len[l];
         // SoA source: src[k * C + c]; AoS source: src[DIM * c + k]
-        for( int k = 0; k < DIM; ++k ) { acc[k] = src_stride_soa ? src[k * C + path[depth - 1]] : src[(int64_t)DIM * path[depth - 1] + k]; }
-        for( int d = depth - 2; d >= 0; --d )
+        auto val = [&]( int32_t c, int k ) -> double { return src_stride_soa ? src[k * C + c] : src[(int64_t)DIM * c + k]; };
+        double acc[DIM];
+        const int32_t r = anc[l];
+        for( int k = 0; k < DIM; ++k ) { acc[k] = val( r, k ); }
+        #pragma unroll 1
+        for( int d = 1; d < n; ++d )
         {
-            const int32_t cc = path[d];
-            for( int k = 0; k < DIM; ++k ) { acc[k] = acc[k] + ( src_stride_soa ? src[k * C + cc] : src[(int64_t)DIM * cc + k] ); }
+            const int32_t cc = anc[(int64_t)d * L + l];
+            for( int k = 0; k < DIM; ++k ) { acc[k] = acc[k] + val( cc, k ); }
         }
         for( int k = 0; k < DIM; ++k )
         {
@@ -239,8 +241,17 @@ namespace rp
         const int h = (int)(g % W);
         if( l >= L ) { return; }
         const int32_t c = leaf_cluster[l];
+        // the leaf's values in batches of 8 loads (leaves hold at most 32 primitives), summed in order
+        const int32_t b = C_begin[c], e = C_end[c];
         double s = 0;
-        for( int32_t i = C_begin[c]; i < C_end[c]; ++i ) { s += P_in[W * (int64_t)i + h]; }
+        for( int32_t i0 = b; i0 < e; i0 += 8 )
+        {
+            double v[8];
+            #pragma unroll
+            for( int u = 0; u < 8; ++u ) { v[u] = ( i0 + u < e ) ? P_in[W * (int64_t)( i0 + u ) + h] : 0.0; }
+            #pragma unroll
+            for( int u = 0; u < 8; ++u ) { if( i0 + u < e ) { s += v[u]; } }
+        }
         C_in[W * (int64_t)c + h] = s;
     }
 
@@ -439,11 +450,11 @@ namespace rp
     void launch_leaf_origin( int64_t L, int64_t C, const int32_t * leaf_cluster, const double * cx, double * origin, hipStream_t s )
     { if( L ) { hipLaunchKernelGGL( k_leaf_origin, grid1(L), dim3(256), 0, s, L, C, leaf_cluster, cx, origin ); } }
 
-    void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * ff_dx, double * leaf_far, hipStream_t s )
-    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<10>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, ff_dx, (int64_t)1, leaf_far, 1 ); } }
+    void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * ff_dx, double * leaf_far, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<10>), grid1(L), dim3(256), 0, s, L, C, leaf_anc, leaf_anc_len, ff_dx, (int64_t)1, leaf_far, 1 ); } }
 
-    void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s )
-    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<12>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
+    void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * C_out, double * leaf_out, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<12>), grid1(L), dim3(256), 0, s, L, C, leaf_anc, leaf_anc_len, C_out, (int64_t)0, leaf_out, 0 ); } }
 
     void launch_hulls( int n, int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim, const int32_t * S,
                        const double * V, const double * charges, const double * nf_dx, const double * vf_dh, const double * leaf_far, double * B, hipStream_t s )
@@ -485,11 +496,11 @@ namespace rp
         }
     }
 
-    void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s )
-    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<3>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, C_out, (int64_t)0, leaf_out, 0 ); } }
+    void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * C_out, double * leaf_out, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<3>), grid1(L), dim3(256), 0, s, L, C, leaf_anc, leaf_anc_len, C_out, (int64_t)0, leaf_out, 0 ); } }
 
-    void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * src, double * leaf_out, hipStream_t s )
-    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<1>), grid1(L), dim3(256), 0, s, L, C, leaf_cluster, parent, src, (int64_t)0, leaf_out, 0 ); } }
+    void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * src, double * leaf_out, hipStream_t s )
+    { if( L ) { hipLaunchKernelGGL( (k_leaf_path_sum<1>), grid1(L), dim3(256), 0, s, L, C, leaf_anc, leaf_anc_len, src, (int64_t)0, leaf_out, 0 ); } }
 
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s )
     {

@@ -72,7 +72,7 @@ namespace rp
             // Far field first, so that the near-field SpMM epilogue can add the percolated cluster
             // output of its leaf and apply the post-operator (S_out = per-vertex-slot contributions).
             launch_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->ff_vals.get(), ctx->ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
-            launch_leaf_out( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
+            launch_leaf_out( L, C, ctx->leaf_anc.get(), ctx->leaf_anc_len.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
         }
         double * S_out = ctx->P_out.get();   // T x (n+1) x 3 <= 12 T
         if( time_nf ) { RP_HIP( hipEventRecord( ctx->ev[6][0], ctx->stream ) ); }
@@ -100,7 +100,7 @@ namespace rp
     static void collect_far( rp_ctx * ctx )
     {
         ctx->leaf_far.require( (size_t)(10 * ctx->L) );
-        launch_leaf_far( ctx->L, ctx->C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->ff_dx.get(), ctx->leaf_far.get(), ctx->stream );
+        launch_leaf_far( ctx->L, ctx->C, ctx->leaf_anc.get(), ctx->leaf_anc_len.get(), ctx->ff_dx.get(), ctx->leaf_far.get(), ctx->stream );
     }
 }
 
@@ -230,6 +230,15 @@ int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
         }
         for( auto x : leaf_of_prim ) { require( x >= 0, "leaves do not cover all primitives" ); }
         for( int64_t i = 0; i < T; ++i ) { require( P_ordering[i] >= 0 && P_ordering[i] < T, "bad P_ordering" ); }
+        // root-to-leaf paths, root first, depth-major so that the leaves of a wavefront read one row
+        std::vector<int32_t> anc( (size_t)( maxd + 1 ) * (size_t)L ), anc_len( (size_t)L );
+        for( int64_t l = 0; l < L; ++l )
+        {
+            int32_t c = leaves[(size_t)l];
+            const int n = depth[(size_t)c] + 1;
+            anc_len[(size_t)l] = n;
+            for( int d = n - 1; d >= 0; --d, c = parent[(size_t)c] ) { anc[(size_t)d * (size_t)L + (size_t)l] = c; }
+        }
         // interior clusters grouped by depth, deepest first
         std::vector<int32_t> level_ptr( (size_t)maxd + 2, 0 );
         for( int64_t c = 0; c < C; ++c ) { if( C_left[c] >= 0 ) { ++level_ptr[(size_t)(maxd - depth[(size_t)c]) + 1]; } }
@@ -259,6 +268,8 @@ int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
         ctx->C_begin.upload( C_begin, (size_t)C, ctx->stream );
         ctx->C_end.upload( C_end, (size_t)C, ctx->stream );
         ctx->C_parent.upload( parent.data(), (size_t)C, ctx->stream );
+        ctx->leaf_anc.upload( anc.data(), anc.size(), ctx->stream );
+        ctx->leaf_anc_len.upload( anc_len.data(), anc_len.size(), ctx->stream );
         ctx->P_ordering.upload( P_ordering, (size_t)T, ctx->stream );
         ctx->leaf_of_prim.upload( leaf_of_prim.data(), (size_t)T, ctx->stream );
         ctx->leaf_cluster.upload( leaves.data(), (size_t)L, ctx->stream );

@@ -183,6 +183,9 @@ struct rp_ctx
 
     // tree
     rp::DBuf<int32_t> C_left, C_right, C_begin, C_end, C_parent, P_ordering;
+    // root-to-leaf paths: leaf_anc[d * L + l] = the cluster at depth d on leaf l's path (d = 0: the
+    // root), leaf_anc_len[l] entries (the leaf's depth + 1)
+    rp::DBuf<int32_t> leaf_anc, leaf_anc_len;
     rp::DBuf<int32_t> leaf_of_prim;      // tree position -> leaf cluster id
     rp::DBuf<int32_t> leaf_cluster;      // leaf index -> cluster id
     rp::DBuf<int32_t> pre_cols;          // T x (n+1)

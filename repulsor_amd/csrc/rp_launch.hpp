@@ -54,8 +54,8 @@ namespace rp
     void launch_prep_prims( int64_t T, int nd, int size, const double * near, double * a, double * x, double * P, double * rec, hipStream_t s );
     void launch_prep_clusters( int64_t C, const double * far, double * a, double * x, double * P, hipStream_t s );
     void launch_leaf_origin( int64_t L, int64_t C, const int32_t * leaf_cluster, const double * cx, double * origin, hipStream_t s );
-    void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * ff_dx, double * leaf_far, hipStream_t s );
-    void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s );
+    void launch_leaf_far( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * ff_dx, double * leaf_far, hipStream_t s );
+    void launch_leaf_out( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * C_out, double * leaf_out, hipStream_t s );
     void launch_hulls( int n, int64_t T, int64_t L, const int32_t * P_ordering, const int32_t * leaf_of_prim, const int32_t * S,
                        const double * V, const double * charges, const double * nf_dx, const double * vf_dh, const double * leaf_far, double * B, hipStream_t s );
     void launch_assemble( int64_t V, const int64_t * vc_ptr, const int32_t * vc_corner, const double * B, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
@@ -65,7 +65,7 @@ namespace rp
     // the whole up-sweep (all depth levels, deepest first) in launches of three levels each
     void launch_up_tree( const std::vector<int32_t> & level_ptr, const int32_t * level_nodes, const int32_t * C_left, const int32_t * C_right,
                          double * C_in, hipStream_t s, int W );
-    void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * C_out, double * leaf_out, hipStream_t s );
+    void launch_leaf_out3( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * C_out, double * leaf_out, hipStream_t s );
     void launch_nf_spmm_post( bool mixed, int size, int64_t T, int64_t L, const int32_t * lb, const int32_t * le,
                               const int64_t * cp, const int32_t * cpr, const uint32_t * cm, const int64_t * vp,
                               const void * vals, const double * diag, const double * xin,
@@ -236,7 +236,7 @@ namespace rp
     void launch_mlp_backward( int64_t C, int F, const float * feats, const float * p, const double * g_lam, double * part, double * grad, hipStream_t s );
     void launch_adam( int NP, float * p, float * m, float * v, const double * grad, float lr, float b1, float b2, float eps, int t, hipStream_t s );
     // root-path sums of one value per cluster, per leaf (k_tree.hip)
-    void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_cluster, const int32_t * parent, const double * src, double * leaf_out, hipStream_t s );
+    void launch_leaf_path_sum1( int64_t L, int64_t C, const int32_t * leaf_anc, const int32_t * leaf_anc_len, const double * src, double * leaf_out, hipStream_t s );
 
     // cluster tree and block cluster tree from coordinates (k_build.hip); the squared separation parameters
     void build_topology( BuiltTopology & B, int dom_dim, int64_t V, int64_t T, const double * V_coords, const int32_t * simplices,

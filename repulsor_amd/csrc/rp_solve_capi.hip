@@ -29,7 +29,7 @@ namespace rp
             launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream, 3 );
             launch_up_tree( ctx->h_level_ptr, ctx->level_nodes.get(), ctx->C_left.get(), ctx->C_right.get(), ctx->C_in.get(), ctx->stream, 3 );
             launch_pl_ff_spmm( mixed, C, ctx->far_rp.get(), ctx->far_ci.get(), ctx->pl_ff_vals.get(), ctx->pl_ff_diag.get(), ctx->C_in.get(), ctx->C_out.get(), ctx->stream );
-            launch_leaf_out3( L, C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
+            launch_leaf_out3( L, C, ctx->leaf_anc.get(), ctx->leaf_anc_len.get(), ctx->C_out.get(), ctx->leaf_out.get(), ctx->stream );
         }
         double * S_out = ctx->P_out.get();   // T x (n+1) x 3
         if( do_nf )
@@ -142,7 +142,7 @@ namespace rp
         ctx->lp_logd.require( (size_t)ctx->V ); ctx->lp_gp.require( (size_t)ctx->T );
         ctx->lp_params16.require( (size_t)mlp_param_count( F ) );
         launch_mlp_infer( bf16, ctx->C, F, ctx->lp_feats.get(), ctx->lp_params.get(), ctx->lp_params16.get(), ctx->lp_lam.get(), ctx->stream );
-        launch_leaf_path_sum1( ctx->L, ctx->C, ctx->leaf_cluster.get(), ctx->C_parent.get(), ctx->lp_lam.get(), ctx->lp_leaf.get(), ctx->stream );
+        launch_leaf_path_sum1( ctx->L, ctx->C, ctx->leaf_anc.get(), ctx->leaf_anc_len.get(), ctx->lp_lam.get(), ctx->lp_leaf.get(), ctx->stream );
         launch_vertex_scale( ctx->V, ctx->T, ctx->size, ctx->pre_cols.get(), ctx->vp_ptr.get(), ctx->vp_entry.get(), ctx->leaf_of_prim.get(),
                              ctx->lp_leaf.get(), ctx->lp_logd.get(), ctx->lp_gp.get(), ctx->lp_sqrtd.get(), ctx->stream );
         RP_HIP( hipGetLastError() );
