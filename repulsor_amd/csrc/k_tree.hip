@@ -307,6 +307,28 @@ namespace rp
         C_in[W * (int64_t)c + h] = C_in[W * (int64_t)C_left[c] + h] + C_in[W * (int64_t)C_right[c] + h];
     }
 
+    // The top levels of PercolateUp in one workgroup: every level whose internal clusters number at most
+    // UP_TOP_MAX, deepest first, one barrier between levels (the sums of a level are written by the
+    // workgroup's own threads and read after the barrier), the same pair sums as one level per launch.
+    constexpr int UP_TOP_MAX = 512;
+    constexpr int UP_TOP_LEVELS = 48;
+    struct UpTopLevels { int n; int32_t ptr[UP_TOP_LEVELS + 1]; };
+
+    __global__ __launch_bounds__(1024) void k_up_top( UpTopLevels lv, int W, const int32_t * nodes, const int32_t * C_left, const int32_t * C_right, double * C_in )
+    {
+        for( int l = 0; l < lv.n; ++l )
+        {
+            const int32_t b = lv.ptr[l], cnt = ( lv.ptr[l + 1] - b ) * W;
+            for( int32_t g = threadIdx.x; g < cnt; g += 1024 )
+            {
+                const int32_t c = nodes[b + g / W];
+                const int h = g % W;
+                C_in[W * (int64_t)c + h] = C_in[W * (int64_t)C_left[c] + h] + C_in[W * (int64_t)C_right[c] + h];
+            }
+            __syncthreads();
+        }
+    }
+
     template<typename T>
     __device__ __forceinline__ T wave_sum64( T v )
     {
@@ -483,16 +505,27 @@ namespace rp
     void launch_up_tree( const std::vector<int32_t> & level_ptr, const int32_t * level_nodes, const int32_t * C_left, const int32_t * C_right,
                          double * C_in, hipStream_t s, int W )
     {
-        // level index l holds the internal clusters of depth maxd - l; bands of three levels end at l = 2, 5, ...
+        // level index l holds the internal clusters of depth maxd - l; the levels from top on (at most
+        // UP_TOP_MAX clusters each, the root end) run in one workgroup, the deeper ones in bands of three
+        // levels ending at l = 2, 5, ... and at top - 1
         const int nl = (int)level_ptr.size() - 1;
+        int top = nl;
+        while( top > 0 && nl - top < UP_TOP_LEVELS && level_ptr[(size_t)top] - level_ptr[(size_t)top - 1] <= UP_TOP_MAX ) { --top; }
         int prev = -1;
-        for( int l = 0; l < nl; ++l )
+        for( int l = 0; l < top; ++l )
         {
-            if( l % 3 != 2 && l != nl - 1 ) { continue; }
+            if( l % 3 != 2 && l != top - 1 ) { continue; }
             const int64_t b = level_ptr[(size_t)l], e = level_ptr[(size_t)l + 1];
             const int band = l - prev;
             prev = l;
             if( e > b ) { hipLaunchKernelGGL( k_up_band, grid1(W * ( e - b )), dim3(256), 0, s, e - b, W, band, level_nodes + b, C_left, C_right, C_in ); }
+        }
+        if( top < nl )
+        {
+            UpTopLevels lv;
+            lv.n = nl - top;
+            for( int l = 0; l <= lv.n; ++l ) { lv.ptr[l] = level_ptr[(size_t)( top + l )]; }
+            hipLaunchKernelGGL( k_up_top, dim3(1), dim3(1024), 0, s, lv, W, level_nodes, C_left, C_right, C_in );
         }
     }
 
