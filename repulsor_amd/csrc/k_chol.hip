@@ -148,10 +148,17 @@ namespace rp
                 }
                 c = PFS;
             }
-            for( ; c < ce; ++c )
+            // the remaining block values of the row in guarded batches of eight loads (sums in column order)
+            for( ; c < ce; c += 8 )
             {
-                const double l = Ls[(int64_t)c * r + a];
-                for( int k = 0; k < 3; ++k ) { acc[k] += l * fr[3 * c + k]; }
+                double lv[8];
+                #pragma unroll
+                for( int u = 0; u < 8; ++u ) { lv[u] = ( c + u < ce ) ? Ls[(int64_t)( c + u ) * r + a] : 0.0; }
+                #pragma unroll
+                for( int u = 0; u < 8; ++u )
+                {
+                    if( c + u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * fr[3 * ( c + u ) + k]; } }
+                }
             }
             if( a < w ) { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = acc[k]; } }
             else        { for( int k = 0; k < 3; ++k ) { z[3 * a + k] = fr[3 * a + k] - acc[k]; } }
@@ -197,11 +204,17 @@ namespace rp
         {
             if( cb + u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * zt[3 * ( cb + u ) + k]; } }
         }
-        #pragma unroll 8
-        for( int c = cb + PF; c < ce; ++c )
+        // the remaining block values in guarded batches of eight loads (sums in column order)
+        for( int c = cb + PF; c < ce; c += 8 )
         {
-            const double l = Ls[(int64_t)c * r + a];
-            for( int k = 0; k < 3; ++k ) { acc[k] += l * zt[3 * c + k]; }
+            double lv[8];
+            #pragma unroll
+            for( int u = 0; u < 8; ++u ) { lv[u] = ( c + u < ce ) ? Ls[(int64_t)( c + u ) * r + a] : 0.0; }
+            #pragma unroll
+            for( int u = 0; u < 8; ++u )
+            {
+                if( c + u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * zt[3 * ( c + u ) + k]; } }
+            }
         }
         for( int k = 0; k < 3; ++k ) { part[( wv * 64 + lane ) * 3 + k] = acc[k]; }
         __syncthreads();
@@ -258,11 +271,17 @@ namespace rp
         {
             if( u >= lane && u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * u + k]; } }
         }
-        #pragma unroll 8
-        for( int i = ( lane > PFS ? lane : PFS ); i < r; ++i )
+        // the remaining rows in guarded batches of eight loads (sums in row order)
+        for( int i = ( lane > PFS ? lane : PFS ); i < r; i += 8 )
         {
-            const double l = Lt[(int64_t)i * w + lane];
-            for( int k = 0; k < 3; ++k ) { acc[k] += l * vs[3 * i + k]; }
+            double lv[8];
+            #pragma unroll
+            for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u < r ) ? Lt[(int64_t)( i + u ) * w + lane] : 0.0; }
+            #pragma unroll
+            for( int u = 0; u < 8; ++u )
+            {
+                if( i + u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * vs[3 * ( i + u ) + k]; } }
+            }
         }
         for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = acc[k]; }
     }
@@ -327,10 +346,16 @@ namespace rp
                 const int i = ib + gl + u * G;
                 if( i < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * v[3 * ( i - i0 ) + k]; } }
             }
-            for( int i = ib + gl + CHOL_PF * G; i < r; i += G )
+            for( int i = ib + gl + CHOL_PF * G; i < r; i += 8 * G )
             {
-                const double l = col[i];
-                for( int k = 0; k < 3; ++k ) { acc[k] += l * v[3 * ( i - i0 ) + k]; }
+                double lv[8];
+                #pragma unroll
+                for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u * G < r ) ? col[i + u * G] : 0.0; }
+                #pragma unroll
+                for( int u = 0; u < 8; ++u )
+                {
+                    if( i + u * G < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * v[3 * ( i + u * G - i0 ) + k]; } }
+                }
             }
         }
         else
@@ -353,11 +378,17 @@ namespace rp
                 if( c < w )
                 {
                     const int ib = max( c, i0 );
-                    #pragma unroll 8
-                    for( int i = ib + gl; i < i0 + in; i += G )
+                    const int ie = i0 + in;
+                    for( int i = ib + gl; i < ie; i += 8 * G )
                     {
-                        const double l = col[i];
-                        for( int k = 0; k < 3; ++k ) { acc[k] += l * v[3 * ( i - i0 ) + k]; }
+                        double lv[8];
+                        #pragma unroll
+                        for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u * G < ie ) ? col[i + u * G] : 0.0; }
+                        #pragma unroll
+                        for( int u = 0; u < 8; ++u )
+                        {
+                            if( i + u * G < ie ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * v[3 * ( i + u * G - i0 ) + k]; } }
+                        }
                     }
                 }
             }
