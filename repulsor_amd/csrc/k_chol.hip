@@ -313,7 +313,7 @@ namespace rp
         const int64_t rb = tk.rb;
         const double * z = F.z + 3 * rb;
         const int32_t * R = F.rows + rb;
-        const int G = chol_pow2_at_least( r < 64 ? r : 64 );
+        const int G = chol_bwd_group( r, WIDE );
         const int grp = threadIdx.x / G, gl = threadIdx.x % G;
         const int c = tk.k + grp;
         const double * col = F.vals + tk.vp + (int64_t)c * r;

@@ -107,6 +107,18 @@ namespace rp
     constexpr int CHOL_NARROW_WIDTH = 128; // levels up to this width run the 4-wave forward (k_chol_fwd_big<4>)
     constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a wavefront per supernode
     constexpr int CHOL_FWD_ROWS = 64;     // front rows per forward task (one per lane)
+    constexpr int CHOL_BWD_RPL = 8;       // rows per lane the backward tasks of the narrow larger levels aim at
+
+    // Lanes per column in the backward tasks of the larger levels (a 256-thread task takes 256 / G
+    // columns): on the wide levels min(64, r rounded up to a power of two); on the others the power of
+    // two covering r / CHOL_BWD_RPL rows per lane (one batch of block-value loads), from 4 to 64.
+    __host__ __device__ inline int chol_bwd_group( int r, bool wide )
+    {
+        const int need = wide ? r : ( r + CHOL_BWD_RPL - 1 ) / CHOL_BWD_RPL;
+        int g = wide ? 1 : 4;
+        while( g < need && g < 64 ) { g <<= 1; }
+        return g;
+    }
     void launch_chol_form_m( const CholTask * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s );
     void launch_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp, hipStream_t s );
     void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s );   // G lanes per supernode, G >= its width

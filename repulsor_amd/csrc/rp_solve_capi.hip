@@ -829,13 +829,12 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         }
         // Levels whose supernodes all have at most CHOL_SMALL_ROWS rows run a lane group per supernode;
         // the others run task lists: forward (supernode, 64 front rows), backward (supernode, as many
-        // columns as groups of G = min(64, r rounded up to a power of two) lanes fit in 256 threads).
+        // columns as groups of G = chol_bwd_group( r, wide ) lanes fit in 256 threads).
         // A third list, (supernode, 64 update rows) over all supernodes, forms M = L_off inv(L_ss).
         std::vector<char> small( (size_t)nl, 0 );
         std::vector<CholTask> tasks;
         std::vector<int64_t> tptr[3];
         for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
-        auto group_of = []( int64_t len ) { int g = 1; while( g < len && g < 64 ) { g <<= 1; } return g; };
         auto task = [&]( int32_t s, int32_t k )
         {
             CholTask t;
@@ -858,7 +857,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                     for( int32_t q0 = 0; q0 < std::max( r - w, 1 ); q0 += 64 ) { tk[2].push_back( task( s, q0 ) ); }
                     if( small[(size_t)l] ) { continue; }
                     for( int32_t a0 = 0; a0 < r; a0 += CHOL_FWD_ROWS ) { tk[0].push_back( task( s, a0 ) ); }
-                    const int per = 256 / group_of( r );
+                    const int per = 256 / chol_bwd_group( r, wmax[(size_t)l] > CHOL_NARROW_WIDTH );
                     for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( task( s, c0 ) ); }
                 }
                 for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
