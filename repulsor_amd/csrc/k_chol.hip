@@ -166,8 +166,9 @@ namespace rp
     }
 
     // Forward, larger levels: one workgroup of NW waves per task (supernode, 64 front rows); the waves
-    // split the columns and add their partial sums in LDS. NW = 4 serves levels whose supernodes are at
-    // most CHOL_NARROW_WIDTH columns wide (the launch picks NW by the level's widest supernode).
+    // split the columns and add their partial sums in LDS. NW = 1, 2 or 4 serves levels whose supernodes
+    // are at most CHOL_NARROW_WIDTH columns wide, 16 the wider ones (the launch picks NW by the level's
+    // widest supernode).
     template<int NW>
     __global__ __launch_bounds__(64 * NW) void k_chol_fwd_big( CholDev F, const CholTask * tasks )
     {
@@ -232,10 +233,11 @@ namespace rp
     }
 
     // ---------------------------------------------------------------------------------------------
-    // Backward, small levels: a group of G lanes per supernode (G at least the level's widest supernode);
-    // the lanes stage v in LDS (rows lane, lane + G, ...), then lane c < w forms
-    // x_c = sum over rows i >= c of L(i, c) v_i, reading the transposed copy of the block (row-major, so
-    // the lanes of a row read consecutive addresses), its first rows loaded before the staging.
+    // Backward, small levels: a group of G lanes per supernode (G follows the level's mean row count);
+    // the lanes stage v in LDS (rows lane, lane + G, ...), then lane c forms x_c = sum over rows i >= c
+    // of L(i, c) v_i for the columns c = lane, lane + G, ..., reading the transposed copy of the block
+    // (row-major, so the lanes of a row read consecutive addresses), the first column's first rows
+    // loaded before the staging.
     template<int G>
     __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count, int rs )
     {
@@ -264,26 +266,35 @@ namespace rp
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-        if( lane >= w ) { return; }
-        double acc[3] = { 0.0, 0.0, 0.0 };
-        #pragma unroll
-        for( int u = 0; u < PFS; ++u )
+        // lane c takes the columns c, c + G, ...: each column's sum is one lane's, in row order, whatever G
+        for( int c = lane; c < w; c += G )
         {
-            if( u >= lane && u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * u + k]; } }
-        }
-        // the remaining rows in guarded batches of eight loads (sums in row order)
-        for( int i = ( lane > PFS ? lane : PFS ); i < r; i += 8 )
-        {
-            double lv[8];
-            #pragma unroll
-            for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u < r ) ? Lt[(int64_t)( i + u ) * w + lane] : 0.0; }
-            #pragma unroll
-            for( int u = 0; u < 8; ++u )
+            double acc[3] = { 0.0, 0.0, 0.0 };
+            int i = c;
+            if( c == lane )
             {
-                if( i + u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * vs[3 * ( i + u ) + k]; } }
+                // the lane's first column: its first PFS rows were loaded before the staging
+                #pragma unroll
+                for( int u = 0; u < PFS; ++u )
+                {
+                    if( u >= c && u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * u + k]; } }
+                }
+                i = c > PFS ? c : PFS;
             }
+            // the remaining rows in guarded batches of eight loads (sums in row order)
+            for( ; i < r; i += 8 )
+            {
+                double lv[8];
+                #pragma unroll
+                for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u < r ) ? Lt[(int64_t)( i + u ) * w + c] : 0.0; }
+                #pragma unroll
+                for( int u = 0; u < 8; ++u )
+                {
+                    if( i + u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * vs[3 * ( i + u ) + k]; } }
+                }
+            }
+            for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + c ) + k] = acc[k]; }
         }
-        for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + lane ) + k] = acc[k]; }
     }
 
     // Transposed (row-major) copy of the blocks of the small levels: Lt[i * w + c] = L(i, c).
@@ -432,12 +443,16 @@ namespace rp
         else               { hipLaunchKernelGGL( k_chol_fwd_small<64>, dim3( nb ), dim3( 256 ), shm, s, F, nodes, count, rmax ); }
     }
 
-    void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s )
+    void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int wmax, hipStream_t s )
     {
         if( count <= 0 ) { return; }
-        // wide: the level's widest supernode has more than CHOL_NARROW_WIDTH columns
-        if( wide ) { hipLaunchKernelGGL( (k_chol_fwd_big<16>), dim3( (unsigned)count ), dim3( 1024 ), 0, s, F, tasks ); }
-        else       { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
+        // waves per task by the level's widest supernode: the columns split over 16 waves on the wide
+        // levels (more than CHOL_NARROW_WIDTH columns), over 4, 2 or 1 on the others (at most 32 columns
+        // per wave), so that narrow levels launch few wavefronts
+        if( wmax > CHOL_NARROW_WIDTH ) { hipLaunchKernelGGL( (k_chol_fwd_big<16>), dim3( (unsigned)count ), dim3( 1024 ), 0, s, F, tasks ); }
+        else if( wmax > 64 )           { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
+        else if( wmax > 32 )           { hipLaunchKernelGGL( (k_chol_fwd_big<2>),  dim3( (unsigned)count ), dim3( 128 ),  0, s, F, tasks ); }
+        else                           { hipLaunchKernelGGL( (k_chol_fwd_big<1>),  dim3( (unsigned)count ), dim3( 64 ),   0, s, F, tasks ); }
     }
 
     void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s )

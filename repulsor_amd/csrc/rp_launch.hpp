@@ -122,7 +122,7 @@ namespace rp
     void launch_chol_form_m( const CholTask * tasks, int64_t count, const double * Lin, double * Lout, hipStream_t s );
     void launch_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp, hipStream_t s );
     void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s );   // G lanes per supernode, G >= its width
-    void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, bool wide, hipStream_t s );
+    void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int wmax, hipStream_t s );
     void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s );   // G lanes per supernode, G >= its width
     void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s );
     void launch_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t, hipStream_t s );

@@ -78,7 +78,7 @@ namespace rp
             else
             {
                 const int64_t tb = ctx->h1_task_ptr[0][l], te = ctx->h1_task_ptr[0][l + 1];
-                launch_chol_fwd_big( F, tasks + tb, te - tb, ctx->h1_level_wmax[l] > CHOL_NARROW_WIDTH, ctx->stream );
+                launch_chol_fwd_big( F, tasks + tb, te - tb, ctx->h1_level_wmax[l], ctx->stream );
             }
         }
         for( size_t l = nl; l-- > 0; )
@@ -816,12 +816,12 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 rmax[(size_t)h] = std::max( rmax[(size_t)h], (int)( row_ptr[s + 1] - row_ptr[s] ) );
                 rsum[(size_t)h] += row_ptr[s + 1] - row_ptr[s];
             }
-            // small levels: G lanes per supernode, the power of two covering the level's mean row count and
-            // its widest supernode (one lane per column backward), at least 8; rows beyond G loop
+            // small levels: G lanes per supernode, the power of two covering the level's mean row count,
+            // at least 8; rows (forward) and columns (backward) beyond G loop
             for( int32_t l = 0; l < nl; ++l )
             {
                 const int64_t cnt = lptr[(size_t)l + 1] - lptr[(size_t)l];
-                const int64_t need = std::max<int64_t>( { (int64_t)8, (int64_t)wmax[(size_t)l], cnt > 0 ? ( rsum[(size_t)l] + cnt - 1 ) / cnt : 1 } );
+                const int64_t need = std::max<int64_t>( (int64_t)8, cnt > 0 ? ( rsum[(size_t)l] + cnt - 1 ) / cnt : 1 );
                 int g = 8;
                 while( g < need && g < 64 ) { g <<= 1; }
                 gsz[(size_t)l] = g;
