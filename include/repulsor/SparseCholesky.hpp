@@ -97,7 +97,7 @@ namespace Repulsor
     inline SupernodalFactor SparseCholeskyFactor( const Int n, const std::vector<LInt> & rp, const std::vector<Int> & ci,
                                                   const std::vector<double> & av, const std::vector<Int> & perm0,
                                                   const int threads = 1, const Int relax_cols = 16, const Int max_width = 2048,
-                                                  const bool chain_merge = true, const Int bottom_levels = 2 )
+                                                  const bool chain_merge = true, const Int bottom_levels = 3 )
     {
         ScopedTimer timer("SparseCholesky::Factor");
         SupernodalFactor F;
@@ -190,8 +190,9 @@ namespace Repulsor
             // the maximal subtrees of supernodes at heights below bottom_levels become one supernode each
             // (a subtree is a contiguous column range in postorder; explicit zeros where the structures
             // differ): the tree loses bottom_levels - 1 levels at its widest, lowest part. Measured on the
-            // GPU solves at 1.31M triangles: 2 (3 % more stored values) 0.749 -> 0.726 ms per solve; 3 and
-            // 4 were slower (0.81, 0.85 ms: the merged fronts leave the lane-group scheme)
+            // GPU solves at 1.31M triangles: 2 (3 % more stored values) 0.749 -> 0.726 ms per solve; with
+            // the batched loads and per-sweep lane-group choice of the GPU solves, 3 (21 -> 20 levels, 7 %
+            // more stored values) 0.605 -> 0.582 ms
             const Int m = (Int)sb.size() - 1;
             std::vector<Int> sof( (size_t)n );
             for( Int t = 0; t < m; ++t ) { for( Int j = sb[(size_t)t]; j < sb[(size_t)t + 1]; ++j ) { sof[(size_t)j] = t; } }

@@ -279,7 +279,8 @@ struct rp_ctx
     std::vector<int>     h1_level_wmax;
     std::vector<int>     h1_level_rmax;
     std::vector<int>     h1_level_g;       // small levels: lanes per supernode
-    std::vector<char>    h1_level_small;           // levels run a wavefront per supernode
+    std::vector<char>    h1_level_small;           // forward: the level runs a lane group per supernode
+    std::vector<char>    h1_level_small_bwd;       // backward: the same
     rp::DBuf<rp::CholTask> h1_tasks;               // forward, backward and M-forming tasks
     rp::DBuf<rp::CholTask> h1_sdesc;               // per supernode, in level order
     std::vector<int64_t> h1_task_ptr[3];           // per kind (forward, backward, forming M): level offsets into h1_tasks

@@ -105,7 +105,9 @@ namespace rp
     };
     constexpr int CHOL_MAX_WIDTH = 2048;  // columns per supernode
     constexpr int CHOL_NARROW_WIDTH = 128; // levels up to this width run the 4-wave forward (k_chol_fwd_big<4>)
-    constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a wavefront per supernode
+    constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a lane group per supernode
+    constexpr int CHOL_SMALL_ROWS_MANY = 96;  // the backward also on levels of at most this many rows ...
+    constexpr int CHOL_MANY_SUPERNODES = 16384;  // ... that hold at least this many supernodes
     constexpr int CHOL_FWD_ROWS = 64;     // front rows per forward task (one per lane)
     constexpr int CHOL_BWD_RPL = 8;       // rows per lane the backward tasks of the narrow larger levels aim at
 

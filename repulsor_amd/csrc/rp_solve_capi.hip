@@ -83,7 +83,7 @@ namespace rp
         }
         for( size_t l = nl; l-- > 0; )
         {
-            if( ctx->h1_level_small[l] )
+            if( ctx->h1_level_small_bwd[l] )
             {
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
                 launch_chol_bwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_g[l], ctx->h1_level_rmax[l], ctx->stream );
@@ -827,11 +827,13 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 gsz[(size_t)l] = g;
             }
         }
-        // Levels whose supernodes all have at most CHOL_SMALL_ROWS rows run a lane group per supernode;
-        // the others run task lists: forward (supernode, 64 front rows), backward (supernode, as many
-        // columns as groups of G = chol_bwd_group( r, wide ) lanes fit in 256 threads).
+        // Levels whose supernodes all have at most CHOL_SMALL_ROWS rows run a lane group per supernode
+        // (the backward also levels of at most CHOL_SMALL_ROWS_MANY rows with at least
+        // CHOL_MANY_SUPERNODES supernodes); the others run task lists: forward (supernode, 64 front rows),
+        // backward (supernode, as many columns as groups of G = chol_bwd_group( r, wide ) lanes fit in
+        // 256 threads).
         // A third list, (supernode, 64 update rows) over all supernodes, forms M = L_off inv(L_ss).
-        std::vector<char> small( (size_t)nl, 0 );
+        std::vector<char> small( (size_t)nl, 0 ), small_bwd( (size_t)nl, 0 );
         std::vector<CholTask> tasks;
         std::vector<int64_t> tptr[3];
         for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
@@ -849,16 +851,22 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
             for( int32_t l = 0; l < nl; ++l )
             {
                 small[(size_t)l] = rmax[(size_t)l] <= CHOL_SMALL_ROWS;
+                // the backward's lane groups also serve levels of many slightly longer supernodes (one
+                // task per supernode would launch too many wavefronts there)
+                small_bwd[(size_t)l] = small[(size_t)l] || ( rmax[(size_t)l] <= CHOL_SMALL_ROWS_MANY &&
+                                                             lptr[(size_t)l + 1] - lptr[(size_t)l] >= CHOL_MANY_SUPERNODES );
                 for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k )
                 {
                     const int32_t s = lnodes[(size_t)k];
                     const int32_t w = sn_begin[s + 1] - sn_begin[s];
                     const int32_t r = (int32_t)( row_ptr[s + 1] - row_ptr[s] );
                     for( int32_t q0 = 0; q0 < std::max( r - w, 1 ); q0 += 64 ) { tk[2].push_back( task( s, q0 ) ); }
-                    if( small[(size_t)l] ) { continue; }
-                    for( int32_t a0 = 0; a0 < r; a0 += CHOL_FWD_ROWS ) { tk[0].push_back( task( s, a0 ) ); }
-                    const int per = 256 / chol_bwd_group( r, wmax[(size_t)l] > CHOL_NARROW_WIDTH );
-                    for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( task( s, c0 ) ); }
+                    if( !small[(size_t)l] ) { for( int32_t a0 = 0; a0 < r; a0 += CHOL_FWD_ROWS ) { tk[0].push_back( task( s, a0 ) ); } }
+                    if( !small_bwd[(size_t)l] )
+                    {
+                        const int per = 256 / chol_bwd_group( r, wmax[(size_t)l] > CHOL_NARROW_WIDTH );
+                        for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( task( s, c0 ) ); }
+                    }
                 }
                 for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
             }
@@ -884,17 +892,18 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
             const int64_t mb = tptr[2][0], me = tptr[2][(size_t)nl];
             launch_chol_form_m( ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
             RP_HIP( hipGetLastError() );
-            // row-major copies of the blocks of every small level (same offsets as in h1_vals)
+            // row-major copies of the blocks of every level whose backward runs lane groups (same offsets
+            // as in h1_vals)
             int64_t extent = 1;
             for( int32_t l = 0; l < nl; ++l )
             {
-                if( !small[(size_t)l] ) { continue; }
+                if( !small_bwd[(size_t)l] ) { continue; }
                 for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k ) { extent = std::max<int64_t>( extent, sdesc[(size_t)k].vp + (int64_t)sdesc[(size_t)k].w * sdesc[(size_t)k].r ); }
             }
             ctx->h1_vals_t.require( (size_t)extent );
             for( int32_t l = 0; l < nl; ++l )
             {
-                if( !small[(size_t)l] ) { continue; }
+                if( !small_bwd[(size_t)l] ) { continue; }
                 launch_chol_transpose( ctx->h1_sdesc.get() + lptr[(size_t)l], lptr[(size_t)l + 1] - lptr[(size_t)l], ctx->h1_vals.get(), ctx->h1_vals_t.get(), ctx->stream );
             }
             RP_HIP( hipGetLastError() );
@@ -916,6 +925,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_level_rmax = rmax;
         ctx->h1_level_g = gsz;
         ctx->h1_level_small = small;
+        ctx->h1_level_small_bwd = small_bwd;
         for( int kind = 0; kind < 3; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
         ctx->h1_n = n;
         ctx->h1_ns = ns;
