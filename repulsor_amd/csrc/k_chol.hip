@@ -232,6 +232,67 @@ namespace rp
         }
     }
 
+    // Forward, wide levels of few supernodes, in two launches. First the merged front of every row of
+    // the level's supernodes into fr (tasks: supernode, 64 rows; one lane per row), then the products
+    // over tasks of 16 rows: a row's columns are split over 64 groups of at most 32 columns each (one
+    // batch of block-value loads per lane), the own rows of the front staged in LDS from fr, and the 64
+    // partial sums of a row added in group order.
+    __global__ __launch_bounds__(64) void k_chol_fwd_front( CholDev F, const CholTask * tasks, double * fr )
+    {
+        const CholTask tk = tasks[blockIdx.x];
+        const int a = tk.k + (int)threadIdx.x;
+        if( a >= tk.r ) { return; }
+        double v[3];
+        chol_front_row( F, tk.rb, tk.f, tk.w, a, v );
+        for( int k = 0; k < 3; ++k ) { fr[3 * ( tk.rb + a ) + k] = v[k]; }
+    }
+
+    __global__ __launch_bounds__(1024) void k_chol_fwd_split( CholDev F, const CholTask * tasks, const double * fr )
+    {
+        constexpr int RB = 16;          // rows per task
+        constexpr int NG = 1024 / RB;   // column groups per row
+        constexpr int CW = ( CHOL_MAX_WIDTH + NG - 1 ) / NG;   // columns per group at most
+        __shared__ double zt[3 * CHOL_MAX_WIDTH];
+        __shared__ double part[NG][RB][3];
+        const CholTask tk = tasks[blockIdx.x];
+        const int f = tk.f, w = tk.w, r = tk.r;
+        const int64_t rb = tk.rb;
+        const int tid = threadIdx.x;
+        const int i = tid % RB, g = tid / RB;
+        const int a = tk.k + i;
+        const double * Ls = F.vals + tk.vp;
+        const int cw = ( w + NG - 1 ) / NG;
+        const int cb = g * cw;
+        const int ce = ( a < r ) ? min( ( a < w ) ? a + 1 : w, cb + cw ) : cb;
+        // the lane's block values do not depend on the front: they go out before the staging
+        double lp[CW];
+        #pragma unroll
+        for( int u = 0; u < CW; ++u ) { lp[u] = ( cb + u < ce ) ? Ls[(int64_t)( cb + u ) * r + a] : 0.0; }
+        for( int c = tid; c < w; c += 1024 )
+        {
+            for( int k = 0; k < 3; ++k ) { zt[3 * c + k] = fr[3 * ( rb + c ) + k]; }
+        }
+        __syncthreads();
+        double acc[3] = { 0.0, 0.0, 0.0 };
+        #pragma unroll
+        for( int u = 0; u < CW; ++u )
+        {
+            if( cb + u < ce ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * zt[3 * ( cb + u ) + k]; } }
+        }
+        for( int k = 0; k < 3; ++k ) { part[g][i][k] = acc[k]; }
+        __syncthreads();
+        if( tid < RB && a < r )
+        {
+            double * z = F.z + 3 * rb;
+            for( int k = 0; k < 3; ++k )
+            {
+                double sum = part[0][tid][k];
+                for( int u = 1; u < NG; ++u ) { sum += part[u][tid][k]; }
+                z[3 * a + k] = ( a < w ) ? sum : fr[3 * ( rb + a ) + k] - sum;
+            }
+        }
+    }
+
     // ---------------------------------------------------------------------------------------------
     // Backward, small levels: a group of G lanes per supernode (G follows the level's mean row count);
     // the lanes stage v in LDS (rows lane, lane + G, ...), then lane c forms x_c = sum over rows i >= c
@@ -453,6 +514,14 @@ namespace rp
         else if( wmax > 64 )           { hipLaunchKernelGGL( (k_chol_fwd_big<4>),  dim3( (unsigned)count ), dim3( 256 ),  0, s, F, tasks ); }
         else if( wmax > 32 )           { hipLaunchKernelGGL( (k_chol_fwd_big<2>),  dim3( (unsigned)count ), dim3( 128 ),  0, s, F, tasks ); }
         else                           { hipLaunchKernelGGL( (k_chol_fwd_big<1>),  dim3( (unsigned)count ), dim3( 64 ),   0, s, F, tasks ); }
+    }
+
+    void launch_chol_fwd_split( const CholDev & F, const CholTask * front_tasks, int64_t front_count,
+                                const CholTask * tasks, int64_t count, double * fr, hipStream_t s )
+    {
+        if( front_count <= 0 || count <= 0 ) { return; }
+        hipLaunchKernelGGL( k_chol_fwd_front, dim3( (unsigned)front_count ), dim3( 64 ), 0, s, F, front_tasks, fr );
+        hipLaunchKernelGGL( k_chol_fwd_split, dim3( (unsigned)count ), dim3( 1024 ), 0, s, F, tasks, fr );
     }
 
     void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s )

@@ -272,7 +272,7 @@ struct rp_ctx
     // supernodal Cholesky factor of the H^1 metric
     rp::DBuf<int32_t> h1_sn_begin, h1_rows, h1_kid_ptr, h1_kids, h1_cmap, h1_perm, h1_level_nodes;
     rp::DBuf<int64_t> h1_row_ptr, h1_val_ptr;
-    rp::DBuf<double>  h1_vals, h1_z, h1_xp, h1_bp;  // h1_vals: inv(L_ss) over M = L_off inv(L_ss) per supernode
+    rp::DBuf<double>  h1_vals, h1_z, h1_xp, h1_bp, h1_fr;  // h1_vals: inv(L_ss) over M = L_off inv(L_ss) per supernode
     rp::DBuf<double>  h1_vals_t;                     // row-major copies of the small levels' blocks
     rp::DBuf<int32_t> h1_pull_ptr, h1_pull_src;      // per front row: the child update rows that map to it
     std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
@@ -283,7 +283,8 @@ struct rp_ctx
     std::vector<char>    h1_level_small_bwd;       // backward: the same
     rp::DBuf<rp::CholTask> h1_tasks;               // forward, backward and M-forming tasks
     rp::DBuf<rp::CholTask> h1_sdesc;               // per supernode, in level order
-    std::vector<int64_t> h1_task_ptr[3];           // per kind (forward, backward, forming M): level offsets into h1_tasks
+    std::vector<int64_t> h1_task_ptr[4];           // per kind (forward, backward, forming M, split forward): level offsets into h1_tasks
+    std::vector<char>    h1_level_split;           // the forward of the level runs in two launches (front, then 16-row tasks)
     int64_t h1_n = 0, h1_ns = 0;
     bool have_h1 = false;
 

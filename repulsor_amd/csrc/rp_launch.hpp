@@ -108,6 +108,8 @@ namespace rp
     constexpr int CHOL_SMALL_ROWS = 64;   // levels whose supernodes have at most this many rows run a lane group per supernode
     constexpr int CHOL_SMALL_ROWS_MANY = 96;  // the backward also on levels of at most this many rows ...
     constexpr int CHOL_MANY_SUPERNODES = 16384;  // ... that hold at least this many supernodes
+    constexpr int CHOL_SPLIT_TASKS = 64;  // wide levels with fewer 64-row forward tasks run the split forward
+    constexpr int CHOL_SPLIT_ROWS = 16;   // rows per task of the split forward
     constexpr int CHOL_FWD_ROWS = 64;     // front rows per forward task (one per lane)
     constexpr int CHOL_BWD_RPL = 8;       // rows per lane the backward tasks of the narrow larger levels aim at
 
@@ -125,6 +127,8 @@ namespace rp
     void launch_chol_gather( int64_t n, const int32_t * perm, double alpha, const double * B, int64_t ldB, double * bp, hipStream_t s );
     void launch_chol_fwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s );   // G lanes per supernode, G >= its width
     void launch_chol_fwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int wmax, hipStream_t s );
+    void launch_chol_fwd_split( const CholDev & F, const CholTask * front_tasks, int64_t front_count,
+                                const CholTask * tasks, int64_t count, double * fr, hipStream_t s );
     void launch_chol_bwd_small( const CholDev & F, const CholTask * nodes, int64_t count, int G, int rmax, hipStream_t s );   // G lanes per supernode, G >= its width
     void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s );
     void launch_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t, hipStream_t s );

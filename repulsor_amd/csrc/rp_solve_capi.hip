@@ -75,6 +75,12 @@ namespace rp
                 const int64_t b = ctx->h1_level_ptr[l], e = ctx->h1_level_ptr[l + 1];
                 launch_chol_fwd_small( F, ctx->h1_sdesc.get() + b, e - b, ctx->h1_level_g[l], ctx->h1_level_rmax[l], ctx->stream );
             }
+            else if( ctx->h1_level_split[l] )
+            {
+                const int64_t tb = ctx->h1_task_ptr[0][l], te = ctx->h1_task_ptr[0][l + 1];
+                const int64_t sb = ctx->h1_task_ptr[3][l], se = ctx->h1_task_ptr[3][l + 1];
+                launch_chol_fwd_split( F, tasks + tb, te - tb, tasks + sb, se - sb, ctx->h1_fr.get(), ctx->stream );
+            }
             else
             {
                 const int64_t tb = ctx->h1_task_ptr[0][l], te = ctx->h1_task_ptr[0][l + 1];
@@ -833,10 +839,10 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         // backward (supernode, as many columns as groups of G = chol_bwd_group( r, wide ) lanes fit in
         // 256 threads).
         // A third list, (supernode, 64 update rows) over all supernodes, forms M = L_off inv(L_ss).
-        std::vector<char> small( (size_t)nl, 0 ), small_bwd( (size_t)nl, 0 );
+        std::vector<char> small( (size_t)nl, 0 ), small_bwd( (size_t)nl, 0 ), split( (size_t)nl, 0 );
         std::vector<CholTask> tasks;
-        std::vector<int64_t> tptr[3];
-        for( int kind = 0; kind < 3; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
+        std::vector<int64_t> tptr[4];
+        for( int kind = 0; kind < 4; ++kind ) { tptr[kind].assign( (size_t)nl + 1, 0 ); }
         auto task = [&]( int32_t s, int32_t k )
         {
             CholTask t;
@@ -847,7 +853,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         std::vector<CholTask> sdesc( (size_t)ns );
         for( size_t k = 0; k < lnodes.size(); ++k ) { sdesc[k] = task( lnodes[k], 0 ); }
         {
-            std::vector<CholTask> tk[3];
+            std::vector<CholTask> tk[4];
             for( int32_t l = 0; l < nl; ++l )
             {
                 small[(size_t)l] = rmax[(size_t)l] <= CHOL_SMALL_ROWS;
@@ -868,10 +874,22 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                         for( int32_t c0 = 0; c0 < w; c0 += per ) { tk[1].push_back( task( s, c0 ) ); }
                     }
                 }
-                for( int kind = 0; kind < 3; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
+                // wide levels of few forward tasks run the split forward: 16-row tasks after the front launch
+                if( !small[(size_t)l] && wmax[(size_t)l] > CHOL_NARROW_WIDTH &&
+                    (int64_t)tk[0].size() - tptr[0][(size_t)l] < CHOL_SPLIT_TASKS )
+                {
+                    split[(size_t)l] = 1;
+                    for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k )
+                    {
+                        const int32_t s = lnodes[(size_t)k];
+                        const int32_t r = (int32_t)( row_ptr[s + 1] - row_ptr[s] );
+                        for( int32_t a0 = 0; a0 < r; a0 += CHOL_SPLIT_ROWS ) { tk[3].push_back( task( s, a0 ) ); }
+                    }
+                }
+                for( int kind = 0; kind < 4; ++kind ) { tptr[kind][(size_t)l + 1] = (int64_t)tk[kind].size(); }
             }
             int64_t off = 0;
-            for( int kind = 0; kind < 3; ++kind )
+            for( int kind = 0; kind < 4; ++kind )
             {
                 for( auto & x : tptr[kind] ) { x += off; }
                 tasks.insert( tasks.end(), tk[kind].begin(), tk[kind].end() );
@@ -918,6 +936,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_perm.upload( perm, (size_t)n, ctx->stream );
         ctx->h1_level_nodes.upload( lnodes.data(), lnodes.size(), ctx->stream );
         ctx->h1_z.require( (size_t)(3 * nrows) );
+        ctx->h1_fr.require( (size_t)(3 * nrows) );
         ctx->h1_xp.require( (size_t)(3 * (int64_t)n) );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->h1_level_ptr = lptr;
@@ -926,7 +945,8 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_level_g = gsz;
         ctx->h1_level_small = small;
         ctx->h1_level_small_bwd = small_bwd;
-        for( int kind = 0; kind < 3; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
+        for( int kind = 0; kind < 4; ++kind ) { ctx->h1_task_ptr[kind] = tptr[kind]; }
+        ctx->h1_level_split = split;
         ctx->h1_n = n;
         ctx->h1_ns = ns;
         ctx->have_h1 = true;
