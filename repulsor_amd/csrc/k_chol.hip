@@ -295,10 +295,9 @@ namespace rp
 
     // ---------------------------------------------------------------------------------------------
     // Backward, small levels: a group of G lanes per supernode (G follows the level's mean row count);
-    // the lanes stage v in LDS (rows lane, lane + G, ...), then lane c forms x_c = sum over rows i >= c
-    // of L(i, c) v_i for the columns c = lane, lane + G, ..., reading the transposed copy of the block
-    // (row-major, so the lanes of a row read consecutive addresses), the first column's first rows
-    // loaded before the staging.
+    // the lanes stage v in LDS (rows lane, lane + G, ...), then x_c = sum over rows i >= c of L(i, c) v_i
+    // for every column, each by Lc lanes (as many as the group has to spare, up to 16), reading the
+    // transposed copy of the block (row-major), the first rows loaded before the staging.
     template<int G>
     __global__ __launch_bounds__(256) void k_chol_bwd_small( CholDev F, const CholTask * desc, int64_t count, int rs )
     {
@@ -311,10 +310,20 @@ namespace rp
         const int f = d.f, w = d.w, r = d.r;
         const int64_t rb = d.rb;
         const double * Lt = F.vals_t + d.vp;   // Lt[i * w + c] = L(i, c)
+        // Lc lanes per column (the largest power of two up to 16 with Lc w <= G): lane (cl, sub) takes the
+        // rows c + sub, c + sub + Lc, ... of the columns c = cl, cl + G / Lc, ...; the Lc partial sums of a
+        // column are added by an aligned shuffle tree
+        int Lc = 1;
+        while( Lc < 16 && 2 * Lc * w <= G ) { Lc <<= 1; }
+        const int cl = lane / Lc, sub = lane % Lc, ncs = G / Lc;
         constexpr int PFS = 16;
         double lp[PFS];
         #pragma unroll
-        for( int u = 0; u < PFS; ++u ) { lp[u] = ( lane < w && u >= lane && u < r ) ? Lt[(int64_t)u * w + lane] : 0.0; }
+        for( int u = 0; u < PFS; ++u )
+        {
+            const int i = cl + sub + u * Lc;
+            lp[u] = ( cl < w && i < r ) ? Lt[(int64_t)i * w + cl] : 0.0;
+        }
         for( int i = lane; i < r; i += G )
         {
             if( i < w ) { for( int k = 0; k < 3; ++k ) { vs[3 * i + k] = F.z[3 * ( rb + i ) + k]; } }
@@ -327,34 +336,35 @@ namespace rp
         __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
-        // lane c takes the columns c, c + G, ...: each column's sum is one lane's, in row order, whatever G
-        for( int c = lane; c < w; c += G )
+        for( int c = cl; c < w; c += ncs )
         {
             double acc[3] = { 0.0, 0.0, 0.0 };
-            int i = c;
-            if( c == lane )
+            int i = c + sub;
+            if( c == cl )
             {
-                // the lane's first column: its first PFS rows were loaded before the staging
+                // the slot's first column: its first PFS rows of this lane were loaded before the staging
                 #pragma unroll
                 for( int u = 0; u < PFS; ++u )
                 {
-                    if( u >= c && u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * u + k]; } }
+                    const int iu = c + sub + u * Lc;
+                    if( iu < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lp[u] * vs[3 * iu + k]; } }
                 }
-                i = c > PFS ? c : PFS;
+                i = c + sub + PFS * Lc;
             }
-            // the remaining rows in guarded batches of eight loads (sums in row order)
-            for( ; i < r; i += 8 )
+            // the remaining rows of the lane in guarded batches of eight loads (sums in row order)
+            for( ; i < r; i += 8 * Lc )
             {
                 double lv[8];
                 #pragma unroll
-                for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u < r ) ? Lt[(int64_t)( i + u ) * w + c] : 0.0; }
+                for( int u = 0; u < 8; ++u ) { lv[u] = ( i + u * Lc < r ) ? Lt[(int64_t)( i + u * Lc ) * w + c] : 0.0; }
                 #pragma unroll
                 for( int u = 0; u < 8; ++u )
                 {
-                    if( i + u < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * vs[3 * ( i + u ) + k]; } }
+                    if( i + u * Lc < r ) { for( int k = 0; k < 3; ++k ) { acc[k] += lv[u] * vs[3 * ( i + u * Lc ) + k]; } }
                 }
             }
-            for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + c ) + k] = acc[k]; }
+            for( int k = 0; k < 3; ++k ) { acc[k] = group_sum( acc[k], Lc ); }
+            if( sub == 0 ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + c ) + k] = acc[k]; } }
         }
     }
 
