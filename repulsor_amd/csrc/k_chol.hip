@@ -80,17 +80,28 @@ namespace rp
     // ---------------------------------------------------------------------------------------------
     // The merged front value of row a of supernode s: the supernode's own right-hand side (rows a < w,
     // already multiplied by alpha and permuted) or 0, plus the children's update rows that map to it, in
-    // child order (pull lists built at upload, the same additions in the same order as pushing the
+    // child order (pull entries built at upload, the same additions in the same order as pushing the
     // children one after another).
     __device__ __forceinline__ void chol_front_row( const CholDev & F, int64_t rb, int f, int w, int a, double v[3] )
     {
+        const int2 pr = F.pull2[rb + a];
         if( a < w ) { for( int k = 0; k < 3; ++k ) { v[k] = F.bp[3 * (int64_t)( f + a ) + k]; } }
         else        { for( int k = 0; k < 3; ++k ) { v[k] = 0.0; } }
-        const int32_t e1 = F.pull_ptr[rb + a + 1];
-        for( int32_t e = F.pull_ptr[rb + a]; e < e1; ++e )
+        if( pr.x >= 0 )
         {
-            const int64_t q = F.pull_src[e];
-            for( int k = 0; k < 3; ++k ) { v[k] += F.z[3 * q + k]; }
+            double u0[3], u1[3] = { 0.0, 0.0, 0.0 };
+            for( int k = 0; k < 3; ++k ) { u0[k] = F.z[3 * (int64_t)pr.x + k]; }
+            if( pr.y >= 0 ) { for( int k = 0; k < 3; ++k ) { u1[k] = F.z[3 * (int64_t)pr.y + k]; } }
+            for( int k = 0; k < 3; ++k ) { v[k] += u0[k]; }
+            if( pr.y >= 0 ) { for( int k = 0; k < 3; ++k ) { v[k] += u1[k]; } }
+        }
+        else if( pr.x < -1 )
+        {
+            for( int32_t e = -2 - pr.x; e < pr.y; ++e )
+            {
+                const int64_t q = F.pull_src[e];
+                for( int k = 0; k < 3; ++k ) { v[k] += F.z[3 * q + k]; }
+            }
         }
     }
 

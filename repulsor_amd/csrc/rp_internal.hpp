@@ -274,7 +274,7 @@ struct rp_ctx
     rp::DBuf<int64_t> h1_row_ptr, h1_val_ptr;
     rp::DBuf<double>  h1_vals, h1_z, h1_xp, h1_bp, h1_fr;  // h1_vals: inv(L_ss) over M = L_off inv(L_ss) per supernode
     rp::DBuf<double>  h1_vals_t;                     // row-major copies of the small levels' blocks
-    rp::DBuf<int32_t> h1_pull_ptr, h1_pull_src;      // per front row: the child update rows that map to it
+    rp::DBuf<int32_t> h1_pull2, h1_pull_src;         // per front row: the child update rows that map to it
     std::vector<int64_t> h1_level_ptr;   // supernodes grouped by height (leaves first)
     std::vector<int>     h1_level_wmax;
     std::vector<int>     h1_level_rmax;

@@ -97,8 +97,10 @@ namespace rp
         const int32_t * kids;       // children in increasing order
         const int32_t * cmap;       // per row entry: position in the parent's row list
         const int32_t * perm;       // new -> old
-        const int32_t * pull_ptr;   // rows + 1: per front row, its sources among the children's update rows
-        const int32_t * pull_src;   // source rows (global row index), in child order
+        const int2    * pull2;      // per front row: its sources among the children's update rows (global
+                                    // row indices, child order; -1 for none) when it has at most two, else
+                                    // x = -2 - (first position in pull_src) and y = the end of its run there
+        const int32_t * pull_src;   // source rows of the front rows with more than two, in child order
         const double  * bp;         // 3 x n: alpha B(perm)
         double        * z;          // 3 x rows: front vectors
         double        * xp;         // 3 x n: solution in the new numbering

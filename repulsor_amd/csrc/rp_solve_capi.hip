@@ -57,7 +57,7 @@ namespace rp
         F.val_ptr = ctx->h1_val_ptr.get(); F.vals = ctx->h1_vals.get(); F.vals_t = ctx->h1_vals_t.get();
         F.kid_ptr = ctx->h1_kid_ptr.get(); F.kids = ctx->h1_kids.get(); F.cmap = ctx->h1_cmap.get();
         F.perm = ctx->h1_perm.get(); F.z = ctx->h1_z.get(); F.xp = ctx->h1_xp.get();
-        F.pull_ptr = ctx->h1_pull_ptr.get(); F.pull_src = ctx->h1_pull_src.get(); F.bp = ctx->h1_bp.get();
+        F.pull2 = reinterpret_cast<const int2 *>( ctx->h1_pull2.get() ); F.pull_src = ctx->h1_pull_src.get(); F.bp = ctx->h1_bp.get();
         return F;
     }
 
@@ -804,6 +804,20 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 }
             }
         }
+        // the sources of a front row in one 8-byte entry (almost every row has at most two), so that the
+        // front merge waits for one index load instead of two
+        std::vector<int32_t> pull2( (size_t)( 2 * nrows ) );
+        for( int64_t q = 0; q < nrows; ++q )
+        {
+            const int32_t e0 = pull_ptr[(size_t)q], e1 = pull_ptr[(size_t)q + 1];
+            int32_t * o = pull2.data() + 2 * q;
+            if( e1 - e0 <= 2 )
+            {
+                o[0] = ( e1 > e0 ) ? pull_src[(size_t)e0] : -1;
+                o[1] = ( e1 > e0 + 1 ) ? pull_src[(size_t)e0 + 1] : -1;
+            }
+            else { o[0] = -2 - e0; o[1] = e1; }
+        }
         int32_t nl = 0;
         for( auto h : height ) { nl = std::max( nl, h + 1 ); }
         std::vector<int64_t> lptr( (size_t)nl + 1, 0 );
@@ -930,7 +944,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_kid_ptr.upload( kid_ptr.data(), kid_ptr.size(), ctx->stream );
         ctx->h1_kids.upload( kids.data(), kids.size(), ctx->stream );
         ctx->h1_cmap.upload( cmap.data(), cmap.size(), ctx->stream );
-        ctx->h1_pull_ptr.upload( pull_ptr.data(), pull_ptr.size(), ctx->stream );
+        ctx->h1_pull2.upload( pull2.data(), pull2.size(), ctx->stream );
         ctx->h1_pull_src.upload( pull_src.data(), pull_src.size(), ctx->stream );
         ctx->h1_bp.require( (size_t)( 3 * (int64_t)n ) );
         ctx->h1_perm.upload( perm, (size_t)n, ctx->stream );
