@@ -585,7 +585,6 @@ int rp_pl_compute( rp_ctx * ctx, double s, int precision )
         const int64_t T = ctx->T, C = ctx->C, L = ctx->L;
         const double s_exp = -0.5 * n - s;
         const Expo ex = make_expo( 0.0, 0.0, n, s_exp );
-        const bool mixed = precision == RP_MIXED;
         ctx->pl_nf_vals.require( (size_t)std::max<int64_t>( 1, ctx->nf_slots ) );
         ctx->pl_nf_diag.require( (size_t)T );
         ctx->pl_ff_vals.require( (size_t)std::max<int64_t>( 1, ctx->far_nnz ) );
