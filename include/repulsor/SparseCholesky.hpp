@@ -97,7 +97,7 @@ namespace Repulsor
     inline SupernodalFactor SparseCholeskyFactor( const Int n, const std::vector<LInt> & rp, const std::vector<Int> & ci,
                                                   const std::vector<double> & av, const std::vector<Int> & perm0,
                                                   const int threads = 1, const Int relax_cols = 16, const Int max_width = 2048,
-                                                  const bool chain_merge = true, const Int bottom_levels = 3 )
+                                                  const bool chain_merge = true, const Int bottom_levels = 4 )
     {
         ScopedTimer timer("SparseCholesky::Factor");
         SupernodalFactor F;
@@ -192,7 +192,9 @@ namespace Repulsor
             // differ): the tree loses bottom_levels - 1 levels at its widest, lowest part. Measured on the
             // GPU solves at 1.31M triangles: 2 (3 % more stored values) 0.749 -> 0.726 ms per solve; with
             // the batched loads and per-sweep lane-group choice of the GPU solves, 3 (21 -> 20 levels, 7 %
-            // more stored values) 0.605 -> 0.582 ms
+            // more stored values) 0.605 -> 0.582 ms; with the single-entry front pulls, 4 (19 levels, 68.4M
+            // instead of 60.3M stored values) 15 us less per solve, while 5 and 6 (18 and 17 levels, 85M and
+            // 117M values, lowest supernodes up to 182 and 304 rows) doubled the solve time
             const Int m = (Int)sb.size() - 1;
             std::vector<Int> sof( (size_t)n );
             for( Int t = 0; t < m; ++t ) { for( Int j = sb[(size_t)t]; j < sb[(size_t)t + 1]; ++j ) { sof[(size_t)j] = t; } }
