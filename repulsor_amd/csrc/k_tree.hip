@@ -426,11 +426,13 @@ namespace rp
     }
 
     // Y[v] = alpha * sum_{(i,slot) of v, increasing i} S_out[i][slot] + beta * Y[v]   (mi_post.Dot)
+    // perm non-null: row j of Y takes vertex perm[j] (the H^1 factor's numbering inside the preconditioner)
     __global__ void k_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out,
-                                 double alpha, double beta, double * Y, int64_t ldY )
+                                 double alpha, double beta, double * Y, int64_t ldY, const int32_t * perm )
     {
-        const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-        if( v >= V ) { return; }
+        const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( j >= V ) { return; }
+        const int64_t v = perm ? (int64_t)perm[j] : j;
         double s0 = 0, s1 = 0, s2 = 0;
         const int64_t kb = vp_ptr[v], ke = vp_ptr[v + 1];
         // up to eight entries (a vertex's valence is about six) are gathered at once, then summed in order
@@ -449,9 +451,15 @@ namespace rp
             #pragma unroll
             for( int u = 0; u < 8; ++u ) { if( e[u] >= 0 ) { s0 += t[u][0]; s1 += t[u][1]; s2 += t[u][2]; } }
         }
-        double * y = Y + ldY * v;
+        double * y = Y + ldY * j;
         if( beta == 0.0 ) { y[0] = alpha * s0; y[1] = alpha * s1; y[2] = alpha * s2; }
         else { y[0] = alpha * s0 + beta * y[0]; y[1] = alpha * s1 + beta * y[1]; y[2] = alpha * s2 + beta * y[2]; }
+    }
+
+    __global__ void k_map_index( int64_t n, const int32_t * map, const int32_t * in, int32_t * out )
+    {
+        const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        if( g < n ) { out[g] = map[in[g]]; }
     }
 
     __global__ void k_fill( double * p, int64_t n, double v )
@@ -551,8 +559,12 @@ namespace rp
         else            { hipLaunchKernelGGL( (k_post_prim<2>), grid1(T), dim3(256), 0, s, T, leaf_idx_of_prim, pre_vals, leaf_out, use_far, S_out ); }
     }
 
-    void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )
-    { if( V ) { hipLaunchKernelGGL( k_post_vert, grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, S_out, alpha, beta, Y, ldY ); } }
+    void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s,
+                           const int32_t * perm )
+    { if( V ) { hipLaunchKernelGGL( k_post_vert, grid1(V), dim3(256), 0, s, V, vp_ptr, vp_entry, S_out, alpha, beta, Y, ldY, perm ); } }
+
+    void launch_map_index( int64_t n, const int32_t * map, const int32_t * in, int32_t * out, hipStream_t s )
+    { if( n ) { hipLaunchKernelGGL( k_map_index, grid1(n), dim3(256), 0, s, n, map, in, out ); } }
 
     void launch_fill( double * p, int64_t n, double v, hipStream_t s )
     { if( n ) { hipLaunchKernelGGL( k_fill, grid1(n), dim3(256), 0, s, p, n, v ); } }

@@ -274,6 +274,7 @@ int rp_upload_tree( rp_ctx * ctx, int32_t cluster_count,
         ctx->leaf_of_prim.upload( leaf_of_prim.data(), (size_t)T, ctx->stream );
         ctx->leaf_cluster.upload( leaves.data(), (size_t)L, ctx->stream );
         ctx->pre_cols.upload( pre_cols, (size_t)(T * size), ctx->stream );
+        ++ctx->tree_version;
         ctx->pre_vals.upload( pre_vals, (size_t)(T * 4 * size), ctx->stream );
         ctx->vp_ptr.upload( vptr.data(), vptr.size(), ctx->stream );
         ctx->vp_entry.upload( ventry.data(), ventry.size(), ctx->stream );

@@ -294,7 +294,11 @@ struct rp_ctx
 
     // two-level preconditioner P0 + w diag(A)^-1: exact metric diagonal and the weight
     rp::DBuf<double> dA, dA_path;          // V, 2C
-    int64_t metric_version = 0, pl_version = 0, h1_version = 0;
+    int64_t metric_version = 0, pl_version = 0, h1_version = 0, tree_version = 0;
+    // inside the preconditioner the H^1 solves hand the pseudo-Laplacian their vectors in the factor's
+    // numbering: pre_cols mapped through the inverse permutation, rebuilt when the tree or factor changes
+    rp::DBuf<int32_t> h1_iperm, pre_cols_h1;
+    int64_t pre_cols_h1_key[2] = { -1, -1 };
     int64_t diag_version = -1;
     double pc_weight_setting = 0.0;        // 0: P0 only, > 0: fixed weight, < 0: estimated per state
     double pc_weight = 0.0;                // weight in use

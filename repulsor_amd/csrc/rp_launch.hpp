@@ -72,7 +72,10 @@ namespace rp
                               const double * leaf_out, int use_far, const double * pre_vals, double * S_out, hipStream_t s );
     void launch_ff_spmm( bool mixed, int64_t C, const int64_t * rp, const int32_t * ci, const void * vals, const double * diag, const double * xin, double * yout, hipStream_t s );
     void launch_post_prim( int size, int64_t T, const int32_t * leaf_idx_of_prim, const double * pre_vals, const double * leaf_out, int use_far, double * S_out, hipStream_t s );
-    void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
+    // perm non-null: row j of Y takes vertex perm[j] (Y in that numbering)
+    void launch_post_vert( int64_t V, const int64_t * vp_ptr, const int32_t * vp_entry, const double * S_out, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s,
+                           const int32_t * perm = nullptr );
+    void launch_map_index( int64_t n, const int32_t * map, const int32_t * in, int32_t * out, hipStream_t s );   // out[i] = map[in[i]]
     void launch_fill( double * p, int64_t n, double v, hipStream_t s );
 
     // pseudo-Laplacian (k_pl.hip)

@@ -14,7 +14,10 @@ namespace rp
     // Y = alpha L X + beta Y (device vectors). Reference: MetricDimRestricted::MultiplyMetric with the
     // LowOrder pre/post (src/Metrics/MetricDimRestricted.hpp:109-139) around
     // PseudoLaplacian_Traversor::MultiplyMetric (src/Metrics/PseudoLaplacian/PseudoLaplacian_Traversor.hpp:143-300).
-    static void pl_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy, bool nf_flag, bool ff_flag )
+    // xcols (may be null): the pre-operator's vertex index per primitive slot in place of pre_cols (X in
+    // another numbering); yperm (may be null): row j of Y takes vertex yperm[j].
+    static void pl_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy, bool nf_flag, bool ff_flag,
+                          const int32_t * xcols = nullptr, const int32_t * yperm = nullptr )
     {
         const int64_t T = ctx->T, V = ctx->V, C = ctx->C, L = ctx->L;
         const bool mixed = ctx->pl_precision != RP_FP64;   // the far field: FP32 unless FP64
@@ -23,7 +26,7 @@ namespace rp
         ctx->leaf_out.require( (size_t)(12 * L) );
         const bool do_nf = nf_flag && ctx->nf_cols > 0;
         const bool do_ff = ff_flag && ctx->far_nnz > 0;
-        launch_pl_pre( ctx->size, T, ctx->pre_cols.get(), ctx->pre_vals.get(), Xd, ldx, ctx->P_in.get(), ctx->stream );
+        launch_pl_pre( ctx->size, T, xcols ? xcols : ctx->pre_cols.get(), ctx->pre_vals.get(), Xd, ldx, ctx->P_in.get(), ctx->stream );
         if( do_ff )
         {
             launch_up_leaves( L, ctx->leaf_cluster.get(), ctx->C_begin.get(), ctx->C_end.get(), ctx->P_in.get(), ctx->C_in.get(), ctx->stream, 3 );
@@ -47,7 +50,7 @@ namespace rp
             launch_vf_pl_spmm_post( ctx->size, ctx->vf_nrows, ctx->vf_rows.get(), ctx->vf_rp.get(), ctx->vf_ci.get(), ctx->pl_vf_vals.get(),
                                     ctx->pl_vf_diag.get(), ctx->P_in.get(), ctx->pre_vals.get(), S_out, ctx->stream );
         }
-        launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream );
+        launch_post_vert( V, ctx->vp_ptr.get(), ctx->vp_entry.get(), S_out, alpha, beta, Yd, ldy, ctx->stream, yperm );
     }
 
     static CholDev chol_dev( rp_ctx * ctx )
@@ -62,12 +65,14 @@ namespace rp
     }
 
     // X = alpha H^-1 B + beta X (device vectors, V x 3): one launch per elimination level and sweep.
+    // Bd null: the right-hand side is already in h1_bp (alpha B in the factor's numbering); Xd null: the
+    // solution stays in h1_xp (factor numbering).
     static void h1_solve( rp_ctx * ctx, double alpha, const double * Bd, int64_t ldb, double beta, double * Xd, int64_t ldx )
     {
         const CholDev F = chol_dev( ctx );
         const size_t nl = ctx->h1_level_small.size();
         const CholTask * tasks = ctx->h1_tasks.get();
-        launch_chol_gather( ctx->h1_n, ctx->h1_perm.get(), alpha, Bd, ldb, ctx->h1_bp.get(), ctx->stream );
+        if( Bd ) { launch_chol_gather( ctx->h1_n, ctx->h1_perm.get(), alpha, Bd, ldb, ctx->h1_bp.get(), ctx->stream ); }
         for( size_t l = 0; l < nl; ++l )
         {
             if( ctx->h1_level_small[l] )
@@ -100,19 +105,39 @@ namespace rp
                 launch_chol_bwd_big( F, tasks + tb, te - tb, ctx->h1_level_rmax[l], ctx->h1_level_wmax[l] > CHOL_NARROW_WIDTH, ctx->stream );
             }
         }
-        launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream );
+        if( Xd ) { launch_chol_scatter( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), beta, Xd, ldx, ctx->stream ); }
     }
 
     // Y = alpha P0 X + beta Y with P0 = H^-1 L_{2-s} H^-1.
     // Reference: TangentPointMetric0::multiply_preconditioner (src/Metrics/TangentPointMetric0.hpp:88-126).
+    // Between the two H^1 solves the vectors stay in the factor's numbering: the first solve leaves its
+    // solution in h1_xp, the pseudo-Laplacian's pre-operator reads it through pre_cols mapped by the
+    // inverse permutation and its vertex gather writes row j = vertex perm[j] straight into h1_bp, the
+    // second solve's right-hand side; the same values and sums as permuting back and forth in between.
     static void pc0_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
     {
         const int64_t V = ctx->V;
-        ctx->pc_tmp.require( (size_t)(3 * V) );
-        double * Z = ctx->pc_tmp.get();
-        h1_solve( ctx, alpha, Xd, ldx, 0.0, Z, 3 );
-        if( alpha != 0.0 ) { pl_apply( ctx, 1.0, Z, 3, 0.0, Z, 3, true, true ); }
-        h1_solve( ctx, 1.0, Z, 3, beta, Yd, ldy );
+        if( alpha == 0.0 || ctx->h1_n != V )
+        {
+            ctx->pc_tmp.require( (size_t)(3 * V) );
+            double * Z = ctx->pc_tmp.get();
+            h1_solve( ctx, alpha, Xd, ldx, 0.0, Z, 3 );
+            if( alpha != 0.0 ) { pl_apply( ctx, 1.0, Z, 3, 0.0, Z, 3, true, true ); }
+            h1_solve( ctx, 1.0, Z, 3, beta, Yd, ldy );
+            return;
+        }
+        if( ctx->pre_cols_h1_key[0] != ctx->tree_version || ctx->pre_cols_h1_key[1] != ctx->h1_version )
+        {
+            const int64_t ne = ctx->T * ctx->size;
+            ctx->pre_cols_h1.require( (size_t)std::max<int64_t>( 1, ne ) );
+            launch_map_index( ne, ctx->h1_iperm.get(), ctx->pre_cols.get(), ctx->pre_cols_h1.get(), ctx->stream );
+            RP_HIP( hipGetLastError() );
+            ctx->pre_cols_h1_key[0] = ctx->tree_version;
+            ctx->pre_cols_h1_key[1] = ctx->h1_version;
+        }
+        h1_solve( ctx, alpha, Xd, ldx, 0.0, nullptr, 0 );
+        pl_apply( ctx, 1.0, ctx->h1_xp.get(), 3, 0.0, ctx->h1_bp.get(), 3, true, true, ctx->pre_cols_h1.get(), ctx->h1_perm.get() );
+        h1_solve( ctx, 1.0, nullptr, 0, beta, Yd, ldy );
     }
 
     // Exact diagonal of the assembled metric (cached per assembly).
@@ -947,6 +972,12 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_pull_src.upload( pull_src.data(), pull_src.size(), ctx->stream );
         ctx->h1_bp.require( (size_t)( 3 * (int64_t)n ) );
         ctx->h1_perm.upload( perm, (size_t)n, ctx->stream );
+        {
+            std::vector<int32_t> iperm( (size_t)n );
+            for( int64_t j = 0; j < n; ++j ) { iperm[(size_t)perm[j]] = (int32_t)j; }
+            ctx->h1_iperm.upload( iperm.data(), iperm.size(), ctx->stream );
+            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+        }
         ctx->h1_level_nodes.upload( lnodes.data(), lnodes.size(), ctx->stream );
         ctx->h1_z.require( (size_t)(3 * nrows) );
         ctx->h1_fr.require( (size_t)(3 * nrows) );
