@@ -492,6 +492,37 @@ namespace rp
         if( c < w && gl == 0 ) { for( int k = 0; k < 3; ++k ) { F.xp[3 * (int64_t)( f + c ) + k] = acc[k]; } }
     }
 
+    // The learned preconditioner's ends fused into the permutations (the same products as scaling
+    // separately): in, bp = sqrt_d B(perm); out, with t = xp + (c / dA) sqrt_d X (the diagonal term, c = 0:
+    // none), Y(perm) = alpha sqrt_d t + beta Y(perm).
+    __global__ void k_chol_gather_scaled( int64_t n, const int32_t * perm, const double * sd, const double * B, int64_t ldB, double * bp )
+    {
+        const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        if( j >= n ) { return; }
+        const int64_t o = perm[j];
+        const double c = 1.0 * sd[o];
+        for( int k = 0; k < 3; ++k ) { bp[3 * j + k] = c * B[ldB * o + k]; }
+    }
+
+    __global__ void k_chol_scatter_scaled( int64_t n, const int32_t * perm, const double * xp, const double * sd, const double * X, int64_t ldX,
+                                           double cw, const double * dA, double alpha, double beta, double * Y, int64_t ldY )
+    {
+        const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+        if( j >= n ) { return; }
+        const int64_t o = perm[j];
+        const double s1 = 1.0 * sd[o];
+        const double sj = ( cw != 0.0 && dA[o] > 0.0 ) ? cw / dA[o] : 0.0;
+        const double c = alpha * sd[o];
+        double * y = Y + ldY * o;
+        for( int k = 0; k < 3; ++k )
+        {
+            double t = xp[3 * j + k];
+            if( cw != 0.0 ) { t += sj * ( s1 * X[ldX * o + k] ); }
+            const double val = c * t;
+            y[k] = ( beta == 0.0 ) ? val : val + beta * y[k];
+        }
+    }
+
     __global__ void k_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY )
     {
         const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -577,6 +608,19 @@ namespace rp
     {
         if( count <= 0 ) { return; }
         hipLaunchKernelGGL( k_chol_transpose, dim3( (unsigned)count ), dim3( 256 ), 0, s, desc, count, vals, vals_t );
+    }
+
+    void launch_chol_gather_scaled( int64_t n, const int32_t * perm, const double * sd, const double * B, int64_t ldB, double * bp, hipStream_t s )
+    {
+        if( n <= 0 ) { return; }
+        hipLaunchKernelGGL( k_chol_gather_scaled, dim3( (unsigned)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, n, perm, sd, B, ldB, bp );
+    }
+
+    void launch_chol_scatter_scaled( int64_t n, const int32_t * perm, const double * xp, const double * sd, const double * X, int64_t ldX,
+                                     double cw, const double * dA, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s )
+    {
+        if( n <= 0 ) { return; }
+        hipLaunchKernelGGL( k_chol_scatter_scaled, dim3( (unsigned)( ( n + 255 ) / 256 ) ), dim3( 256 ), 0, s, n, perm, xp, sd, X, ldX, cw, dA, alpha, beta, Y, ldY );
     }
 
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s )

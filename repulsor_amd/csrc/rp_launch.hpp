@@ -138,6 +138,9 @@ namespace rp
     void launch_chol_bwd_big( const CholDev & F, const CholTask * tasks, int64_t count, int rmax, bool wide, hipStream_t s );
     void launch_chol_transpose( const CholTask * desc, int64_t count, const double * vals, double * vals_t, hipStream_t s );
     void launch_chol_scatter( int64_t n, const int32_t * perm, const double * xp, double beta, double * Y, int64_t ldY, hipStream_t s );
+    void launch_chol_gather_scaled( int64_t n, const int32_t * perm, const double * sd, const double * B, int64_t ldB, double * bp, hipStream_t s );
+    void launch_chol_scatter_scaled( int64_t n, const int32_t * perm, const double * xp, const double * sd, const double * X, int64_t ldX,
+                                     double cw, const double * dA, double alpha, double beta, double * Y, int64_t ldY, hipStream_t s );
 
     // Krylov vector kernels (k_solve.hip)
     void launch_dot3( int64_t n, const double * u, const double * v, double * part, double * out, hipStream_t s );   // part: 3 x 1024

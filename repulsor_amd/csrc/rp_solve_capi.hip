@@ -114,18 +114,11 @@ namespace rp
     // solution in h1_xp, the pseudo-Laplacian's pre-operator reads it through pre_cols mapped by the
     // inverse permutation and its vertex gather writes row j = vertex perm[j] straight into h1_bp, the
     // second solve's right-hand side; the same values and sums as permuting back and forth in between.
-    static void pc0_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    static bool pc0_fused( rp_ctx * ctx ) { return ctx->h1_n == ctx->V; }
+
+    // The pseudo-Laplacian step of P0 in the factor's numbering: h1_bp = L h1_xp.
+    static void pc0_middle( rp_ctx * ctx )
     {
-        const int64_t V = ctx->V;
-        if( alpha == 0.0 || ctx->h1_n != V )
-        {
-            ctx->pc_tmp.require( (size_t)(3 * V) );
-            double * Z = ctx->pc_tmp.get();
-            h1_solve( ctx, alpha, Xd, ldx, 0.0, Z, 3 );
-            if( alpha != 0.0 ) { pl_apply( ctx, 1.0, Z, 3, 0.0, Z, 3, true, true ); }
-            h1_solve( ctx, 1.0, Z, 3, beta, Yd, ldy );
-            return;
-        }
         if( ctx->pre_cols_h1_key[0] != ctx->tree_version || ctx->pre_cols_h1_key[1] != ctx->h1_version )
         {
             const int64_t ne = ctx->T * ctx->size;
@@ -135,8 +128,23 @@ namespace rp
             ctx->pre_cols_h1_key[0] = ctx->tree_version;
             ctx->pre_cols_h1_key[1] = ctx->h1_version;
         }
-        h1_solve( ctx, alpha, Xd, ldx, 0.0, nullptr, 0 );
         pl_apply( ctx, 1.0, ctx->h1_xp.get(), 3, 0.0, ctx->h1_bp.get(), 3, true, true, ctx->pre_cols_h1.get(), ctx->h1_perm.get() );
+    }
+
+    static void pc0_apply( rp_ctx * ctx, double alpha, const double * Xd, int64_t ldx, double beta, double * Yd, int64_t ldy )
+    {
+        const int64_t V = ctx->V;
+        if( alpha == 0.0 || !pc0_fused( ctx ) )
+        {
+            ctx->pc_tmp.require( (size_t)(3 * V) );
+            double * Z = ctx->pc_tmp.get();
+            h1_solve( ctx, alpha, Xd, ldx, 0.0, Z, 3 );
+            if( alpha != 0.0 ) { pl_apply( ctx, 1.0, Z, 3, 0.0, Z, 3, true, true ); }
+            h1_solve( ctx, 1.0, Z, 3, beta, Yd, ldy );
+            return;
+        }
+        h1_solve( ctx, alpha, Xd, ldx, 0.0, nullptr, 0 );
+        pc0_middle( ctx );
         h1_solve( ctx, 1.0, nullptr, 0, beta, Yd, ldy );
     }
 
@@ -247,6 +255,19 @@ namespace rp
     {
         if( !ctx->lp_enabled ) { pc2_apply( ctx, alpha, Xd, ldx, beta, Yd, ldy ); return; }
         const int64_t V = ctx->V;
+        if( alpha != 0.0 && pc0_fused( ctx ) )
+        {
+            // D^{1/2} folded into the first solve's gather, the diagonal term and the outer D^{1/2} into the
+            // last solve's scatter (the same products as the separate steps below)
+            launch_chol_gather_scaled( ctx->h1_n, ctx->h1_perm.get(), ctx->lp_sqrtd.get(), Xd, ldx, ctx->h1_bp.get(), ctx->stream );
+            h1_solve( ctx, 1.0, nullptr, 0, 0.0, nullptr, 0 );
+            pc0_middle( ctx );
+            h1_solve( ctx, 1.0, nullptr, 0, 0.0, nullptr, 0 );
+            const double cw = ctx->pc_weight > 0.0 ? 1.0 * ctx->pc_weight : 0.0;
+            launch_chol_scatter_scaled( ctx->h1_n, ctx->h1_perm.get(), ctx->h1_xp.get(), ctx->lp_sqrtd.get(), Xd, ldx,
+                                        cw, ctx->dA.get(), alpha, beta, Yd, ldy, ctx->stream );
+            return;
+        }
         ctx->lp_t1.require( (size_t)(3 * V) ); ctx->lp_t2.require( (size_t)(3 * V) );
         launch_row_scale( V, ctx->lp_sqrtd.get(), Xd, ldx, 1.0, 0.0, ctx->lp_t1.get(), 3, ctx->stream );
         pc2_apply( ctx, 1.0, ctx->lp_t1.get(), 3, 0.0, ctx->lp_t2.get(), 3 );
