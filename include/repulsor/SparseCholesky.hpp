@@ -373,24 +373,42 @@ namespace Repulsor
                 }
                 const Int t0 = kb + nb;
                 if( t0 >= r ) { continue; }
-                pan.assign( (size_t)(r - t0) * NB, 0.0 );
+                // the panel copied k-major (row i of the panel at [k][i - t0], padded by 4 zero rows), so that
+                // the trailing update runs 4 x 4 register blocks of entries: each entry's dot product over
+                // the panel still sums k = 0, 1, ... into its own accumulator (the same sums as one entry at
+                // a time), but 16 independent sums proceed together instead of one dependent chain
+                const Int np = r - t0 + 4;
+                pan.assign( (size_t)np * NB, 0.0 );
                 for( Int k = 0; k < nb; ++k )
                 {
                     const double * ck = &Fm[(size_t)(kb + k) * r];
-                    for( Int i = t0; i < r; ++i ) { pan[(size_t)(i - t0) * NB + k] = ck[i]; }
+                    double * pk = &pan[(size_t)k * np];
+                    for( Int i = t0; i < r; ++i ) { pk[i - t0] = ck[i]; }
                 }
                 auto upd_cols = [&]( Int jb, Int je )
                 {
-                    for( Int jj = jb; jj < je; ++jj )
+                    for( Int j0 = jb; j0 < je; j0 += 4 )
                     {
-                        const double * pj = &pan[(size_t)(jj - t0) * NB];
-                        double * cj = &Fm[(size_t)jj * r];
-                        for( Int i = jj; i < r; ++i )
+                        const Int jn = std::min<Int>( 4, je - j0 );
+                        for( Int i0 = j0; i0 < r; i0 += 4 )
                         {
-                            const double * pi = &pan[(size_t)(i - t0) * NB];
-                            double acc = 0.0;
-                            for( Int k = 0; k < NB; ++k ) { acc += pi[k] * pj[k]; }
-                            cj[i] -= acc;
+                            double acc[4][4] = { { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 } };
+                            for( Int k = 0; k < NB; ++k )
+                            {
+                                const double * pk = &pan[(size_t)k * np];
+                                const double a0 = pk[i0 - t0], a1 = pk[i0 - t0 + 1], a2 = pk[i0 - t0 + 2], a3 = pk[i0 - t0 + 3];
+                                const double b0 = pk[j0 - t0], b1 = pk[j0 - t0 + 1], b2 = pk[j0 - t0 + 2], b3 = pk[j0 - t0 + 3];
+                                acc[0][0] += a0 * b0; acc[0][1] += a0 * b1; acc[0][2] += a0 * b2; acc[0][3] += a0 * b3;
+                                acc[1][0] += a1 * b0; acc[1][1] += a1 * b1; acc[1][2] += a1 * b2; acc[1][3] += a1 * b3;
+                                acc[2][0] += a2 * b0; acc[2][1] += a2 * b1; acc[2][2] += a2 * b2; acc[2][3] += a2 * b3;
+                                acc[3][0] += a3 * b0; acc[3][1] += a3 * b1; acc[3][2] += a3 * b2; acc[3][3] += a3 * b3;
+                            }
+                            const Int in = std::min<Int>( 4, r - i0 );
+                            for( Int v = 0; v < jn; ++v )
+                            {
+                                double * cj = &Fm[(size_t)( j0 + v ) * r];
+                                for( Int u = 0; u < in; ++u ) { if( i0 + u >= j0 + v ) { cj[i0 + u] -= acc[u][v]; } }
+                            }
                         }
                     }
                 };
