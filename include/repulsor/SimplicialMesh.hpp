@@ -92,6 +92,7 @@ namespace Repulsor
         mutable std::unique_ptr<DeviceContext> dev;
         mutable bool dev_topology = false;
         mutable bool dev_data     = false;
+        mutable bool dev_h1_structure = false;   // the device holds the structure of h1_factor (its values may be older)
         // after a device-resident SemiStaticUpdate the device holds newer coordinates (coords_stale);
         // after any SemiStaticUpdate the host tree rows are recomputed only when a host computation
         // needs them (rows_stale)
@@ -380,7 +381,9 @@ namespace Repulsor
             Base_T::ClearCache();
             device_state.clear();
             h1.reset();
-            h1_factor.reset();
+            // a moved mesh keeps the H^1 pattern: the factor keeps its symbolic structure and is refactorised
+            // numerically on its next use
+            if( h1_factor ) { h1_values_stale = true; }
         }
 
         // Y = alpha * dE + beta * Y for the differential last computed on the device
@@ -398,6 +401,8 @@ namespace Repulsor
         double h1_mass_weight      = 1.0;
         mutable std::unique_ptr<CSRMatrix> h1;
         mutable std::unique_ptr<SupernodalFactor> h1_factor;
+        mutable bool h1_values_stale = false;   // h1_factor's structure is current, its values belong to older coordinates
+        mutable size_t h1_pattern_nnz = 0;      // nonzeros of the H^1 pattern h1_factor was analysed for
         mutable std::unique_ptr<std::vector<Int>> nd_ordering;   // persistent (depends on the tree only)
 
         mutable Int_ h1_solver_id = 0;
@@ -405,8 +410,8 @@ namespace Repulsor
     public:
         double H1StiffnessWeight() const override { return h1_stiffness_weight; }
         double H1MassWeight()      const override { return h1_mass_weight; }
-        void SetH1StiffnessWeight( const double w ) override { h1_stiffness_weight = w; h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
-        void SetH1MassWeight( const double w ) override      { h1_mass_weight = w;      h1.reset(); h1_factor.reset(); device_state.erase("h1"); }
+        void SetH1StiffnessWeight( const double w ) override { h1_stiffness_weight = w; h1.reset(); if( h1_factor ) { h1_values_stale = true; } device_state.erase("h1"); }
+        void SetH1MassWeight( const double w ) override      { h1_mass_weight = w;      h1.reset(); if( h1_factor ) { h1_values_stale = true; } device_state.erase("h1"); }
 
         // Ordering of the H^1 Cholesky factorisation (SimplicialMesh::SetH1SolverID, src/SimplicialMesh/H1Metric.hpp:174-257):
         // 0 = nested dissection on the cluster tree, the reference's default and the only ordering this
@@ -437,10 +442,24 @@ namespace Repulsor
 
         const SupernodalFactor & H1Solver() const
         {
+            if( h1_factor && h1_values_stale )
+            {
+                const CSRMatrix & H = H1Metric();
+                if( H.ci.size() == h1_pattern_nnz && (Int)H.n == h1_factor->n )
+                {
+                    // same pattern: numeric refactorisation on the kept structure (bitwise a fresh factor)
+                    SparseCholeskyNumeric( *h1_factor, H.rp, H.ci, H.v, threads );
+                    h1_values_stale = false;
+                }
+                else { h1_factor.reset(); }
+            }
             if( !h1_factor )
             {
                 const CSRMatrix & H = H1Metric();
                 h1_factor = std::make_unique<SupernodalFactor>( SparseCholeskyFactor( H.n, H.rp, H.ci, H.v, NestedDissection(), threads ) );
+                h1_pattern_nnz = H.ci.size();
+                h1_values_stale = false;
+                dev_h1_structure = false;
             }
             return *h1_factor;
         }
@@ -450,7 +469,7 @@ namespace Repulsor
         // The device context, created on first use (nothing uploaded yet).
         rp_ctx * Context() const
         {
-            if( !dev ) { dev = std::make_unique<DeviceContext>( device ); dev_topology = false; dev_data = false; }
+            if( !dev ) { dev = std::make_unique<DeviceContext>( device ); dev_topology = false; dev_data = false; dev_h1_structure = false; }
             return dev->get();
         }
 
@@ -496,10 +515,20 @@ namespace Repulsor
             if( !device_state.count( "h1" ) )
             {
                 const SupernodalFactor & F = H1Solver();
-                std::vector<int32_t> perm( F.perm.begin(), F.perm.end() );
-                auto st = rp_upload_h1_factor( c, F.n, F.ns, perm.data(), F.sn_begin.data(), F.row_ptr.data(), F.rows.data(),
-                                               F.val_ptr.data(), F.vals.data(), F.sn_parent.data() );
-                if( st != RP_OK ) { throw std::runtime_error( std::string("rp_upload_h1_factor: ") + rp_last_error() ); }
+                if( dev_h1_structure )
+                {
+                    // the device already holds this factor's structure: new values only
+                    auto st = rp_update_h1_values( c, F.vals.data() );
+                    if( st != RP_OK ) { throw std::runtime_error( std::string("rp_update_h1_values: ") + rp_last_error() ); }
+                }
+                else
+                {
+                    std::vector<int32_t> perm( F.perm.begin(), F.perm.end() );
+                    auto st = rp_upload_h1_factor( c, F.n, F.ns, perm.data(), F.sn_begin.data(), F.row_ptr.data(), F.rows.data(),
+                                                   F.val_ptr.data(), F.vals.data(), F.sn_parent.data() );
+                    if( st != RP_OK ) { throw std::runtime_error( std::string("rp_upload_h1_factor: ") + rp_last_error() ); }
+                    dev_h1_structure = true;
+                }
                 device_state["h1"] = 1;
             }
             return c;

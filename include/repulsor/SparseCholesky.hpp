@@ -94,6 +94,228 @@ namespace Repulsor
     };
 
     // A: symmetric CSR (full pattern, with diagonal), perm0: fill-reducing ordering new -> old.
+    // Numeric phase (reference: Solver_T::NumericFactorization, src/SimplicialMesh/H1Metric.hpp:256): the
+    // values of F for the matrix (rp, ci, av) on the structure F already holds (ordering, supernodes, row
+    // sets). A matrix of the same pattern with new values (the H^1 metric of a moved mesh) is refactorised
+    // without redoing the symbolic analysis; the result equals a fresh SparseCholeskyFactor bit for bit.
+    inline void SparseCholeskyNumeric( SupernodalFactor & F, const std::vector<LInt> & rp, const std::vector<Int> & ci,
+                                       const std::vector<double> & av, const int threads = 1 )
+    {
+        ScopedTimer timer("SparseCholesky::Numeric");
+        const Int n = F.n, ns = F.ns;
+        const std::vector<Int> & sb = F.sn_begin;
+        std::vector<std::vector<Int>> kids( (size_t)ns );
+        for( Int s = 0; s < ns; ++s ) { if( F.sn_parent[(size_t)s] >= 0 ) { kids[(size_t)F.sn_parent[(size_t)s]].push_back( s ); } }
+        if( F.vals.size() != (size_t)F.val_ptr.back() ) { F.vals.assign( (size_t)F.val_ptr.back(), 0.0 ); }
+        // ------------------------------------------------ numeric multifrontal factorisation
+        // Fronts are processed as soon as all their children are done (a ready queue shared by the
+        // worker threads); update matrices (lower triangles, column-major dense) are kept until the
+        // parent consumes them. Inside a front the first w columns are factorised by blocked
+        // right-looking Cholesky: panel of NB columns, then the trailing update with the panel
+        // copied row-major so that every entry update is one contiguous dot product.
+        std::vector<std::vector<double>> upd( (size_t)ns );
+        std::vector<std::atomic<Int>> pending( (size_t)ns );
+        for( Int s = 0; s < ns; ++s ) { pending[(size_t)s].store( (Int)kids[(size_t)s].size() ); }
+        std::mutex qm;
+        std::vector<Int> ready;
+        for( Int s = 0; s < ns; ++s ) { if( kids[(size_t)s].empty() ) { ready.push_back( s ); } }
+        std::reverse( ready.begin(), ready.end() );
+        std::atomic<Int> done{0};
+        std::atomic<bool> failed{false};
+        std::vector<double> flops_t( (size_t)std::max( 1, threads ), 0.0 );
+        constexpr Int NB = 32;
+
+        auto front = [&]( const Int s, std::vector<Int> & pos, double & fl )
+        {
+            const Int * R = F.rows.data() + F.row_ptr[(size_t)s];
+            const Int r = F.RowCount( s );
+            const Int f = sb[(size_t)s], l = sb[(size_t)s + 1];
+            const Int w = l - f;
+            for( Int q = 0; q < r; ++q ) { pos[(size_t)R[(size_t)q]] = q; }
+            std::vector<double> Fm( (size_t)r * r, 0.0 );   // column-major, lower part used
+            for( Int j = f; j < l; ++j )
+            {
+                const Int oj = F.perm[(size_t)j];
+                const Int cj = j - f;
+                for( LInt k = rp[(size_t)oj]; k < rp[(size_t)oj + 1]; ++k )
+                {
+                    const Int i = F.iperm[(size_t)ci[(size_t)k]];
+                    if( i >= j ) { Fm[(size_t)cj * r + pos[(size_t)i]] += av[(size_t)k]; }
+                }
+            }
+            for( Int c : kids[(size_t)s] )
+            {
+                const Int * Rc = F.rows.data() + F.row_ptr[(size_t)c];
+                const Int wc = sb[(size_t)c + 1] - sb[(size_t)c];
+                const Int uc = F.RowCount( c ) - wc;
+                const std::vector<double> & U = upd[(size_t)c];
+                for( Int b = 0; b < uc; ++b )
+                {
+                    const Int pb = pos[(size_t)Rc[(size_t)(wc + b)]];
+                    double * col = &Fm[(size_t)pb * r];
+                    const double * uu = &U[(size_t)b * uc];
+                    for( Int a = b; a < uc; ++a ) { col[pos[(size_t)Rc[(size_t)(wc + a)]]] += uu[a]; }
+                }
+                std::vector<double>().swap( upd[(size_t)c] );
+            }
+            std::vector<double> pan;   // row-major panel copy r x NB
+            for( Int kb = 0; kb < w; kb += NB )
+            {
+                const Int nb = std::min( NB, w - kb );
+                for( Int k = kb; k < kb + nb; ++k )
+                {
+                    double * ck = &Fm[(size_t)k * r];
+                    const double d = ck[k];
+                    if( !(d > 0) ) { failed = true; return; }
+                    const double ldk = std::sqrt( d );
+                    ck[k] = ldk;
+                    const double inv = 1.0 / ldk;
+                    for( Int i = k + 1; i < r; ++i ) { ck[i] *= inv; }
+                    for( Int jj = k + 1; jj < kb + nb; ++jj )
+                    {
+                        const double ljk = ck[jj];
+                        double * cj = &Fm[(size_t)jj * r];
+                        for( Int i = jj; i < r; ++i ) { cj[i] -= ck[i] * ljk; }
+                    }
+                }
+                const Int t0 = kb + nb;
+                if( t0 >= r ) { continue; }
+                // the panel copied k-major (row i of the panel at [k][i - t0], padded by 4 zero rows), so that
+                // the trailing update runs 4 x 4 register blocks of entries: each entry's dot product over
+                // the panel still sums k = 0, 1, ... into its own accumulator (the same sums as one entry at
+                // a time), but 16 independent sums proceed together instead of one dependent chain
+                const Int np = r - t0 + 4;
+                pan.assign( (size_t)np * NB, 0.0 );
+                for( Int k = 0; k < nb; ++k )
+                {
+                    const double * ck = &Fm[(size_t)(kb + k) * r];
+                    double * pk = &pan[(size_t)k * np];
+                    for( Int i = t0; i < r; ++i ) { pk[i - t0] = ck[i]; }
+                }
+                auto upd_cols = [&]( Int jb, Int je )
+                {
+                    for( Int j0 = jb; j0 < je; j0 += 4 )
+                    {
+                        const Int jn = std::min<Int>( 4, je - j0 );
+                        for( Int i0 = j0; i0 < r; i0 += 4 )
+                        {
+                            double acc[4][4] = { { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 } };
+                            for( Int k = 0; k < NB; ++k )
+                            {
+                                const double * pk = &pan[(size_t)k * np];
+                                const double a0 = pk[i0 - t0], a1 = pk[i0 - t0 + 1], a2 = pk[i0 - t0 + 2], a3 = pk[i0 - t0 + 3];
+                                const double b0 = pk[j0 - t0], b1 = pk[j0 - t0 + 1], b2 = pk[j0 - t0 + 2], b3 = pk[j0 - t0 + 3];
+                                acc[0][0] += a0 * b0; acc[0][1] += a0 * b1; acc[0][2] += a0 * b2; acc[0][3] += a0 * b3;
+                                acc[1][0] += a1 * b0; acc[1][1] += a1 * b1; acc[1][2] += a1 * b2; acc[1][3] += a1 * b3;
+                                acc[2][0] += a2 * b0; acc[2][1] += a2 * b1; acc[2][2] += a2 * b2; acc[2][3] += a2 * b3;
+                                acc[3][0] += a3 * b0; acc[3][1] += a3 * b1; acc[3][2] += a3 * b2; acc[3][3] += a3 * b3;
+                            }
+                            const Int in = std::min<Int>( 4, r - i0 );
+                            for( Int v = 0; v < jn; ++v )
+                            {
+                                double * cj = &Fm[(size_t)( j0 + v ) * r];
+                                for( Int u = 0; u < in; ++u ) { if( i0 + u >= j0 + v ) { cj[i0 + u] -= acc[u][v]; } }
+                            }
+                        }
+                    }
+                };
+                const Int ncols = r - t0;
+                if( (double)ncols * ncols * NB > 4e7 && threads > 1 )
+                {
+                    // split the columns of the trailing triangle into chunks of equal work
+                    const int tc = threads;
+                    std::vector<Int> cut( (size_t)tc + 1, r );
+                    cut[0] = t0;
+                    const double tot = 0.5 * (double)ncols * ncols;
+                    for( int q = 1; q < tc; ++q )
+                    {
+                        const double target = tot * (1.0 - (double)q / tc);   // work of columns >= cut is ~ (r-cut)^2/2
+                        cut[(size_t)q] = std::max( cut[(size_t)q - 1], (Int)( r - std::sqrt( 2.0 * target ) ) );
+                    }
+                    ParallelDo( tc, [&]( int q ) { upd_cols( cut[(size_t)q], cut[(size_t)q + 1] ); } );
+                }
+                else
+                {
+                    upd_cols( t0, r );
+                }
+                fl += 2.0 * (double)nb * ncols * ncols * 0.5 + (double)nb * nb * r;
+            }
+            double * Ls = &F.vals[(size_t)F.val_ptr[(size_t)s]];
+            for( Int c = 0; c < w; ++c ) { for( Int q = 0; q < r; ++q ) { Ls[(size_t)c * r + q] = (q >= c) ? Fm[(size_t)c * r + q] : 0.0; } }
+            {
+                // column-major inverse of the lower diagonal block, column c by forward substitution on
+                // e_c in column (axpy) order; wide blocks split their columns over the threads in chunks
+                // of equal work (column c costs about (w - c)^2 / 2)
+                std::vector<double> inv( (size_t)w * w, 0.0 );
+                auto inv_cols = [&]( Int cb, Int ce )
+                {
+                    for( Int c = cb; c < ce; ++c )
+                    {
+                        double * x = &inv[(size_t)c * w];
+                        x[c] = 1.0;
+                        for( Int b = c; b < w; ++b )
+                        {
+                            const double * lb = &Ls[(size_t)b * r];
+                            const double xb = x[b] / lb[b];
+                            x[b] = xb;
+                            for( Int a = b + 1; a < w; ++a ) { x[a] -= lb[a] * xb; }
+                        }
+                    }
+                };
+                if( w > 256 && threads > 1 )
+                {
+                    const int tc = threads;
+                    std::vector<Int> cut( (size_t)tc + 1, w );
+                    cut[0] = 0;
+                    const double tot = (double)w * w * w / 6.0;
+                    for( int q = 1; q < tc; ++q )
+                    {
+                        const double target = tot * ( 1.0 - (double)q / tc );   // work of the columns >= cut is ~ (w - cut)^3 / 6
+                        cut[(size_t)q] = std::max( cut[(size_t)q - 1], (Int)( w - std::cbrt( 6.0 * target ) ) );
+                    }
+                    ParallelDo( tc, [&]( int q ) { inv_cols( cut[(size_t)q], cut[(size_t)q + 1] ); } );
+                }
+                else { inv_cols( 0, w ); }
+                for( Int c = 0; c < w; ++c ) { for( Int a = 0; a < w; ++a ) { Ls[(size_t)c * r + a] = inv[(size_t)c * w + a]; } }
+            }
+            const Int u = r - w;
+            if( u > 0 && F.sn_parent[(size_t)s] >= 0 )
+            {
+                std::vector<double> & U = upd[(size_t)s];
+                U.assign( (size_t)u * u, 0.0 );
+                for( Int b = 0; b < u; ++b ) { for( Int a = b; a < u; ++a ) { U[(size_t)b * u + a] = Fm[(size_t)(w + b) * r + (w + a)]; } }
+            }
+            for( Int q = 0; q < r; ++q ) { pos[(size_t)R[(size_t)q]] = -1; }
+        };
+
+        const int tc = std::max( 1, threads );
+        ParallelDo( tc, [&]( int t )
+        {
+            std::vector<Int> pos( (size_t)n, -1 );
+            while( done.load() < ns && !failed.load() )
+            {
+                Int s = -1;
+                {
+                    std::lock_guard<std::mutex> lk( qm );
+                    if( !ready.empty() ) { s = ready.back(); ready.pop_back(); }
+                }
+                if( s < 0 ) { std::this_thread::yield(); continue; }
+                front( s, pos, flops_t[(size_t)t] );
+                const Int p = F.sn_parent[(size_t)s];
+                if( p >= 0 && pending[(size_t)p].fetch_sub( 1 ) == 1 )
+                {
+                    std::lock_guard<std::mutex> lk( qm );
+                    ready.push_back( p );
+                }
+                done.fetch_add( 1 );
+            }
+        });
+        if( failed.load() ) { throw std::runtime_error( "SparseCholesky: matrix is not positive definite" ); }
+        double flops = 0;
+        for( double x : flops_t ) { flops += x; }
+        F.flops = flops;
+    }
+
     inline SupernodalFactor SparseCholeskyFactor( const Int n, const std::vector<LInt> & rp, const std::vector<Int> & ci,
                                                   const std::vector<double> & av, const std::vector<Int> & perm0,
                                                   const int threads = 1, const Int relax_cols = 16, const Int max_width = 2048,
@@ -291,7 +513,6 @@ namespace Repulsor
             F.val_ptr[(size_t)s + 1] = F.val_ptr[(size_t)s] + r * w;
             F.nnz_L += w * (w + 1) / 2 + (r - w) * w;
         }
-        F.vals.assign( (size_t)F.val_ptr.back(), 0.0 );
         // levels (height in the supernode tree)
         F.sn_level.assign( (size_t)ns, 0 );
         for( Int s = 0; s < ns; ++s )
@@ -300,214 +521,7 @@ namespace Repulsor
             if( p >= 0 ) { F.sn_level[(size_t)p] = std::max( F.sn_level[(size_t)p], F.sn_level[(size_t)s] + 1 ); }
             F.levels = std::max( F.levels, F.sn_level[(size_t)s] + 1 );
         }
-        // ------------------------------------------------ numeric multifrontal factorisation
-        // Fronts are processed as soon as all their children are done (a ready queue shared by the
-        // worker threads); update matrices (lower triangles, column-major dense) are kept until the
-        // parent consumes them. Inside a front the first w columns are factorised by blocked
-        // right-looking Cholesky: panel of NB columns, then the trailing update with the panel
-        // copied row-major so that every entry update is one contiguous dot product.
-        std::vector<std::vector<double>> upd( (size_t)ns );
-        std::vector<std::atomic<Int>> pending( (size_t)ns );
-        for( Int s = 0; s < ns; ++s ) { pending[(size_t)s].store( (Int)kids[(size_t)s].size() ); }
-        std::mutex qm;
-        std::vector<Int> ready;
-        for( Int s = 0; s < ns; ++s ) { if( kids[(size_t)s].empty() ) { ready.push_back( s ); } }
-        std::reverse( ready.begin(), ready.end() );
-        std::atomic<Int> done{0};
-        std::atomic<bool> failed{false};
-        std::vector<double> flops_t( (size_t)std::max( 1, threads ), 0.0 );
-        constexpr Int NB = 32;
-
-        auto front = [&]( const Int s, std::vector<Int> & pos, double & fl )
-        {
-            const std::vector<Int> & R = RS[(size_t)s];
-            const Int r = (Int)R.size();
-            const Int f = sb[(size_t)s], l = sb[(size_t)s + 1];
-            const Int w = l - f;
-            for( Int q = 0; q < r; ++q ) { pos[(size_t)R[(size_t)q]] = q; }
-            std::vector<double> Fm( (size_t)r * r, 0.0 );   // column-major, lower part used
-            for( Int j = f; j < l; ++j )
-            {
-                const Int oj = F.perm[(size_t)j];
-                const Int cj = j - f;
-                for( LInt k = rp[(size_t)oj]; k < rp[(size_t)oj + 1]; ++k )
-                {
-                    const Int i = F.iperm[(size_t)ci[(size_t)k]];
-                    if( i >= j ) { Fm[(size_t)cj * r + pos[(size_t)i]] += av[(size_t)k]; }
-                }
-            }
-            for( Int c : kids[(size_t)s] )
-            {
-                const std::vector<Int> & Rc = RS[(size_t)c];
-                const Int wc = sb[(size_t)c + 1] - sb[(size_t)c];
-                const Int uc = (Int)Rc.size() - wc;
-                const std::vector<double> & U = upd[(size_t)c];
-                for( Int b = 0; b < uc; ++b )
-                {
-                    const Int pb = pos[(size_t)Rc[(size_t)(wc + b)]];
-                    double * col = &Fm[(size_t)pb * r];
-                    const double * uu = &U[(size_t)b * uc];
-                    for( Int a = b; a < uc; ++a ) { col[pos[(size_t)Rc[(size_t)(wc + a)]]] += uu[a]; }
-                }
-                std::vector<double>().swap( upd[(size_t)c] );
-            }
-            std::vector<double> pan;   // row-major panel copy r x NB
-            for( Int kb = 0; kb < w; kb += NB )
-            {
-                const Int nb = std::min( NB, w - kb );
-                for( Int k = kb; k < kb + nb; ++k )
-                {
-                    double * ck = &Fm[(size_t)k * r];
-                    const double d = ck[k];
-                    if( !(d > 0) ) { failed = true; return; }
-                    const double ldk = std::sqrt( d );
-                    ck[k] = ldk;
-                    const double inv = 1.0 / ldk;
-                    for( Int i = k + 1; i < r; ++i ) { ck[i] *= inv; }
-                    for( Int jj = k + 1; jj < kb + nb; ++jj )
-                    {
-                        const double ljk = ck[jj];
-                        double * cj = &Fm[(size_t)jj * r];
-                        for( Int i = jj; i < r; ++i ) { cj[i] -= ck[i] * ljk; }
-                    }
-                }
-                const Int t0 = kb + nb;
-                if( t0 >= r ) { continue; }
-                // the panel copied k-major (row i of the panel at [k][i - t0], padded by 4 zero rows), so that
-                // the trailing update runs 4 x 4 register blocks of entries: each entry's dot product over
-                // the panel still sums k = 0, 1, ... into its own accumulator (the same sums as one entry at
-                // a time), but 16 independent sums proceed together instead of one dependent chain
-                const Int np = r - t0 + 4;
-                pan.assign( (size_t)np * NB, 0.0 );
-                for( Int k = 0; k < nb; ++k )
-                {
-                    const double * ck = &Fm[(size_t)(kb + k) * r];
-                    double * pk = &pan[(size_t)k * np];
-                    for( Int i = t0; i < r; ++i ) { pk[i - t0] = ck[i]; }
-                }
-                auto upd_cols = [&]( Int jb, Int je )
-                {
-                    for( Int j0 = jb; j0 < je; j0 += 4 )
-                    {
-                        const Int jn = std::min<Int>( 4, je - j0 );
-                        for( Int i0 = j0; i0 < r; i0 += 4 )
-                        {
-                            double acc[4][4] = { { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 }, { 0.0, 0.0, 0.0, 0.0 } };
-                            for( Int k = 0; k < NB; ++k )
-                            {
-                                const double * pk = &pan[(size_t)k * np];
-                                const double a0 = pk[i0 - t0], a1 = pk[i0 - t0 + 1], a2 = pk[i0 - t0 + 2], a3 = pk[i0 - t0 + 3];
-                                const double b0 = pk[j0 - t0], b1 = pk[j0 - t0 + 1], b2 = pk[j0 - t0 + 2], b3 = pk[j0 - t0 + 3];
-                                acc[0][0] += a0 * b0; acc[0][1] += a0 * b1; acc[0][2] += a0 * b2; acc[0][3] += a0 * b3;
-                                acc[1][0] += a1 * b0; acc[1][1] += a1 * b1; acc[1][2] += a1 * b2; acc[1][3] += a1 * b3;
-                                acc[2][0] += a2 * b0; acc[2][1] += a2 * b1; acc[2][2] += a2 * b2; acc[2][3] += a2 * b3;
-                                acc[3][0] += a3 * b0; acc[3][1] += a3 * b1; acc[3][2] += a3 * b2; acc[3][3] += a3 * b3;
-                            }
-                            const Int in = std::min<Int>( 4, r - i0 );
-                            for( Int v = 0; v < jn; ++v )
-                            {
-                                double * cj = &Fm[(size_t)( j0 + v ) * r];
-                                for( Int u = 0; u < in; ++u ) { if( i0 + u >= j0 + v ) { cj[i0 + u] -= acc[u][v]; } }
-                            }
-                        }
-                    }
-                };
-                const Int ncols = r - t0;
-                if( (double)ncols * ncols * NB > 4e7 && threads > 1 )
-                {
-                    // split the columns of the trailing triangle into chunks of equal work
-                    const int tc = threads;
-                    std::vector<Int> cut( (size_t)tc + 1, r );
-                    cut[0] = t0;
-                    const double tot = 0.5 * (double)ncols * ncols;
-                    for( int q = 1; q < tc; ++q )
-                    {
-                        const double target = tot * (1.0 - (double)q / tc);   // work of columns >= cut is ~ (r-cut)^2/2
-                        cut[(size_t)q] = std::max( cut[(size_t)q - 1], (Int)( r - std::sqrt( 2.0 * target ) ) );
-                    }
-                    ParallelDo( tc, [&]( int q ) { upd_cols( cut[(size_t)q], cut[(size_t)q + 1] ); } );
-                }
-                else
-                {
-                    upd_cols( t0, r );
-                }
-                fl += 2.0 * (double)nb * ncols * ncols * 0.5 + (double)nb * nb * r;
-            }
-            double * Ls = &F.vals[(size_t)F.val_ptr[(size_t)s]];
-            for( Int c = 0; c < w; ++c ) { for( Int q = 0; q < r; ++q ) { Ls[(size_t)c * r + q] = (q >= c) ? Fm[(size_t)c * r + q] : 0.0; } }
-            {
-                // column-major inverse of the lower diagonal block, column c by forward substitution on
-                // e_c in column (axpy) order; wide blocks split their columns over the threads in chunks
-                // of equal work (column c costs about (w - c)^2 / 2)
-                std::vector<double> inv( (size_t)w * w, 0.0 );
-                auto inv_cols = [&]( Int cb, Int ce )
-                {
-                    for( Int c = cb; c < ce; ++c )
-                    {
-                        double * x = &inv[(size_t)c * w];
-                        x[c] = 1.0;
-                        for( Int b = c; b < w; ++b )
-                        {
-                            const double * lb = &Ls[(size_t)b * r];
-                            const double xb = x[b] / lb[b];
-                            x[b] = xb;
-                            for( Int a = b + 1; a < w; ++a ) { x[a] -= lb[a] * xb; }
-                        }
-                    }
-                };
-                if( w > 256 && threads > 1 )
-                {
-                    const int tc = threads;
-                    std::vector<Int> cut( (size_t)tc + 1, w );
-                    cut[0] = 0;
-                    const double tot = (double)w * w * w / 6.0;
-                    for( int q = 1; q < tc; ++q )
-                    {
-                        const double target = tot * ( 1.0 - (double)q / tc );   // work of the columns >= cut is ~ (w - cut)^3 / 6
-                        cut[(size_t)q] = std::max( cut[(size_t)q - 1], (Int)( w - std::cbrt( 6.0 * target ) ) );
-                    }
-                    ParallelDo( tc, [&]( int q ) { inv_cols( cut[(size_t)q], cut[(size_t)q + 1] ); } );
-                }
-                else { inv_cols( 0, w ); }
-                for( Int c = 0; c < w; ++c ) { for( Int a = 0; a < w; ++a ) { Ls[(size_t)c * r + a] = inv[(size_t)c * w + a]; } }
-            }
-            const Int u = r - w;
-            if( u > 0 && F.sn_parent[(size_t)s] >= 0 )
-            {
-                std::vector<double> & U = upd[(size_t)s];
-                U.assign( (size_t)u * u, 0.0 );
-                for( Int b = 0; b < u; ++b ) { for( Int a = b; a < u; ++a ) { U[(size_t)b * u + a] = Fm[(size_t)(w + b) * r + (w + a)]; } }
-            }
-            for( Int q = 0; q < r; ++q ) { pos[(size_t)R[(size_t)q]] = -1; }
-        };
-
-        const int tc = std::max( 1, threads );
-        ParallelDo( tc, [&]( int t )
-        {
-            std::vector<Int> pos( (size_t)n, -1 );
-            while( done.load() < ns && !failed.load() )
-            {
-                Int s = -1;
-                {
-                    std::lock_guard<std::mutex> lk( qm );
-                    if( !ready.empty() ) { s = ready.back(); ready.pop_back(); }
-                }
-                if( s < 0 ) { std::this_thread::yield(); continue; }
-                front( s, pos, flops_t[(size_t)t] );
-                const Int p = F.sn_parent[(size_t)s];
-                if( p >= 0 && pending[(size_t)p].fetch_sub( 1 ) == 1 )
-                {
-                    std::lock_guard<std::mutex> lk( qm );
-                    ready.push_back( p );
-                }
-                done.fetch_add( 1 );
-            }
-        });
-        if( failed.load() ) { throw std::runtime_error( "SparseCholesky: matrix is not positive definite" ); }
-        double flops = 0;
-        for( double x : flops_t ) { flops += x; }
-        F.flops = flops;
-        (void)threads;
+        SparseCholeskyNumeric( F, rp, ci, av, threads );
         return F;
     }
 

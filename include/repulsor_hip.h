@@ -258,6 +258,13 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                          const int64_t * row_ptr, const int32_t * rows, const int64_t * val_ptr, const double * vals,
                          const int32_t * sn_parent );
 
+/* New values (val_ptr[ns] doubles, the layout above) for the factor structure the last rp_upload_h1_factor
+ * uploaded: the numeric refactorisation of a moved mesh, whose H^1 metric keeps its pattern, reuses every
+ * structure the device built from the symbolic factor (kept until rp_upload_mesh).
+ * Replaces: Solver_T::NumericFactorization inside SimplicialMesh::H1Solver (reference
+ * src/SimplicialMesh/H1Metric.hpp:256). */
+int rp_update_h1_values( rp_ctx * ctx, const double * vals );
+
 /* X = alpha * H^-1 B + beta * X (nrhs must be 3).
  * Replaces: H1Solver().Solve<3>(alpha, B, ldB, beta, X, ldX, nrhs) (reference src/Metrics/TangentPointMetric0.hpp:104-124). */
 int rp_h1_solve( rp_ctx * ctx, double alpha, const double * B, int64_t ldB, double beta, double * X, int64_t ldX,

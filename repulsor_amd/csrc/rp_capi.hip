@@ -185,7 +185,7 @@ int rp_upload_mesh( rp_ctx * ctx, int dom_dim, int64_t vertex_count, int64_t sim
         ctx->vc_corner.upload( corner.data(), corner.size(), ctx->stream );
         RP_HIP( hipStreamSynchronize( ctx->stream ) );
         ctx->have_tree = ctx->have_near = ctx->have_far = ctx->have_data = false;
-        ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = false;
+        ctx->have_diff = ctx->have_metric = ctx->have_pl = ctx->have_h1 = ctx->have_h1_structure = false;
     });
 }
 

@@ -286,7 +286,9 @@ struct rp_ctx
     std::vector<int64_t> h1_task_ptr[4];           // per kind (forward, backward, forming M, split forward): level offsets into h1_tasks
     std::vector<char>    h1_level_split;           // the forward of the level runs in two launches (front, then 16-row tasks)
     int64_t h1_n = 0, h1_ns = 0;
-    bool have_h1 = false;
+    int64_t h1_nvals = 0, h1_vt_extent = 1;   // stored factor values; slots of the row-major copies
+    bool have_h1 = false;             // factor values of the current coordinates
+    bool have_h1_structure = false;   // the structure of the last uploaded factor (kept while the topology is)
 
     // Krylov solver state
     rp::DBuf<double> cg_x, cg_r, cg_p, cg_Ap, cg_z, cg_b, pc_tmp, cg_scal, cg_part;

@@ -762,6 +762,47 @@ int rp_pl_multiply( rp_ctx * ctx, double alpha, const double * X, int64_t ldX, d
     });
 }
 
+namespace
+{
+    // The factor's values on the structure of its tasks: the uploaded blocks hold L_off below inv(L_ss);
+    // the device keeps M = L_off inv(L_ss) there, and row-major copies of the blocks of every level whose
+    // backward runs lane groups (same offsets as in h1_vals; descriptors h1_sdesc, levels lptr).
+    void h1_load_values( rp_ctx * ctx, const double * vals, int64_t nvals, int64_t mb, int64_t me,
+                         const std::vector<int64_t> & lptr, const std::vector<char> & small_bwd, int64_t vt_extent )
+    {
+        rp::DBuf<double> lin;
+        lin.upload( vals, (size_t)nvals, ctx->stream );
+        ctx->h1_vals.require( (size_t)nvals );
+        launch_chol_form_m( ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
+        RP_HIP( hipGetLastError() );
+        ctx->h1_vals_t.require( (size_t)vt_extent );
+        const int64_t nl = (int64_t)small_bwd.size();
+        for( int64_t l = 0; l < nl; ++l )
+        {
+            if( !small_bwd[(size_t)l] ) { continue; }
+            launch_chol_transpose( ctx->h1_sdesc.get() + lptr[(size_t)l], lptr[(size_t)l + 1] - lptr[(size_t)l], ctx->h1_vals.get(), ctx->h1_vals_t.get(), ctx->stream );
+        }
+        RP_HIP( hipGetLastError() );
+        RP_HIP( hipStreamSynchronize( ctx->stream ) );
+    }
+}
+
+int rp_update_h1_values( rp_ctx * ctx, const double * vals )
+{
+    return guarded( "rp_update_h1_values", [&]()
+    {
+        require( ctx != nullptr, "ctx is NULL" );
+        require( ctx->have_h1_structure, "no H1 factor structure uploaded (call rp_upload_h1_factor)", RP_ENOTREADY );
+        require( vals != nullptr, "missing factor values" );
+        RP_HIP( hipSetDevice( ctx->device ) );
+        const size_t nl = ctx->h1_level_small_bwd.size();
+        h1_load_values( ctx, vals, ctx->h1_nvals, ctx->h1_task_ptr[2][0], ctx->h1_task_ptr[2][nl], ctx->h1_level_ptr,
+                        ctx->h1_level_small_bwd, ctx->h1_vt_extent );
+        ctx->have_h1 = true;
+        ++ctx->h1_version;
+    });
+}
+
 int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * perm, const int32_t * sn_begin,
                          const int64_t * row_ptr, const int32_t * rows, const int64_t * val_ptr, const double * vals,
                          const int32_t * sn_parent )
@@ -962,13 +1003,6 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_rows.upload( rows, (size_t)nrows, ctx->stream );
         ctx->h1_val_ptr.upload( val_ptr, (size_t)ns + 1, ctx->stream );
         {
-            // the uploaded blocks hold L_off below inv(L_ss); the device keeps M = L_off inv(L_ss) there
-            rp::DBuf<double> lin;
-            lin.upload( vals, (size_t)val_ptr[ns], ctx->stream );
-            ctx->h1_vals.require( (size_t)val_ptr[ns] );
-            const int64_t mb = tptr[2][0], me = tptr[2][(size_t)nl];
-            launch_chol_form_m( ctx->h1_tasks.get() + mb, me - mb, lin.get(), ctx->h1_vals.get(), ctx->stream );
-            RP_HIP( hipGetLastError() );
             // row-major copies of the blocks of every level whose backward runs lane groups (same offsets
             // as in h1_vals)
             int64_t extent = 1;
@@ -977,14 +1011,9 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
                 if( !small_bwd[(size_t)l] ) { continue; }
                 for( int64_t k = lptr[(size_t)l]; k < lptr[(size_t)l + 1]; ++k ) { extent = std::max<int64_t>( extent, sdesc[(size_t)k].vp + (int64_t)sdesc[(size_t)k].w * sdesc[(size_t)k].r ); }
             }
-            ctx->h1_vals_t.require( (size_t)extent );
-            for( int32_t l = 0; l < nl; ++l )
-            {
-                if( !small_bwd[(size_t)l] ) { continue; }
-                launch_chol_transpose( ctx->h1_sdesc.get() + lptr[(size_t)l], lptr[(size_t)l + 1] - lptr[(size_t)l], ctx->h1_vals.get(), ctx->h1_vals_t.get(), ctx->stream );
-            }
-            RP_HIP( hipGetLastError() );
-            RP_HIP( hipStreamSynchronize( ctx->stream ) );
+            h1_load_values( ctx, vals, val_ptr[ns], tptr[2][0], tptr[2][(size_t)nl], lptr, small_bwd, extent );
+            ctx->h1_nvals = val_ptr[ns];
+            ctx->h1_vt_extent = extent;
         }
         ctx->h1_kid_ptr.upload( kid_ptr.data(), kid_ptr.size(), ctx->stream );
         ctx->h1_kids.upload( kids.data(), kids.size(), ctx->stream );
@@ -1015,6 +1044,7 @@ int rp_upload_h1_factor( rp_ctx * ctx, int32_t n, int32_t ns, const int32_t * pe
         ctx->h1_n = n;
         ctx->h1_ns = ns;
         ctx->have_h1 = true;
+        ctx->have_h1_structure = true;
         ++ctx->h1_version;
     });
 }

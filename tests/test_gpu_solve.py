@@ -102,6 +102,26 @@ def test_h1_solve_device(name):
     assert rel(h1_matrix(M) @ M.h1_solve(B), B) <= 1e-11
 
 
+@pytest.mark.parametrize("name", ["torus_2400", "trefoil_3000"])
+def test_h1_solve_device_after_update(name):
+    """After SemiStaticUpdate the device keeps the factor's structure and takes the numerically
+    refactorised values only (rp_update_h1_values): the device solve matches the moved mesh's H^1
+    system and the host solve with the same factor."""
+    wl, M = _mesh(name)
+    print(f"case {name}", flush=True)
+    B = random_field(M.V, seed=13)
+    M.h1_solve(B)                                # factor structure on the device
+    X0 = M.VertexCoordinates()
+    M.SemiStaticUpdate(X0 + 0.01 * np.random.default_rng(3).standard_normal(X0.shape))
+    X = M.h1_solve(B)
+    H = h1_matrix(M)
+    assert rel(H @ X, B) <= 1e-11
+    Xh, _ = M.h1_solve_host(B)
+    assert rel(X, Xh) <= 1e-12
+    ref = spla.spsolve(H.tocsc(), B)
+    assert rel(X, ref) <= 1e-10
+
+
 @pytest.mark.parametrize("name", CASES)
 def test_preconditioner(name):
     wl, M = _mesh(name)

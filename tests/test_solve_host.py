@@ -87,6 +87,26 @@ def test_h1_metric_closed_forms(which, torus, curve):
 
 
 @pytest.mark.parametrize("which", ["torus", "curve"])
+def test_numeric_refactorisation_after_update(which):
+    """A moved mesh keeps its H^1 pattern: SemiStaticUpdate keeps the factor's symbolic structure and its
+    next use refactorises the values only (SparseCholeskyNumeric). The solve after the move must solve the
+    moved mesh's H^1 system, not the old one (which differs by far more than the tolerance)."""
+    X, S = R.make_torus(40, 20, 1.0, 0.5, 1e-3, 1) if which == "torus" else R.make_trefoil(600, 16, 0.02, 3)
+    M = _mesh(X, S)
+    rng = np.random.default_rng(11)
+    B = rng.standard_normal((M.V, 3))
+    X_old, _ = M.h1_solve_host(B)
+    H_old = h1_matrix(M)
+    X2 = X + 0.01 * rng.standard_normal(X.shape)
+    M.SemiStaticUpdate(X2)
+    X_new, _ = M.h1_solve_host(B)
+    H_new = h1_matrix(M)
+    assert rel(H_new @ X_new, B) <= 1e-12
+    assert rel(X_new, spla.spsolve(H_new.tocsc(), B)) <= 1e-10
+    assert rel(H_new @ X_old, B) > 1e-6      # the move changed the system
+
+
+@pytest.mark.parametrize("which", ["torus", "curve"])
 def test_cholesky_factor_solves(which, torus, curve):
     X, S, M = torus if which == "torus" else curve
     perm = M.nd_ordering()
